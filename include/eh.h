@@ -1,0 +1,257 @@
+/*
+ * eh.h -- C ABI of the MI355X (gfx950) set-intersection pattern engine.
+ *
+ * This is the drop-in boundary for EmptyHeaded's hot path: SparseMatrix
+ * construction with a per-neighbourhood layout choice, the ops::set_intersect
+ * family, and the undirected_triangle_counting / n_clique apps.  Every entry
+ * point takes plain pointers and sizes, returns an eh_status, and never throws.
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference repository root).
+ *
+ * Threading: a context owns one HIP stream on one device.  Calls on different
+ * contexts may run concurrently; calls on one context must be serialised by
+ * the caller.  Graphs, matrices and batches belong to the context that made
+ * them.
+ */
+#ifndef EH_H
+#define EH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EH_ABI_VERSION 1
+
+typedef enum {
+  EH_OK = 0,
+  EH_ERR_INVALID = 1,     /* bad argument, malformed CSR, unknown layout or type pair */
+  EH_ERR_OOM = 2,         /* device or host allocation failed */
+  EH_ERR_HIP = 3,         /* HIP runtime error */
+  EH_ERR_IO = 4,          /* file could not be read or written */
+  EH_ERR_UNSUPPORTED = 5, /* request outside what this build implements */
+  EH_ERR_CAPACITY = 6     /* caller-provided buffer too small */
+} eh_status;
+
+/* Set type tags, same values as common::type (src/common.hpp:171-181). */
+typedef enum { EH_BITSET = 0, EH_PSHORT = 1, EH_UINTEGER = 2, EH_HYBRID = 5 } eh_set_type;
+
+/* Matrix layout policy (src/emptyheaded.hpp:45-56 --layout, plus the
+ * selectors this engine adds). */
+typedef enum {
+  EH_LAYOUT_UINT = 0,     /* every row uinteger            (--layout=uint)   */
+  EH_LAYOUT_PSHORT = 1,   /* every row pshort              (--layout=pshort) */
+  EH_LAYOUT_BITSET = 2,   /* every row bitset              (--layout=bs)     */
+  EH_LAYOUT_HYBRID = 3,   /* hybrid::get_type, uint/bitset (--layout=hybrid) */
+  EH_LAYOUT_HYBRID3 = 4,  /* hybrid with compute_type's pshort band          */
+  EH_LAYOUT_LEARNED = 5,  /* per-row choice by the learned selector          */
+  EH_LAYOUT_COST = 6      /* per-row choice by the byte-cost model           */
+} eh_layout_policy;
+
+/* Edge selection applied by the matrix build (SparseMatrix.hpp:249-266). */
+typedef enum {
+  EH_FILTER_ALL = 0,   /* keep every entry (edge_selection == true)        */
+  EH_FILTER_LOWER = 1, /* keep nbr < node (apps' orientation predicate)    */
+  EH_FILTER_MASK = 2   /* keep entries whose byte in edge_mask is nonzero  */
+} eh_edge_filter;
+
+/* Execution schedule of the counting kernels. */
+typedef enum {
+  EH_SCHED_PLAIN = 0, /* reference dispatch: each row staged in its own layout,
+                         per-pair kernel by (typeA,typeB) and the size ratio  */
+  EH_SCHED_TUNED = 1  /* source rows staged as LDS bitmaps when they fit,
+                         autotuned task split and lane grouping            */
+} eh_schedule;
+
+typedef struct eh_ctx eh_ctx;
+typedef struct eh_graph eh_graph;
+typedef struct eh_matrix eh_matrix;
+typedef struct eh_batch eh_batch;
+
+/* Host view of one set, in the reference's in-memory formats (Set.hpp:17-21):
+ * UINTEGER data = uint32[cardinality]; BITSET data = [u64 first word][u64 words];
+ * PSHORT data = u16 stream of [prefix, count-1, low16...] partitions. */
+typedef struct {
+  const void *data;
+  uint64_t cardinality;
+  uint64_t number_of_bytes;
+  int32_t type; /* eh_set_type */
+  int32_t reserved;
+} eh_row_view;
+
+typedef struct {
+  uint32_t n;            /* vertices */
+  uint32_t max_degree;   /* largest row */
+  uint64_t entries;      /* stored entries (2x undirected edges for a graph) */
+} eh_graph_info;
+
+typedef struct {
+  uint32_t n;
+  uint32_t max_row;
+  uint64_t cardinality;  /* kept entries (oriented edges under EH_FILTER_LOWER) */
+  uint64_t rows_by_type[3]; /* non-empty rows typed BITSET, PSHORT, UINTEGER */
+  uint64_t payload_bytes;   /* bytes of the B-side payload (uint + pshort + bitset) */
+  uint64_t ref_row_bytes;   /* bytes the reference's row images would take */
+  uint64_t wedges;          /* W = sum over kept (i,j) of card(j) */
+  double build_ms;          /* device time of the build (filter+stats+layouts) */
+} eh_matrix_info;
+
+typedef struct {
+  int32_t schedule;       /* eh_schedule */
+  int32_t lanes_per_row;  /* 0 = autotuned/default; else 4,8,16,32,64 */
+  uint32_t task_work;     /* 0 = default; target work units per task */
+  uint32_t lds_words;     /* 0 = default; per-wave LDS budget in 32-bit words */
+  int32_t waves_per_block;/* 0 = default (4) */
+  int32_t repeat;         /* run the counting kernel this many times (>=1); stats average */
+  int32_t reserved[2];
+} eh_run_opts;
+
+typedef struct {
+  double kernel_ms;       /* average device time of one counting pass (HIP events) */
+  double sched_ms;        /* device time to build the task schedule (0 if cached) */
+  uint64_t tasks;
+  uint64_t alg_bytes;     /* algorithmic bytes of one pass (canonical uint32 CSR) */
+  uint64_t pairs[4];      /* (i,j) pairs by B-side type: U, PS, BS, empty */
+  uint64_t work_units;    /* elements + words touched on the B side */
+  int32_t schedule;
+  int32_t lanes_per_row;
+  uint32_t task_work;
+  uint32_t lds_words;
+} eh_run_stats;
+
+/* Generator specification (this engine's seeded synthetic graphs). */
+typedef enum { EH_GEN_ER = 0, EH_GEN_RMAT = 1, EH_GEN_CHUNG_LU = 2 } eh_gen_kind;
+typedef struct {
+  int32_t kind;          /* eh_gen_kind */
+  uint32_t n;            /* vertices (ER, Chung-Lu) */
+  uint32_t scale;        /* RMAT: n = 2^scale */
+  uint32_t edge_factor;  /* RMAT: draws = edge_factor * n */
+  uint64_t m;            /* ER: pair draws; Chung-Lu: endpoint-pair draws */
+  uint64_t seed;
+  double a, b, c;        /* RMAT quadrant probabilities (d = 1-a-b-c) */
+  double gamma;          /* Chung-Lu degree exponent */
+  uint32_t communities;  /* Chung-Lu: planted communities */
+  uint32_t community_size;
+  int32_t relabel;       /* 1 = degree relabel (degree desc, id asc) */
+  int32_t reserved;
+} eh_gen_spec;
+
+/* ---------------------------------------------------------------- context */
+int eh_abi_version(void);
+const char *eh_status_str(eh_status s);
+const char *eh_last_error(void);            /* thread-local detail of the last failure */
+/* No reference counterpart: the reference has no device (src/emptyheaded.hpp:33-82 runs on the host). */
+eh_status eh_ctx_create(int device, eh_ctx **out);
+void eh_ctx_destroy(eh_ctx *ctx);
+eh_status eh_ctx_device_name(eh_ctx *ctx, char *buf, size_t cap);
+
+/* ---------------------------------------------------------------- graphs */
+/* MutableGraph::undirectedFromBinary (src/MutableGraph.hpp:316-360): host
+ * symmetric CSR (rows sorted, no self-loops) uploaded to the device. */
+eh_status eh_graph_from_csr(eh_ctx *ctx, uint32_t n, const uint64_t *row_off, const uint32_t *nbrs,
+                            eh_graph **out);
+/* Edge list -> canonical symmetric graph (loops dropped, duplicates merged). */
+eh_status eh_graph_from_edges(eh_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *src,
+                              const uint32_t *dst, int relabel, eh_graph **out);
+/* Seeded device generator + canonicalisation (+ degree relabel). */
+eh_status eh_graph_generate(eh_ctx *ctx, const eh_gen_spec *spec, eh_graph **out);
+/* MutableGraph::reorder_by_degree (src/MutableGraph.hpp:20-32,100-123,268-272). */
+eh_status eh_graph_relabel_by_degree(eh_graph *g);
+eh_status eh_graph_info_get(const eh_graph *g, eh_graph_info *out);
+/* copy the symmetric CSR to host (row_off: n+1 entries, nbrs: entries) */
+eh_status eh_graph_download(const eh_graph *g, uint64_t *row_off, uint32_t *nbrs);
+/* device new->old id map after a relabel (n entries; identity otherwise) */
+eh_status eh_graph_download_perm(const eh_graph *g, uint32_t *new2old);
+void eh_graph_free(eh_graph *g);
+
+/* ---------------------------------------------------------------- matrices */
+/* SparseMatrix<T,R>::build (src/SparseMatrix.hpp:90-93,268-290,406-494) from a
+ * device graph; edge_mask is only read for EH_FILTER_MASK (one byte per stored
+ * entry, host memory). */
+eh_status eh_matrix_build(eh_graph *g, eh_edge_filter filter, const uint8_t *edge_mask,
+                          eh_layout_policy policy, eh_matrix **out);
+/* Same from a host symmetric CSR (upload + build). */
+eh_status eh_matrix_build_host(eh_ctx *ctx, uint32_t n, const uint64_t *row_off, const uint32_t *nbrs,
+                               eh_edge_filter filter, const uint8_t *edge_mask,
+                               eh_layout_policy policy, eh_matrix **out);
+eh_status eh_matrix_info_get(eh_matrix *m, eh_matrix_info *out);
+/* SparseMatrix::get_row (src/SparseMatrix.hpp:156-160): copies row's set, in
+ * its reference format, into buf (cap bytes); view->data points into buf. */
+eh_status eh_matrix_row(eh_matrix *m, uint32_t row, void *buf, size_t cap, eh_row_view *view);
+/* kept entries as a host CSR (row_off: n+1, nbrs: cardinality) */
+eh_status eh_matrix_download(eh_matrix *m, uint64_t *row_off, uint32_t *nbrs);
+/* per-row B-side layout types (n bytes, eh_set_type values) */
+eh_status eh_matrix_row_types(eh_matrix *m, uint8_t *types);
+void eh_matrix_free(eh_matrix *m);
+
+/* ---------------------------------------------------------------- set ops */
+/* ops::set_intersect(Set<X>* C, const Set<Y>* A, const Set<Z>* B)
+ * (src/set/ops/intersection.hpp:829,880,956,1019,1023,1059,1062,1321,1325,2067):
+ * intersect two host sets on the device.  materialize=0 only counts (out may
+ * be NULL); otherwise the result is written to out_buf in the reference's
+ * result layout (U for pairs with a uint side, BS for BSxBS, PS for PSxPS and
+ * PSxBS) and described by *out. */
+eh_status eh_intersect(eh_ctx *ctx, const eh_row_view *a, const eh_row_view *b, int materialize,
+                       void *out_buf, size_t out_cap, eh_row_view *out);
+/* Many pairs of rows of one matrix at once: counts[p] = |row(a[p]) & row(b[p])|. */
+eh_status eh_intersect_rows(eh_matrix *m, const uint32_t *a_rows, const uint32_t *b_rows,
+                            uint64_t npairs, uint64_t *counts);
+
+/* ---------------------------------------------------------------- apps */
+/* undirected_triangle_counting<T,R>::run (apps/undirected_triangle_counting.cpp:27-65):
+ * sum over kept (i,j) of |row(i) & row(j)|. */
+eh_status eh_triangle_count(eh_matrix *m, const eh_run_opts *opts, uint64_t *count,
+                            eh_run_stats *stats);
+/* n_clique<T,R>::run (apps/n_clique.cpp:52-101), query_depth = k (3..8). */
+eh_status eh_clique_count(eh_matrix *m, uint32_t k, const eh_run_opts *opts, uint64_t *count,
+                          eh_run_stats *stats);
+
+/* ---------------------------------------------------------------- batches */
+/* No reference counterpart (the reference processes one graph per process,
+ * src/emptyheaded.hpp:33-82): many independent graphs in one arena. */
+eh_status eh_batch_generate(eh_ctx *ctx, uint64_t seed, uint32_t ngraphs, eh_layout_policy policy,
+                            eh_batch **out);
+eh_status eh_batch_info(eh_batch *b, uint32_t *ngraphs, uint64_t *total_vertices,
+                        uint64_t *total_edges);
+/* per-graph vertex counts and edge-draw counts used by the generator */
+eh_status eh_batch_graph_params(eh_batch *b, uint32_t *n, uint64_t *m_draws);
+/* graph g's canonical symmetric CSR, local ids (row_off: n_g+1) */
+eh_status eh_batch_graph_download(eh_batch *b, uint32_t g, uint64_t *row_off, uint32_t *nbrs,
+                                  uint64_t cap);
+/* mode 0: one persistent launch over every graph, replayed from a captured
+ * hipGraph; mode 1: one launch sequence per graph.  tri/k4 get ngraphs counts
+ * each (either may be NULL). */
+eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *opts, uint64_t *tri,
+                         uint64_t *k4, double *ms);
+void eh_batch_free(eh_batch *b);
+
+/* ---------------------------------------------------------------- learned selector */
+/* No reference counterpart: the reference's selector is the fixed density rule
+ * (src/set/layouts/hybrid.hpp:73-100).  FP32 MLP 16->64->64->4, ReLU. */
+#define EH_MLP_IN 16
+#define EH_MLP_HIDDEN 64
+#define EH_MLP_OUT 4
+#define EH_MLP_PARAMS 5508
+eh_status eh_selector_set_params(eh_ctx *ctx, const float *params);
+eh_status eh_selector_get_params(eh_ctx *ctx, float *params);
+/* per-row features (n x 16 floats) of a matrix's rows */
+eh_status eh_matrix_features(eh_matrix *m, float *features);
+/* engine timings: per-row cycles spent on B-side reads of each layout; fills
+ * targets (n x 4, log2 cycles per use) and mask (n x 4, 1 where measured) */
+eh_status eh_matrix_profile_layouts(eh_matrix *m, float *targets, float *mask);
+/* device forward over a host batch (x: batch x 16, y: batch x 4) */
+eh_status eh_mlp_forward(eh_ctx *ctx, const float *params, const float *x, int batch, float *y);
+/* device loss + gradient (grad: 5508 floats) */
+eh_status eh_mlp_loss_grad(eh_ctx *ctx, const float *params, const float *x, const float *target,
+                           const float *mask, int batch, float *loss, float *grad);
+/* device Adam training of the context's selector on host samples */
+eh_status eh_selector_train(eh_ctx *ctx, const float *x, const float *target, const float *mask,
+                            int nsamples, int epochs, int batch, float lr, uint64_t seed,
+                            float *final_loss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

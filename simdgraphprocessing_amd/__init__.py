@@ -1,0 +1,464 @@
+"""simdgraphprocessing_amd -- MI355X (gfx950) set-intersection pattern engine.
+
+Python binding over the C ABI in ``include/eh.h`` (``libeh.so``, built from
+``csrc/`` with hipcc).  Every call goes to the HIP library; there is no CPU
+fallback: importing works without a GPU, but creating a :class:`Context` fails
+loudly when the library or the device is missing.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libeh.so")
+
+EH_BITSET, EH_PSHORT, EH_UINTEGER = 0, 1, 2
+TYPE_NAMES = {0: "bitset", 1: "pshort", 2: "uint"}
+LAYOUTS = {"uint": 0, "pshort": 1, "bs": 2, "bitset": 2, "hybrid": 3, "hybrid3": 4, "learned": 5,
+           "cost": 6}
+FILTERS = {"all": 0, "lower": 1, "mask": 2}
+SCHEDULES = {"plain": 0, "tuned": 1}
+MLP_PARAMS = 5508
+
+
+class EHError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{msg} (status {status})")
+        self.status = status
+
+
+class RowView(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("cardinality", C.c_uint64), ("number_of_bytes", C.c_uint64),
+                ("type", C.c_int32), ("reserved", C.c_int32)]
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("max_degree", C.c_uint32), ("entries", C.c_uint64)]
+
+
+class MatrixInfo(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("max_row", C.c_uint32), ("cardinality", C.c_uint64),
+                ("rows_by_type", C.c_uint64 * 3), ("payload_bytes", C.c_uint64),
+                ("ref_row_bytes", C.c_uint64), ("wedges", C.c_uint64), ("build_ms", C.c_double)]
+
+
+class RunOpts(C.Structure):
+    _fields_ = [("schedule", C.c_int32), ("lanes_per_row", C.c_int32), ("task_work", C.c_uint32),
+                ("lds_words", C.c_uint32), ("waves_per_block", C.c_int32), ("repeat", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("sched_ms", C.c_double), ("tasks", C.c_uint64),
+                ("alg_bytes", C.c_uint64), ("pairs", C.c_uint64 * 4), ("work_units", C.c_uint64),
+                ("schedule", C.c_int32), ("lanes_per_row", C.c_int32), ("task_work", C.c_uint32),
+                ("lds_words", C.c_uint32)]
+
+    def as_dict(self):
+        return {"kernel_ms": self.kernel_ms, "sched_ms": self.sched_ms, "tasks": self.tasks,
+                "alg_bytes": self.alg_bytes, "pairs": list(self.pairs), "work_units": self.work_units,
+                "schedule": self.schedule, "lanes_per_row": self.lanes_per_row,
+                "task_work": self.task_work, "lds_words": self.lds_words}
+
+
+class GenSpec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("n", C.c_uint32), ("scale", C.c_uint32),
+                ("edge_factor", C.c_uint32), ("m", C.c_uint64), ("seed", C.c_uint64),
+                ("a", C.c_double), ("b", C.c_double), ("c", C.c_double), ("gamma", C.c_double),
+                ("communities", C.c_uint32), ("community_size", C.c_uint32), ("relabel", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+# every symbol declared in include/eh.h
+EXPORTS = [
+    "eh_abi_version", "eh_status_str", "eh_last_error", "eh_ctx_create", "eh_ctx_destroy",
+    "eh_ctx_device_name", "eh_graph_from_csr", "eh_graph_from_edges", "eh_graph_generate",
+    "eh_graph_relabel_by_degree", "eh_graph_info_get", "eh_graph_download", "eh_graph_download_perm",
+    "eh_graph_free", "eh_matrix_build", "eh_matrix_build_host", "eh_matrix_info_get", "eh_matrix_row",
+    "eh_matrix_download", "eh_matrix_row_types", "eh_matrix_free", "eh_intersect", "eh_intersect_rows",
+    "eh_triangle_count", "eh_clique_count", "eh_batch_generate", "eh_batch_info",
+    "eh_batch_graph_params", "eh_batch_graph_download", "eh_batch_count", "eh_batch_free",
+    "eh_selector_set_params", "eh_selector_get_params", "eh_matrix_features",
+    "eh_matrix_profile_layouts", "eh_mlp_forward", "eh_mlp_loss_grad", "eh_selector_train",
+]
+
+_lib = None
+_vp = C.c_void_p
+_pp = C.POINTER(C.c_void_p)
+
+
+def build(verbose=False):
+    """Compile libeh.so for gfx950 (hipcc; cross-compiles without a GPU)."""
+    cmd = ["make", "-C", os.path.join(HERE, "csrc"), f"-j{min(os.cpu_count() or 4, 16)}"]
+    r = subprocess.run(cmd, capture_output=not verbose, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("libeh build failed:\n" + (r.stdout or "")[-4000:] + (r.stderr or "")[-4000:])
+
+
+def lib():
+    """Load libeh.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run simdgraphprocessing_amd.build() first")
+    L = C.CDLL(LIB_PATH)
+    L.eh_status_str.restype = C.c_char_p
+    L.eh_last_error.restype = C.c_char_p
+    L.eh_ctx_create.argtypes = [C.c_int, _pp]
+    L.eh_ctx_destroy.argtypes = [_vp]
+    L.eh_ctx_destroy.restype = None
+    L.eh_ctx_device_name.argtypes = [_vp, C.c_char_p, C.c_size_t]
+    L.eh_graph_from_csr.argtypes = [_vp, C.c_uint32, _vp, _vp, _pp]
+    L.eh_graph_from_edges.argtypes = [_vp, C.c_uint32, C.c_uint64, _vp, _vp, C.c_int, _pp]
+    L.eh_graph_generate.argtypes = [_vp, C.POINTER(GenSpec), _pp]
+    L.eh_graph_relabel_by_degree.argtypes = [_vp]
+    L.eh_graph_info_get.argtypes = [_vp, C.POINTER(GraphInfo)]
+    L.eh_graph_download.argtypes = [_vp, _vp, _vp]
+    L.eh_graph_download_perm.argtypes = [_vp, _vp]
+    L.eh_graph_free.argtypes = [_vp]
+    L.eh_graph_free.restype = None
+    L.eh_matrix_build.argtypes = [_vp, C.c_int, _vp, C.c_int, _pp]
+    L.eh_matrix_build_host.argtypes = [_vp, C.c_uint32, _vp, _vp, C.c_int, _vp, C.c_int, _pp]
+    L.eh_matrix_info_get.argtypes = [_vp, C.POINTER(MatrixInfo)]
+    L.eh_matrix_row.argtypes = [_vp, C.c_uint32, _vp, C.c_size_t, C.POINTER(RowView)]
+    L.eh_matrix_download.argtypes = [_vp, _vp, _vp]
+    L.eh_matrix_row_types.argtypes = [_vp, _vp]
+    L.eh_matrix_free.argtypes = [_vp]
+    L.eh_matrix_free.restype = None
+    L.eh_intersect.argtypes = [_vp, C.POINTER(RowView), C.POINTER(RowView), C.c_int, _vp, C.c_size_t,
+                               C.POINTER(RowView)]
+    L.eh_intersect_rows.argtypes = [_vp, _vp, _vp, C.c_uint64, _vp]
+    L.eh_triangle_count.argtypes = [_vp, C.POINTER(RunOpts), C.POINTER(C.c_uint64), C.POINTER(RunStats)]
+    L.eh_clique_count.argtypes = [_vp, C.c_uint32, C.POINTER(RunOpts), C.POINTER(C.c_uint64),
+                                  C.POINTER(RunStats)]
+    L.eh_batch_generate.argtypes = [_vp, C.c_uint64, C.c_uint32, C.c_int, _pp]
+    L.eh_batch_info.argtypes = [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.eh_batch_graph_params.argtypes = [_vp, _vp, _vp]
+    L.eh_batch_graph_download.argtypes = [_vp, C.c_uint32, _vp, _vp, C.c_uint64]
+    L.eh_batch_count.argtypes = [_vp, C.c_int, C.POINTER(RunOpts), _vp, _vp, C.POINTER(C.c_double)]
+    L.eh_batch_free.argtypes = [_vp]
+    L.eh_batch_free.restype = None
+    L.eh_selector_set_params.argtypes = [_vp, _vp]
+    L.eh_selector_get_params.argtypes = [_vp, _vp]
+    L.eh_matrix_features.argtypes = [_vp, _vp]
+    L.eh_matrix_profile_layouts.argtypes = [_vp, _vp, _vp]
+    L.eh_mlp_forward.argtypes = [_vp, _vp, _vp, C.c_int, _vp]
+    L.eh_mlp_loss_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_float), _vp]
+    L.eh_selector_train.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64,
+                                    C.POINTER(C.c_float)]
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != 0:
+        L = lib()
+        raise EHError(status, f"{L.eh_status_str(status).decode()}: {L.eh_last_error().decode()}")
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _opts(schedule="tuned", lanes=0, task_work=0, lds_words=0, waves=0, repeat=1):
+    o = RunOpts()
+    o.schedule = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
+    o.lanes_per_row = lanes
+    o.task_work = task_work
+    o.lds_words = lds_words
+    o.waves_per_block = waves
+    o.repeat = repeat
+    return o
+
+
+class Context:
+    """A device plus one HIP stream (eh_ctx)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = C.c_void_p()
+        _check(L.eh_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().eh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def device_name(self):
+        buf = C.create_string_buffer(256)
+        _check(lib().eh_ctx_device_name(self.h, buf, 256))
+        return buf.value.decode()
+
+    # ---- graphs
+    def graph_from_csr(self, n, off, nbr):
+        off = np.ascontiguousarray(off, np.uint64)
+        nbr = np.ascontiguousarray(nbr, np.uint32)
+        h = C.c_void_p()
+        _check(lib().eh_graph_from_csr(self.h, n, _ptr(off), _ptr(nbr), C.byref(h)))
+        return Graph(self, h)
+
+    def graph_from_edges(self, n, src, dst, relabel=True):
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        h = C.c_void_p()
+        _check(lib().eh_graph_from_edges(self.h, n, len(src), _ptr(src), _ptr(dst), int(relabel), C.byref(h)))
+        return Graph(self, h)
+
+    def generate(self, kind, relabel=True, **kw):
+        s = GenSpec()
+        s.kind = {"er": 0, "rmat": 1, "chung_lu": 2}[kind]
+        s.relabel = int(relabel)
+        s.a, s.b, s.c = kw.get("a", 0.57), kw.get("b", 0.19), kw.get("c", 0.19)
+        s.gamma = kw.get("gamma", 2.1)
+        s.n = kw.get("n", 0)
+        s.scale = kw.get("scale", 0)
+        s.edge_factor = kw.get("edge_factor", 16)
+        s.m = kw.get("m", 0)
+        s.seed = kw.get("seed", 0)
+        s.communities = kw.get("communities", 0)
+        s.community_size = kw.get("community_size", 256)
+        h = C.c_void_p()
+        _check(lib().eh_graph_generate(self.h, C.byref(s), C.byref(h)))
+        return Graph(self, h)
+
+    def generate_er(self, n, m, seed, relabel=True):
+        return self.generate("er", relabel, n=n, m=m, seed=seed)
+
+    def generate_rmat(self, scale, edge_factor, seed, a=0.57, b=0.19, c=0.19, relabel=True):
+        return self.generate("rmat", relabel, scale=scale, edge_factor=edge_factor, seed=seed, a=a, b=b, c=c)
+
+    def generate_chung_lu(self, n, mean_degree, gamma, seed, communities=0, community_size=256, relabel=True):
+        return self.generate("chung_lu", relabel, n=n, m=n * mean_degree // 2, gamma=gamma, seed=seed,
+                             communities=communities, community_size=community_size)
+
+    def matrix_from_csr(self, n, off, nbr, filter="lower", policy="hybrid", mask=None):
+        off = np.ascontiguousarray(off, np.uint64)
+        nbr = np.ascontiguousarray(nbr, np.uint32)
+        mk = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        h = C.c_void_p()
+        _check(lib().eh_matrix_build_host(self.h, n, _ptr(off), _ptr(nbr), FILTERS[filter], _ptr(mk),
+                                          LAYOUTS[policy], C.byref(h)))
+        return SparseMatrix(self, h)
+
+    # ---- set ops
+    def intersect(self, a, b, materialize=True, out_cap=None):
+        """a, b: (type, cardinality, payload bytes) in the reference formats."""
+        L = lib()
+        keep = []
+
+        def view(s):
+            t, card, payload = s
+            p = np.ascontiguousarray(payload, np.uint8)
+            keep.append(p)
+            return RowView(p.ctypes.data if len(p) else None, card, len(p), t, 0)
+
+        va, vb = view(a), view(b)
+        cap = out_cap if out_cap is not None else 16 * (a[1] + b[1]) + len(a[2]) + len(b[2]) + 64
+        out = np.zeros(cap, np.uint8)
+        ov = RowView()
+        _check(L.eh_intersect(self.h, C.byref(va), C.byref(vb), int(materialize), _ptr(out), cap, C.byref(ov)))
+        return int(ov.cardinality), int(ov.type), out[:ov.number_of_bytes].copy()
+
+    # ---- learned selector
+    def set_selector(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == MLP_PARAMS
+        _check(lib().eh_selector_set_params(self.h, _ptr(p)))
+
+    def get_selector(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        _check(lib().eh_selector_get_params(self.h, _ptr(p)))
+        return p
+
+    def mlp_forward(self, params, x):
+        p = np.ascontiguousarray(params, np.float32)
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], 4), np.float32)
+        _check(lib().eh_mlp_forward(self.h, _ptr(p), _ptr(x), x.shape[0], _ptr(y)))
+        return y
+
+    def mlp_loss_grad(self, params, x, target, mask):
+        p = np.ascontiguousarray(params, np.float32)
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        m = np.ascontiguousarray(mask, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        _check(lib().eh_mlp_loss_grad(self.h, _ptr(p), _ptr(x), _ptr(t), _ptr(m), x.shape[0], C.byref(loss), _ptr(g)))
+        return float(loss.value), g
+
+    def train_selector(self, x, target, mask, epochs=200, batch=256, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        m = np.ascontiguousarray(mask, np.float32)
+        loss = C.c_float()
+        _check(lib().eh_selector_train(self.h, _ptr(x), _ptr(t), _ptr(m), x.shape[0], epochs, batch, lr, seed,
+                                       C.byref(loss)))
+        return float(loss.value)
+
+    # ---- batches
+    def generate_batch(self, seed, ngraphs, policy="hybrid"):
+        h = C.c_void_p()
+        _check(lib().eh_batch_generate(self.h, seed, ngraphs, LAYOUTS[policy], C.byref(h)))
+        return Batch(self, h)
+
+
+class Graph:
+    """Canonical symmetric graph on the device (eh_graph)."""
+
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().eh_graph_free(self.h)
+            self.h = None
+
+    def info(self):
+        gi = GraphInfo()
+        _check(lib().eh_graph_info_get(self.h, C.byref(gi)))
+        return {"n": gi.n, "max_degree": gi.max_degree, "entries": gi.entries}
+
+    def download(self):
+        i = self.info()
+        off = np.zeros(i["n"] + 1, np.uint64)
+        nbr = np.zeros(max(i["entries"], 1), np.uint32)
+        _check(lib().eh_graph_download(self.h, _ptr(off), _ptr(nbr)))
+        return off, nbr[:i["entries"]]
+
+    def perm(self):
+        p = np.zeros(self.info()["n"], np.uint32)
+        _check(lib().eh_graph_download_perm(self.h, _ptr(p)))
+        return p
+
+    def relabel_by_degree(self):
+        _check(lib().eh_graph_relabel_by_degree(self.h))
+
+    def build_matrix(self, filter="lower", policy="hybrid", mask=None):
+        mk = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        h = C.c_void_p()
+        _check(lib().eh_matrix_build(self.h, FILTERS[filter], _ptr(mk), LAYOUTS[policy], C.byref(h)))
+        return SparseMatrix(self.ctx, h)
+
+
+class SparseMatrix:
+    """Built per-row layouts on the device (eh_matrix)."""
+
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().eh_matrix_free(self.h)
+            self.h = None
+
+    def info(self):
+        mi = MatrixInfo()
+        _check(lib().eh_matrix_info_get(self.h, C.byref(mi)))
+        return {"n": mi.n, "max_row": mi.max_row, "cardinality": mi.cardinality,
+                "rows_by_type": {"bitset": mi.rows_by_type[0], "pshort": mi.rows_by_type[1],
+                                 "uint": mi.rows_by_type[2]},
+                "payload_bytes": mi.payload_bytes, "ref_row_bytes": mi.ref_row_bytes,
+                "wedges": mi.wedges, "build_ms": mi.build_ms}
+
+    def row(self, i, cap=1 << 22):
+        buf = np.zeros(cap, np.uint8)
+        v = RowView()
+        _check(lib().eh_matrix_row(self.h, i, _ptr(buf), cap, C.byref(v)))
+        return int(v.type), int(v.cardinality), buf[:v.number_of_bytes].copy()
+
+    def download(self):
+        i = self.info()
+        off = np.zeros(i["n"] + 1, np.uint64)
+        nbr = np.zeros(max(i["cardinality"], 1), np.uint32)
+        _check(lib().eh_matrix_download(self.h, _ptr(off), _ptr(nbr)))
+        return off, nbr[:i["cardinality"]]
+
+    def row_types(self):
+        t = np.zeros(self.info()["n"], np.uint8)
+        _check(lib().eh_matrix_row_types(self.h, _ptr(t)))
+        return t
+
+    def features(self):
+        f = np.zeros((self.info()["n"], 16), np.float32)
+        _check(lib().eh_matrix_features(self.h, _ptr(f)))
+        return f
+
+    def profile_layouts(self):
+        n = self.info()["n"]
+        t = np.zeros((n, 4), np.float32)
+        m = np.zeros((n, 4), np.float32)
+        _check(lib().eh_matrix_profile_layouts(self.h, _ptr(t), _ptr(m)))
+        return t, m
+
+    def triangle_count(self, schedule="tuned", **kw):
+        c = C.c_uint64()
+        st = RunStats()
+        o = _opts(schedule, **kw)
+        _check(lib().eh_triangle_count(self.h, C.byref(o), C.byref(c), C.byref(st)))
+        return int(c.value), st.as_dict()
+
+    def clique_count(self, k=4, schedule="tuned", **kw):
+        c = C.c_uint64()
+        st = RunStats()
+        o = _opts(schedule, **kw)
+        _check(lib().eh_clique_count(self.h, k, C.byref(o), C.byref(c), C.byref(st)))
+        return int(c.value), st.as_dict()
+
+    def intersect_rows(self, a_rows, b_rows):
+        a = np.ascontiguousarray(a_rows, np.uint32)
+        b = np.ascontiguousarray(b_rows, np.uint32)
+        out = np.zeros(len(a), np.uint64)
+        _check(lib().eh_intersect_rows(self.h, _ptr(a), _ptr(b), len(a), _ptr(out)))
+        return out
+
+
+class Batch:
+    """Many independent graphs in one device arena (eh_batch)."""
+
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().eh_batch_free(self.h)
+            self.h = None
+
+    def info(self):
+        g = C.c_uint32()
+        v = C.c_uint64()
+        e = C.c_uint64()
+        _check(lib().eh_batch_info(self.h, C.byref(g), C.byref(v), C.byref(e)))
+        return {"graphs": g.value, "vertices": v.value, "edges": e.value}
+
+    def params(self):
+        ng = self.info()["graphs"]
+        n = np.zeros(ng, np.uint32)
+        m = np.zeros(ng, np.uint64)
+        _check(lib().eh_batch_graph_params(self.h, _ptr(n), _ptr(m)))
+        return n, m
+
+    def graph(self, g, cap=1 << 24):
+        n = int(self.params()[0][g])
+        off = np.zeros(n + 1, np.uint64)
+        nbr = np.zeros(cap, np.uint32)
+        _check(lib().eh_batch_graph_download(self.h, g, _ptr(off), _ptr(nbr), cap))
+        return n, off, nbr[:int(off[-1])].copy()
+
+    def count(self, mode="persistent", schedule="tuned", triangles=True, cliques=True, **kw):
+        ng = self.info()["graphs"]
+        tri = np.zeros(ng, np.uint64) if triangles else None
+        k4 = np.zeros(ng, np.uint64) if cliques else None
+        ms = C.c_double()
+        o = _opts(schedule, **kw)
+        _check(lib().eh_batch_count(self.h, {"persistent": 0, "per_graph": 1}[mode], C.byref(o), _ptr(tri),
+                                    _ptr(k4), C.byref(ms)))
+        return tri, k4, ms.value

@@ -1,0 +1,916 @@
+// eh_count.hip -- triangle and 4-clique counting kernels for gfx950 (wave64).
+//
+// Work unit: a task = (source row i, j-range [jb, je) of row(i), graph id, staging kind).
+// One wavefront runs one task at a time, pulling tasks from an atomic queue
+// (persistent grid):
+//   1. stage A = row(i) into the wave's LDS slice, in one of four kinds:
+//        BITMAP  64-bit-word bitmap over [word(first), word(last)] plus the list
+//                (the reference bitset's word span, bitset.hpp:65-96)
+//        LIST    the sorted uint32 list (binary-search membership)
+//        PSDIR   the list plus a directory of 16-bit prefix partitions
+//                (pshort partitions, pshort.hpp:46-78)
+//        GLOBAL  nothing staged; membership by binary search in HBM
+//   2. for each j of the range, intersect A with B = row(j) read from HBM in
+//      B's own layout (uint / pshort / bitset); 64 lanes split B's elements or
+//      words and test membership against A, counts reduce per wave.
+// Triangle counting follows apps/undirected_triangle_counting.cpp:39-61 (sum of
+// |A & B| over kept (i,j)); the 4-clique count follows apps/n_clique.cpp:30-50
+// with query_depth 4: C = A & row(j) is materialised in LDS and every k in C
+// adds |C & row(k)|.
+//
+// Pair semantics follow the reference dispatcher (intersection.hpp:2066-2118):
+//   U x U    lanes over B, binary search in A     (set_intersect_standard/galloping)
+//   U x BS   lanes over the uint side probing bits (:1023-1058)
+//   BS x BS  64-bit AND + popcount over the overlapping words (:880-955)
+//   PS x PS  prefix match, then 16-bit runs        (:829-878)
+//   U x PS   prefix-aligned run search             (:1062-1173)
+//   PS x BS  pshort elements probing bits          (:956-1017)
+#include "eh_internal.hpp"
+
+namespace {
+
+constexpr uint32_t KIND_BITMAP = 0, KIND_LIST = 1, KIND_PSDIR = 2, KIND_GLOBAL = 3;
+constexpr uint32_t KIND_SHIFT = 28;
+constexpr uint32_t GID_MASK = (1u << KIND_SHIFT) - 1;
+
+struct CountArgs {
+  const uint32_t *__restrict__ card;
+  const uint32_t *__restrict__ aoff;
+  const uint32_t *__restrict__ unbr;
+  const RowDesc *__restrict__ desc;
+  const uint64_t *__restrict__ bsw;
+  const uint16_t *__restrict__ psd;
+  const uint4 *__restrict__ tasks;
+  uint64_t ntasks;
+  uint32_t *next;
+  unsigned long long *counts;
+  uint32_t lds_words;
+  uint32_t weighted;  // add card(x) per common element (byte model of the 4-clique pass)
+};
+
+__device__ __forceinline__ uint32_t rfl(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint32_t rdl(uint32_t x, uint32_t l) { return __builtin_amdgcn_readlane(x, l); }
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+__device__ __forceinline__ uint32_t mbcnt(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t lane, uint32_t *total) {
+  uint32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  *total = __shfl(x, 63, 64);
+  return x - v;
+}
+
+// membership in a sorted array: largest position whose value <= x, then compare
+template <class P>
+__device__ __forceinline__ uint32_t sorted_contains(P L, uint32_t n, uint32_t x) {
+  if (n == 0) return 0;
+  uint32_t base = 0, len = n;
+  while (len > 1) {
+    uint32_t half = len >> 1;
+    base = (L[base + half] <= x) ? base + half : base;
+    len -= half;
+  }
+  return L[base] == x;
+}
+
+// A-side state of one staged set (row(i) for triangles; also C for 4-cliques)
+struct AState {
+  uint32_t *bm;           // LDS bitmap (u32 words); bit x at bm[(x >> 5) - base32]
+  uint32_t base32;        // 2 * aw0
+  uint32_t aw0, anw;      // 64-bit word span
+  uint32_t lo, span;      // first element, last - first
+  const uint32_t *list;   // LDS list (or HBM list for KIND_GLOBAL)
+  uint32_t n;
+  const uint32_t *dpre;   // PSDIR: partition prefixes (P), starts (P + 1)
+  const uint32_t *dstart;
+  uint32_t P;
+};
+
+template <uint32_t KIND>
+__device__ __forceinline__ uint32_t memb(const AState &A, uint32_t x) {
+  if (x - A.lo > A.span) return 0;
+  if constexpr (KIND == KIND_BITMAP) {
+    return (A.bm[(x >> 5) - A.base32] >> (x & 31u)) & 1u;
+  } else if constexpr (KIND == KIND_PSDIR) {
+    const uint32_t hi = x >> 16;
+    uint32_t base = 0, len = A.P;
+    while (len > 1) {
+      uint32_t half = len >> 1;
+      base = (A.dpre[base + half] <= hi) ? base + half : base;
+      len -= half;
+    }
+    if (A.dpre[base] != hi) return 0;
+    const uint32_t s = A.dstart[base], e = A.dstart[base + 1];
+    return sorted_contains(A.list + s, e - s, x);
+  } else {
+    return sorted_contains(A.list, A.n, x);
+  }
+}
+
+template <uint32_t KIND, bool W>
+__device__ __forceinline__ uint64_t hit_value(const AState &A, const CountArgs &a, uint32_t x) {
+  uint32_t h = memb<KIND>(A, x);
+  if constexpr (W) return h ? (uint64_t)a.card[x] : 0ull;
+  else return h;
+}
+
+// ---------------------------------------------------------------- count-only intersections
+
+// |A & B| for a uint B row: 16-byte loads, four probes per lane per step
+template <uint32_t KIND, bool W>
+__device__ __forceinline__ uint64_t isect_b_uint(const AState &A, const CountArgs &a, const RowDesc &dj,
+                                                 uint32_t lane) {
+  const uint4 *B = reinterpret_cast<const uint4 *>(a.unbr) + dj.off;
+  const uint32_t n4 = (dj.card + 3u) >> 2;
+  uint64_t c = 0;
+  for (uint32_t v = lane; v < n4; v += 64) {
+    const uint4 x = B[v];
+    c += hit_value<KIND, W>(A, a, x.x) + hit_value<KIND, W>(A, a, x.y) + hit_value<KIND, W>(A, a, x.z) +
+         hit_value<KIND, W>(A, a, x.w);
+  }
+  return c;
+}
+
+// locate A's partition for prefix `pre` (uniform); returns false when absent
+__device__ __forceinline__ bool psdir_find(const AState &A, uint32_t pre, uint32_t *s, uint32_t *e) {
+  uint32_t base = 0, len = A.P;
+  while (len > 1) {
+    uint32_t half = len >> 1;
+    base = (A.dpre[base + half] <= pre) ? base + half : base;
+    len -= half;
+  }
+  if (A.P == 0 || A.dpre[base] != pre) return false;
+  *s = A.dstart[base];
+  *e = A.dstart[base + 1];
+  return true;
+}
+
+template <uint32_t KIND, bool W>
+__device__ __forceinline__ uint64_t isect_b_pshort(const AState &A, const CountArgs &a, const RowDesc &dj,
+                                                   uint32_t lane) {
+  const uint16_t *S = a.psd + (uint64_t)dj.off * 4;
+  const uint32_t len = desc_aux(dj);
+  const uint32_t a_hi = A.lo + A.span;
+  uint64_t c = 0;
+  uint32_t pos = 0;
+  while (pos < len) {
+    const uint32_t pre = rfl(S[pos]);
+    const uint32_t cnt = rfl(S[pos + 1]) + 1u;
+    const uint32_t base = pre << 16;
+    if (base + 0xFFFFu >= A.lo && base <= a_hi) {
+      if constexpr (KIND == KIND_PSDIR) {
+        // PS x PS: only partitions with a matching prefix are read
+        uint32_t s, e;
+        if (psdir_find(A, pre, &s, &e)) {
+          for (uint32_t k = lane; k < cnt; k += 64) {
+            const uint32_t x = base | S[pos + 2 + k];
+            uint32_t h = sorted_contains(A.list + s, e - s, x);
+            if constexpr (W) c += h ? (uint64_t)a.card[x] : 0ull;
+            else c += h;
+          }
+        }
+      } else {
+        for (uint32_t k = lane; k < cnt; k += 64) c += hit_value<KIND, W>(A, a, base | S[pos + 2 + k]);
+      }
+    }
+    pos += cnt + 2u;
+  }
+  return c;
+}
+
+template <uint32_t KIND, bool W>
+__device__ __forceinline__ uint64_t isect_b_bitset(const AState &A, const CountArgs &a, const RowDesc &dj,
+                                                   uint32_t lane) {
+  const uint32_t bw0 = dj.w0, bnw = desc_aux(dj);
+  const uint64_t *BW = a.bsw + dj.off;
+  uint64_t c = 0;
+  if constexpr (KIND == KIND_BITMAP) {
+    const uint32_t s = max(A.aw0, bw0);
+    const uint32_t e = min(A.aw0 + A.anw, bw0 + bnw);
+    for (uint32_t w = s + lane; w < e; w += 64) {
+      const uint32_t i2 = 2u * (w - A.aw0);
+      const uint64_t aw = (uint64_t)A.bm[i2] | ((uint64_t)A.bm[i2 + 1] << 32);
+      uint64_t r = aw & BW[w - bw0];
+      if constexpr (W) {
+        while (r) {
+          const uint32_t b = (uint32_t)__builtin_ctzll(r);
+          c += a.card[64u * w + b];
+          r &= r - 1;
+        }
+      } else {
+        c += (uint64_t)__popcll(r);
+      }
+    }
+  } else {
+    // probe B's bits with A's elements
+    for (uint32_t k = lane; k < A.n; k += 64) {
+      const uint32_t x = A.list[k];
+      const uint32_t w = (x >> 6) - bw0;
+      if (w < bnw && ((BW[w] >> (x & 63u)) & 1ull)) {
+        if constexpr (W) c += a.card[x];
+        else c += 1;
+      }
+    }
+  }
+  return c;
+}
+
+template <uint32_t KIND, bool W>
+__device__ __forceinline__ uint64_t isect_count(const AState &A, const CountArgs &a, const RowDesc &dj,
+                                                uint32_t lane) {
+  if (dj.card == 0) return 0;
+  const uint32_t t = desc_type(dj);
+  if (t == EH_U) return isect_b_uint<KIND, W>(A, a, dj, lane);
+  if (t == EH_PS) return isect_b_pshort<KIND, W>(A, a, dj, lane);
+  return isect_b_bitset<KIND, W>(A, a, dj, lane);
+}
+
+// ---------------------------------------------------------------- materialising intersection
+
+// C = A & B into LDS list `out` (sorted); returns |C| (uniform)
+template <uint32_t KIND>
+__device__ __forceinline__ uint32_t isect_write(const AState &A, const CountArgs &a, const RowDesc &dj,
+                                                uint32_t lane, uint32_t *out) {
+  if (dj.card == 0) return 0;
+  const uint32_t t = desc_type(dj);
+  uint32_t nc = 0;
+  if (t == EH_U) {
+    const uint32_t *B = a.unbr + (uint64_t)dj.off * 4;
+    for (uint32_t k0 = 0; k0 < dj.card; k0 += 64) {
+      const uint32_t k = k0 + lane;
+      uint32_t x = 0, h = 0;
+      if (k < dj.card) { x = B[k]; h = memb<KIND>(A, x); }
+      const uint64_t m = __ballot(h);
+      if (h) out[nc + mbcnt(m)] = x;
+      nc += (uint32_t)__popcll(m);
+    }
+  } else if (t == EH_PS) {
+    const uint16_t *S = a.psd + (uint64_t)dj.off * 4;
+    const uint32_t len = desc_aux(dj);
+    uint32_t pos = 0;
+    while (pos < len) {
+      const uint32_t pre = rfl(S[pos]);
+      const uint32_t cnt = rfl(S[pos + 1]) + 1u;
+      const uint32_t base = pre << 16;
+      for (uint32_t k0 = 0; k0 < cnt; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        uint32_t x = 0, h = 0;
+        if (k < cnt) { x = base | S[pos + 2 + k]; h = memb<KIND>(A, x); }
+        const uint64_t m = __ballot(h);
+        if (h) out[nc + mbcnt(m)] = x;
+        nc += (uint32_t)__popcll(m);
+      }
+      pos += cnt + 2u;
+    }
+  } else {
+    const uint32_t bw0 = dj.w0, bnw = desc_aux(dj);
+    const uint64_t *BW = a.bsw + dj.off;
+    if constexpr (KIND == KIND_BITMAP) {
+      const uint32_t s = max(A.aw0, bw0);
+      const uint32_t e = min(A.aw0 + A.anw, bw0 + bnw);
+      for (uint32_t w0 = s; w0 < e; w0 += 64) {
+        const uint32_t w = w0 + lane;
+        uint64_t r = 0;
+        if (w < e) {
+          const uint32_t i2 = 2u * (w - A.aw0);
+          r = ((uint64_t)A.bm[i2] | ((uint64_t)A.bm[i2 + 1] << 32)) & BW[w - bw0];
+        }
+        uint32_t tot;
+        uint32_t p = nc + wave_excl_scan((uint32_t)__popcll(r), lane, &tot);
+        while (r) {
+          const uint32_t b = (uint32_t)__builtin_ctzll(r);
+          out[p++] = 64u * w + b;
+          r &= r - 1;
+        }
+        nc += tot;
+      }
+    } else {
+      for (uint32_t k0 = 0; k0 < A.n; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        uint32_t x = 0, h = 0;
+        if (k < A.n) {
+          x = A.list[k];
+          const uint32_t w = (x >> 6) - bw0;
+          h = (w < bnw) ? (uint32_t)((BW[w] >> (x & 63u)) & 1ull) : 0u;
+        }
+        const uint64_t m = __ballot(h);
+        if (h) out[nc + mbcnt(m)] = x;
+        nc += (uint32_t)__popcll(m);
+      }
+    }
+  }
+  wave_sync();
+  return nc;
+}
+
+// ---------------------------------------------------------------- staging
+
+// Stage row(i) as `kind` into LDS slice L; returns the A state and the number
+// of LDS words used.
+__device__ __forceinline__ uint32_t stage_a(const CountArgs &a, uint32_t kind, uint32_t i, uint32_t lane,
+                                            uint32_t *L, AState &A) {
+  const uint32_t dA = a.card[i];
+  const uint32_t *G = a.unbr + (uint64_t)a.aoff[i] * 4;
+  const uint32_t lo = rfl(G[0]);
+  const uint32_t hi = rfl(G[dA - 1]);
+  A.lo = lo;
+  A.span = hi - lo;
+  A.n = dA;
+  A.aw0 = lo >> 6;
+  A.anw = (hi >> 6) - (lo >> 6) + 1u;
+  A.base32 = 2u * A.aw0;
+  A.P = 0;
+  A.dpre = A.dstart = nullptr;
+  A.bm = L;
+  if (kind == KIND_GLOBAL) {
+    A.list = G;
+    return 0;
+  }
+  uint32_t used = 0;
+  if (kind == KIND_BITMAP) {
+    const uint32_t nw32 = 2u * A.anw;
+    for (uint32_t w = lane; w < nw32; w += 64) L[w] = 0u;
+    used = nw32;
+  }
+  uint32_t *list = L + used;
+  for (uint32_t k = lane; k < dA; k += 64) list[k] = G[k];
+  used += dA;
+  wave_sync();
+  if (kind == KIND_BITMAP) {
+    for (uint32_t k = lane; k < dA; k += 64) {
+      const uint32_t x = list[k];
+      atomicOr(&L[(x >> 5) - A.base32], 1u << (x & 31u));
+    }
+  } else if (kind == KIND_PSDIR) {
+    // directory of 16-bit prefixes: partition p starts where the prefix changes
+    uint32_t *dpre = L + used;
+    uint32_t np = 0;
+    for (uint32_t k0 = 0; k0 < dA; k0 += 64) {
+      const uint32_t k = k0 + lane;
+      uint32_t f = 0;
+      if (k < dA) f = (k == 0) || ((list[k] >> 16) != (list[k - 1] >> 16));
+      np += (uint32_t)__popcll(__ballot(f));
+    }
+    uint32_t *dstart = dpre + np;
+    uint32_t q = 0;
+    for (uint32_t k0 = 0; k0 < dA; k0 += 64) {
+      const uint32_t k = k0 + lane;
+      uint32_t f = 0;
+      if (k < dA) f = (k == 0) || ((list[k] >> 16) != (list[k - 1] >> 16));
+      const uint64_t m = __ballot(f);
+      if (f) {
+        dpre[q + mbcnt(m)] = list[k] >> 16;
+        dstart[q + mbcnt(m)] = k;
+      }
+      q += (uint32_t)__popcll(m);
+    }
+    if (lane == 0) dstart[np] = dA;
+    A.P = np;
+    A.dpre = dpre;
+    A.dstart = dstart;
+    used += 2u * np + 1u;
+  }
+  wave_sync();
+  A.list = list;
+  return used;
+}
+
+// C-state over a materialised LDS list (sorted) for the 4-clique inner level;
+// for bitmap kinds the C bitmap shares A's word span.
+template <uint32_t CKIND>
+__device__ __forceinline__ void make_cstate(const AState &A, uint32_t *clist, uint32_t nc, uint32_t *cbm,
+                                            uint32_t lane, AState &Cs) {
+  Cs.list = clist;
+  Cs.n = nc;
+  Cs.lo = clist[0];
+  Cs.span = clist[nc - 1] - clist[0];
+  Cs.P = 0;
+  Cs.dpre = Cs.dstart = nullptr;
+  Cs.aw0 = A.aw0;
+  Cs.anw = A.anw;
+  Cs.base32 = A.base32;
+  Cs.bm = cbm;
+  if constexpr (CKIND == KIND_BITMAP) {
+    const uint32_t nw32 = 2u * A.anw;
+    for (uint32_t w = lane; w < nw32; w += 64) cbm[w] = 0u;
+    wave_sync();
+    for (uint32_t k = lane; k < nc; k += 64) {
+      const uint32_t x = clist[k];
+      atomicOr(&cbm[(x >> 5) - A.base32], 1u << (x & 31u));
+    }
+    wave_sync();
+  }
+}
+
+// ---------------------------------------------------------------- task bodies
+
+template <uint32_t KIND, bool W>
+__device__ uint64_t task_triangles(const CountArgs &a, const AState &A, uint32_t jb, uint32_t je,
+                                   uint32_t lane) {
+  uint64_t acc = 0;
+  for (uint32_t jc = jb; jc < je; jc += 64) {
+    const uint32_t nj = min(64u, je - jc);
+    uint32_t jl = 0;
+    RowDesc dl = {0u, 0u, 0u, 0u};
+    if (lane < nj) {
+      jl = A.list[jc + lane];
+      dl = a.desc[jl];
+    }
+    for (uint32_t t = 0; t < nj; t++) {
+      RowDesc dj;
+      dj.card = rdl(dl.card, t);
+      dj.tinfo = rdl(dl.tinfo, t);
+      dj.w0 = rdl(dl.w0, t);
+      dj.off = rdl(dl.off, t);
+      acc += isect_count<KIND, W>(A, a, dj, lane);
+    }
+  }
+  return acc;
+}
+
+template <uint32_t KIND>
+__device__ uint64_t task_cliques4(const CountArgs &a, const AState &A, uint32_t jb, uint32_t je,
+                                  uint32_t lane, uint32_t *free_lds) {
+  constexpr uint32_t CKIND = (KIND == KIND_BITMAP) ? KIND_BITMAP : KIND_LIST;
+  uint32_t *cbm = free_lds;
+  uint32_t *clist = free_lds + ((CKIND == KIND_BITMAP) ? 2u * A.anw : 0u);
+  uint64_t acc = 0;
+  for (uint32_t jc = jb; jc < je; jc += 64) {
+    const uint32_t nj = min(64u, je - jc);
+    uint32_t jl = 0;
+    RowDesc dl = {0u, 0u, 0u, 0u};
+    if (lane < nj) {
+      jl = A.list[jc + lane];
+      dl = a.desc[jl];
+    }
+    for (uint32_t t = 0; t < nj; t++) {
+      RowDesc dj;
+      dj.card = rdl(dl.card, t);
+      dj.tinfo = rdl(dl.tinfo, t);
+      dj.w0 = rdl(dl.w0, t);
+      dj.off = rdl(dl.off, t);
+      const uint32_t nc = rfl(isect_write<KIND>(A, a, dj, lane, clist));
+      if (nc == 0) continue;
+      AState Cs;
+      make_cstate<CKIND>(A, clist, nc, cbm, lane, Cs);
+      for (uint32_t kc = 0; kc < nc; kc += 64) {
+        const uint32_t nk = min(64u, nc - kc);
+        RowDesc kl = {0u, 0u, 0u, 0u};
+        if (lane < nk) kl = a.desc[clist[kc + lane]];
+        for (uint32_t u = 0; u < nk; u++) {
+          RowDesc dk;
+          dk.card = rdl(kl.card, u);
+          dk.tinfo = rdl(kl.tinfo, u);
+          dk.w0 = rdl(kl.w0, u);
+          dk.off = rdl(kl.off, u);
+          acc += isect_count<CKIND, false>(Cs, a, dk, lane);
+        }
+      }
+      wave_sync();
+    }
+  }
+  return acc;
+}
+
+template <int K, bool W>
+__device__ uint64_t run_task(const CountArgs &a, uint32_t kind, uint32_t i, uint32_t jb, uint32_t je,
+                             uint32_t lane, uint32_t *L) {
+  AState A;
+  const uint32_t used = stage_a(a, kind, i, lane, L, A);
+  if constexpr (K == 3) {
+    switch (kind) {
+      case KIND_BITMAP: return task_triangles<KIND_BITMAP, W>(a, A, jb, je, lane);
+      case KIND_LIST: return task_triangles<KIND_LIST, W>(a, A, jb, je, lane);
+      case KIND_PSDIR: return task_triangles<KIND_PSDIR, W>(a, A, jb, je, lane);
+      default: return task_triangles<KIND_GLOBAL, W>(a, A, jb, je, lane);
+    }
+  } else {
+    switch (kind) {
+      case KIND_BITMAP: return task_cliques4<KIND_BITMAP>(a, A, jb, je, lane, L + used);
+      case KIND_LIST: return task_cliques4<KIND_LIST>(a, A, jb, je, lane, L + used);
+      case KIND_PSDIR: return task_cliques4<KIND_PSDIR>(a, A, jb, je, lane, L + used);
+      default: return task_cliques4<KIND_GLOBAL>(a, A, jb, je, lane, L + used);
+    }
+  }
+}
+
+template <int K, bool W>
+__global__ void __launch_bounds__(256) k_count(CountArgs a) {
+  extern __shared__ uint32_t lds_all[];
+  const uint32_t lane = lane_id();
+  uint32_t *L = lds_all + (threadIdx.x >> 6) * a.lds_words;
+  uint64_t acc = 0;
+  uint32_t cur = 0;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(a.next, 1u);
+    t = rfl(t);
+    if (t >= a.ntasks) break;
+    const uint4 task = a.tasks[t];
+    const uint32_t i = rfl(task.x), jb = rfl(task.y), je = rfl(task.z), w = rfl(task.w);
+    const uint32_t gid = w & GID_MASK, kind = w >> KIND_SHIFT;
+    if (gid != cur) {
+      const uint64_t s = wave_sum_u64(acc);
+      if (lane == 0 && s) atomicAdd(&a.counts[cur], (unsigned long long)s);
+      acc = 0;
+      cur = gid;
+    }
+    acc += run_task<K, W>(a, kind, i, jb, je, lane, L);
+  }
+  const uint64_t s = wave_sum_u64(acc);
+  if (lane == 0 && s) atomicAdd(&a.counts[cur], (unsigned long long)s);
+}
+
+__global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
+}
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------- schedule kernels
+
+struct SchedArgs {
+  const uint32_t *card;
+  const uint32_t *aoff;
+  const uint32_t *unbr;
+  const RowDesc *desc;
+  const uint32_t *graph_of;
+  uint32_t n;
+  int tuned;
+  int k;
+  uint32_t lds_small, lds_big;
+  uint64_t task_work;
+  uint64_t *sortkey;      // out: (class << 62) | (order << 32) | i
+  uint32_t *ntask;        // out: tasks of row i
+  uint32_t *rowinfo;      // out: kind << 28 | graph id
+  unsigned long long *stats;  // [0..3] pair types, [4] work units, [5] wedges
+};
+
+// one wave per row: per-row work and pair statistics
+__global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+  unsigned long long loc[6] = {0, 0, 0, 0, 0, 0};
+  for (uint32_t i = wid; i < s.n; i += nw) {
+    const uint32_t dA = s.card[i];
+    uint64_t work = 0;
+    uint32_t lo = 0, hi = 0;
+    if (dA) {
+      const uint32_t *G = s.unbr + (uint64_t)s.aoff[i] * 4;
+      lo = G[0];
+      hi = G[dA - 1];
+      for (uint32_t k = lane; k < dA; k += 64) {
+        const RowDesc d = s.desc[G[k]];
+        const uint32_t t = desc_type(d);
+        uint64_t units;
+        if (d.card == 0) { loc[3]++; units = 0; }
+        else if (t == EH_U) { loc[0]++; units = d.card; }
+        else if (t == EH_PS) { loc[1]++; units = desc_aux(d); }
+        else { loc[2]++; units = desc_aux(d); }
+        loc[5] += d.card;
+        work += units + 16u;
+      }
+      work = wave_sum_u64(work);
+      loc[4] += (lane == 0) ? work - 16ull * dA : 0ull;
+    }
+    if (lane != 0) continue;
+    if (dA == 0) {
+      s.sortkey[i] = (3ull << 62) | i;
+      s.ntask[i] = 0;
+      s.rowinfo[i] = 0;
+      continue;
+    }
+    // staging kind and LDS need
+    const uint32_t anw = (hi >> 6) - (lo >> 6) + 1u;
+    const RowDesc da = s.desc[i];
+    const uint32_t ta = desc_type(da);
+    const uint32_t extra = (s.k >= 4) ? 1u : 0u;  // the 4-clique level set needs room too
+    const uint64_t need_bitmap = 2ull * anw + dA + extra * (2ull * anw + dA);
+    const uint64_t need_list = (uint64_t)dA + extra * dA;
+    const uint64_t need_psdir = 3ull * dA + 1 + extra * dA;
+    uint32_t kind, cls;
+    uint64_t need;
+    if (s.tuned) {
+      if (need_bitmap <= s.lds_small) { kind = KIND_BITMAP; need = need_bitmap; }
+      else if (need_bitmap <= s.lds_big && dA >= 64) { kind = KIND_BITMAP; need = need_bitmap; }
+      else { kind = KIND_LIST; need = need_list; }
+    } else {
+      if (ta == EH_BS) { kind = KIND_BITMAP; need = need_bitmap; }
+      else if (ta == EH_PS) { kind = KIND_PSDIR; need = need_psdir; }
+      else { kind = KIND_LIST; need = need_list; }
+    }
+    if (need <= s.lds_small) cls = 1;
+    else if (need <= s.lds_big) cls = 0;
+    else if (s.k == 3) { kind = KIND_GLOBAL; cls = 1; }
+    else { kind = KIND_GLOBAL; cls = 2; }  // unsupported: flagged by the host
+    const uint32_t gid = s.graph_of ? s.graph_of[i] : 0u;
+    s.rowinfo[i] = (kind << KIND_SHIFT) | gid;
+    uint32_t nt = 1;
+    if (s.tuned && s.task_work) {
+      uint64_t q = (work + s.task_work - 1) / s.task_work;
+      nt = (uint32_t)min<uint64_t>(max<uint64_t>(q, 1ull), (uint64_t)dA);
+    }
+    s.ntask[i] = nt;
+    const uint64_t order = s.tuned ? (0x3FFFFFFFull - min<uint64_t>(work, 0x3FFFFFFFull)) : 0ull;
+    s.sortkey[i] = ((uint64_t)cls << 62) | (order << 32) | i;
+  }
+  for (int q = 0; q < 6; q++) {
+    const unsigned long long v = wave_sum_u64(loc[q]);
+    if (lane == 0 && v) atomicAdd(&s.stats[q], v);
+  }
+}
+
+__global__ void k_gather_ntask(const uint64_t *sorted, const uint32_t *ntask, uint32_t n, uint32_t *nt_sorted,
+                               uint32_t *nbig_rows) {
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const uint64_t key = sorted[q];
+    nt_sorted[q] = ntask[(uint32_t)key];
+    if ((key >> 62) == 0) atomicAdd(nbig_rows, 1u);
+    if ((key >> 62) == 2) atomicAdd(nbig_rows + 1, 1u);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) nt_sorted[n] = 0;
+}
+
+__global__ void k_expand_tasks(const uint64_t *sorted, const uint32_t *nt_sorted, const uint64_t *toff,
+                               const uint32_t *card, const uint32_t *rowinfo, uint32_t n, uint4 *tasks) {
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const uint32_t nt = nt_sorted[q];
+    if (!nt) continue;
+    const uint32_t i = (uint32_t)sorted[q];
+    const uint32_t d = card[i];
+    const uint64_t base = toff[q];
+    for (uint32_t p = 0; p < nt; p++) {
+      const uint32_t jb = (uint32_t)(((uint64_t)p * d) / nt);
+      const uint32_t je = (uint32_t)(((uint64_t)(p + 1) * d) / nt);
+      tasks[base + p] = make_uint4(i, jb, je, rowinfo[i]);
+    }
+  }
+}
+
+inline unsigned grid_for(uint64_t work, unsigned block = 256) {
+  uint64_t g = ceil_div(work ? work : 1, block);
+  return (unsigned)(g > 65536 ? 65536 : g);
+}
+
+struct SchedStats {
+  unsigned long long v[6];
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+
+TuneParams tune_defaults(eh_matrix *m, int sched, const eh_run_opts *opts) {
+  TuneParams tp;
+  tp.lds_small = 2048;
+  tp.lds_big = 16384;
+  tp.waves_per_block = 4;
+  tp.lanes = 64;
+  if (sched == EH_SCHED_TUNED) {
+    uint64_t waves = (uint64_t)(m->ctx->num_cus ? m->ctx->num_cus : 256) * 16;
+    uint64_t tw = m->wedges / (waves * 8 + 1);
+    tp.task_work = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(tw, 2048), 1u << 26);
+  } else {
+    tp.task_work = 0;
+  }
+  if (opts) {
+    if (opts->task_work) tp.task_work = opts->task_work;
+    if (opts->lds_words) tp.lds_small = opts->lds_words;
+    if (opts->waves_per_block > 0) tp.waves_per_block = opts->waves_per_block;
+    if (opts->lanes_per_row > 0) tp.lanes = opts->lanes_per_row;
+  }
+  if (tp.lds_small < 256) tp.lds_small = 256;
+  if ((uint64_t)tp.lds_small * tp.waves_per_block * 4 > 160 * 1024) tp.waves_per_block = 1;
+  if (tp.lds_big < tp.lds_small) tp.lds_big = tp.lds_small;
+  return tp;
+}
+
+static uint64_t g_last_stats[6];
+
+eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, Schedule &s) {
+  eh_ctx *ctx = m->ctx;
+  hipStream_t st = ctx->stream;
+  const uint32_t n = m->n;
+  DevBuf key, key2, ntask, rowinfo, stats, ntsorted, toff, cnt2;
+  EH_CHECK(key.alloc(8ull * n + 8));
+  EH_CHECK(key2.alloc(8ull * n + 8));
+  EH_CHECK(ntask.alloc(4ull * n + 4));
+  EH_CHECK(rowinfo.alloc(4ull * n + 4));
+  EH_CHECK(stats.alloc(64));
+  EH_CHECK(cnt2.alloc(8));
+  EH_HIP(hipEventRecord(ctx->ev0, st));
+  k_zero_u64<<<1, 64, 0, st>>>(stats.as<unsigned long long>(), 8);
+  k_zero_u32<<<1, 64, 0, st>>>(cnt2.as<uint32_t>(), 2);
+  SchedArgs sa;
+  sa.card = m->dcard.as<uint32_t>();
+  sa.aoff = m->daoff.as<uint32_t>();
+  sa.unbr = m->dunbr.as<uint32_t>();
+  sa.desc = m->ddesc.as<RowDesc>();
+  sa.graph_of = m->dgraph_of.p ? m->dgraph_of.as<uint32_t>() : nullptr;
+  sa.n = n;
+  sa.tuned = sched == EH_SCHED_TUNED;
+  sa.k = k;
+  sa.lds_small = tp.lds_small;
+  sa.lds_big = tp.lds_big;
+  sa.task_work = tp.task_work;
+  sa.sortkey = key.as<uint64_t>();
+  sa.ntask = ntask.as<uint32_t>();
+  sa.rowinfo = rowinfo.as<uint32_t>();
+  sa.stats = stats.as<unsigned long long>();
+  k_row_work<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(sa);
+  EH_HIP(hipGetLastError());
+  EH_CHECK(prim_sort_u64(ctx, key.as<uint64_t>(), key2.as<uint64_t>(), n, 64));
+  EH_CHECK(ntsorted.alloc(4ull * n + 8));
+  EH_CHECK(toff.alloc(8ull * n + 16));
+  k_gather_ntask<<<grid_for(n), 256, 0, st>>>(key.as<uint64_t>(), ntask.as<uint32_t>(), n,
+                                             ntsorted.as<uint32_t>(), cnt2.as<uint32_t>());
+  EH_CHECK(prim_exscan_u32_to_u64(ctx, ntsorted.as<uint32_t>(), toff.as<uint64_t>(), (uint64_t)n + 1));
+  uint64_t ntasks = 0;
+  uint32_t hcnt[2] = {0, 0};
+  EH_HIP(hipMemcpyAsync(&ntasks, toff.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, st));
+  EH_HIP(hipMemcpyAsync(hcnt, cnt2.p, 8, hipMemcpyDeviceToHost, st));
+  SchedStats hs;
+  EH_HIP(hipMemcpyAsync(hs.v, stats.p, 48, hipMemcpyDeviceToHost, st));
+  EH_HIP(hipStreamSynchronize(st));
+  if (hcnt[1]) {
+    eh_set_error("%u rows need more LDS than the %u-word budget for a %d-clique pass", hcnt[1], tp.lds_big, k);
+    return EH_ERR_CAPACITY;
+  }
+  uint64_t nbig = 0;
+  if (hcnt[0]) EH_HIP(hipMemcpy(&nbig, toff.as<uint64_t>() + hcnt[0], 8, hipMemcpyDeviceToHost));
+  EH_CHECK(s.tasks.alloc(16 * ntasks + 16));
+  k_expand_tasks<<<grid_for(n), 256, 0, st>>>(key.as<uint64_t>(), ntsorted.as<uint32_t>(), toff.as<uint64_t>(),
+                                              m->dcard.as<uint32_t>(), rowinfo.as<uint32_t>(), n,
+                                              s.tasks.as<uint4>());
+  EH_HIP(hipGetLastError());
+  EH_HIP(hipEventRecord(ctx->ev1, st));
+  EH_HIP(hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  EH_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  s.ms = ms;
+  s.n_big = nbig;
+  s.n_small = ntasks - nbig;
+  s.valid = true;
+  s.sched = sched;
+  s.task_work = tp.task_work;
+  s.lds_small = tp.lds_small;
+  s.lds_big = tp.lds_big;
+  for (int q = 0; q < 6; q++) g_last_stats[q] = hs.v[q];
+  return EH_OK;
+}
+
+namespace {
+
+template <int K, bool W>
+eh_status launch_count(eh_ctx *ctx, const CountArgs &base, const uint4 *tasks, uint64_t ntasks,
+                       uint32_t lds_words, int waves, uint32_t *d_next) {
+  if (ntasks == 0) return EH_OK;
+  CountArgs a = base;
+  a.tasks = tasks;
+  a.ntasks = ntasks;
+  a.next = d_next;
+  a.lds_words = lds_words;
+  const size_t lds_bytes = (size_t)lds_words * 4 * waves;
+  const int threads = 64 * waves;
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count<K, W>, threads, lds_bytes);
+  if (e != hipSuccess || per_cu < 1) per_cu = 1;
+  uint64_t blocks = (uint64_t)per_cu * (ctx->num_cus ? ctx->num_cus : 256);
+  const uint64_t need = ceil_div(ntasks, (uint64_t)waves);
+  if (blocks > need) blocks = need;
+  k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
+  hipLaunchKernelGGL((k_count<K, W>), dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a);
+  EH_HIP(hipGetLastError());
+  return EH_OK;
+}
+
+}  // namespace
+
+eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, Schedule &s,
+                          uint64_t *d_counts, uint32_t *d_next) {
+  (void)sched;
+  CountArgs a;
+  a.card = m->dcard.as<uint32_t>();
+  a.aoff = m->daoff.as<uint32_t>();
+  a.unbr = m->dunbr.as<uint32_t>();
+  a.desc = m->ddesc.as<RowDesc>();
+  a.bsw = m->dbsw.as<uint64_t>();
+  a.psd = m->dpsd.as<uint16_t>();
+  a.counts = reinterpret_cast<unsigned long long *>(d_counts);
+  a.weighted = 0;
+  const uint4 *t = s.tasks.as<uint4>();
+  const bool weighted = k == -3;
+  if (k == 4) {
+    EH_CHECK((launch_count<4, false>(m->ctx, a, t, s.n_big, s.lds_big, 1, d_next)));
+    EH_CHECK((launch_count<4, false>(m->ctx, a, t + s.n_big, s.n_small, s.lds_small, tp.waves_per_block,
+                                     d_next + 1)));
+  } else if (weighted) {
+    EH_CHECK((launch_count<3, true>(m->ctx, a, t, s.n_big, s.lds_big, 1, d_next)));
+    EH_CHECK((launch_count<3, true>(m->ctx, a, t + s.n_big, s.n_small, s.lds_small, tp.waves_per_block,
+                                    d_next + 1)));
+  } else {
+    EH_CHECK((launch_count<3, false>(m->ctx, a, t, s.n_big, s.lds_big, 1, d_next)));
+    EH_CHECK((launch_count<3, false>(m->ctx, a, t + s.n_big, s.n_small, s.lds_small, tp.waves_per_block,
+                                     d_next + 1)));
+  }
+  return EH_OK;
+}
+
+// k: 3 triangles, 4 four-cliques, -3 weighted triangle pass (sum of card(x) over
+// common x; the per-triangle byte term of the 4-clique model)
+eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_counts, uint32_t ncounts,
+                    uint64_t *count, eh_run_stats *stats, bool capture_ok) {
+  (void)capture_ok;
+  eh_ctx *ctx = m->ctx;
+  hipStream_t st = ctx->stream;
+  const int sched = opts ? opts->schedule : EH_SCHED_TUNED;
+  if (sched != EH_SCHED_PLAIN && sched != EH_SCHED_TUNED) { eh_set_error("bad schedule %d", sched); return EH_ERR_INVALID; }
+  const int kk = (k == 4) ? 4 : 3;
+  TuneParams tp = tune_defaults(m, sched, opts);
+  Schedule &s = m->sched[sched];
+  const int skey = kk * 8 + sched;
+  double sched_ms = 0;
+  if (!s.valid || s.task_work != tp.task_work || s.lds_small != tp.lds_small || s.lds_big != tp.lds_big ||
+      (int)s.sched != skey) {
+    s = Schedule();
+    EH_CHECK(schedule_build(m, sched, kk, tp, s));
+    s.sched = skey;
+    sched_ms = s.ms;
+  }
+  DevBuf own_counts, next;
+  unsigned long long *dc = reinterpret_cast<unsigned long long *>(d_counts);
+  if (!dc) {
+    EH_CHECK(own_counts.alloc(8ull * (ncounts ? ncounts : 1)));
+    dc = own_counts.as<unsigned long long>();
+  }
+  EH_CHECK(next.alloc(16));
+  const int repeat = (opts && opts->repeat > 1) ? opts->repeat : 1;
+  float total_ms = 0;
+  for (int r = 0; r < repeat; r++) {
+    k_zero_u64<<<grid_for(ncounts ? ncounts : 1), 256, 0, st>>>(dc, ncounts ? ncounts : 1);
+    EH_HIP(hipEventRecord(ctx->ev0, st));
+    EH_CHECK(schedule_launch(m, k, sched, tp, s, reinterpret_cast<uint64_t *>(dc), next.as<uint32_t>()));
+    EH_HIP(hipEventRecord(ctx->ev1, st));
+    EH_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    EH_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    total_ms += ms;
+  }
+  if (count) {
+    std::vector<uint64_t> h(ncounts ? ncounts : 1);
+    EH_HIP(hipMemcpy(h.data(), dc, 8 * h.size(), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < h.size(); q++) count[q] = h[q];
+  }
+  if (stats) {
+    stats->kernel_ms = total_ms / repeat;
+    stats->sched_ms = sched_ms;
+    stats->tasks = s.n_big + s.n_small;
+    stats->alg_bytes = 4ull * (m->card + m->wedges) + 8ull * ((uint64_t)m->n + 1);
+    if (sched_ms > 0) {
+      for (int q = 0; q < 4; q++) stats->pairs[q] = g_last_stats[q];
+      stats->work_units = g_last_stats[4];
+    }
+    stats->schedule = sched;
+    stats->lanes_per_row = tp.lanes;
+    stats->task_work = tp.task_work;
+    stats->lds_words = tp.lds_small;
+  }
+  return EH_OK;
+}
+
+extern "C" eh_status eh_triangle_count(eh_matrix *m, const eh_run_opts *opts, uint64_t *count,
+                                       eh_run_stats *stats) {
+  if (!m || !count) { eh_set_error("eh_triangle_count: null argument"); return EH_ERR_INVALID; }
+  EH_HIP(hipSetDevice(m->ctx->device));
+  if (stats) *stats = eh_run_stats{};
+  return count_run(m, 3, opts, nullptr, 1, count, stats, false);
+}
+
+extern "C" eh_status eh_clique_count(eh_matrix *m, uint32_t k, const eh_run_opts *opts, uint64_t *count,
+                                     eh_run_stats *stats) {
+  if (!m || !count) { eh_set_error("eh_clique_count: null argument"); return EH_ERR_INVALID; }
+  if (k != 3 && k != 4) { eh_set_error("eh_clique_count: k=%u not supported (3 or 4)", k); return EH_ERR_UNSUPPORTED; }
+  EH_HIP(hipSetDevice(m->ctx->device));
+  if (stats) *stats = eh_run_stats{};
+  return count_run(m, (int)k, opts, nullptr, 1, count, stats, false);
+}

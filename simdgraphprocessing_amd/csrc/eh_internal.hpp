@@ -1,0 +1,186 @@
+// eh_internal.hpp -- shared declarations of the gfx950 set-intersection engine.
+//
+// Device layout of a built matrix (one contiguous set of HBM buffers per matrix):
+//   card[n]      u32   kept entries per row (d+ under the nbr<node filter)
+//   aoff[n]      u32   row start in `unbr`, in units of 4 elements (16-byte aligned rows)
+//   unbr[]       u32   canonical sorted uint32 rows, each padded to a multiple of 4
+//                      with 0xFFFFFFFF (never a vertex id)
+//   desc[n]      RowDesc (16 B): the B-side payload of the row in its chosen layout
+//   bsw[]        u64   bitset words (reference bitset: word index = value >> 6)
+//   psd[]        u16   pshort streams [prefix, count-1, low16 x count] per partition,
+//                      each row 8-byte aligned
+// The uint rows are always present (they drive the j-enumeration of a source
+// row); the pshort/bitset payloads exist for rows whose selected layout is
+// that type.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/eh.h"
+
+#define EH_U 2u
+#define EH_PS 1u
+#define EH_BS 0u
+#define EH_PAD_VALUE 0xFFFFFFFFu
+
+struct RowDesc {
+  uint32_t card;   // element count
+  uint32_t tinfo;  // bits 0-1: type (0 BS, 1 PS, 2 U); bits 2-31: BS words / PS stream shorts
+  uint32_t w0;     // BS: first u64 word index; PS: number of partitions
+  uint32_t off;    // U: element offset/4; BS: word offset; PS: short offset/4
+};
+static_assert(sizeof(RowDesc) == 16, "RowDesc must be 16 bytes");
+
+__host__ __device__ inline uint32_t desc_type(const RowDesc &d) { return d.tinfo & 3u; }
+__host__ __device__ inline uint32_t desc_aux(const RowDesc &d) { return d.tinfo >> 2; }
+
+// ---------------------------------------------------------------- errors
+
+void eh_set_error(const char *fmt, ...);
+
+#define EH_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      eh_set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      return e_ == hipErrorOutOfMemory ? EH_ERR_OOM : EH_ERR_HIP;                     \
+    }                                                                                 \
+  } while (0)
+
+#define EH_CHECK(expr)                  \
+  do {                                  \
+    eh_status s_ = (expr);              \
+    if (s_ != EH_OK) return s_;         \
+  } while (0)
+
+// ---------------------------------------------------------------- device buffers
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  eh_status alloc(size_t b) {
+    release();
+    if (b == 0) b = 16;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      eh_set_error("hipMalloc(%zu): %s", b, hipGetErrorString(e));
+      return EH_ERR_OOM;
+    }
+    bytes = b;
+    return EH_OK;
+  }
+  template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+// ---------------------------------------------------------------- handles
+
+struct TuneParams {
+  int lanes = 64;          // lanes cooperating on one B row
+  uint32_t task_work = 0;  // work units per task (0: derived)
+  uint32_t lds_small = 2048;
+  uint32_t lds_big = 16384;
+  int waves_per_block = 4;
+};
+
+struct eh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int num_cus = 0;
+  std::string name;
+  std::vector<float> mlp;  // selector parameters (EH_MLP_PARAMS) or empty
+  DevBuf scratch;          // reusable temp storage for primitives
+  DevBuf counter;          // small device counters
+};
+
+struct eh_graph {
+  eh_ctx *ctx = nullptr;
+  uint32_t n = 0;
+  uint64_t entries = 0;  // stored entries (2x undirected edges)
+  uint32_t max_degree = 0;
+  DevBuf off;      // u64[n+1]
+  DevBuf nbr;      // u32[entries]
+  DevBuf new2old;  // u32[n] (empty => identity)
+};
+
+struct Schedule {
+  bool valid = false;
+  int sched = -1;
+  uint32_t task_work = 0;
+  uint32_t lds_small = 0, lds_big = 0;
+  DevBuf tasks;  // uint4 {i, jb, je, graph}
+  uint64_t n_small = 0, n_big = 0;  // tasks [0,n_big) need the big LDS budget
+  double ms = 0;
+};
+
+struct eh_matrix {
+  eh_ctx *ctx = nullptr;
+  uint32_t n = 0;
+  uint64_t card = 0;
+  uint32_t max_row = 0;
+  int policy = 0;
+  uint64_t unbr_words = 0;  // u32 entries in unbr (padded)
+  uint64_t bs_words = 0;
+  uint64_t ps_shorts = 0;
+  uint64_t rows_by_type[3] = {0, 0, 0};
+  uint64_t ref_row_bytes = 0;
+  uint64_t wedges = 0;
+  double build_ms = 0;
+  DevBuf dcard, daoff, dunbr, ddesc, dbsw, dpsd;
+  DevBuf dgraph_of;  // u32 per row: graph id (batch arenas); empty => all 0
+  Schedule sched[2];
+};
+
+// ---------------------------------------------------------------- primitives (eh_prim.hip)
+
+eh_status prim_sort_u64(eh_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int end_bit);
+eh_status prim_sort_pairs_u32(eh_ctx *ctx, uint32_t *keys, uint32_t *keys_tmp, uint32_t *vals,
+                              uint32_t *vals_tmp, uint64_t n);
+eh_status prim_unique_u64(eh_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n,
+                          uint64_t *n_out);
+eh_status prim_exscan_u64(eh_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n);
+eh_status prim_exscan_u32_to_u64(eh_ctx *ctx, const uint32_t *in, uint64_t *out, uint64_t n);
+eh_status prim_reduce_max_u32(eh_ctx *ctx, const uint32_t *in, uint64_t n, uint32_t *out);
+eh_status prim_reduce_sum_u64(eh_ctx *ctx, const uint64_t *in, uint64_t n, uint64_t *out);
+
+// ---------------------------------------------------------------- internal entry points
+
+eh_status graph_canonicalize(eh_ctx *ctx, uint32_t n, DevBuf &keys, uint64_t nkeys, int relabel,
+                             eh_graph **out);
+eh_status graph_relabel(eh_graph *g);
+eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const uint32_t *d_nbr,
+                           uint64_t entries, eh_edge_filter filter, const uint8_t *d_mask,
+                           eh_layout_policy policy, const uint32_t *d_graph_of, eh_matrix **out);
+eh_status matrix_select_learned(eh_matrix *m, uint8_t *d_types);
+eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_counts,
+                    uint32_t ncounts, uint64_t *count, eh_run_stats *stats, bool capture_ok);
+eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, Schedule &s);
+eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, Schedule &s,
+                          uint64_t *d_counts, uint32_t *d_next);
+TuneParams tune_defaults(eh_matrix *m, int sched, const eh_run_opts *opts);
+
+// timing helper: device ms between two events recorded on ctx->stream
+double ctx_elapsed_ms(eh_ctx *ctx);
+
+inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }

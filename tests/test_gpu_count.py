@@ -1,0 +1,73 @@
+"""GPU parity: triangle and 4-clique counts through libeh.so against the oracle
+and the reference's own fixture values (test/undirected_triangle_counting_test.cpp:11,
+test/n_clique_test.cpp:11)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FB_TRIANGLES = 1612010
+FB_4CLIQUES = 30004668
+POLICIES = ["uint", "pshort", "bs", "hybrid", "hybrid3", "cost"]
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+@pytest.mark.parametrize("schedule", ["plain", "tuned"])
+def test_facebook_triangles(ctx, facebook, policy, schedule):
+    M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "lower", policy)
+    c, st = M.triangle_count(schedule)
+    print(f"facebook {policy}/{schedule}: {c} in {st['kernel_ms']:.3f} ms", flush=True)
+    assert c == FB_TRIANGLES
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+@pytest.mark.parametrize("schedule", ["plain", "tuned"])
+def test_facebook_4cliques(ctx, facebook, policy, schedule):
+    M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "lower", policy)
+    c, st = M.clique_count(4, schedule)
+    print(f"facebook 4-clique {policy}/{schedule}: {c} in {st['kernel_ms']:.3f} ms", flush=True)
+    assert c == FB_4CLIQUES
+
+
+def test_facebook_ref_row_bytes(ctx, facebook):
+    # the reference prints "ROW DATA SIZE (Bytes): 569724" for the hybrid build
+    M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "lower", "hybrid")
+    om = O.Matrix(facebook["n"], facebook["off"], facebook["nbr"], "hybrid")
+    assert M.info()["ref_row_bytes"] == om.bytes() == 569724
+    assert M.info()["rows_by_type"]["bitset"] == om.type_counts()["bitset"]
+
+
+def test_er_c1_parity(ctx):
+    """ER G(10000, 100000 draws), seed 1: device generator + build + count vs oracle."""
+    g = ctx.generate_er(10000, 100000, 1)
+    off, nbr = g.download()
+    src, dst = O.gen_er(10000, 100000, 1)
+    ref = O.canonical_oriented(10000, src, dst)
+    roff, rnbr = ref["relabel"][1], ref["relabel"][2]
+    assert np.array_equal(off, roff) and np.array_equal(nbr, rnbr)
+    M = g.build_matrix("lower", "hybrid")
+    om = O.Matrix(10000, roff, rnbr, "hybrid")
+    exp, _ = om.triangles(8)
+    for sched in ("plain", "tuned"):
+        c, st = M.triangle_count(sched)
+        assert c == exp, sched
+        k4, _ = M.clique_count(4, sched)
+        assert k4 == om.cliques(4, 8)[0]
+
+
+@pytest.mark.parametrize("scale", [12, 16])
+def test_rmat_parity(ctx, scale):
+    g = ctx.generate_rmat(scale, 16, 2)
+    off, nbr = g.download()
+    src, dst = O.gen_rmat(scale, 16, 2)
+    ref = O.canonical_oriented(1 << scale, src, dst)
+    assert np.array_equal(off, ref["relabel"][1]) and np.array_equal(nbr, ref["relabel"][2])
+    om = O.Matrix(1 << scale, off, nbr, "hybrid")
+    exp, _ = om.triangles(8)
+    for policy in ("hybrid", "hybrid3", "cost", "uint", "bs", "pshort"):
+        M = g.build_matrix("lower", policy)
+        for sched in ("plain", "tuned"):
+            c, _ = M.triangle_count(sched)
+            assert c == exp, (policy, sched)
