@@ -105,7 +105,9 @@ typedef struct {
   uint32_t lds_words;     /* 0 = default; per-wave LDS budget in 32-bit words */
   int32_t waves_per_block;/* 0 = default (4) */
   int32_t repeat;         /* run the counting kernel this many times (>=1); stats average */
-  int32_t reserved[2];
+  int32_t small_degree;   /* TUNED on oriented matrices: rows with at most this many
+                             entries are counted edge-centrically; 0 = default, <0 = off */
+  int32_t reserved;
 } eh_run_opts;
 
 typedef struct {
@@ -204,9 +206,15 @@ eh_status eh_intersect_rows(eh_matrix *m, const uint32_t *a_rows, const uint32_t
  * sum over kept (i,j) of |row(i) & row(j)|. */
 eh_status eh_triangle_count(eh_matrix *m, const eh_run_opts *opts, uint64_t *count,
                             eh_run_stats *stats);
-/* n_clique<T,R>::run (apps/n_clique.cpp:52-101), query_depth = k (3..8). */
+/* n_clique<T,R>::run (apps/n_clique.cpp:52-101), query_depth = k (3 or 4). */
 eh_status eh_clique_count(eh_matrix *m, uint32_t k, const eh_run_opts *opts, uint64_t *count,
                           eh_run_stats *stats);
+/* No reference counterpart (the reference times whole phases only,
+ * src/common.hpp:87-99): one counting pass with the shader-clock cycles of
+ * every task recorded; task_rows/task_kinds/task_cycles get ntasks entries
+ * (cap is their capacity).  The engine-generated timings the selector trains on. */
+eh_status eh_count_profile(eh_matrix *m, uint32_t k, const eh_run_opts *opts, uint32_t *task_rows,
+                           uint32_t *task_kinds, uint64_t *task_cycles, uint64_t cap, uint64_t *ntasks);
 
 /* ---------------------------------------------------------------- batches */
 /* No reference counterpart (the reference processes one graph per process,

@@ -47,7 +47,7 @@ class MatrixInfo(C.Structure):
 class RunOpts(C.Structure):
     _fields_ = [("schedule", C.c_int32), ("lanes_per_row", C.c_int32), ("task_work", C.c_uint32),
                 ("lds_words", C.c_uint32), ("waves_per_block", C.c_int32), ("repeat", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("small_degree", C.c_int32), ("reserved", C.c_int32)]
 
 
 class RunStats(C.Structure):
@@ -78,7 +78,7 @@ EXPORTS = [
     "eh_graph_relabel_by_degree", "eh_graph_info_get", "eh_graph_download", "eh_graph_download_perm",
     "eh_graph_free", "eh_matrix_build", "eh_matrix_build_host", "eh_matrix_info_get", "eh_matrix_row",
     "eh_matrix_download", "eh_matrix_row_types", "eh_matrix_free", "eh_intersect", "eh_intersect_rows",
-    "eh_triangle_count", "eh_clique_count", "eh_batch_generate", "eh_batch_info",
+    "eh_triangle_count", "eh_clique_count", "eh_count_profile", "eh_batch_generate", "eh_batch_info",
     "eh_batch_graph_params", "eh_batch_graph_download", "eh_batch_count", "eh_batch_free",
     "eh_selector_set_params", "eh_selector_get_params", "eh_matrix_features",
     "eh_matrix_profile_layouts", "eh_mlp_forward", "eh_mlp_loss_grad", "eh_selector_train",
@@ -134,6 +134,8 @@ def lib():
     L.eh_triangle_count.argtypes = [_vp, C.POINTER(RunOpts), C.POINTER(C.c_uint64), C.POINTER(RunStats)]
     L.eh_clique_count.argtypes = [_vp, C.c_uint32, C.POINTER(RunOpts), C.POINTER(C.c_uint64),
                                   C.POINTER(RunStats)]
+    L.eh_count_profile.argtypes = [_vp, C.c_uint32, C.POINTER(RunOpts), _vp, _vp, _vp, C.c_uint64,
+                                   C.POINTER(C.c_uint64)]
     L.eh_batch_generate.argtypes = [_vp, C.c_uint64, C.c_uint32, C.c_int, _pp]
     L.eh_batch_info.argtypes = [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.eh_batch_graph_params.argtypes = [_vp, _vp, _vp]
@@ -163,8 +165,9 @@ def _ptr(a):
     return C.c_void_p(a.ctypes.data) if a is not None else None
 
 
-def _opts(schedule="tuned", lanes=0, task_work=0, lds_words=0, waves=0, repeat=1):
+def _opts(schedule="tuned", lanes=0, task_work=0, lds_words=0, waves=0, repeat=1, small_degree=0):
     o = RunOpts()
+    o.small_degree = small_degree
     o.schedule = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
     o.lanes_per_row = lanes
     o.task_work = task_work
@@ -412,6 +415,20 @@ class SparseMatrix:
         o = _opts(schedule, **kw)
         _check(lib().eh_clique_count(self.h, k, C.byref(o), C.byref(c), C.byref(st)))
         return int(c.value), st.as_dict()
+
+    def profile(self, k=3, schedule="tuned", **kw):
+        """One counting pass with per-task shader-clock cycles: (rows, kinds, cycles)."""
+        o = _opts(schedule, **kw)
+        nt = C.c_uint64()
+        st = lib().eh_count_profile(self.h, k, C.byref(o), None, None, None, 0, C.byref(nt))
+        if st not in (0, 6):
+            _check(st)
+        n = int(nt.value)
+        rows = np.zeros(max(n, 1), np.uint32)
+        kinds = np.zeros(max(n, 1), np.uint32)
+        cyc = np.zeros(max(n, 1), np.uint64)
+        _check(lib().eh_count_profile(self.h, k, C.byref(o), _ptr(rows), _ptr(kinds), _ptr(cyc), n, C.byref(nt)))
+        return rows[:n], kinds[:n], cyc[:n]
 
     def intersect_rows(self, a_rows, b_rows):
         a = np.ascontiguousarray(a_rows, np.uint32)

@@ -279,6 +279,7 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   m->ctx = ctx;
   m->n = n;
   m->policy = policy;
+  m->filter = filter;
   eh_status s = EH_OK;
   auto fail = [&](eh_status e) { delete m; return e; };
   DevBuf pc, poff, type, parts, bs_n, ps_n, bsoff, psoff, refb, stats, wed;

@@ -29,8 +29,9 @@
 
 namespace {
 
-constexpr uint32_t KIND_BITMAP = 0, KIND_LIST = 1, KIND_PSDIR = 2, KIND_GLOBAL = 3;
+constexpr uint32_t KIND_BITMAP = 0, KIND_LIST = 1, KIND_PSDIR = 2, KIND_GLOBAL = 3, KIND_HASH = 4;
 constexpr uint32_t KIND_SHIFT = 28;
+constexpr uint32_t HASH_EMPTY = 0xFFFFFFFFu;
 constexpr uint32_t GID_MASK = (1u << KIND_SHIFT) - 1;
 
 struct CountArgs {
@@ -45,7 +46,10 @@ struct CountArgs {
   uint32_t *next;
   unsigned long long *counts;
   uint32_t lds_words;
-  uint32_t weighted;  // add card(x) per common element (byte model of the 4-clique pass)
+  uint32_t tuned;     // flattened per-pair schedule (else the reference dispatch)
+  const uint32_t *graph_of;         // per-row graph id of batch arenas (null: one graph)
+  unsigned long long *task_cycles;  // optional: shader clock cycles spent per task
+  uint64_t task_base;               // index of tasks[0] in the whole schedule
 };
 
 __device__ __forceinline__ uint32_t rfl(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -102,6 +106,8 @@ struct AState {
   const uint32_t *dpre;   // PSDIR: partition prefixes (P), starts (P + 1)
   const uint32_t *dstart;
   uint32_t P;
+  uint32_t *ht;           // HASH: LDS table of 2^hlg slots
+  uint32_t hlg;
 };
 
 template <uint32_t KIND>
@@ -120,6 +126,15 @@ __device__ __forceinline__ uint32_t memb(const AState &A, uint32_t x) {
     if (A.dpre[base] != hi) return 0;
     const uint32_t s = A.dstart[base], e = A.dstart[base + 1];
     return sorted_contains(A.list + s, e - s, x);
+  } else if constexpr (KIND == KIND_HASH) {
+    const uint32_t mask = (1u << A.hlg) - 1u;
+    uint32_t s = (x * 2654435761u) >> (32u - A.hlg);
+    for (;;) {
+      const uint32_t v = A.ht[s];
+      if (v == x) return 1;
+      if (v == HASH_EMPTY) return 0;
+      s = (s + 1u) & mask;
+    }
   } else {
     return sorted_contains(A.list, A.n, x);
   }
@@ -339,6 +354,8 @@ __device__ __forceinline__ uint32_t stage_a(const CountArgs &a, uint32_t kind, u
   A.base32 = 2u * A.aw0;
   A.P = 0;
   A.dpre = A.dstart = nullptr;
+  A.ht = nullptr;
+  A.hlg = 0;
   A.bm = L;
   if (kind == KIND_GLOBAL) {
     A.list = G;
@@ -404,6 +421,8 @@ __device__ __forceinline__ void make_cstate(const AState &A, uint32_t *clist, ui
   Cs.span = clist[nc - 1] - clist[0];
   Cs.P = 0;
   Cs.dpre = Cs.dstart = nullptr;
+  Cs.ht = nullptr;
+  Cs.hlg = 0;
   Cs.aw0 = A.aw0;
   Cs.anw = A.anw;
   Cs.base32 = A.base32;
@@ -490,9 +509,226 @@ __device__ uint64_t task_cliques4(const CountArgs &a, const AState &A, uint32_t 
   return acc;
 }
 
+// ---------------------------------------------------------------- tuned path: flattened pair work
+//
+// For a chunk of up to 64 keys (the j's of a task, or the k's of a 4-clique
+// level set) every pair (S, row(key)) is classified by the cheaper of
+//   PROBE   S's elements inside B's word span test B's bitset bits (|S in span| items)
+//   AND     S's LDS bitmap AND B's words over the overlapping span (overlap words)
+//   STREAM  B's uint elements (16-byte vectors) test membership in S (|B|/4 items)
+// and the items of all pairs of one kind are flattened over the 64 lanes, so a
+// wave keeps 64 independent loads in flight instead of one pair at a time.
+// pshort B rows are walked partition by partition, one pair after another.
+
+constexpr uint32_t PASS_PROBE = 0, PASS_AND = 1, PASS_STREAM = 2;
+constexpr uint32_t META_WORDS = 260;
+constexpr uint32_t SMALL_META_WORDS = 196;
+
+__device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t *L, uint32_t n, uint32_t x) {
+  uint32_t lo = 0, len = n;
+  while (len > 0) {
+    const uint32_t half = len >> 1;
+    if (L[lo + half] < x) { lo += half + 1; len -= half + 1; }
+    else len = half;
+  }
+  return lo;
+}
+
+template <int PASS, uint32_t KIND, bool W>
+__device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &S, uint32_t c, uint32_t p0,
+                                              uint32_t p1, uint32_t p2, uint32_t lane, uint32_t *meta) {
+  if (__ballot(c != 0u) == 0ull) return 0;
+  uint32_t tot;
+  const uint32_t base = wave_excl_scan(c, lane, &tot);
+  tot = rfl(tot);
+  if (tot == 0) return 0;
+  uint32_t *mbase = meta, *m0 = meta + 65, *m1 = meta + 130, *m2 = meta + 195;
+  mbase[lane] = base;
+  m0[lane] = p0;
+  m1[lane] = p1;
+  m2[lane] = p2;
+  if (lane == 0) mbase[64] = tot;
+  wave_sync();
+  uint64_t acc = 0;
+  for (uint32_t t = lane; t < tot; t += 64) {
+    uint32_t q = 0;
+#pragma unroll
+    for (uint32_t step = 32; step; step >>= 1)
+      if (mbase[q + step] <= t) q += step;
+    const uint32_t r = t - mbase[q];
+    if constexpr (PASS == PASS_PROBE) {
+      const uint32_t x = S.list[m0[q] + r];
+      const uint64_t word = a.bsw[m1[q] + ((x >> 6) - m2[q])];
+      const uint32_t h = (uint32_t)((word >> (x & 63u)) & 1ull);
+      if constexpr (W) acc += h ? (uint64_t)a.card[x] : 0ull;
+      else acc += h;
+    } else if constexpr (PASS == PASS_AND) {
+      const uint32_t w = m0[q] + r;
+      const uint32_t i2 = 2u * (w - S.aw0);
+      uint64_t v = ((uint64_t)S.bm[i2] | ((uint64_t)S.bm[i2 + 1] << 32)) & a.bsw[m1[q] + (w - m2[q])];
+      if constexpr (W) {
+        while (v) {
+          acc += a.card[64u * w + (uint32_t)__builtin_ctzll(v)];
+          v &= v - 1;
+        }
+      } else {
+        acc += (uint64_t)__popcll(v);
+      }
+    } else {
+      const uint4 x = reinterpret_cast<const uint4 *>(a.unbr)[(uint64_t)m1[q] + r];
+      acc += hit_value<KIND, W>(S, a, x.x) + hit_value<KIND, W>(S, a, x.y) + hit_value<KIND, W>(S, a, x.z) +
+             hit_value<KIND, W>(S, a, x.w);
+    }
+  }
+  wave_sync();
+  return acc;
+}
+
+// sum over keys K[kb..ke) of |S & row(key)|
+template <uint32_t KIND, bool W>
+__device__ uint64_t sum_isect(const CountArgs &a, const AState &S, const uint32_t *K, uint32_t kb, uint32_t ke,
+                              uint32_t lane, uint32_t *meta) {
+  uint64_t acc = 0;
+  for (uint32_t kc = kb; kc < ke; kc += 64) {
+    const uint32_t nk = min(64u, ke - kc);
+    RowDesc d = {0u, 0u, 0u, 0u};
+    if (lane < nk) d = a.desc[K[kc + lane]];
+    const uint32_t t = desc_type(d);
+    uint32_t pc = 0, ps = 0, ac = 0, as = 0, sc = 0, seq = 0;
+    if (d.card) {
+      if (t == EH_BS) {
+        const uint32_t bw0 = d.w0, bnw = desc_aux(d);
+        const uint64_t xlo = (uint64_t)bw0 * 64ull, xhi = (uint64_t)(bw0 + bnw) * 64ull;
+        const uint32_t s = (xlo > S.lo + S.span) ? S.n : lds_lower_bound(S.list, S.n, (uint32_t)xlo);
+        const uint32_t e = (xhi > 0xFFFFFFFFull) ? S.n : lds_lower_bound(S.list, S.n, (uint32_t)xhi);
+        pc = e - s;
+        ps = s;
+        if constexpr (KIND == KIND_BITMAP) {
+          const uint32_t os = max(S.aw0, bw0), oe = min(S.aw0 + S.anw, bw0 + bnw);
+          const uint32_t ov = oe > os ? oe - os : 0u;
+          if (ov < pc) { ac = ov; as = os; pc = 0; }
+        }
+      } else if (t == EH_U) {
+        sc = (d.card + 3u) >> 2;
+      } else {
+        seq = 1;
+      }
+    }
+    acc += flat_pass<PASS_PROBE, KIND, W>(a, S, pc, ps, d.off, d.w0, lane, meta);
+    if constexpr (KIND == KIND_BITMAP) acc += flat_pass<PASS_AND, KIND, W>(a, S, ac, as, d.off, d.w0, lane, meta);
+    acc += flat_pass<PASS_STREAM, KIND, W>(a, S, sc, 0u, d.off, 0u, lane, meta);
+    uint64_t sm = __ballot(seq);
+    while (sm) {
+      const uint32_t q = (uint32_t)__builtin_ctzll(sm);
+      sm &= sm - 1;
+      RowDesc dq;
+      dq.card = rdl(d.card, q);
+      dq.tinfo = rdl(d.tinfo, q);
+      dq.w0 = rdl(d.w0, q);
+      dq.off = rdl(d.off, q);
+      acc += isect_b_pshort<KIND, W>(S, a, dq, lane);
+    }
+  }
+  return acc;
+}
+
+__device__ __forceinline__ uint32_t hash_lg(uint32_t n) {
+  uint32_t lg = 4;
+  while ((1u << lg) < 2u * n) lg++;
+  return lg;
+}
+
+// LDS hash table over a sorted list (linear probing, multiplicative hash)
+__device__ __forceinline__ void build_hash(const uint32_t *list, uint32_t n, uint32_t *ht, uint32_t lg,
+                                           uint32_t lane) {
+  const uint32_t cap = 1u << lg, mask = cap - 1u;
+  for (uint32_t w = lane; w < cap; w += 64) ht[w] = HASH_EMPTY;
+  wave_sync();
+  for (uint32_t k = lane; k < n; k += 64) {
+    const uint32_t x = list[k];
+    uint32_t s = (x * 2654435761u) >> (32u - lg);
+    while (atomicCAS(&ht[s], HASH_EMPTY, x) != HASH_EMPTY) s = (s + 1u) & mask;
+  }
+  wave_sync();
+}
+
+// stage a tuned source row: list + (bitmap | hash | nothing)
+__device__ __forceinline__ uint32_t stage_tuned(const CountArgs &a, uint32_t kind, uint32_t i, uint32_t lane,
+                                                uint32_t *L, AState &A) {
+  if (kind != KIND_HASH) return stage_a(a, kind, i, lane, L, A);
+  const uint32_t used = stage_a(a, KIND_LIST, i, lane, L, A);
+  const uint32_t lg = hash_lg(A.n);
+  A.ht = L + used;
+  A.hlg = lg;
+  build_hash(A.list, A.n, A.ht, lg, lane);
+  return used + (1u << lg);
+}
+
+template <uint32_t KIND>
+__device__ uint64_t task_cliques4_tuned(const CountArgs &a, const AState &A, uint32_t jb, uint32_t je,
+                                        uint32_t lane, uint32_t *free_lds, uint32_t *meta) {
+  uint64_t acc = 0;
+  uint32_t *clist = free_lds;
+  uint32_t *cmemb = free_lds + A.n;
+  for (uint32_t jc = jb; jc < je; jc += 64) {
+    const uint32_t nj = min(64u, je - jc);
+    RowDesc dl = {0u, 0u, 0u, 0u};
+    if (lane < nj) dl = a.desc[A.list[jc + lane]];
+    for (uint32_t t = 0; t < nj; t++) {
+      RowDesc dj;
+      dj.card = rdl(dl.card, t);
+      dj.tinfo = rdl(dl.tinfo, t);
+      dj.w0 = rdl(dl.w0, t);
+      dj.off = rdl(dl.off, t);
+      const uint32_t nc = rfl(isect_write<KIND>(A, a, dj, lane, clist));
+      if (nc == 0) continue;
+      AState C;
+      if constexpr (KIND == KIND_BITMAP) {
+        make_cstate<KIND_BITMAP>(A, clist, nc, cmemb, lane, C);
+        acc += sum_isect<KIND_BITMAP, false>(a, C, clist, 0, nc, lane, meta);
+      } else if constexpr (KIND == KIND_HASH) {
+        make_cstate<KIND_LIST>(A, clist, nc, cmemb, lane, C);
+        C.hlg = hash_lg(nc);
+        C.ht = cmemb;
+        build_hash(clist, nc, cmemb, C.hlg, lane);
+        acc += sum_isect<KIND_HASH, false>(a, C, clist, 0, nc, lane, meta);
+      } else {
+        make_cstate<KIND_LIST>(A, clist, nc, cmemb, lane, C);
+        acc += sum_isect<KIND_LIST, false>(a, C, clist, 0, nc, lane, meta);
+      }
+      wave_sync();
+    }
+  }
+  return acc;
+}
+
 template <int K, bool W>
+__device__ uint64_t run_task_tuned(const CountArgs &a, uint32_t kind, uint32_t i, uint32_t jb, uint32_t je,
+                                   uint32_t lane, uint32_t *L) {
+  uint32_t *meta = L;
+  uint32_t *Ls = L + META_WORDS;
+  AState A;
+  const uint32_t used = stage_tuned(a, kind, i, lane, Ls, A);
+  if constexpr (K == 3) {
+    switch (kind) {
+      case KIND_BITMAP: return sum_isect<KIND_BITMAP, W>(a, A, A.list, jb, je, lane, meta);
+      case KIND_HASH: return sum_isect<KIND_HASH, W>(a, A, A.list, jb, je, lane, meta);
+      case KIND_LIST: return sum_isect<KIND_LIST, W>(a, A, A.list, jb, je, lane, meta);
+      default: return task_triangles<KIND_GLOBAL, W>(a, A, jb, je, lane);
+    }
+  } else {
+    switch (kind) {
+      case KIND_BITMAP: return task_cliques4_tuned<KIND_BITMAP>(a, A, jb, je, lane, Ls + used, meta);
+      case KIND_HASH: return task_cliques4_tuned<KIND_HASH>(a, A, jb, je, lane, Ls + used, meta);
+      default: return task_cliques4_tuned<KIND_LIST>(a, A, jb, je, lane, Ls + used, meta);
+    }
+  }
+}
+
+template <int K, bool W, bool TUNED>
 __device__ uint64_t run_task(const CountArgs &a, uint32_t kind, uint32_t i, uint32_t jb, uint32_t je,
                              uint32_t lane, uint32_t *L) {
+  if constexpr (TUNED) return run_task_tuned<K, W>(a, kind, i, jb, je, lane, L);
   AState A;
   const uint32_t used = stage_a(a, kind, i, lane, L, A);
   if constexpr (K == 3) {
@@ -512,7 +748,124 @@ __device__ uint64_t run_task(const CountArgs &a, uint32_t kind, uint32_t i, uint
   }
 }
 
+// ---------------------------------------------------------------- small source rows
+//
+// Rows with few kept entries are counted edge-centrically: for an oriented
+// matrix (every row j only holds ids below j) the triangles closed at row i are
+// the pairs a < b of row(i) with row(i)[a] in row(row(i)[b]), and the
+// 4-cliques are the triples a < b < c whose three pairs are all edges.  Items
+// (pairs or triples) of 64 rows are flattened over the lanes; each item is a
+// handful of membership tests against rows in HBM, with no LDS staging.
+
+// membership of x in row(j) as stored in HBM
+__device__ __forceinline__ uint32_t row_contains(const CountArgs &a, const RowDesc &d, uint32_t x) {
+  if (d.card == 0) return 0;
+  const uint32_t t = desc_type(d);
+  if (t == EH_BS) {
+    const uint32_t w = (x >> 6) - d.w0;
+    return (w < desc_aux(d)) ? (uint32_t)((a.bsw[d.off + w] >> (x & 63u)) & 1ull) : 0u;
+  }
+  if (t == EH_U) return sorted_contains(a.unbr + (uint64_t)d.off * 4, d.card, x);
+  const uint16_t *S = a.psd + (uint64_t)d.off * 4;
+  const uint32_t len = desc_aux(d), hx = x >> 16;
+  uint32_t pos = 0;
+  while (pos < len) {
+    const uint32_t pre = S[pos], cnt = (uint32_t)S[pos + 1] + 1u;
+    if (pre == hx) return sorted_contains(S + pos + 2, cnt, x & 0xFFFFu);
+    if (pre > hx) return 0;
+    pos += cnt + 2u;
+  }
+  return 0;
+}
+
+// r -> (p, q) with 0 <= p < q, r = q(q-1)/2 + p
+__device__ __forceinline__ void tri_decode(uint32_t r, uint32_t *p, uint32_t *q) {
+  uint32_t b = (uint32_t)sqrtf(2.0f * (float)r);
+  while ((b * (b + 1u)) / 2u <= r) b++;
+  while (b > 1u && (b * (b - 1u)) / 2u > r) b--;
+  if (b < 1u) b = 1u;
+  *q = b;
+  *p = r - (b * (b - 1u)) / 2u;
+}
+
+// r -> (p, q, s) with p < q < s, r = C(s,3) + C(q,2) + p
+__device__ __forceinline__ void tet_decode(uint32_t r, uint32_t *p, uint32_t *q, uint32_t *s) {
+  uint32_t c = (uint32_t)cbrtf(6.0f * (float)r);
+  while (((c + 1u) * c * (c - 1u)) / 6u <= r) c++;
+  while (c > 2u && (c * (c - 1u) * (c - 2u)) / 6u > r) c--;
+  if (c < 2u) c = 2u;
+  *s = c;
+  tri_decode(r - (c * (c - 1u) * (c - 2u)) / 6u, p, q);
+}
+
 template <int K, bool W>
+__global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t *__restrict__ rows, uint32_t nrows) {
+  extern __shared__ uint32_t lds_all[];
+  const uint32_t lane = lane_id();
+  uint32_t *meta = lds_all + (threadIdx.x >> 6) * SMALL_META_WORDS;
+  uint32_t *mbase = meta, *moff = meta + 65, *rcnt = meta + 130;
+  const bool per_graph = a.graph_of != nullptr;
+  uint64_t acc = 0;
+  for (;;) {
+    uint32_t c0 = 0;
+    if (lane == 0) c0 = atomicAdd(a.next, 64u);
+    c0 = rfl(c0);
+    if (c0 >= nrows) break;
+    const uint32_t nr = min(64u, nrows - c0);
+    uint32_t items = 0, off = 0, gid = 0;
+    if (lane < nr) {
+      const uint32_t i = rows[c0 + lane];
+      const uint32_t d = a.card[i];
+      off = a.aoff[i];
+      if (per_graph) gid = a.graph_of[i];
+      if constexpr (K == 3) items = (d * (d - 1u)) / 2u;
+      else items = d >= 3u ? (d * (d - 1u) * (d - 2u)) / 6u : 0u;
+    }
+    uint32_t tot;
+    const uint32_t base = wave_excl_scan(items, lane, &tot);
+    tot = rfl(tot);
+    mbase[lane] = base;
+    moff[lane] = off;
+    rcnt[lane] = 0;
+    if (lane == 0) mbase[64] = tot;
+    wave_sync();
+    for (uint32_t t = lane; t < tot; t += 64) {
+      uint32_t q = 0;
+#pragma unroll
+      for (uint32_t step = 32; step; step >>= 1)
+        if (mbase[q + step] <= t) q += step;
+      const uint32_t r = t - mbase[q];
+      const uint32_t *R = a.unbr + (uint64_t)moff[q] * 4;
+      uint64_t h;
+      if constexpr (K == 3) {
+        uint32_t p, s;
+        tri_decode(r, &p, &s);
+        const uint32_t x = R[p], j = R[s];
+        const uint32_t m = row_contains(a, a.desc[j], x);
+        if constexpr (W) h = m ? (uint64_t)a.card[x] : 0ull;
+        else h = m;
+      } else {
+        uint32_t p, s, u;
+        tet_decode(r, &p, &s, &u);
+        const uint32_t x = R[p], y = R[s], z = R[u];
+        const RowDesc dz = a.desc[z];
+        h = (row_contains(a, dz, y) && row_contains(a, dz, x) && row_contains(a, a.desc[y], x)) ? 1ull : 0ull;
+      }
+      if (per_graph) {
+        if (h) atomicAdd(&rcnt[q], (uint32_t)h);
+      } else {
+        acc += h;
+      }
+    }
+    wave_sync();
+    if (per_graph && lane < nr && rcnt[lane]) atomicAdd(&a.counts[gid], (unsigned long long)rcnt[lane]);
+    wave_sync();
+  }
+  const uint64_t s = wave_sum_u64(acc);
+  if (lane == 0 && s) atomicAdd(&a.counts[0], (unsigned long long)s);
+}
+
+template <int K, bool W, bool TUNED>
 __global__ void __launch_bounds__(256) k_count(CountArgs a) {
   extern __shared__ uint32_t lds_all[];
   const uint32_t lane = lane_id();
@@ -533,7 +886,14 @@ __global__ void __launch_bounds__(256) k_count(CountArgs a) {
       acc = 0;
       cur = gid;
     }
-    acc += run_task<K, W>(a, kind, i, jb, je, lane, L);
+    if (a.task_cycles) {
+      const uint64_t c0 = clock64();
+      acc += run_task<K, W, TUNED>(a, kind, i, jb, je, lane, L);
+      const uint64_t c1 = clock64();
+      if (lane == 0) a.task_cycles[a.task_base + t] = c1 - c0;
+    } else {
+      acc += run_task<K, W, TUNED>(a, kind, i, jb, je, lane, L);
+    }
   }
   const uint64_t s = wave_sum_u64(acc);
   if (lane == 0 && s) atomicAdd(&a.counts[cur], (unsigned long long)s);
@@ -559,6 +919,8 @@ struct SchedArgs {
   int k;
   uint32_t lds_small, lds_big;
   uint64_t task_work;
+  uint32_t small_deg;     // rows with 1..small_deg entries go to the edge-centric kernel
+  uint32_t *is_small;     // out: 1 for those rows
   uint64_t *sortkey;      // out: (class << 62) | (order << 32) | i
   uint32_t *ntask;        // out: tasks of row i
   uint32_t *rowinfo;      // out: kind << 28 | graph id
@@ -594,7 +956,9 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
       loc[4] += (lane == 0) ? work - 16ull * dA : 0ull;
     }
     if (lane != 0) continue;
-    if (dA == 0) {
+    const bool small = dA > 0 && dA <= s.small_deg;
+    s.is_small[i] = small ? 1u : 0u;
+    if (dA == 0 || small) {
       s.sortkey[i] = (3ull << 62) | i;
       s.ntask[i] = 0;
       s.rowinfo[i] = 0;
@@ -611,9 +975,20 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
     uint32_t kind, cls;
     uint64_t need;
     if (s.tuned) {
-      if (need_bitmap <= s.lds_small) { kind = KIND_BITMAP; need = need_bitmap; }
-      else if (need_bitmap <= s.lds_big && dA >= 64) { kind = KIND_BITMAP; need = need_bitmap; }
-      else { kind = KIND_LIST; need = need_list; }
+      // flattened schedule: LDS holds the pair metadata, the list and a bitmap or
+      // hash table of the source row (twice for the 4-clique level set)
+      const uint64_t cap = 1ull << hash_lg(dA);
+      const uint64_t rep = 1ull + extra;
+      const uint64_t nb = META_WORDS + rep * (2ull * anw + dA);
+      const uint64_t nh = META_WORDS + rep * (cap + dA);
+      const uint64_t nl = META_WORDS + rep * dA;
+      if (nb <= s.lds_small && 2ull * anw <= 4ull * cap) { kind = KIND_BITMAP; need = nb; }
+      else if (nh <= s.lds_small) { kind = KIND_HASH; need = nh; }
+      else if (nb <= s.lds_small) { kind = KIND_BITMAP; need = nb; }
+      else if (nl <= s.lds_small) { kind = KIND_LIST; need = nl; }
+      else if (s.k == 3) { kind = KIND_GLOBAL; need = 0; }
+      else if (nh <= s.lds_big) { kind = KIND_HASH; need = nh; }
+      else { kind = KIND_LIST; need = nl; }
     } else {
       if (ta == EH_BS) { kind = KIND_BITMAP; need = need_bitmap; }
       else if (ta == EH_PS) { kind = KIND_PSDIR; need = need_psdir; }
@@ -667,6 +1042,11 @@ __global__ void k_expand_tasks(const uint64_t *sorted, const uint32_t *nt_sorted
   }
 }
 
+__global__ void k_scatter_small(const uint32_t *is_small, const uint64_t *pos, uint32_t n, uint32_t *rows) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (is_small[i]) rows[pos[i]] = i;
+}
+
 inline unsigned grid_for(uint64_t work, unsigned block = 256) {
   uint64_t g = ceil_div(work ? work : 1, block);
   return (unsigned)(g > 65536 ? 65536 : g);
@@ -680,7 +1060,7 @@ struct SchedStats {
 
 // ---------------------------------------------------------------- host side
 
-TuneParams tune_defaults(eh_matrix *m, int sched, const eh_run_opts *opts) {
+TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts) {
   TuneParams tp;
   tp.lds_small = 2048;
   tp.lds_big = 16384;
@@ -693,7 +1073,11 @@ TuneParams tune_defaults(eh_matrix *m, int sched, const eh_run_opts *opts) {
   } else {
     tp.task_work = 0;
   }
+  // rows this short are cheaper as explicit wedges (triangles) or triples (4-cliques)
+  tp.small_deg = (sched == EH_SCHED_TUNED && m->filter == EH_FILTER_LOWER) ? (k == 4 ? 8u : 32u) : 0u;
   if (opts) {
+    if (opts->small_degree < 0) tp.small_deg = 0;
+    else if (opts->small_degree > 0 && tp.small_deg) tp.small_deg = (uint32_t)std::min(opts->small_degree, k == 4 ? 64 : 256);
     if (opts->task_work) tp.task_work = opts->task_work;
     if (opts->lds_words) tp.lds_small = opts->lds_words;
     if (opts->waves_per_block > 0) tp.waves_per_block = opts->waves_per_block;
@@ -705,13 +1089,14 @@ TuneParams tune_defaults(eh_matrix *m, int sched, const eh_run_opts *opts) {
   return tp;
 }
 
-static uint64_t g_last_stats[6];
-
 eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, Schedule &s) {
   eh_ctx *ctx = m->ctx;
   hipStream_t st = ctx->stream;
   const uint32_t n = m->n;
-  DevBuf key, key2, ntask, rowinfo, stats, ntsorted, toff, cnt2;
+  DevBuf key, key2, ntask, rowinfo, stats, ntsorted, toff, cnt2, is_small, spos;
+  EH_CHECK(is_small.alloc(4ull * n + 8));
+  EH_CHECK(spos.alloc(8ull * n + 16));
+  k_zero_u32<<<grid_for((uint64_t)n + 1), 256, 0, st>>>(is_small.as<uint32_t>(), n + 1);
   EH_CHECK(key.alloc(8ull * n + 8));
   EH_CHECK(key2.alloc(8ull * n + 8));
   EH_CHECK(ntask.alloc(4ull * n + 4));
@@ -733,12 +1118,24 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   sa.lds_small = tp.lds_small;
   sa.lds_big = tp.lds_big;
   sa.task_work = tp.task_work;
+  sa.small_deg = tp.small_deg;
+  sa.is_small = is_small.as<uint32_t>();
   sa.sortkey = key.as<uint64_t>();
   sa.ntask = ntask.as<uint32_t>();
   sa.rowinfo = rowinfo.as<uint32_t>();
   sa.stats = stats.as<unsigned long long>();
   k_row_work<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(sa);
   EH_HIP(hipGetLastError());
+  EH_CHECK(prim_exscan_u32_to_u64(ctx, is_small.as<uint32_t>(), spos.as<uint64_t>(), (uint64_t)n + 1));
+  uint64_t nsmall = 0;
+  EH_HIP(hipMemcpyAsync(&nsmall, spos.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, st));
+  EH_HIP(hipStreamSynchronize(st));
+  EH_CHECK(s.small_rows.alloc(4 * nsmall + 16));
+  if (nsmall)
+    k_scatter_small<<<grid_for(n), 256, 0, st>>>(is_small.as<uint32_t>(), spos.as<uint64_t>(), n,
+                                                s.small_rows.as<uint32_t>());
+  s.n_small_rows = nsmall;
+  s.small_deg = tp.small_deg;
   EH_CHECK(prim_sort_u64(ctx, key.as<uint64_t>(), key2.as<uint64_t>(), n, 64));
   EH_CHECK(ntsorted.alloc(4ull * n + 8));
   EH_CHECK(toff.alloc(8ull * n + 16));
@@ -775,7 +1172,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   s.task_work = tp.task_work;
   s.lds_small = tp.lds_small;
   s.lds_big = tp.lds_big;
-  for (int q = 0; q < 6; q++) g_last_stats[q] = hs.v[q];
+  for (int q = 0; q < 6; q++) s.stats[q] = hs.v[q];
   return EH_OK;
 }
 
@@ -783,9 +1180,11 @@ namespace {
 
 template <int K, bool W>
 eh_status launch_count(eh_ctx *ctx, const CountArgs &base, const uint4 *tasks, uint64_t ntasks,
-                       uint32_t lds_words, int waves, uint32_t *d_next) {
+                       uint32_t lds_words, int waves, uint32_t *d_next, uint64_t task_base) {
+  auto kern = base.tuned ? k_count<K, W, true> : k_count<K, W, false>;
   if (ntasks == 0) return EH_OK;
   CountArgs a = base;
+  a.task_base = task_base;
   a.tasks = tasks;
   a.ntasks = ntasks;
   a.next = d_next;
@@ -793,13 +1192,33 @@ eh_status launch_count(eh_ctx *ctx, const CountArgs &base, const uint4 *tasks, u
   const size_t lds_bytes = (size_t)lds_words * 4 * waves;
   const int threads = 64 * waves;
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count<K, W>, threads, lds_bytes);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds_bytes);
   if (e != hipSuccess || per_cu < 1) per_cu = 1;
   uint64_t blocks = (uint64_t)per_cu * (ctx->num_cus ? ctx->num_cus : 256);
   const uint64_t need = ceil_div(ntasks, (uint64_t)waves);
   if (blocks > need) blocks = need;
   k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
-  hipLaunchKernelGGL((k_count<K, W>), dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a);
+  EH_HIP(hipGetLastError());
+  return EH_OK;
+}
+
+template <int K, bool W>
+eh_status launch_small(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows, uint64_t nrows, uint32_t *d_next) {
+  if (nrows == 0) return EH_OK;
+  CountArgs a = base;
+  a.next = d_next;
+  const int waves = 4, threads = 256;
+  const size_t lds_bytes = (size_t)SMALL_META_WORDS * 4 * waves;
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_small_rows<K, W>, threads, lds_bytes);
+  if (e != hipSuccess || per_cu < 1) per_cu = 1;
+  uint64_t blocks = (uint64_t)per_cu * (ctx->num_cus ? ctx->num_cus : 256);
+  const uint64_t need = ceil_div(ceil_div(nrows, 64), (uint64_t)waves);
+  if (blocks > need) blocks = need;
+  k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
+  hipLaunchKernelGGL((k_small_rows<K, W>), dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a, rows,
+                     (uint32_t)nrows);
   EH_HIP(hipGetLastError());
   return EH_OK;
 }
@@ -807,9 +1226,10 @@ eh_status launch_count(eh_ctx *ctx, const CountArgs &base, const uint4 *tasks, u
 }  // namespace
 
 eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, Schedule &s,
-                          uint64_t *d_counts, uint32_t *d_next) {
-  (void)sched;
+                          uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles) {
   CountArgs a;
+  a.task_cycles = reinterpret_cast<unsigned long long *>(d_task_cycles);
+  a.task_base = 0;
   a.card = m->dcard.as<uint32_t>();
   a.aoff = m->daoff.as<uint32_t>();
   a.unbr = m->dunbr.as<uint32_t>();
@@ -817,21 +1237,26 @@ eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, 
   a.bsw = m->dbsw.as<uint64_t>();
   a.psd = m->dpsd.as<uint16_t>();
   a.counts = reinterpret_cast<unsigned long long *>(d_counts);
-  a.weighted = 0;
+  a.tuned = sched == EH_SCHED_TUNED ? 1u : 0u;
+  a.graph_of = m->dgraph_of.p ? m->dgraph_of.as<uint32_t>() : nullptr;
   const uint4 *t = s.tasks.as<uint4>();
   const bool weighted = k == -3;
+  const uint32_t *sr = s.small_rows.as<uint32_t>();
+  if (k == 4) EH_CHECK((launch_small<4, false>(m->ctx, a, sr, s.n_small_rows, d_next + 2)));
+  else if (weighted) EH_CHECK((launch_small<3, true>(m->ctx, a, sr, s.n_small_rows, d_next + 2)));
+  else EH_CHECK((launch_small<3, false>(m->ctx, a, sr, s.n_small_rows, d_next + 2)));
   if (k == 4) {
-    EH_CHECK((launch_count<4, false>(m->ctx, a, t, s.n_big, s.lds_big, 1, d_next)));
+    EH_CHECK((launch_count<4, false>(m->ctx, a, t, s.n_big, s.lds_big, 1, d_next, 0)));
     EH_CHECK((launch_count<4, false>(m->ctx, a, t + s.n_big, s.n_small, s.lds_small, tp.waves_per_block,
-                                     d_next + 1)));
+                                     d_next + 1, s.n_big)));
   } else if (weighted) {
-    EH_CHECK((launch_count<3, true>(m->ctx, a, t, s.n_big, s.lds_big, 1, d_next)));
+    EH_CHECK((launch_count<3, true>(m->ctx, a, t, s.n_big, s.lds_big, 1, d_next, 0)));
     EH_CHECK((launch_count<3, true>(m->ctx, a, t + s.n_big, s.n_small, s.lds_small, tp.waves_per_block,
-                                    d_next + 1)));
+                                    d_next + 1, s.n_big)));
   } else {
-    EH_CHECK((launch_count<3, false>(m->ctx, a, t, s.n_big, s.lds_big, 1, d_next)));
+    EH_CHECK((launch_count<3, false>(m->ctx, a, t, s.n_big, s.lds_big, 1, d_next, 0)));
     EH_CHECK((launch_count<3, false>(m->ctx, a, t + s.n_big, s.n_small, s.lds_small, tp.waves_per_block,
-                                     d_next + 1)));
+                                     d_next + 1, s.n_big)));
   }
   return EH_OK;
 }
@@ -839,19 +1264,18 @@ eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, 
 // k: 3 triangles, 4 four-cliques, -3 weighted triangle pass (sum of card(x) over
 // common x; the per-triangle byte term of the 4-clique model)
 eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_counts, uint32_t ncounts,
-                    uint64_t *count, eh_run_stats *stats, bool capture_ok) {
-  (void)capture_ok;
+                    uint64_t *count, eh_run_stats *stats, uint64_t *d_task_cycles) {
   eh_ctx *ctx = m->ctx;
   hipStream_t st = ctx->stream;
   const int sched = opts ? opts->schedule : EH_SCHED_TUNED;
   if (sched != EH_SCHED_PLAIN && sched != EH_SCHED_TUNED) { eh_set_error("bad schedule %d", sched); return EH_ERR_INVALID; }
   const int kk = (k == 4) ? 4 : 3;
-  TuneParams tp = tune_defaults(m, sched, opts);
+  TuneParams tp = tune_defaults(m, sched, kk, opts);
   Schedule &s = m->sched[sched];
   const int skey = kk * 8 + sched;
   double sched_ms = 0;
   if (!s.valid || s.task_work != tp.task_work || s.lds_small != tp.lds_small || s.lds_big != tp.lds_big ||
-      (int)s.sched != skey) {
+      (int)s.sched != skey || s.small_deg != tp.small_deg) {
     s = Schedule();
     EH_CHECK(schedule_build(m, sched, kk, tp, s));
     s.sched = skey;
@@ -869,7 +1293,8 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
   for (int r = 0; r < repeat; r++) {
     k_zero_u64<<<grid_for(ncounts ? ncounts : 1), 256, 0, st>>>(dc, ncounts ? ncounts : 1);
     EH_HIP(hipEventRecord(ctx->ev0, st));
-    EH_CHECK(schedule_launch(m, k, sched, tp, s, reinterpret_cast<uint64_t *>(dc), next.as<uint32_t>()));
+    EH_CHECK(schedule_launch(m, k, sched, tp, s, reinterpret_cast<uint64_t *>(dc), next.as<uint32_t>(),
+                             d_task_cycles));
     EH_HIP(hipEventRecord(ctx->ev1, st));
     EH_HIP(hipEventSynchronize(ctx->ev1));
     float ms = 0;
@@ -886,10 +1311,8 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
     stats->sched_ms = sched_ms;
     stats->tasks = s.n_big + s.n_small;
     stats->alg_bytes = 4ull * (m->card + m->wedges) + 8ull * ((uint64_t)m->n + 1);
-    if (sched_ms > 0) {
-      for (int q = 0; q < 4; q++) stats->pairs[q] = g_last_stats[q];
-      stats->work_units = g_last_stats[4];
-    }
+    for (int q = 0; q < 4; q++) stats->pairs[q] = s.stats[q];
+    stats->work_units = s.stats[4];
     stats->schedule = sched;
     stats->lanes_per_row = tp.lanes;
     stats->task_work = tp.task_work;
@@ -903,7 +1326,7 @@ extern "C" eh_status eh_triangle_count(eh_matrix *m, const eh_run_opts *opts, ui
   if (!m || !count) { eh_set_error("eh_triangle_count: null argument"); return EH_ERR_INVALID; }
   EH_HIP(hipSetDevice(m->ctx->device));
   if (stats) *stats = eh_run_stats{};
-  return count_run(m, 3, opts, nullptr, 1, count, stats, false);
+  return count_run(m, 3, opts, nullptr, 1, count, stats, nullptr);
 }
 
 extern "C" eh_status eh_clique_count(eh_matrix *m, uint32_t k, const eh_run_opts *opts, uint64_t *count,
@@ -912,5 +1335,32 @@ extern "C" eh_status eh_clique_count(eh_matrix *m, uint32_t k, const eh_run_opts
   if (k != 3 && k != 4) { eh_set_error("eh_clique_count: k=%u not supported (3 or 4)", k); return EH_ERR_UNSUPPORTED; }
   EH_HIP(hipSetDevice(m->ctx->device));
   if (stats) *stats = eh_run_stats{};
-  return count_run(m, (int)k, opts, nullptr, 1, count, stats, false);
+  return count_run(m, (int)k, opts, nullptr, 1, count, stats, nullptr);
+}
+
+extern "C" eh_status eh_count_profile(eh_matrix *m, uint32_t k, const eh_run_opts *opts, uint32_t *task_rows,
+                                      uint32_t *task_kinds, uint64_t *task_cycles, uint64_t cap, uint64_t *ntasks) {
+  if (!m || !ntasks) { eh_set_error("eh_count_profile: null argument"); return EH_ERR_INVALID; }
+  if (k != 3 && k != 4) { eh_set_error("eh_count_profile: k=%u not supported (3 or 4)", k); return EH_ERR_UNSUPPORTED; }
+  EH_HIP(hipSetDevice(m->ctx->device));
+  const int sched = opts ? opts->schedule : EH_SCHED_TUNED;
+  if (sched != EH_SCHED_PLAIN && sched != EH_SCHED_TUNED) { eh_set_error("bad schedule %d", sched); return EH_ERR_INVALID; }
+  uint64_t count = 0;
+  // first pass builds (or reuses) the schedule; the second records cycles
+  EH_CHECK(count_run(m, (int)k, opts, nullptr, 1, &count, nullptr, nullptr));
+  Schedule &s = m->sched[sched];
+  const uint64_t nt = s.n_big + s.n_small;
+  *ntasks = nt;
+  if (cap < nt) { eh_set_error("eh_count_profile needs room for %llu tasks", (unsigned long long)nt); return EH_ERR_CAPACITY; }
+  DevBuf cyc;
+  EH_CHECK(cyc.alloc(8 * nt + 8));
+  EH_CHECK(count_run(m, (int)k, opts, nullptr, 1, &count, nullptr, cyc.as<uint64_t>()));
+  std::vector<uint4> t(nt);
+  if (nt) EH_HIP(hipMemcpy(t.data(), s.tasks.p, 16 * nt, hipMemcpyDeviceToHost));
+  if (task_cycles && nt) EH_HIP(hipMemcpy(task_cycles, cyc.p, 8 * nt, hipMemcpyDeviceToHost));
+  for (uint64_t q = 0; q < nt; q++) {
+    if (task_rows) task_rows[q] = t[q].x;
+    if (task_kinds) task_kinds[q] = t[q].w >> KIND_SHIFT;
+  }
+  return EH_OK;
 }

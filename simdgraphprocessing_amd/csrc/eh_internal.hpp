@@ -101,6 +101,7 @@ struct TuneParams {
   uint32_t lds_small = 2048;
   uint32_t lds_big = 16384;
   int waves_per_block = 4;
+  uint32_t small_deg = 0;  // rows up to this many entries use the edge-centric kernel
 };
 
 struct eh_ctx {
@@ -132,6 +133,10 @@ struct Schedule {
   DevBuf tasks;  // uint4 {i, jb, je, graph}
   uint64_t n_small = 0, n_big = 0;  // tasks [0,n_big) need the big LDS budget
   double ms = 0;
+  uint64_t stats[6] = {0, 0, 0, 0, 0, 0};  // pairs by B type (U, PS, BS, empty), B units, wedges
+  DevBuf small_rows;        // rows counted by the edge-centric kernel
+  uint64_t n_small_rows = 0;
+  uint32_t small_deg = 0;
 };
 
 struct eh_matrix {
@@ -140,6 +145,7 @@ struct eh_matrix {
   uint64_t card = 0;
   uint32_t max_row = 0;
   int policy = 0;
+  int filter = 0;
   uint64_t unbr_words = 0;  // u32 entries in unbr (padded)
   uint64_t bs_words = 0;
   uint64_t ps_shorts = 0;
@@ -174,11 +180,11 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
                            eh_layout_policy policy, const uint32_t *d_graph_of, eh_matrix **out);
 eh_status matrix_select_learned(eh_matrix *m, uint8_t *d_types);
 eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_counts,
-                    uint32_t ncounts, uint64_t *count, eh_run_stats *stats, bool capture_ok);
+                    uint32_t ncounts, uint64_t *count, eh_run_stats *stats, uint64_t *d_task_cycles);
 eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, Schedule &s);
 eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, Schedule &s,
-                          uint64_t *d_counts, uint32_t *d_next);
-TuneParams tune_defaults(eh_matrix *m, int sched, const eh_run_opts *opts);
+                          uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles);
+TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts);
 
 // timing helper: device ms between two events recorded on ctx->stream
 double ctx_elapsed_ms(eh_ctx *ctx);

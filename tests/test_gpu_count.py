@@ -71,3 +71,26 @@ def test_rmat_parity(ctx, scale):
         for sched in ("plain", "tuned"):
             c, _ = M.triangle_count(sched)
             assert c == exp, (policy, sched)
+
+
+@pytest.mark.parametrize("small", [-1, 2, 8, 16, 32])
+def test_small_row_kernel(ctx, facebook, small):
+    """The edge-centric path for short rows must not change any count."""
+    M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "lower", "hybrid")
+    assert M.triangle_count("tuned", small_degree=small)[0] == FB_TRIANGLES
+    assert M.clique_count(4, "tuned", small_degree=small)[0] == FB_4CLIQUES
+    g = ctx.generate_rmat(14, 16, 5)
+    off, nbr = g.download()
+    om = O.Matrix(1 << 14, off, nbr, "hybrid")
+    for pol in ("hybrid", "cost", "hybrid3"):
+        M = g.build_matrix("lower", pol)
+        assert M.triangle_count("tuned", small_degree=small)[0] == om.triangles(8)[0], pol
+        assert M.clique_count(4, "tuned", small_degree=small)[0] == om.cliques(4, 8)[0], pol
+
+
+def test_unoriented_matrix(ctx, facebook):
+    """EH_FILTER_ALL keeps both directions: every triangle is counted 6 times
+    (sum over ordered (i,j) of |N(i) & N(j)|), and the small-row path stays off."""
+    M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "all", "hybrid")
+    for sched in ("plain", "tuned"):
+        assert M.triangle_count(sched)[0] == 6 * FB_TRIANGLES
