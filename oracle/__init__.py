@@ -99,6 +99,8 @@ def lib():
     L.orc_clique_count_sample.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32,
                                           C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
     L.orc_clique_count_sample.restype = C.c_uint64
+    L.orc_count_rows.argtypes = [C.c_void_p, C.c_int, C.c_int, _u64p]
+    L.orc_count_rows.restype = C.c_uint64
     L.orc_pair_mix.argtypes = [C.c_void_p, _u64p]
     L.orc_pair_mix.restype = C.c_uint64
     L.orc_brute_triangles.argtypes = [C.c_uint32, _u64p, _u32p]
@@ -340,6 +342,11 @@ class Matrix:
         e = C.c_uint64()
         c = lib().orc_clique_count_sample(self.h, k, threads, stride, phase, C.byref(e), C.byref(t))
         return int(c), int(e.value), t.value
+
+    def count_rows(self, k=3, threads=8):
+        out = np.zeros(self.n, np.uint64)
+        total = lib().orc_count_rows(self.h, k, threads, out)
+        return int(total), out
 
     def pair_mix(self):
         mix = np.zeros(4, np.uint64)

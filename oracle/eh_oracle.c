@@ -968,6 +968,7 @@ typedef struct {
   int k;
   uint8_t **bufs;
   size_t buf_bytes;
+  uint64_t *per_row; /* optional: count closed at each source row */
 } tc_ctx;
 
 typedef struct {
@@ -990,6 +991,7 @@ static void tc_row(void *c, int tid, uint64_t i) {
   orc_matrix_get_row(x->M, (uint32_t)i, &A);
   tc_inner t = {x->M, &A, 0};
   set_foreach(&A, tc_visit, &t);
+  if (x->per_row) x->per_row[i] = t.acc;
   x->count[(size_t)tid * ORC_PAD] += t.acc;
   x->edges[(size_t)tid * ORC_PAD] += A.cardinality;
 }
@@ -999,7 +1001,7 @@ uint64_t orc_triangle_count_sample(const orc_matrix *M, int threads, uint32_t st
   if (threads < 1) threads = 1;
   uint64_t *cnt = (uint64_t *)calloc((size_t)threads * ORC_PAD, 8);
   uint64_t *edg = (uint64_t *)calloc((size_t)threads * ORC_PAD, 8);
-  tc_ctx x = {M, cnt, edg, stride, phase, 0, NULL, 0};
+  tc_ctx x = {M, cnt, edg, stride, phase, 0, NULL, 0, NULL};
   double t0 = now_s();
   par_for_range(threads, 0, M->n, 100, tc_row, &x);
   double t1 = now_s();
@@ -1069,6 +1071,7 @@ static void clq_row(void *c, int tid, uint64_t i) {
   orc_matrix_get_row(x->M, (uint32_t)i, &sets[1]);
   clq_visit_ctx v = {&s, 2, 0};
   set_foreach(&sets[1], clq_visit, &v);
+  if (x->per_row) x->per_row[i] = v.acc;
   x->count[(size_t)tid * ORC_PAD] += v.acc;
   x->edges[(size_t)tid * ORC_PAD] += sets[1].cardinality;
 }
@@ -1082,7 +1085,7 @@ uint64_t orc_clique_count_sample(const orc_matrix *M, int k, int threads, uint32
   for (int t = 0; t < threads; t++) bufs[t] = (uint8_t *)malloc(bb * (size_t)k);
   uint64_t *cnt = (uint64_t *)calloc((size_t)threads * ORC_PAD, 8);
   uint64_t *edg = (uint64_t *)calloc((size_t)threads * ORC_PAD, 8);
-  tc_ctx x = {M, cnt, edg, stride, phase, k, bufs, bb};
+  tc_ctx x = {M, cnt, edg, stride, phase, k, bufs, bb, NULL};
   double t0 = now_s();
   par_for_range(threads, 0, M->n, 100, clq_row, &x);
   double t1 = now_s();
@@ -1414,4 +1417,29 @@ double orc_mlp_loss_grad(const float *P, const float *x, const float *target, co
     }
   }
   return loss / lab;
+}
+
+/* per-source-row counts of the triangle (k == 3) or k-clique pass */
+uint64_t orc_count_rows(const orc_matrix *M, int k, int threads, uint64_t *per_row) {
+  if (threads < 1) threads = 1;
+  memset(per_row, 0, 8 * (size_t)M->n);
+  uint64_t *cnt = (uint64_t *)calloc((size_t)threads * ORC_PAD, 8);
+  uint64_t *edg = (uint64_t *)calloc((size_t)threads * ORC_PAD, 8);
+  uint64_t total = 0;
+  if (k == 3) {
+    tc_ctx x = {M, cnt, edg, 1, 0, 0, NULL, 0, per_row};
+    par_for_range(threads, 0, M->n, 100, tc_row, &x);
+  } else {
+    size_t bb = 8 * ((size_t)M->n / 64 + 4) + 8 * ((size_t)M->max_row + 16) + 64;
+    uint8_t **bufs = (uint8_t **)malloc(sizeof(uint8_t *) * (size_t)threads);
+    for (int t = 0; t < threads; t++) bufs[t] = (uint8_t *)malloc(bb * (size_t)k);
+    tc_ctx x = {M, cnt, edg, 1, 0, k, bufs, bb, per_row};
+    par_for_range(threads, 0, M->n, 100, clq_row, &x);
+    for (int t = 0; t < threads; t++) free(bufs[t]);
+    free(bufs);
+  }
+  for (int t = 0; t < threads; t++) total += cnt[(size_t)t * ORC_PAD];
+  free(cnt);
+  free(edg);
+  return total;
 }

@@ -93,6 +93,7 @@ uint64_t orc_clique_count(const orc_matrix *M, int k, int threads, double *secon
 uint64_t orc_clique_count_sample(const orc_matrix *M, int k, int threads, uint32_t stride,
                                  uint32_t phase, uint64_t *edges_done, double *seconds);
 uint64_t orc_pair_mix(const orc_matrix *M, uint64_t mix[4]);
+uint64_t orc_count_rows(const orc_matrix *M, int k, int threads, uint64_t *per_row);
 
 /* ---- independent brute force (symmetric CSR) ---- */
 uint64_t orc_brute_triangles(uint32_t n, const uint64_t *off, const uint32_t *nbr);

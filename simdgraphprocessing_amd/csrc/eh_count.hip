@@ -1225,8 +1225,10 @@ eh_status launch_small(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows,
 
 }  // namespace
 
-eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, Schedule &s,
-                          uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles) {
+eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, const uint32_t *small_rows,
+                       uint64_t n_small_rows, const uint4 *tasks_big, uint64_t n_big, const uint4 *tasks_small,
+                       uint64_t n_small, uint32_t lds_small, uint32_t lds_big, uint64_t *d_counts, uint32_t *d_next,
+                       uint64_t *d_task_cycles) {
   CountArgs a;
   a.task_cycles = reinterpret_cast<unsigned long long *>(d_task_cycles);
   a.task_base = 0;
@@ -1239,26 +1241,31 @@ eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, 
   a.counts = reinterpret_cast<unsigned long long *>(d_counts);
   a.tuned = sched == EH_SCHED_TUNED ? 1u : 0u;
   a.graph_of = m->dgraph_of.p ? m->dgraph_of.as<uint32_t>() : nullptr;
-  const uint4 *t = s.tasks.as<uint4>();
   const bool weighted = k == -3;
-  const uint32_t *sr = s.small_rows.as<uint32_t>();
-  if (k == 4) EH_CHECK((launch_small<4, false>(m->ctx, a, sr, s.n_small_rows, d_next + 2)));
-  else if (weighted) EH_CHECK((launch_small<3, true>(m->ctx, a, sr, s.n_small_rows, d_next + 2)));
-  else EH_CHECK((launch_small<3, false>(m->ctx, a, sr, s.n_small_rows, d_next + 2)));
   if (k == 4) {
-    EH_CHECK((launch_count<4, false>(m->ctx, a, t, s.n_big, s.lds_big, 1, d_next, 0)));
-    EH_CHECK((launch_count<4, false>(m->ctx, a, t + s.n_big, s.n_small, s.lds_small, tp.waves_per_block,
-                                     d_next + 1, s.n_big)));
+    EH_CHECK((launch_small<4, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2)));
+    EH_CHECK((launch_count<4, false>(m->ctx, a, tasks_big, n_big, lds_big, 1, d_next, 0)));
+    EH_CHECK((launch_count<4, false>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,
+                                     n_big)));
   } else if (weighted) {
-    EH_CHECK((launch_count<3, true>(m->ctx, a, t, s.n_big, s.lds_big, 1, d_next, 0)));
-    EH_CHECK((launch_count<3, true>(m->ctx, a, t + s.n_big, s.n_small, s.lds_small, tp.waves_per_block,
-                                    d_next + 1, s.n_big)));
+    EH_CHECK((launch_small<3, true>(m->ctx, a, small_rows, n_small_rows, d_next + 2)));
+    EH_CHECK((launch_count<3, true>(m->ctx, a, tasks_big, n_big, lds_big, 1, d_next, 0)));
+    EH_CHECK((launch_count<3, true>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,
+                                    n_big)));
   } else {
-    EH_CHECK((launch_count<3, false>(m->ctx, a, t, s.n_big, s.lds_big, 1, d_next, 0)));
-    EH_CHECK((launch_count<3, false>(m->ctx, a, t + s.n_big, s.n_small, s.lds_small, tp.waves_per_block,
-                                     d_next + 1, s.n_big)));
+    EH_CHECK((launch_small<3, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2)));
+    EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_big, n_big, lds_big, 1, d_next, 0)));
+    EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,
+                                     n_big)));
   }
   return EH_OK;
+}
+
+eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, Schedule &s,
+                          uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles) {
+  const uint4 *t = s.tasks.as<uint4>();
+  return launch_parts(m, k, sched, tp, s.small_rows.as<uint32_t>(), s.n_small_rows, t, s.n_big, t + s.n_big,
+                      s.n_small, s.lds_small, s.lds_big, d_counts, d_next, d_task_cycles);
 }
 
 // k: 3 triangles, 4 four-cliques, -3 weighted triangle pass (sum of card(x) over
@@ -1271,7 +1278,7 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
   if (sched != EH_SCHED_PLAIN && sched != EH_SCHED_TUNED) { eh_set_error("bad schedule %d", sched); return EH_ERR_INVALID; }
   const int kk = (k == 4) ? 4 : 3;
   TuneParams tp = tune_defaults(m, sched, kk, opts);
-  Schedule &s = m->sched[sched];
+  Schedule &s = m->sched[kk == 4 ? 1 : 0][sched];
   const int skey = kk * 8 + sched;
   double sched_ms = 0;
   if (!s.valid || s.task_work != tp.task_work || s.lds_small != tp.lds_small || s.lds_big != tp.lds_big ||
@@ -1348,7 +1355,7 @@ extern "C" eh_status eh_count_profile(eh_matrix *m, uint32_t k, const eh_run_opt
   uint64_t count = 0;
   // first pass builds (or reuses) the schedule; the second records cycles
   EH_CHECK(count_run(m, (int)k, opts, nullptr, 1, &count, nullptr, nullptr));
-  Schedule &s = m->sched[sched];
+  Schedule &s = m->sched[k == 4 ? 1 : 0][sched];
   const uint64_t nt = s.n_big + s.n_small;
   *ntasks = nt;
   if (cap < nt) { eh_set_error("eh_count_profile needs room for %llu tasks", (unsigned long long)nt); return EH_ERR_CAPACITY; }

@@ -115,6 +115,22 @@ __global__ void k_relabel_keys(const uint32_t *deg, uint32_t n, uint64_t *vk) {
     vk[i] = ((uint64_t)(0xFFFFFFFFu - deg[i]) << 32) | (uint32_t)i;
 }
 
+// batch arenas: (graph, degree desc, id asc) so every graph keeps its id range
+__global__ void k_relabel_keys_grouped(const uint32_t *deg, const uint32_t *vgraph, uint32_t n, uint64_t *vk) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    vk[i] = ((uint64_t)vgraph[i] << 52) | ((uint64_t)(0xFFFFFFu - min(deg[i], 0xFFFFFFu)) << 26) | (uint32_t)i;
+}
+
+__global__ void k_perm_from_sorted_masked(const uint64_t *vk, uint32_t n, uint32_t *new2old, uint32_t *old2new) {
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t o = (uint32_t)(vk[r] & 0x3FFFFFFull);
+    new2old[r] = o;
+    old2new[o] = (uint32_t)r;
+  }
+}
+
 __global__ void k_perm_from_sorted(const uint64_t *vk, uint32_t n, uint32_t *new2old,
                                    uint32_t *old2new) {
   for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n;
@@ -191,7 +207,7 @@ inline unsigned grid_for(uint64_t work, unsigned block = 256) {
 
 // sorted unique undirected keys -> symmetric CSR (optionally relabelled)
 eh_status finish_graph(eh_ctx *ctx, uint32_t n, const uint64_t *ukeys, uint64_t mu, int relabel,
-                       eh_graph *g) {
+                       eh_graph *g, const uint32_t *vgraph) {
   hipStream_t st = ctx->stream;
   DevBuf old2new;
   g->n = n;
@@ -204,11 +220,18 @@ eh_status finish_graph(eh_ctx *ctx, uint32_t n, const uint64_t *ukeys, uint64_t 
     k_degree_from_keys<<<grid_for(mu), 256, 0, st>>>(ukeys, mu, deg.as<uint32_t>());
     EH_CHECK(vk.alloc(8 * (size_t)n + 8));
     EH_CHECK(vk2.alloc(8 * (size_t)n + 8));
-    k_relabel_keys<<<grid_for(n), 256, 0, st>>>(deg.as<uint32_t>(), n, vk.as<uint64_t>());
-    EH_CHECK(prim_sort_u64(ctx, vk.as<uint64_t>(), vk2.as<uint64_t>(), n, 64));
     EH_CHECK(old2new.alloc(4 * (size_t)n + 4));
-    k_perm_from_sorted<<<grid_for(n), 256, 0, st>>>(vk.as<uint64_t>(), n, g->new2old.as<uint32_t>(),
-                                                   old2new.as<uint32_t>());
+    if (vgraph) {
+      k_relabel_keys_grouped<<<grid_for(n), 256, 0, st>>>(deg.as<uint32_t>(), vgraph, n, vk.as<uint64_t>());
+      EH_CHECK(prim_sort_u64(ctx, vk.as<uint64_t>(), vk2.as<uint64_t>(), n, 64));
+      k_perm_from_sorted_masked<<<grid_for(n), 256, 0, st>>>(vk.as<uint64_t>(), n, g->new2old.as<uint32_t>(),
+                                                            old2new.as<uint32_t>());
+    } else {
+      k_relabel_keys<<<grid_for(n), 256, 0, st>>>(deg.as<uint32_t>(), n, vk.as<uint64_t>());
+      EH_CHECK(prim_sort_u64(ctx, vk.as<uint64_t>(), vk2.as<uint64_t>(), n, 64));
+      k_perm_from_sorted<<<grid_for(n), 256, 0, st>>>(vk.as<uint64_t>(), n, g->new2old.as<uint32_t>(),
+                                                     old2new.as<uint32_t>());
+    }
   } else {
     k_iota<<<grid_for(n), 256, 0, st>>>(g->new2old.as<uint32_t>(), n);
   }
@@ -233,7 +256,7 @@ eh_status finish_graph(eh_ctx *ctx, uint32_t n, const uint64_t *ukeys, uint64_t 
 }  // namespace
 
 eh_status graph_canonicalize(eh_ctx *ctx, uint32_t n, DevBuf &keys, uint64_t nkeys, int relabel,
-                             eh_graph **out) {
+                             eh_graph **out, const uint32_t *vgraph) {
   DevBuf tmp, uk;
   EH_CHECK(tmp.alloc(8 * nkeys + 8));
   EH_CHECK(prim_sort_u64(ctx, keys.as<uint64_t>(), tmp.as<uint64_t>(), nkeys, 64));
@@ -247,7 +270,7 @@ eh_status graph_canonicalize(eh_ctx *ctx, uint32_t n, DevBuf &keys, uint64_t nke
   keys.release();
   eh_graph *g = new eh_graph();
   g->ctx = ctx;
-  eh_status s = finish_graph(ctx, n, tmp.as<uint64_t>(), mu, relabel, g);
+  eh_status s = finish_graph(ctx, n, tmp.as<uint64_t>(), mu, relabel, g, vgraph);
   if (s != EH_OK) { delete g; return s; }
   *out = g;
   return EH_OK;
@@ -357,7 +380,7 @@ extern "C" eh_status eh_graph_generate(eh_ctx *ctx, const eh_gen_spec *spec, eh_
     return EH_ERR_INVALID;
   }
   EH_HIP(hipGetLastError());
-  return graph_canonicalize(ctx, n, keys, nkeys, spec->relabel, out);
+  return graph_canonicalize(ctx, n, keys, nkeys, spec->relabel, out, nullptr);
 }
 
 extern "C" eh_status eh_graph_from_edges(eh_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *src,
@@ -375,7 +398,7 @@ extern "C" eh_status eh_graph_from_edges(eh_ctx *ctx, uint32_t n, uint64_t m, co
   k_edges_to_keys<<<grid_for(m), 256, 0, ctx->stream>>>(keys.as<uint64_t>(), ds.as<uint32_t>(),
                                                         dd.as<uint32_t>(), m, n);
   EH_HIP(hipGetLastError());
-  return graph_canonicalize(ctx, n, keys, m, relabel, out);
+  return graph_canonicalize(ctx, n, keys, m, relabel, out, nullptr);
 }
 
 extern "C" eh_status eh_graph_from_csr(eh_ctx *ctx, uint32_t n, const uint64_t *row_off,

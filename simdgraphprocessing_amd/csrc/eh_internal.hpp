@@ -155,7 +155,7 @@ struct eh_matrix {
   double build_ms = 0;
   DevBuf dcard, daoff, dunbr, ddesc, dbsw, dpsd;
   DevBuf dgraph_of;  // u32 per row: graph id (batch arenas); empty => all 0
-  Schedule sched[2];
+  Schedule sched[2][2];  // [triangles, 4-cliques][plain, tuned]
 };
 
 // ---------------------------------------------------------------- primitives (eh_prim.hip)
@@ -173,7 +173,7 @@ eh_status prim_reduce_sum_u64(eh_ctx *ctx, const uint64_t *in, uint64_t n, uint6
 // ---------------------------------------------------------------- internal entry points
 
 eh_status graph_canonicalize(eh_ctx *ctx, uint32_t n, DevBuf &keys, uint64_t nkeys, int relabel,
-                             eh_graph **out);
+                             eh_graph **out, const uint32_t *vgraph);
 eh_status graph_relabel(eh_graph *g);
 eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const uint32_t *d_nbr,
                            uint64_t entries, eh_edge_filter filter, const uint8_t *d_mask,
@@ -185,6 +185,10 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
 eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, Schedule &s,
                           uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles);
 TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts);
+eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, const uint32_t *small_rows,
+                       uint64_t n_small_rows, const uint4 *tasks_big, uint64_t n_big, const uint4 *tasks_small,
+                       uint64_t n_small, uint32_t lds_small, uint32_t lds_big, uint64_t *d_counts, uint32_t *d_next,
+                       uint64_t *d_task_cycles);
 
 // timing helper: device ms between two events recorded on ctx->stream
 double ctx_elapsed_ms(eh_ctx *ctx);

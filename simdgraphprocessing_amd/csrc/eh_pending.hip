@@ -9,12 +9,6 @@ eh_status matrix_select_learned(eh_matrix *, uint8_t *) { EH_PENDING("EH_LAYOUT_
 extern "C" {
 eh_status eh_intersect(eh_ctx *, const eh_row_view *, const eh_row_view *, int, void *, size_t, eh_row_view *) { EH_PENDING("eh_intersect"); }
 eh_status eh_intersect_rows(eh_matrix *, const uint32_t *, const uint32_t *, uint64_t, uint64_t *) { EH_PENDING("eh_intersect_rows"); }
-eh_status eh_batch_generate(eh_ctx *, uint64_t, uint32_t, eh_layout_policy, eh_batch **) { EH_PENDING("eh_batch_generate"); }
-eh_status eh_batch_info(eh_batch *, uint32_t *, uint64_t *, uint64_t *) { EH_PENDING("eh_batch_info"); }
-eh_status eh_batch_graph_params(eh_batch *, uint32_t *, uint64_t *) { EH_PENDING("eh_batch_graph_params"); }
-eh_status eh_batch_graph_download(eh_batch *, uint32_t, uint64_t *, uint32_t *, uint64_t) { EH_PENDING("eh_batch_graph_download"); }
-eh_status eh_batch_count(eh_batch *, int, const eh_run_opts *, uint64_t *, uint64_t *, double *) { EH_PENDING("eh_batch_count"); }
-void eh_batch_free(eh_batch *) {}
 eh_status eh_selector_set_params(eh_ctx *, const float *) { EH_PENDING("eh_selector_set_params"); }
 eh_status eh_selector_get_params(eh_ctx *, float *) { EH_PENDING("eh_selector_get_params"); }
 eh_status eh_matrix_features(eh_matrix *, float *) { EH_PENDING("eh_matrix_features"); }

@@ -1,0 +1,318 @@
+"""CPU tests that pin the oracle (oracle/eh_oracle.c) before it is trusted.
+
+Every restated function is checked against something it does not share code
+with: the reference's own fixture answers, closed forms, brute force over the
+symmetric graph, Python set intersection, hand-computed layout bytes, and --
+where oracle/_ref was built in this container -- the reference's own programs.
+"""
+import itertools
+import random
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+FB_TRIANGLES = 1612010   # test/undirected_triangle_counting_test.cpp:11
+FB_4CLIQUES = 30004668   # test/n_clique_test.cpp:11
+
+
+# ------------------------------------------------------------------ golden fixture
+
+def test_facebook_fixture_shape(facebook):
+    # SNAP ego-Facebook: 4,039 nodes, 88,234 undirected edges
+    assert facebook["n"] == 4039
+    assert len(facebook["nbr"]) == 2 * 88234
+
+
+@pytest.mark.parametrize("policy", ["uint", "pshort", "bs", "hybrid", "hybrid3"])
+def test_facebook_counts_every_layout(facebook, policy):
+    M = O.Matrix(facebook["n"], facebook["off"], facebook["nbr"], policy)
+    assert M.triangles(4)[0] == FB_TRIANGLES
+    assert M.cliques(4, 4)[0] == FB_4CLIQUES
+
+
+def test_facebook_brute_force(facebook):
+    assert O.brute_triangles(facebook["n"], facebook["off"], facebook["nbr"]) == FB_TRIANGLES
+    assert O.brute_cliques(facebook["n"], facebook["off"], facebook["nbr"], 4) == FB_4CLIQUES
+
+
+def test_facebook_row_data_size(facebook):
+    # the reference's hybrid build prints "ROW DATA SIZE (Bytes): 569724" for this file
+    M = O.Matrix(facebook["n"], facebook["off"], facebook["nbr"], "hybrid")
+    assert M.bytes() == 569724
+    tot, mix = M.pair_mix()
+    assert tot == 88234 and mix == [2869, 6138, 76651, 2576]
+
+
+def test_facebook_is_degree_ordered(facebook):
+    deg = np.diff(facebook["off"])
+    assert np.all(deg[:-1] >= deg[1:])
+
+
+# ------------------------------------------------------------------ closed forms
+
+def complete(n):
+    src, dst = zip(*itertools.combinations(range(n), 2))
+    return O.edges_to_csr(n, np.array(src), np.array(dst))
+
+
+@pytest.mark.parametrize("n", [3, 5, 12, 40, 130])
+@pytest.mark.parametrize("policy", ["uint", "bs", "pshort", "hybrid", "hybrid3"])
+def test_complete_graph(n, policy):
+    from math import comb
+    off, nbr = complete(n)
+    M = O.Matrix(n, off, nbr, policy)
+    assert M.triangles(2)[0] == comb(n, 3)
+    assert M.cliques(4, 2)[0] == comb(n, 4)
+    assert M.cliques(5, 2)[0] == comb(n, 5)
+
+
+def test_triangle_free_graphs():
+    n = 200
+    star = O.edges_to_csr(n, np.zeros(n - 1), np.arange(1, n))
+    path = O.edges_to_csr(n, np.arange(n - 1), np.arange(1, n))
+    bip = O.edges_to_csr(n, np.repeat(np.arange(100), 100), np.tile(np.arange(100, 200), 100))
+    for off, nbr in (star, path, bip):
+        assert O.Matrix(n, off, nbr, "hybrid").triangles(2)[0] == 0
+
+
+def test_disjoint_union_adds():
+    from math import comb
+    # K_7 on 0..6 plus K_9 on 100..108 plus a random sparse part on 300..
+    parts = [list(itertools.combinations(range(7), 2)),
+             list(itertools.combinations(range(100, 109), 2))]
+    e = [p for part in parts for p in part]
+    src, dst = np.array([a for a, _ in e]), np.array([b for _, b in e])
+    off, nbr = O.edges_to_csr(400, src, dst)
+    M = O.Matrix(400, off, nbr, "hybrid")
+    assert M.triangles(2)[0] == comb(7, 3) + comb(9, 3)
+    assert M.cliques(4, 2)[0] == comb(7, 4) + comb(9, 4)
+
+
+# ------------------------------------------------------------------ brute force on random graphs
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("policy", ["uint", "bs", "pshort", "hybrid", "hybrid3"])
+def test_random_graphs_vs_brute_force(seed, policy):
+    n = 300
+    src, dst = O.gen_er(n, 3000, seed)
+    r = O.canonical_oriented(n, src, dst)
+    off, nbr = r["sym"]
+    roff, rnbr = r["relabel"][1], r["relabel"][2]
+    M = O.Matrix(n, roff, rnbr, policy)
+    assert M.triangles(3)[0] == O.brute_triangles(n, off, nbr)
+    assert M.cliques(4, 3)[0] == O.brute_cliques(n, off, nbr, 4)
+
+
+def test_relabel_preserves_counts_and_orders_degrees():
+    n = 2000
+    src, dst = O.gen_rmat(11, 8, 7)
+    off, nbr = O.edges_to_csr(n if n > 2048 else 2048, src, dst)
+    n = 2048
+    new2old, roff, rnbr = O.relabel_by_degree(n, off, nbr)
+    deg = np.diff(roff)
+    assert np.all(deg[:-1] >= deg[1:])
+    # ties keep old id order
+    od = np.diff(off)[new2old]
+    for a in range(n - 1):
+        if od[a] == od[a + 1]:
+            assert new2old[a] < new2old[a + 1]
+    assert O.Matrix(n, off, nbr, "hybrid").triangles(2)[0] == O.Matrix(n, roff, rnbr, "hybrid").triangles(2)[0]
+
+
+# ------------------------------------------------------------------ intersections vs Python sets
+
+def gen_set(rng, length, span, run):
+    """Sorted set with runs, in the style of the reference's gen_set
+    (experiments/intersection_benchmark_and_test.cpp:125-163)."""
+    s = set()
+    length = min(length, span // 2)
+    base = rng.randrange(0, max(1, span // 10))
+    while len(s) < length:
+        v = base + rng.randrange(0, span)
+        for t in range(run):
+            s.add(v + t)
+    return sorted(s)[:length]
+
+
+@pytest.mark.parametrize("alg", ["scalar", "standard", "galloping", "select"])
+def test_uint_algorithms(alg):
+    rng = random.Random(11)
+    cases = [([], [1, 2, 3]), ([5], [5]), ([1, 2, 3], [4, 5, 6]), (list(range(0, 1000, 3)), list(range(0, 1000, 5)))]
+    for la, lb, span in [(10, 10, 50), (100, 5000, 20000), (3, 100000, 10 ** 6), (4000, 4000, 10 ** 4),
+                         (127, 129, 1000), (1, 300, 3000)]:
+        for run in (1, 4):
+            cases.append((gen_set(rng, la, span, run), gen_set(rng, lb, span, run)))
+    for a, b in cases:
+        exp = sorted(set(a) & set(b))
+        assert O.isect_uint(alg, a, b) == len(exp)
+        assert list(O.isect_uint(alg, a, b, materialize=True)) == exp
+
+
+TYPES = {"U": O.UINTEGER, "PS": O.PSHORT, "BS": O.BITSET}
+
+
+@pytest.mark.parametrize("ta", ["U", "PS", "BS"])
+@pytest.mark.parametrize("tb", ["U", "PS", "BS"])
+def test_typed_pairs(ta, tb):
+    rng = random.Random(hash((ta, tb)) & 0xFFFF)
+    edge = [([63], [63]), ([64], [63, 64]), ([65535, 65536], [65536, 70000]), ([0, 1, 2], [2, 3]),
+            ([1 << 20], [1 << 20, (1 << 20) + 64])]
+    for _ in range(12):
+        span = rng.choice([100, 5000, 200000, 3 << 20])
+        edge.append((gen_set(rng, rng.randrange(1, 300), span, rng.choice([1, 3])),
+                     gen_set(rng, rng.randrange(1, 300), span, 1)))
+    for a, b in edge:
+        exp = sorted(set(a) & set(b))
+        c, vals, rt = O.set_intersect(a, TYPES[ta], b, TYPES[tb])
+        assert c == len(exp), (ta, tb, a[:5], b[:5])
+        assert list(vals) == exp
+        # result type per the reference dispatcher: U when a uint side is present
+        if "U" in (ta, tb):
+            assert rt == O.UINTEGER
+        elif ta == tb == "BS":
+            assert rt == O.BITSET
+        else:
+            assert rt == O.PSHORT
+
+
+def test_empty_side_is_zero():
+    c, _, _ = O.set_intersect([], O.UINTEGER, [1, 2], O.BITSET)
+    assert c == 0
+
+
+# ------------------------------------------------------------------ layout bytes (hand computed)
+
+def u16(*v):
+    return np.array(v, np.uint16).view(np.uint8)
+
+
+def test_pshort_vector():
+    # pshort of {1,2,65537}: prefix 0, count-1=1, 1, 2, prefix 1, count-1=0, 1 (pshort.hpp:46-78)
+    assert np.array_equal(O.layout_build("pshort", [1, 2, 65537]), u16(0, 1, 1, 2, 1, 0, 1))
+    img = O.row_image([1, 2, 65537], "pshort")
+    assert np.array_equal(img[:4], np.array([14], np.uint32).view(np.uint8))
+    assert np.array_equal(img[4:], u16(0, 1, 1, 2, 1, 0, 1))
+
+
+def test_bitset_vector():
+    # {3, 64, 130}: first word 0, words 0..2 (bitset.hpp:65-96)
+    got = O.layout_build("bitset", [3, 64, 130])
+    exp = np.array([0, 1 << 3, 1, 1 << 2], np.uint64).view(np.uint8)
+    assert np.array_equal(got, exp)
+    got = O.layout_build("bitset", [128, 129])
+    assert np.array_equal(got, np.array([2, 3], np.uint64).view(np.uint8))
+
+
+def test_hybrid_image_and_rule():
+    # hybrid image = u64 tag + flattened sub-layout (hybrid.hpp:132-166)
+    img = O.row_image([5], "hybrid")
+    assert np.array_equal(img, np.concatenate([np.array([O.UINTEGER], np.uint64).view(np.uint8),
+                                               np.array([5], np.uint32).view(np.uint8)]))
+    # density len/(last-first) must exceed 1/256: len 2, range 512 -> uint; range 511 -> bitset
+    assert O.row_type([0, 512]) == O.UINTEGER
+    assert O.row_type([0, 511]) == O.BITSET
+    assert O.row_type([7]) == O.UINTEGER
+    # three-way band: > 1/32 bitset, (1/256, 1/32] pshort
+    assert O.row_type([0, 63], "hybrid3") == O.BITSET
+    assert O.row_type([0, 64], "hybrid3") == O.PSHORT
+    assert O.row_type([0, 512], "hybrid3") == O.UINTEGER
+
+
+@pytest.mark.parametrize("kind,t", [("uint", O.UINTEGER), ("pshort", O.PSHORT), ("bitset", O.BITSET)])
+def test_layout_round_trip(kind, t):
+    rng = random.Random(5)
+    for span in (10, 70000, 5 << 20):
+        for _ in range(20):
+            v = gen_set(rng, rng.randrange(1, 200), span, rng.choice([1, 5]))
+            enc = O.layout_build(kind, v)
+            assert list(O.decode(t, enc, len(v))) == v
+
+
+# ------------------------------------------------------------------ generators
+
+def test_generator_streams_are_deterministic():
+    a = O.gen_rmat(10, 4, 3)
+    b = O.gen_rmat(10, 4, 3)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    c = O.gen_rmat(10, 4, 4)
+    assert not np.array_equal(a[0], c[0])
+
+
+def test_rmat_quadrant_frequencies():
+    # a/b/c/d = .57/.19/.19/.05 at the top level: the top bit of (u, v)
+    src, dst = O.gen_rmat(8, 512, 9)
+    hu, hv = src >> 7, dst >> 7
+    f = [np.mean((hu == 0) & (hv == 0)), np.mean((hu == 0) & (hv == 1)),
+         np.mean((hu == 1) & (hv == 0)), np.mean((hu == 1) & (hv == 1))]
+    for got, exp in zip(f, (0.57, 0.19, 0.19, 0.05)):
+        assert abs(got - exp) < 0.01
+
+
+def test_er_uniform_and_in_range():
+    src, dst = O.gen_er(1000, 200000, 1)
+    assert src.max() < 1000 and dst.max() < 1000
+    counts = np.bincount(src, minlength=1000)
+    assert counts.min() > 100 and counts.max() < 320
+
+
+def test_communities_density():
+    k = O.gen_chung_lu(100000, 4, 2.1, 3, ncomm=10, csize=64)
+    # each community pair is present with probability 1/2
+    n_pairs = 10 * 64 * 63 // 2
+    L = O.lib()
+    got = L.orc_gen_communities(100000, 10, 64, 3, None, None)
+    assert abs(got / n_pairs - 0.5) < 0.03
+    assert len(k[0]) == 100000 * 4 // 2 + got
+
+
+def test_batch_params_ranges():
+    ns, ms = [], []
+    for g in range(4096):
+        n, m = O.batch_params(4, g)
+        assert 64 <= n <= 8192
+        assert 2 * n <= m <= 32 * n
+        ns.append(n)
+        ms.append(m / n * 2)
+    assert 3000 < np.mean(ns) < 5200
+    assert 30 < np.mean(ms) < 38
+
+
+# ------------------------------------------------------------------ MLP reference math
+
+def test_mlp_gradient_finite_differences():
+    rng = np.random.default_rng(0)
+    P = (rng.standard_normal(5508) * 0.2).astype(np.float32)
+    x = rng.standard_normal((7, 16)).astype(np.float32)
+    t = rng.standard_normal((7, 4)).astype(np.float32)
+    m = (rng.random((7, 4)) > 0.3).astype(np.float32)
+    loss, g = O.mlp_loss_grad(P, x, t, m)
+    for idx in [0, 37, 1023, 1024 + 5, 1088 + 100, 5184 + 7, 5184 + 255, 5507]:
+        h = 1e-2
+        Pp, Pm = P.copy(), P.copy()
+        Pp[idx] += h
+        Pm[idx] -= h
+        num = (O.mlp_loss_grad(Pp, x, t, m)[0] - O.mlp_loss_grad(Pm, x, t, m)[0]) / (2 * h)
+        assert abs(num - g[idx]) < 2e-3 * max(1.0, abs(num)), idx
+    y = O.mlp_forward(P, x)
+    lab = m.sum()
+    assert abs(loss - float((m * (y - t) ** 2).sum() / lab)) < 1e-9
+
+
+# ------------------------------------------------------------------ the reference's own programs
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (needs the reference sources)")
+@pytest.mark.parametrize("seed", [1, 2])
+def test_oracle_matches_reference_programs(seed):
+    n = 3000
+    src, dst = O.gen_rmat(11, 12, seed)
+    r = O.canonical_oriented(2048, src, dst)
+    _, off, nbr = r["relabel"]
+    om = O.Matrix(2048, off, nbr, "hybrid")
+    t_ref, _ = O.run_ref("undirected_triangle_counting", 2048, off, nbr, threads=4, layout="hybrid")
+    k_ref, _ = O.run_ref("n_clique", 2048, off, nbr, threads=4, layout="hybrid", k=4)
+    assert om.triangles(4)[0] == t_ref
+    assert om.cliques(4, 4)[0] == k_ref
+    for layout in ("uint", "bs", "pshort"):
+        assert O.run_ref("undirected_triangle_counting", 2048, off, nbr, threads=2, layout=layout)[0] == t_ref
