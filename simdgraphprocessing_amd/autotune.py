@@ -39,16 +39,12 @@ def _store(db):
         pass
 
 
-def candidates(M):
-    w = M.info()["wedges"]
-    base = max(2048, w // (256 * 16 * 8 + 1))
-    works = sorted({max(1024, int(base * f)) for f in (0.25, 0.5, 1, 2, 4)})
-    for lds in (1024, 2048, 4096):
-        for waves in (4, 2):
-            if lds * waves * 4 > 160 * 1024:
-                continue
-            for tw in works:
-                yield {"task_work": tw, "lds_words": lds, "waves": waves}
+def candidates(M, k=3):
+    """Launch/schedule grid: edge-centric row threshold x LDS budget x waves per block."""
+    smalls = (16, 24, 32, 48, 64) if k == 3 else (4, 8, 12, 16, 24)
+    for small in smalls:
+        for lds, waves in ((2048, 2), (2048, 4), (1536, 4), (3072, 2)):
+            yield {"small_degree": small, "lds_words": lds, "waves": waves}
 
 
 def tune(M, k=3, schedule="tuned", repeat=3, budget_s=60.0, use_cache=True, verbose=False):
@@ -64,10 +60,10 @@ def tune(M, k=3, schedule="tuned", repeat=3, budget_s=60.0, use_cache=True, verb
     trials = []
     t0 = time.time()
     best = None
-    for p in candidates(M):
+    for p in candidates(M, k):
         if time.time() - t0 > budget_s:
             break
-        c, st = run(task_work=p["task_work"], lds_words=p["lds_words"], waves=p["waves"], repeat=repeat)
+        c, st = run(repeat=repeat, **p)
         if c != ref:
             raise AssertionError(f"autotune candidate {p} changed the count: {c} != {ref}")
         trials.append((st["kernel_ms"], p))

@@ -1064,7 +1064,7 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
   TuneParams tp;
   tp.lds_small = 2048;
   tp.lds_big = 16384;
-  tp.waves_per_block = 4;
+  tp.waves_per_block = sched == EH_SCHED_TUNED ? 2 : 4;
   tp.lanes = 64;
   if (sched == EH_SCHED_TUNED) {
     uint64_t waves = (uint64_t)(m->ctx->num_cus ? m->ctx->num_cus : 256) * 16;
