@@ -244,11 +244,15 @@ void eh_batch_free(eh_batch *b);
 #define EH_MLP_PARAMS 5508
 eh_status eh_selector_set_params(eh_ctx *ctx, const float *params);
 eh_status eh_selector_get_params(eh_ctx *ctx, float *params);
-/* per-row features (n x 16 floats) of a matrix's rows */
+/* Per-row selector arms (eh_matrix_row_types values): 0 bitset probe, 1 pshort
+ * partition walk, 2 uint binary search, 3 uint galloping search. */
+/* per-row features (n x 16 floats) of an oriented matrix's rows */
 eh_status eh_matrix_features(eh_matrix *m, float *features);
-/* engine timings: per-row cycles spent on B-side reads of each layout; fills
- * targets (n x 4, log2 cycles per use) and mask (n x 4, 1 where measured) */
-eh_status eh_matrix_profile_layouts(eh_matrix *m, float *targets, float *mask);
+/* Engine timings: for every stride-th row that answers membership queries in the
+ * triangle pass, time each arm on the row's own queries (the elements below the row
+ * id of every source row that contains it) with the shader clock; fills targets
+ * (n x 4, log2 cycles per query) and mask (n x 4, 1 where measured). */
+eh_status eh_matrix_profile_layouts(eh_matrix *m, uint32_t stride, float *targets, float *mask);
 /* device forward over a host batch (x: batch x 16, y: batch x 4) */
 eh_status eh_mlp_forward(eh_ctx *ctx, const float *params, const float *x, int batch, float *y);
 /* device loss + gradient (grad: 5508 floats) */

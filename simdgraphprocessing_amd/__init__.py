@@ -146,7 +146,7 @@ def lib():
     L.eh_selector_set_params.argtypes = [_vp, _vp]
     L.eh_selector_get_params.argtypes = [_vp, _vp]
     L.eh_matrix_features.argtypes = [_vp, _vp]
-    L.eh_matrix_profile_layouts.argtypes = [_vp, _vp, _vp]
+    L.eh_matrix_profile_layouts.argtypes = [_vp, C.c_uint32, _vp, _vp]
     L.eh_mlp_forward.argtypes = [_vp, _vp, _vp, C.c_int, _vp]
     L.eh_mlp_loss_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_float), _vp]
     L.eh_selector_train.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64,
@@ -395,11 +395,12 @@ class SparseMatrix:
         _check(lib().eh_matrix_features(self.h, _ptr(f)))
         return f
 
-    def profile_layouts(self):
+    def profile_layouts(self, stride=1):
+        """Engine-timed log2 cycles per membership query for each selector arm."""
         n = self.info()["n"]
         t = np.zeros((n, 4), np.float32)
         m = np.zeros((n, 4), np.float32)
-        _check(lib().eh_matrix_profile_layouts(self.h, _ptr(t), _ptr(m)))
+        _check(lib().eh_matrix_profile_layouts(self.h, stride, _ptr(t), _ptr(m)))
         return t, m
 
     def triangle_count(self, schedule="tuned", **kw):

@@ -157,11 +157,11 @@ __global__ void k_select(TypeArgs a) {
     // reference row image: optional u64 tag, then the flattened sub-layout
     const bool tagged = a.policy >= EH_LAYOUT_HYBRID;
     uint64_t bytes = tagged ? 8ull : 0ull;
-    if (t == EH_U) bytes += 4ull * c;
+    if (is_uint_type(t)) bytes += 4ull * c;
     else if (t == EH_BS) bytes += c ? 4ull + 8ull + 8ull * nw : 0ull;
     else bytes += c ? 4ull + 2ull * shorts : 0ull;
     a.refb[i] = bytes;
-    if (c) atomicAdd(&a.type_rows[t], 1ull);
+    if (c) atomicAdd(&a.type_rows[t == EH_UG ? EH_U : t], 1ull);
   }
 }
 
@@ -436,17 +436,17 @@ extern "C" eh_status eh_matrix_row(eh_matrix *m, uint32_t row, void *buf, size_t
   const uint32_t t = desc_type(d);
   uint64_t bytes = 0;
   if (d.card == 0) bytes = 0;
-  else if (t == EH_U) bytes = 4ull * d.card;
+  else if (is_uint_type(t)) bytes = 4ull * d.card;
   else if (t == EH_BS) bytes = 8ull + 8ull * desc_aux(d);
   else bytes = 2ull * desc_aux(d);
   view->data = buf;
   view->cardinality = d.card;
   view->number_of_bytes = bytes;
-  view->type = (int32_t)t;
+  view->type = (int32_t)(is_uint_type(t) ? EH_U : t);
   view->reserved = 0;
   if (bytes > cap) { eh_set_error("row needs %llu bytes", (unsigned long long)bytes); return EH_ERR_CAPACITY; }
   if (!bytes) return EH_OK;
-  if (t == EH_U) {
+  if (is_uint_type(t)) {
     EH_HIP(hipMemcpy(buf, m->dunbr.as<uint32_t>() + 4ull * d.off, bytes, hipMemcpyDeviceToHost));
   } else if (t == EH_BS) {
     const uint64_t w0 = d.w0;

@@ -253,7 +253,7 @@ __device__ __forceinline__ uint64_t isect_count(const AState &A, const CountArgs
                                                 uint32_t lane) {
   if (dj.card == 0) return 0;
   const uint32_t t = desc_type(dj);
-  if (t == EH_U) return isect_b_uint<KIND, W>(A, a, dj, lane);
+  if (is_uint_type(t)) return isect_b_uint<KIND, W>(A, a, dj, lane);
   if (t == EH_PS) return isect_b_pshort<KIND, W>(A, a, dj, lane);
   return isect_b_bitset<KIND, W>(A, a, dj, lane);
 }
@@ -267,7 +267,7 @@ __device__ __forceinline__ uint32_t isect_write(const AState &A, const CountArgs
   if (dj.card == 0) return 0;
   const uint32_t t = desc_type(dj);
   uint32_t nc = 0;
-  if (t == EH_U) {
+  if (is_uint_type(t)) {
     const uint32_t *B = a.unbr + (uint64_t)dj.off * 4;
     for (uint32_t k0 = 0; k0 < dj.card; k0 += 64) {
       const uint32_t k = k0 + lane;
@@ -522,7 +522,7 @@ __device__ uint64_t task_cliques4(const CountArgs &a, const AState &A, uint32_t 
 
 constexpr uint32_t PASS_PROBE = 0, PASS_AND = 1, PASS_STREAM = 2;
 constexpr uint32_t META_WORDS = 260;
-constexpr uint32_t SMALL_META_WORDS = 196;
+constexpr uint32_t SMALL_META_WORDS = 260;
 
 __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t *L, uint32_t n, uint32_t x) {
   uint32_t lo = 0, len = n;
@@ -608,7 +608,7 @@ __device__ uint64_t sum_isect(const CountArgs &a, const AState &S, const uint32_
           const uint32_t ov = oe > os ? oe - os : 0u;
           if (ov < pc) { ac = ov; as = os; pc = 0; }
         }
-      } else if (t == EH_U) {
+      } else if (is_uint_type(t)) {
         sc = (d.card + 3u) >> 2;
       } else {
         seq = 1;
@@ -753,9 +753,25 @@ __device__ uint64_t run_task(const CountArgs &a, uint32_t kind, uint32_t i, uint
 // Rows with few kept entries are counted edge-centrically: for an oriented
 // matrix (every row j only holds ids below j) the triangles closed at row i are
 // the pairs a < b of row(i) with row(i)[a] in row(row(i)[b]), and the
-// 4-cliques are the triples a < b < c whose three pairs are all edges.  Items
-// (pairs or triples) of 64 rows are flattened over the lanes; each item is a
-// handful of membership tests against rows in HBM, with no LDS staging.
+// 4-cliques are those pairs extended by every c > b whose row holds both.  The
+// pairs of 64 rows are flattened over the lanes; each is a handful of
+// membership tests against rows in HBM, with no LDS staging.
+
+// galloping membership: probe positions 1, 2, 4, ... until one holds a value >= x,
+// then binary-search the last doubling interval (the exponential search the
+// reference's set_intersect_galloping runs over 128-element blocks)
+__device__ __forceinline__ uint32_t gallop_contains(const uint32_t *R, uint32_t n, uint32_t x) {
+  if (n == 0) return 0;
+  if (R[0] >= x) return R[0] == x;
+  uint32_t b = 1;
+  while (b < n && R[b] < x) b <<= 1;
+  uint32_t lo = (b >> 1) + 1, hi = min(b, n - 1) + 1;  // answer in [lo, hi)
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (R[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo < n && R[lo] == x;
+}
 
 // membership of x in row(j) as stored in HBM
 __device__ __forceinline__ uint32_t row_contains(const CountArgs &a, const RowDesc &d, uint32_t x) {
@@ -766,6 +782,7 @@ __device__ __forceinline__ uint32_t row_contains(const CountArgs &a, const RowDe
     return (w < desc_aux(d)) ? (uint32_t)((a.bsw[d.off + w] >> (x & 63u)) & 1ull) : 0u;
   }
   if (t == EH_U) return sorted_contains(a.unbr + (uint64_t)d.off * 4, d.card, x);
+  if (t == EH_UG) return gallop_contains(a.unbr + (uint64_t)d.off * 4, d.card, x);
   const uint16_t *S = a.psd + (uint64_t)d.off * 4;
   const uint32_t len = desc_aux(d), hx = x >> 16;
   uint32_t pos = 0;
@@ -788,22 +805,12 @@ __device__ __forceinline__ void tri_decode(uint32_t r, uint32_t *p, uint32_t *q)
   *p = r - (b * (b - 1u)) / 2u;
 }
 
-// r -> (p, q, s) with p < q < s, r = C(s,3) + C(q,2) + p
-__device__ __forceinline__ void tet_decode(uint32_t r, uint32_t *p, uint32_t *q, uint32_t *s) {
-  uint32_t c = (uint32_t)cbrtf(6.0f * (float)r);
-  while (((c + 1u) * c * (c - 1u)) / 6u <= r) c++;
-  while (c > 2u && (c * (c - 1u) * (c - 2u)) / 6u > r) c--;
-  if (c < 2u) c = 2u;
-  *s = c;
-  tri_decode(r - (c * (c - 1u) * (c - 2u)) / 6u, p, q);
-}
-
 template <int K, bool W>
 __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t *__restrict__ rows, uint32_t nrows) {
   extern __shared__ uint32_t lds_all[];
   const uint32_t lane = lane_id();
   uint32_t *meta = lds_all + (threadIdx.x >> 6) * SMALL_META_WORDS;
-  uint32_t *mbase = meta, *moff = meta + 65, *rcnt = meta + 130;
+  uint32_t *mbase = meta, *moff = meta + 65, *rcnt = meta + 130, *mdeg = meta + 194;
   const bool per_graph = a.graph_of != nullptr;
   uint64_t acc = 0;
   for (;;) {
@@ -812,20 +819,20 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
     c0 = rfl(c0);
     if (c0 >= nrows) break;
     const uint32_t nr = min(64u, nrows - c0);
-    uint32_t items = 0, off = 0, gid = 0;
+    uint32_t items = 0, off = 0, gid = 0, d = 0;
     if (lane < nr) {
       const uint32_t i = rows[c0 + lane];
-      const uint32_t d = a.card[i];
+      d = a.card[i];
       off = a.aoff[i];
       if (per_graph) gid = a.graph_of[i];
-      if constexpr (K == 3) items = (d * (d - 1u)) / 2u;
-      else items = d >= 3u ? (d * (d - 1u) * (d - 2u)) / 6u : 0u;
+      items = (d * (d - 1u)) / 2u;  // pairs; 4-cliques extend the pairs that are edges
     }
     uint32_t tot;
     const uint32_t base = wave_excl_scan(items, lane, &tot);
     tot = rfl(tot);
     mbase[lane] = base;
     moff[lane] = off;
+    mdeg[lane] = d;
     rcnt[lane] = 0;
     if (lane == 0) mbase[64] = tot;
     wave_sync();
@@ -845,11 +852,19 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
         if constexpr (W) h = m ? (uint64_t)a.card[x] : 0ull;
         else h = m;
       } else {
-        uint32_t p, s, u;
-        tet_decode(r, &p, &s, &u);
-        const uint32_t x = R[p], y = R[s], z = R[u];
-        const RowDesc dz = a.desc[z];
-        h = (row_contains(a, dz, y) && row_contains(a, dz, x) && row_contains(a, a.desc[y], x)) ? 1ull : 0ull;
+        uint32_t p, s;
+        tri_decode(r, &p, &s);
+        const uint32_t x = R[p], y = R[s];
+        h = 0;
+        if (row_contains(a, a.desc[y], x)) {
+          // (x, y) closes a triangle with the source: every later z adjacent to both closes a 4-clique
+          const uint32_t d = mdeg[q];
+          for (uint32_t u = s + 1u; u < d; u++) {
+            const uint32_t z = R[u];
+            const RowDesc dz = a.desc[z];
+            if (row_contains(a, dz, y) && row_contains(a, dz, x)) h++;
+          }
+        }
       }
       if (per_graph) {
         if (h) atomicAdd(&rcnt[q], (uint32_t)h);
@@ -946,7 +961,7 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
         const uint32_t t = desc_type(d);
         uint64_t units;
         if (d.card == 0) { loc[3]++; units = 0; }
-        else if (t == EH_U) { loc[0]++; units = d.card; }
+        else if (is_uint_type(t)) { loc[0]++; units = d.card; }
         else if (t == EH_PS) { loc[1]++; units = desc_aux(d); }
         else { loc[2]++; units = desc_aux(d); }
         loc[5] += d.card;

@@ -27,17 +27,21 @@
 #define EH_U 2u
 #define EH_PS 1u
 #define EH_BS 0u
+#define EH_UG 3u  // uint layout, membership by galloping (exponential) search from the front
 #define EH_PAD_VALUE 0xFFFFFFFFu
 
 struct RowDesc {
   uint32_t card;   // element count
-  uint32_t tinfo;  // bits 0-1: type (0 BS, 1 PS, 2 U); bits 2-31: BS words / PS stream shorts
+  uint32_t tinfo;  // bits 0-1: type (0 BS, 1 PS, 2 U, 3 U searched by galloping);
+                   // bits 2-31: BS words / PS stream shorts
   uint32_t w0;     // BS: first u64 word index; PS: number of partitions
   uint32_t off;    // U: element offset/4; BS: word offset; PS: short offset/4
 };
 static_assert(sizeof(RowDesc) == 16, "RowDesc must be 16 bytes");
 
 __host__ __device__ inline uint32_t desc_type(const RowDesc &d) { return d.tinfo & 3u; }
+// the two uint arms share one payload (the canonical row)
+__host__ __device__ inline bool is_uint_type(uint32_t t) { return t >= EH_U; }
 __host__ __device__ inline uint32_t desc_aux(const RowDesc &d) { return d.tinfo >> 2; }
 
 // ---------------------------------------------------------------- errors
