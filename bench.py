@@ -55,6 +55,14 @@ def traffic_from_profiles(config_tag):
     return None
 
 
+def _stride_for(est_full_s, budget_s):
+    """Power-of-two row stride that brings an estimated full CPU pass under the budget."""
+    stride = 1
+    while est_full_s / stride > budget_s and stride < 64:
+        stride *= 2
+    return stride
+
+
 def cpu_baseline(n, off, nbr, gpu_count, budget_s=30.0):
     """The oracle (C restatement of the reference hybrid path) on this host."""
     import oracle as O
@@ -62,10 +70,7 @@ def cpu_baseline(n, off, nbr, gpu_count, budget_s=30.0):
     M = O.Matrix(n, off, nbr, "hybrid", lower=True, threads=threads)
     # probe with a 1/64 vertex sample, then size the measured sample to the budget
     _, e_probe, t_probe = M.triangles_sample(threads, 64, 0)
-    est_full = t_probe * 64
-    stride = 1
-    while est_full / stride > budget_s and stride < 64:
-        stride *= 2
+    stride = _stride_for(t_probe * 64, budget_s)
     if stride == 1:
         c, secs = M.triangles(threads)
         edges = M.cardinality()
@@ -127,9 +132,18 @@ def config_c3(ctx, with_cpu):
         import oracle as O
         off, nbr = g.download()
         om = O.Matrix(i["n"], off, nbr, "hybrid", threads=cpu_threads())
-        kc, edges, secs = om.cliques_sample(4, cpu_threads(), 16, 0)
-        out["cpu"] = {"seconds": secs, "sample": "source rows i % 16 == 0", "sampled_edges": edges,
-                      "edges_per_s": edges / secs, "cores": cpu_threads(), "kind": "port"}
+        threads = cpu_threads()
+        _, _, t_probe = om.cliques_sample(4, threads, 64, 0)
+        stride = _stride_for(t_probe * 64, 30.0)
+        if stride == 1:
+            kc, secs = om.cliques(4, threads)
+            assert kc == c, f"CPU 4-clique count {kc} != GPU {c}"
+            edges, sample = i["cardinality"], "full graph"
+        else:
+            _, edges, secs = om.cliques_sample(4, threads, stride, 0)
+            sample = f"source rows i % {stride} == 0"
+        out["cpu"] = {"seconds": secs, "sample": sample, "edges_per_s": edges / secs,
+                      "cores": threads, "kind": "port", "count_checked": stride == 1}
     return out
 
 
@@ -177,6 +191,20 @@ def config_c4(ctx, with_cpu):
     return out
 
 
+def config_c5(ctx, with_cpu):
+    """Learned selector (trained on engine timings of RMAT 18-21, seeds 10-19) evaluated on RMAT-24,
+    edge factor 16, seed 24: triangles with density-rule vs learned layouts."""
+    from simdgraphprocessing_amd import selector
+    out = selector.evaluate(ctx, scale=24, seed=24, steps=3)
+    if with_cpu:
+        import oracle as O
+        g = ctx.generate_rmat(24, 16, 24)
+        off, nbr = g.download()
+        del g
+        out["cpu"] = cpu_baseline(1 << 24, off, nbr, out["tuned_learned"]["count"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,8 +213,9 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--policy", default="hybrid")
-    ap.add_argument("--configs", default="c1,c3,c4", help="extra configs to report (c1,c3,c4)")
+    ap.add_argument("--policy", default="auto", help="layouts of the timed pass: auto = learned when the "
+                    "trained selector parameters exist, else hybrid")
+    ap.add_argument("--configs", default="c1,c3,c4,c5", help="extra configs to report (c1,c3,c4,c5)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=5)
@@ -197,9 +226,15 @@ def main():
         sys.exit(2)
 
     import simdgraphprocessing_amd as eh
-    from simdgraphprocessing_amd import autotune
+    from simdgraphprocessing_amd import autotune, selector
 
     ctx = eh.Context(0)
+    if args.policy in ("auto", "learned"):
+        if os.path.exists(selector.PARAMS_PATH):
+            ctx.set_selector(selector.load())
+            args.policy = "learned"
+        elif args.policy == "auto":
+            args.policy = "hybrid"
     log(f"device: {ctx.device_name}")
     t0 = time.time()
     g = ctx.generate_rmat(args.scale, args.edge_factor, args.seed)
@@ -217,6 +252,14 @@ def main():
         plain_ms.append(st["kernel_ms"])
     plain_ms = sorted(plain_ms)[len(plain_ms) // 2] if plain_ms else stp["kernel_ms"]
     log(f"plain hybrid: count={cp} kernel {plain_ms:.3f} ms")
+
+    # density-rule layouts on the tuned schedule (the selector's yardstick)
+    Mh = g.build_matrix("lower", "hybrid")
+    ch, sth = Mh.triangle_count("tuned", repeat=5)
+    assert ch == cp
+    tuned_hybrid_ms = sth["kernel_ms"]
+    del Mh
+    log(f"tuned hybrid: kernel {tuned_hybrid_ms:.3f} ms")
 
     # optimised: selected layouts + tuned schedule
     M = g.build_matrix("lower", args.policy)
@@ -260,6 +303,8 @@ def main():
         extra["c3"] = config_c3(ctx, not args.no_cpu)
     if "c4" in wanted:
         extra["c4"] = config_c4(ctx, not args.no_cpu)
+    if "c5" in wanted and args.policy == "learned":
+        extra["c5"] = config_c5(ctx, not args.no_cpu)
 
     out = {
         "metric": "TC edges/s + HBM GB/s, RMAT scale-22; learned/batched vs plain, bit-exact",
@@ -286,6 +331,8 @@ def main():
         "plain": {"policy": "hybrid", "schedule": "plain", "kernel_ms": plain_ms,
                   "edges_per_s": m / (plain_ms * 1e-3)},
         "speedup_vs_plain": plain_ms / kernel_ms,
+        "tuned_hybrid_kernel_ms": tuned_hybrid_ms,
+        "learned_vs_density_rule": tuned_hybrid_ms / kernel_ms,
         "alg_bytes": alg_bytes,
         "rows_by_type": info["rows_by_type"],
         "build_ms": info["build_ms"],

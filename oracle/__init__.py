@@ -286,13 +286,44 @@ def set_intersect(a_vals, a_type, b_vals, b_type, materialize=True):
     return c, vals, Cs.type
 
 
+def set_intersect_bytes(a_vals, a_type, b_vals, b_type):
+    """Materialised typed-set intersection: (count, result type, result bytes)."""
+    L = lib()
+    A, ka = make_set(a_vals, a_type)
+    B, kb = make_set(b_vals, b_type)
+    n = max(len(a_vals), 1) + max(len(b_vals), 1)
+    hi = int(max(np.max(a_vals) if len(a_vals) else 0, np.max(b_vals) if len(b_vals) else 0))
+    out = np.zeros(16 * n + 8 * (hi // 64 + 4) + 64, np.uint8)
+    Cs = OrcSet(out.ctypes.data, 0, 0, 0, 0)
+    c = int(L.orc_set_intersect(C.byref(Cs), C.byref(A), C.byref(B)))
+    return c, int(Cs.type), out[:Cs.number_of_bytes].copy()
+
+
+def set_payload(values, type_):
+    """(type, cardinality, payload bytes) of a typed set in the reference formats."""
+    v = _u32(values)
+    if type_ == UINTEGER:
+        return type_, len(v), v.view(np.uint8).copy()
+    if len(v) == 0:
+        return type_, 0, np.zeros(0, np.uint8)
+    return type_, len(v), layout_build("bitset" if type_ == BITSET else "pshort", v)
+
+
 class Matrix:
     """SparseMatrix built from a symmetric CSR (lower=True keeps nbr < node)."""
 
     def __init__(self, n, off, nbr, policy="hybrid", lower=True, threads=8):
         self.n = n
         self._off, self._nbr = _u64(off), _u32(nbr)
+        off64 = self._off
+        if len(off64) != n + 1 or int(off64[0]) != 0 or int(off64[-1]) > len(self._nbr) or \
+                np.any(off64[1:] < off64[:-1]):
+            raise ValueError("Matrix: malformed CSR offsets")
+        if int(off64[-1]) and int(self._nbr[:int(off64[-1])].max()) >= n:
+            raise ValueError("Matrix: neighbour id out of range")
         self.h = lib().orc_matrix_build(n, self._off, self._nbr, int(lower), POLICY[policy], threads)
+        if not self.h:
+            raise MemoryError("orc_matrix_build could not allocate the row images")
 
     def __del__(self):
         if getattr(self, "h", None):

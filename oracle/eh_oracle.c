@@ -360,6 +360,17 @@ size_t orc_row_image_bound(const uint32_t *data, size_t len) {
   return b;
 }
 
+/* Bound for the layout the policy picks for this row only: the arena of a
+ * large sparse matrix must not reserve a bitset span for every uint row. */
+size_t orc_row_image_bound_policy(const uint32_t *data, size_t len, int policy) {
+  size_t b = 8 + 4 + 16; /* type tag, sub-layout byte count, slack */
+  int t = orc_row_type(data, len, policy);
+  if (t == ORC_UINTEGER) b += 4 * len;
+  else if (t == ORC_PSHORT) b += 6 * len; /* worst case: one partition per element */
+  else if (len) b += 8 * (size_t)((data[len - 1] >> 6) - (data[0] >> 6) + 1) + 8;
+  return b;
+}
+
 /* The flattened sub-layout of one type: uint is raw, bitset/pshort carry a
  * u32 byte count and are empty for empty rows (bitset.hpp:99-109,
  * pshort.hpp:81-91, uinteger.hpp:56-59). */
@@ -845,7 +856,7 @@ static void build_size(void *c, int tid, uint64_t i) {
   uint32_t *buf = b->scratch[tid];
   size_t k = select_row(b, i, buf);
   b->len[i] = (uint32_t)k;
-  b->size[i] = orc_row_image_bound(buf, k);
+  b->size[i] = orc_row_image_bound_policy(buf, k, b->policy);
 }
 
 static void build_fill(void *c, int tid, uint64_t i) {
@@ -876,6 +887,16 @@ orc_matrix *orc_matrix_build(uint32_t n, const uint64_t *off, const uint32_t *nb
   for (uint32_t i = 0; i < n; i++) { M->row_pos[i] = p; p += (size[i] + 7) & ~7ull; }
   M->row_pos[n] = p;
   M->arena = (uint8_t *)malloc(p + 64);
+  if (!M->arena) {
+    fprintf(stderr, "orc_matrix_build: cannot allocate %llu bytes of row images\n", (unsigned long long)(p + 64));
+    for (int t = 0; t < threads; t++) free(scratch[t]);
+    free(scratch);
+    free(size);
+    free(M->row_lengths);
+    free(M->row_pos);
+    free(M);
+    return NULL;
+  }
   b.arena = M->arena;
   b.pos = M->row_pos;
   par_for_range(threads, 0, n, 256, build_fill, &b);

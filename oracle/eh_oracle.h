@@ -59,6 +59,7 @@ size_t orc_row_image(uint8_t *out, const uint32_t *data, size_t len, int policy)
 size_t orc_decode(int type, const uint8_t *payload, size_t card, size_t nbytes, uint32_t *out);
 /* upper bound on the bytes orc_row_image writes for a row */
 size_t orc_row_image_bound(const uint32_t *data, size_t len);
+size_t orc_row_image_bound_policy(const uint32_t *data, size_t len, int policy);
 
 /* ---- sets and intersections ---- */
 typedef struct {

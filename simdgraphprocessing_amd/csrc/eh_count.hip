@@ -997,12 +997,15 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
       const uint64_t nb = META_WORDS + rep * (2ull * anw + dA);
       const uint64_t nh = META_WORDS + rep * (cap + dA);
       const uint64_t nl = META_WORDS + rep * dA;
+      // O(1) membership (bitmap or hash) first, in the small slice, then in the
+      // large one; binary search only when neither fits
       if (nb <= s.lds_small && 2ull * anw <= 4ull * cap) { kind = KIND_BITMAP; need = nb; }
       else if (nh <= s.lds_small) { kind = KIND_HASH; need = nh; }
       else if (nb <= s.lds_small) { kind = KIND_BITMAP; need = nb; }
+      else if (nh <= s.lds_big) { kind = KIND_HASH; need = nh; }
+      else if (nb <= s.lds_big) { kind = KIND_BITMAP; need = nb; }
       else if (nl <= s.lds_small) { kind = KIND_LIST; need = nl; }
       else if (s.k == 3) { kind = KIND_GLOBAL; need = 0; }
-      else if (nh <= s.lds_big) { kind = KIND_HASH; need = nh; }
       else { kind = KIND_LIST; need = nl; }
     } else {
       if (ta == EH_BS) { kind = KIND_BITMAP; need = need_bitmap; }
@@ -1078,7 +1081,8 @@ struct SchedStats {
 TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts) {
   TuneParams tp;
   tp.lds_small = 2048;
-  tp.lds_big = 16384;
+  tp.lds_big = sched == EH_SCHED_TUNED ? 8192 : 16384;
+  tp.big_waves = sched == EH_SCHED_TUNED ? 2 : 1;
   tp.waves_per_block = sched == EH_SCHED_TUNED ? 2 : 4;
   tp.lanes = 64;
   if (sched == EH_SCHED_TUNED) {
@@ -1259,17 +1263,17 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
   const bool weighted = k == -3;
   if (k == 4) {
     EH_CHECK((launch_small<4, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2)));
-    EH_CHECK((launch_count<4, false>(m->ctx, a, tasks_big, n_big, lds_big, 1, d_next, 0)));
+    EH_CHECK((launch_count<4, false>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0)));
     EH_CHECK((launch_count<4, false>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,
                                      n_big)));
   } else if (weighted) {
     EH_CHECK((launch_small<3, true>(m->ctx, a, small_rows, n_small_rows, d_next + 2)));
-    EH_CHECK((launch_count<3, true>(m->ctx, a, tasks_big, n_big, lds_big, 1, d_next, 0)));
+    EH_CHECK((launch_count<3, true>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0)));
     EH_CHECK((launch_count<3, true>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,
                                     n_big)));
   } else {
     EH_CHECK((launch_small<3, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2)));
-    EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_big, n_big, lds_big, 1, d_next, 0)));
+    EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0)));
     EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,
                                      n_big)));
   }

@@ -105,6 +105,7 @@ struct TuneParams {
   uint32_t lds_small = 2048;
   uint32_t lds_big = 16384;
   int waves_per_block = 4;
+  int big_waves = 1;       // waves per block of the large-LDS task class
   uint32_t small_deg = 0;  // rows up to this many entries use the edge-centric kernel
 };
 
