@@ -48,8 +48,9 @@ def traffic_from_profiles(config_tag):
     try:
         with open(path) as f:
             d = json.load(f)
-        if d.get("config") == config_tag:
-            return d.get("bytes_per_pass")
+        entry = d.get("configs", {}).get(config_tag)
+        if entry:
+            return entry.get("bytes_per_pass")
     except (OSError, ValueError):
         pass
     return None
@@ -218,6 +219,8 @@ def main():
     ap.add_argument("--configs", default="c1,c3,c4,c5", help="extra configs to report (c1,c3,c4,c5)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--params", default="", help='fixed launch shape instead of autotuning, e.g. '
+                    '\'{"small_degree": 32, "lds_words": 1536, "waves": 4}\'')
     ap.add_argument("--plain-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -253,21 +256,27 @@ def main():
     plain_ms = sorted(plain_ms)[len(plain_ms) // 2] if plain_ms else stp["kernel_ms"]
     log(f"plain hybrid: count={cp} kernel {plain_ms:.3f} ms")
 
-    # density-rule layouts on the tuned schedule (the selector's yardstick)
-    Mh = g.build_matrix("lower", "hybrid")
-    ch, sth = Mh.triangle_count("tuned", repeat=5)
-    assert ch == cp
-    tuned_hybrid_ms = sth["kernel_ms"]
-    del Mh
-    log(f"tuned hybrid: kernel {tuned_hybrid_ms:.3f} ms")
-
-    # optimised: selected layouts + tuned schedule
-    M = g.build_matrix("lower", args.policy)
+    # candidate layouts on the tuned schedule: the density rule (the selector's
+    # yardstick) and, when trained parameters exist, the learned selector; each gets
+    # its launch shape from the autotuner and the faster one is the timed pass
+    cands = {}
+    for pol in (["hybrid", "learned"] if args.policy == "learned" else [args.policy]):
+        Mc = g.build_matrix("lower", pol)
+        cc, stc = Mc.triangle_count("tuned", repeat=5)
+        assert cc == cp, f"{pol}: tuned count {cc} != plain count {cp}"
+        prm, best = {}, stc["kernel_ms"]
+        if args.params:
+            prm = json.loads(args.params)
+            best = Mc.triangle_count("tuned", repeat=5, **prm)[1]["kernel_ms"]
+        elif not args.no_tune:
+            prm, best, _ = autotune.tune(Mc, 3, "tuned", repeat=2, budget_s=30, use_cache=True)
+            best = min(best, Mc.triangle_count("tuned", repeat=5, **prm)[1]["kernel_ms"])
+        cands[pol] = {"matrix": Mc, "params": prm, "default_ms": stc["kernel_ms"], "tuned_ms": best}
+        log(f"tuned {pol}: default shape {stc['kernel_ms']:.3f} ms, autotuned {prm} {best:.3f} ms")
+    chosen = min(cands, key=lambda k: cands[k]["tuned_ms"])
+    M = cands[chosen]["matrix"]
+    params = cands[chosen]["params"]
     info = M.info()
-    params = {}
-    if not args.no_tune:
-        params, best_ms, _ = autotune.tune(M, 3, "tuned", repeat=2, budget_s=40, use_cache=True)
-        log(f"autotune: {params} {best_ms:.3f} ms")
     kw = dict(params) if params else {}
     for _ in range(args.warmup):
         c, st = M.triangle_count("tuned", **kw)
@@ -284,7 +293,7 @@ def main():
     m = info["cardinality"]
     alg_bytes = st["alg_bytes"]
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
-    config_tag = f"rmat{args.scale}-ef{args.edge_factor}-s{args.seed}-{args.policy}-tuned"
+    config_tag = f"rmat{args.scale}-ef{args.edge_factor}-s{args.seed}-{chosen}-tuned"
     traffic = traffic_from_profiles(config_tag)
 
     cpu = None
@@ -294,6 +303,9 @@ def main():
         log(f"cpu baseline: {cpu['value']:.3e} edges/s on {cpu['cores']} threads ({cpu['sample']})")
         del off, nbr
     del M, Mp, g
+    policies = {k: {"default_shape_ms": v["default_ms"], "autotuned_ms": v["tuned_ms"], "params": v["params"]}
+                for k, v in cands.items()}
+    cands.clear()
 
     extra = {}
     wanted = [c for c in args.configs.split(",") if c]
@@ -321,7 +333,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"triangle count, RMAT scale {args.scale}, edge factor {args.edge_factor}, "
                                f"a/b/c/d .57/.19/.19/.05, seed {args.seed}, degree-oriented",
-                   "layout_policy": args.policy, "schedule": "tuned", "tuned_params": params,
+                   "layout_policy": chosen, "schedule": "tuned", "tuned_params": params,
                    "n": gi["n"], "oriented_edges": m, "wedges": info["wedges"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
@@ -331,8 +343,7 @@ def main():
         "plain": {"policy": "hybrid", "schedule": "plain", "kernel_ms": plain_ms,
                   "edges_per_s": m / (plain_ms * 1e-3)},
         "speedup_vs_plain": plain_ms / kernel_ms,
-        "tuned_hybrid_kernel_ms": tuned_hybrid_ms,
-        "learned_vs_density_rule": tuned_hybrid_ms / kernel_ms,
+        "policies": policies,
         "alg_bytes": alg_bytes,
         "rows_by_type": info["rows_by_type"],
         "build_ms": info["build_ms"],

@@ -90,11 +90,13 @@ _pp = C.POINTER(C.c_void_p)
 
 
 def build(verbose=False):
-    """Compile libeh.so for gfx950 (hipcc; cross-compiles without a GPU)."""
-    cmd = ["make", "-C", os.path.join(HERE, "csrc"), f"-j{min(os.cpu_count() or 4, 16)}"]
-    r = subprocess.run(cmd, capture_output=not verbose, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("libeh build failed:\n" + (r.stdout or "")[-4000:] + (r.stderr or "")[-4000:])
+    """Compile libeh.so for gfx950 (hipcc; cross-compiles without a GPU) and the C++ apps."""
+    jobs = f"-j{min(os.cpu_count() or 4, 16)}"
+    # the device library, then the host C++ apps and tests that link it
+    for sub in ("csrc", "apps"):
+        r = subprocess.run(["make", "-C", os.path.join(HERE, sub), jobs], capture_output=not verbose, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{sub} build failed:\n" + (r.stdout or "")[-4000:] + (r.stderr or "")[-4000:])
 
 
 def lib():

@@ -997,15 +997,15 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
       const uint64_t nb = META_WORDS + rep * (2ull * anw + dA);
       const uint64_t nh = META_WORDS + rep * (cap + dA);
       const uint64_t nl = META_WORDS + rep * dA;
-      // O(1) membership (bitmap or hash) first, in the small slice, then in the
-      // large one; binary search only when neither fits
+      // everything that can stays in the small slice (the large-LDS class runs at
+      // a fraction of the occupancy): O(1) membership first, then a staged list,
+      // then binary search in HBM for triangles
       if (nb <= s.lds_small && 2ull * anw <= 4ull * cap) { kind = KIND_BITMAP; need = nb; }
       else if (nh <= s.lds_small) { kind = KIND_HASH; need = nh; }
       else if (nb <= s.lds_small) { kind = KIND_BITMAP; need = nb; }
-      else if (nh <= s.lds_big) { kind = KIND_HASH; need = nh; }
-      else if (nb <= s.lds_big) { kind = KIND_BITMAP; need = nb; }
       else if (nl <= s.lds_small) { kind = KIND_LIST; need = nl; }
       else if (s.k == 3) { kind = KIND_GLOBAL; need = 0; }
+      else if (nh <= s.lds_big) { kind = KIND_HASH; need = nh; }
       else { kind = KIND_LIST; need = nl; }
     } else {
       if (ta == EH_BS) { kind = KIND_BITMAP; need = need_bitmap; }
@@ -1081,8 +1081,8 @@ struct SchedStats {
 TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts) {
   TuneParams tp;
   tp.lds_small = 2048;
-  tp.lds_big = sched == EH_SCHED_TUNED ? 8192 : 16384;
-  tp.big_waves = sched == EH_SCHED_TUNED ? 2 : 1;
+  tp.lds_big = 16384;
+  tp.big_waves = 1;
   tp.waves_per_block = sched == EH_SCHED_TUNED ? 2 : 4;
   tp.lanes = 64;
   if (sched == EH_SCHED_TUNED) {

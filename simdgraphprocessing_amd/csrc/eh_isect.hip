@@ -207,7 +207,8 @@ k_isect_rows(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr, c
       const uint32_t t = a; a = b; b = t;
       const uint32_t u = na; na = nb; nb = u;
     }
-    const uint32_t *S = unbr + aoff[a], *L = unbr + aoff[b];
+    // row offsets count 16-byte units of the padded uint arena
+    const uint32_t *S = unbr + (uint64_t)aoff[a] * 4, *L = unbr + (uint64_t)aoff[b] * 4;
     uint32_t hits = 0;
     for (uint32_t k = lane; k < na; k += 64) hits += list_contains(L, nb, S[k]) ? 1u : 0u;
     for (int o = 32; o; o >>= 1) hits += __shfl_xor(hits, o, 64);
