@@ -206,7 +206,9 @@ eh_status eh_intersect_rows(eh_matrix *m, const uint32_t *a_rows, const uint32_t
  * sum over kept (i,j) of |row(i) & row(j)|. */
 eh_status eh_triangle_count(eh_matrix *m, const eh_run_opts *opts, uint64_t *count,
                             eh_run_stats *stats);
-/* n_clique<T,R>::run (apps/n_clique.cpp:52-101), query_depth = k (3 or 4). */
+/* n_clique<T,R>::run (apps/n_clique.cpp:52-101), query_depth = k, 3 <= k <= 16:
+ * k = 3, 4 run the tuned/plain schedules of eh_run_opts; k >= 5 runs the general
+ * level-set recursion (opts are not used). */
 eh_status eh_clique_count(eh_matrix *m, uint32_t k, const eh_run_opts *opts, uint64_t *count,
                           eh_run_stats *stats);
 /* No reference counterpart (the reference times whole phases only,

@@ -1,6 +1,6 @@
 // n_clique -- the app of apps/n_clique.cpp:1-107 on the device: the matrix is built
 // with the orientation nbr < node and the recursion of apply_function down to
-// query_depth (3 or 4) runs as one counting pass.
+// query_depth (3 to 16) runs as one counting pass.
 #define WRITE_VECTOR 1
 
 #include "emptyheaded/emptyheaded.hpp"

@@ -102,66 +102,76 @@ struct TypeArgs {
 };
 
 __global__ void k_select(TypeArgs a) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < a.n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t c = a.card[i];
-    const uint32_t *r = a.unbr + a.poff[i];
-    uint32_t lo = c ? r[0] : 0, hi = c ? r[c - 1] : 0;
-    const uint32_t nw = c ? (hi >> 6) - (lo >> 6) + 1u : 0u;
-    uint32_t parts = 0;
-    const bool need_parts = a.policy == EH_LAYOUT_PSHORT || a.policy == EH_LAYOUT_HYBRID3 ||
-                            a.policy == EH_LAYOUT_COST || a.policy == EH_LAYOUT_LEARNED;
-    if (c && need_parts) {
-      parts = 1;
-      for (uint32_t k = 1; k < c; k++) parts += (r[k] >> 16) != (r[k - 1] >> 16);
-    }
-    uint32_t t = EH_U;
-    switch (a.policy) {
-      case EH_LAYOUT_UINT: t = EH_U; break;
-      case EH_LAYOUT_PSHORT: t = EH_PS; break;
-      case EH_LAYOUT_BITSET: t = EH_BS; break;
-      case EH_LAYOUT_HYBRID:
-      case EH_LAYOUT_HYBRID3: {
-        // hybrid::get_type: rows of length <= 1 or zero range stay uint
-        t = EH_U;
-        if (c > 1 && hi - lo > 0) {
-          const double density = (double)c / (double)(hi - lo);
-          if (a.policy == EH_LAYOUT_HYBRID) {
-            t = density > (1.0 / 256.0) ? EH_BS : EH_U;
-          } else {
-            t = density > (1.0 / 32.0) ? EH_BS : (density > (1.0 / 256.0) ? EH_PS : EH_U);
+  // wave-uniform grid stride, so the per-type row counts are added once per wave
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < a.n; base += stride) {
+    const uint64_t i = base + (threadIdx.x & 63u);
+    const bool valid = i < a.n;
+    uint32_t counted = 3u;  // type index this lane adds to type_rows (3 = none)
+    if (valid) {
+      const uint32_t c = a.card[i];
+      const uint32_t *r = a.unbr + a.poff[i];
+      uint32_t lo = c ? r[0] : 0, hi = c ? r[c - 1] : 0;
+      const uint32_t nw = c ? (hi >> 6) - (lo >> 6) + 1u : 0u;
+      uint32_t parts = 0;
+      const bool need_parts = a.policy == EH_LAYOUT_PSHORT || a.policy == EH_LAYOUT_HYBRID3 ||
+                              a.policy == EH_LAYOUT_COST || a.policy == EH_LAYOUT_LEARNED;
+      if (c && need_parts) {
+        parts = 1;
+        for (uint32_t k = 1; k < c; k++) parts += (r[k] >> 16) != (r[k - 1] >> 16);
+      }
+      uint32_t t = EH_U;
+      switch (a.policy) {
+        case EH_LAYOUT_UINT: t = EH_U; break;
+        case EH_LAYOUT_PSHORT: t = EH_PS; break;
+        case EH_LAYOUT_BITSET: t = EH_BS; break;
+        case EH_LAYOUT_HYBRID:
+        case EH_LAYOUT_HYBRID3: {
+          // hybrid::get_type: rows of length <= 1 or zero range stay uint
+          t = EH_U;
+          if (c > 1 && hi - lo > 0) {
+            const double density = (double)c / (double)(hi - lo);
+            if (a.policy == EH_LAYOUT_HYBRID) {
+              t = density > (1.0 / 256.0) ? EH_BS : EH_U;
+            } else {
+              t = density > (1.0 / 32.0) ? EH_BS : (density > (1.0 / 256.0) ? EH_PS : EH_U);
+            }
           }
+          break;
         }
-        break;
+        case EH_LAYOUT_COST: {
+          // fewest B-side bytes: uint 4c, pshort 2c + 4 parts, bitset 8 words
+          const uint64_t bu = 4ull * c, bp = 2ull * c + 4ull * parts, bb = 8ull * nw;
+          t = EH_U;
+          uint64_t best = bu;
+          if (c > 1 && bp < best) { t = EH_PS; best = bp; }
+          if (c > 1 && bb < best) { t = EH_BS; best = bb; }
+          break;
+        }
+        case EH_LAYOUT_LEARNED:
+          t = a.forced ? a.forced[i] : EH_U;
+          if (c <= 1) t = EH_U;
+          break;
+        default: t = EH_U;
       }
-      case EH_LAYOUT_COST: {
-        // fewest B-side bytes: uint 4c, pshort 2c + 4 parts, bitset 8 words
-        const uint64_t bu = 4ull * c, bp = 2ull * c + 4ull * parts, bb = 8ull * nw;
-        t = EH_U;
-        uint64_t best = bu;
-        if (c > 1 && bp < best) { t = EH_PS; best = bp; }
-        if (c > 1 && bb < best) { t = EH_BS; best = bb; }
-        break;
-      }
-      case EH_LAYOUT_LEARNED:
-        t = a.forced ? a.forced[i] : EH_U;
-        if (c <= 1) t = EH_U;
-        break;
-      default: t = EH_U;
+      a.type[i] = (uint8_t)t;
+      a.parts[i] = parts;
+      const uint32_t shorts = c + 2u * parts;
+      a.bs_n[i] = (t == EH_BS && c) ? nw : 0u;
+      a.ps_n[i] = (t == EH_PS && c) ? ((shorts + 3u) & ~3u) : 0u;
+      // reference row image: optional u64 tag, then the flattened sub-layout
+      const bool tagged = a.policy >= EH_LAYOUT_HYBRID;
+      uint64_t bytes = tagged ? 8ull : 0ull;
+      if (is_uint_type(t)) bytes += 4ull * c;
+      else if (t == EH_BS) bytes += c ? 4ull + 8ull + 8ull * nw : 0ull;
+      else bytes += c ? 4ull + 2ull * shorts : 0ull;
+      a.refb[i] = bytes;
+      if (c) counted = t == EH_UG ? EH_U : t;
     }
-    a.type[i] = (uint8_t)t;
-    a.parts[i] = parts;
-    const uint32_t shorts = c + 2u * parts;
-    a.bs_n[i] = (t == EH_BS && c) ? nw : 0u;
-    a.ps_n[i] = (t == EH_PS && c) ? ((shorts + 3u) & ~3u) : 0u;
-    // reference row image: optional u64 tag, then the flattened sub-layout
-    const bool tagged = a.policy >= EH_LAYOUT_HYBRID;
-    uint64_t bytes = tagged ? 8ull : 0ull;
-    if (is_uint_type(t)) bytes += 4ull * c;
-    else if (t == EH_BS) bytes += c ? 4ull + 8ull + 8ull * nw : 0ull;
-    else bytes += c ? 4ull + 2ull * shorts : 0ull;
-    a.refb[i] = bytes;
-    if (c) atomicAdd(&a.type_rows[t == EH_UG ? EH_U : t], 1ull);
+    for (uint32_t q = 0; q < 3; q++) {
+      const uint64_t b = __ballot(counted == q);
+      if ((threadIdx.x & 63u) == 0 && b) atomicAdd(&a.type_rows[q], (unsigned long long)__popcll(b));
+    }
   }
 }
 

@@ -1358,10 +1358,17 @@ extern "C" eh_status eh_triangle_count(eh_matrix *m, const eh_run_opts *opts, ui
 extern "C" eh_status eh_clique_count(eh_matrix *m, uint32_t k, const eh_run_opts *opts, uint64_t *count,
                                      eh_run_stats *stats) {
   if (!m || !count) { eh_set_error("eh_clique_count: null argument"); return EH_ERR_INVALID; }
-  if (k != 3 && k != 4) { eh_set_error("eh_clique_count: k=%u not supported (3 or 4)", k); return EH_ERR_UNSUPPORTED; }
   EH_HIP(hipSetDevice(m->ctx->device));
   if (stats) *stats = eh_run_stats{};
-  return count_run(m, (int)k, opts, nullptr, 1, count, stats, nullptr);
+  if (k == 3 || k == 4) return count_run(m, (int)k, opts, nullptr, 1, count, stats, nullptr);
+  // deeper cliques: the general level-set recursion
+  double ms = 0;
+  EH_CHECK(clique_count_deep(m, k, count, &ms));
+  if (stats) {
+    stats->kernel_ms = ms;
+    stats->schedule = opts ? opts->schedule : EH_SCHED_TUNED;
+  }
+  return EH_OK;
 }
 
 extern "C" eh_status eh_count_profile(eh_matrix *m, uint32_t k, const eh_run_opts *opts, uint32_t *task_rows,

@@ -99,12 +99,34 @@ __global__ void k_edges_to_keys(uint64_t *keys, const uint32_t *src, const uint3
   }
 }
 
+// Keys sorted by their high word: equal highs form runs, so a wave adds each run
+// it holds with one atomic (hub rows would otherwise serialise on one counter).
+// Called by every lane of a wave together; `valid` marks lanes holding a key.
+__device__ __forceinline__ void add_sorted_runs(uint32_t hi, bool valid, uint32_t *cnt) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t prev = __shfl_up(hi, 1, 64);
+  const bool start = valid && (lane == 0 || prev != hi);
+  const uint64_t starts = __ballot(start);
+  const uint64_t valids = __ballot(valid);
+  if (start) {
+    const uint64_t later = lane == 63 ? 0ull : (starts >> (lane + 1));
+    const uint32_t next = later ? lane + 1 + (uint32_t)__builtin_ctzll(later) : 64u;
+    const uint64_t upto = next == 64 ? ~0ull : ((1ull << next) - 1ull);
+    const uint64_t span = upto & ~((1ull << lane) - 1ull);
+    atomicAdd(&cnt[hi], (uint32_t)__popcll(valids & span));
+  }
+}
+
+// degrees from sorted unique (min << 32 | max) keys
 __global__ void k_degree_from_keys(const uint64_t *keys, uint64_t m, uint32_t *deg) {
-  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m;
-       e += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t k = keys[e];
-    atomicAdd(&deg[(uint32_t)(k >> 32)], 1u);
-    atomicAdd(&deg[(uint32_t)k], 1u);
+  // wave-uniform grid stride: every lane of a wave runs the same iterations
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < m; base += stride) {
+    const uint64_t e = base + (threadIdx.x & 63u);
+    const bool valid = e < m;
+    const uint64_t k = valid ? keys[e] : 0ull;
+    add_sorted_runs((uint32_t)(k >> 32), valid, deg);
+    if (valid) atomicAdd(&deg[(uint32_t)k], 1u);
   }
 }
 
@@ -153,10 +175,15 @@ __global__ void k_both_directions(const uint64_t *ukeys, uint64_t m, const uint3
   }
 }
 
+// row lengths from keys sorted by row (row << 32 | nbr)
 __global__ void k_row_counts(const uint64_t *keys, uint64_t m, uint32_t *cnt) {
-  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m;
-       e += (uint64_t)gridDim.x * blockDim.x)
-    atomicAdd(&cnt[(uint32_t)(keys[e] >> 32)], 1u);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < m; base += stride) {
+    const uint64_t e = base + (threadIdx.x & 63u);
+    const bool valid = e < m;
+    const uint64_t k = valid ? keys[e] : 0ull;
+    add_sorted_runs((uint32_t)(k >> 32), valid, cnt);
+  }
 }
 
 __global__ void k_low_words(const uint64_t *keys, uint64_t m, uint32_t *nbr) {

@@ -165,6 +165,9 @@ struct eh_matrix {
 
 // ---------------------------------------------------------------- primitives (eh_prim.hip)
 
+// k-cliques for k >= 3 by the general level-set recursion (eh_clique.hip)
+eh_status clique_count_deep(eh_matrix *m, uint32_t k, uint64_t *count, double *ms);
+
 eh_status prim_sort_u64(eh_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int end_bit);
 eh_status prim_sort_pairs_u32(eh_ctx *ctx, uint32_t *keys, uint32_t *keys_tmp, uint32_t *vals,
                               uint32_t *vals_tmp, uint64_t n);

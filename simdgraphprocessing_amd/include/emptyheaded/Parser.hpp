@@ -39,7 +39,7 @@ inline void printUsage(const std::string &app) {
   std::cout << "\t\t--layout=<uint,pshort,bs,hybrid,hybrid3,cost,learned>" << std::endl;
   std::cout << "\tOPTIONAL: --schedule=<tuned,plain>" << std::endl;
   if (app == "n_clique") {
-    std::cout << "\tOPTIONAL: --n=<size of cliques to find, 3 or 4, default is 4>" << std::endl;
+    std::cout << "\tOPTIONAL: --n=<size of cliques to find, 3 to 16, default is 4>" << std::endl;
     std::cout << "*takes a undirected graph as input*" << std::endl;
   }
   std::exit(0);

@@ -94,3 +94,54 @@ def test_unoriented_matrix(ctx, facebook):
     M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "all", "hybrid")
     for sched in ("plain", "tuned"):
         assert M.triangle_count(sched)[0] == 6 * FB_TRIANGLES
+
+
+# ------------------------------------------------------------------ deeper cliques (query_depth > 4)
+
+@pytest.mark.parametrize("k", [5, 6])
+def test_facebook_deep_cliques(ctx, facebook, k):
+    """n_clique with --n=5, 6 (apps/n_clique.cpp:52-101): the general level-set kernel
+    against the oracle's recursion."""
+    om = O.Matrix(facebook["n"], facebook["off"], facebook["nbr"], "hybrid")
+    exp, _ = om.cliques(k, 8)
+    for policy in ("hybrid", "pshort"):
+        M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "lower", policy)
+        c, st = M.clique_count(k, "tuned")
+        print(f"facebook {k}-cliques {policy}: {c} in {st['kernel_ms']:.1f} ms", flush=True)
+        assert c == exp
+
+
+def test_deep_cliques_closed_forms(ctx):
+    """A hub joined to 5,000 vertices, ten of which form K_10: the k-cliques are those
+    of K_11, C(11, k); the hub's row (5,000 entries) forces the HBM spill of the levels."""
+    from math import comb
+    n = 5001
+    hub = n - 1
+    src = [hub] * (n - 1) + [a for a in range(10) for b in range(a + 1, 10)]
+    dst = list(range(n - 1)) + [b for a in range(10) for b in range(a + 1, 10)]
+    off, nbr = O.edges_to_csr(n, np.array(src), np.array(dst))
+    M = ctx.matrix_from_csr(n, off, nbr, "lower", "hybrid")
+    for k in (5, 6, 8, 11, 12):
+        assert M.clique_count(k, "tuned")[0] == comb(11, k), k
+
+
+def test_deep_cliques_unoriented_and_random(ctx):
+    """EH_FILTER_ALL counts ordered cliques (k! each); random RMAT graphs match the oracle."""
+    from math import factorial
+    src, dst = O.gen_rmat(11, 12, 5)
+    r = O.canonical_oriented(2048, src, dst)
+    _, off, nbr = r["relabel"]
+    om = O.Matrix(2048, off, nbr, "hybrid")
+    M = ctx.matrix_from_csr(2048, off, nbr, "lower", "hybrid")
+    Ma = ctx.matrix_from_csr(2048, off, nbr, "all", "hybrid")
+    for k in (5, 6):
+        exp, _ = om.cliques(k, 8)
+        assert M.clique_count(k, "tuned")[0] == exp
+        assert Ma.clique_count(k, "tuned")[0] == factorial(k) * exp
+
+
+def test_deep_cliques_reject_large_k(ctx, facebook):
+    import simdgraphprocessing_amd as eh
+    M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "lower", "hybrid")
+    with pytest.raises(eh.EHError):
+        M.clique_count(17, "tuned")

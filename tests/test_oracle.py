@@ -329,5 +329,7 @@ def test_oracle_matches_reference_programs(seed):
     k_ref, _ = O.run_ref("n_clique", 2048, off, nbr, threads=4, layout="hybrid", k=4)
     assert om.triangles(4)[0] == t_ref
     assert om.cliques(4, 4)[0] == k_ref
+    # deeper query_depth through the same recursion (apps/n_clique.cpp:30-50)
+    assert om.cliques(5, 4)[0] == O.run_ref("n_clique", 2048, off, nbr, threads=4, layout="hybrid", k=5)[0]
     for layout in ("uint", "bs", "pshort"):
         assert O.run_ref("undirected_triangle_counting", 2048, off, nbr, threads=2, layout=layout)[0] == t_ref
