@@ -102,8 +102,9 @@ typedef struct {
   int32_t schedule;       /* eh_schedule */
   int32_t lanes_per_row;  /* 0 = autotuned/default; else 4,8,16,32,64 */
   uint32_t task_work;     /* 0 = default; target work units per task */
-  uint32_t lds_words;     /* 0 = default; per-wave LDS budget in 32-bit words */
-  int32_t waves_per_block;/* 0 = default (4) */
+  uint32_t lds_words;     /* 0 = default; per-wave LDS budget in 32-bit words (256..16384) */
+  int32_t waves_per_block;/* 0 = default (tuned 2, plain 4); at most 4, and at most
+                             64 KB of LDS slices per block */
   int32_t repeat;         /* run the counting kernel this many times (>=1); stats average */
   int32_t small_degree;   /* TUNED on oriented matrices: rows with at most this many
                              entries are counted edge-centrically; 0 = default, <0 = off */

@@ -534,14 +534,66 @@ __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t *L, uint32_t 
   return lo;
 }
 
+// One element of a pass for a pair with metadata (v0, v1, v2):
+//   PROBE   A's element S.list[v0 + r] tested in B's bitset words (bsw[v1 ..], first word v2)
+//   AND     word v0 + r of A's LDS bitmap AND-ed with B's word (bsw[v1 ..], first word v2)
+//   STREAM  B's elements unbr[4 (v1 + r) .. +4] tested against A's LDS structure
+template <int PASS, uint32_t KIND, bool W>
+__device__ __forceinline__ uint64_t pass_elem(const CountArgs &a, const AState &S, uint32_t v0, uint32_t v1,
+                                              uint32_t v2, uint32_t r) {
+  if constexpr (PASS == PASS_PROBE) {
+    const uint32_t x = S.list[v0 + r];
+    const uint64_t word = a.bsw[v1 + ((x >> 6) - v2)];
+    const uint32_t h = (uint32_t)((word >> (x & 63u)) & 1ull);
+    if constexpr (W) return h ? (uint64_t)a.card[x] : 0ull;
+    else return h;
+  } else if constexpr (PASS == PASS_AND) {
+    const uint32_t w = v0 + r;
+    const uint32_t i2 = 2u * (w - S.aw0);
+    uint64_t v = ((uint64_t)S.bm[i2] | ((uint64_t)S.bm[i2 + 1] << 32)) & a.bsw[v1 + (w - v2)];
+    if constexpr (W) {
+      uint64_t acc = 0;
+      while (v) {
+        acc += a.card[64u * w + (uint32_t)__builtin_ctzll(v)];
+        v &= v - 1;
+      }
+      return acc;
+    } else {
+      return (uint64_t)__popcll(v);
+    }
+  } else {
+    const uint4 x = reinterpret_cast<const uint4 *>(a.unbr)[(uint64_t)v1 + r];
+    return hit_value<KIND, W>(S, a, x.x) + hit_value<KIND, W>(S, a, x.y) + hit_value<KIND, W>(S, a, x.z) +
+           hit_value<KIND, W>(S, a, x.w);
+  }
+}
+
+// One pass over up to 64 pairs (one per lane, c elements each).  Whole rounds of 64
+// elements run pair by pair with the pair's metadata wave-uniform (read from its
+// lane into scalar registers); the remainders (< 64 per pair) are flattened: an
+// exclusive scan gives every remainder element a slot t, lane t % 64 takes it and
+// finds its pair by a 6-step search over the scanned bases.
 template <int PASS, uint32_t KIND, bool W>
 __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &S, uint32_t c, uint32_t p0,
                                               uint32_t p1, uint32_t p2, uint32_t lane, uint32_t *meta) {
   if (__ballot(c != 0u) == 0ull) return 0;
+  uint64_t acc = 0;
+  const uint32_t full = c & ~63u;
+  uint64_t big = __ballot(full != 0u);
+  while (big) {
+    const uint32_t q = (uint32_t)__builtin_ctzll(big);
+    big &= big - 1;
+    const uint32_t fq = rdl(full, q), q0 = rdl(p0, q), q1 = rdl(p1, q), q2 = rdl(p2, q);
+    for (uint32_t r = lane; r < fq; r += 64) acc += pass_elem<PASS, KIND, W>(a, S, q0, q1, q2, r);
+  }
+  // remainders start after the whole rounds: PROBE/AND index by v0, STREAM by v1
+  c -= full;
+  if constexpr (PASS == PASS_STREAM) p1 += full;
+  else p0 += full;
+  if (__ballot(c != 0u) == 0ull) return acc;
   uint32_t tot;
   const uint32_t base = wave_excl_scan(c, lane, &tot);
   tot = rfl(tot);
-  if (tot == 0) return 0;
   uint32_t *mbase = meta, *m0 = meta + 65, *m1 = meta + 130, *m2 = meta + 195;
   mbase[lane] = base;
   m0[lane] = p0;
@@ -549,36 +601,12 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
   m2[lane] = p2;
   if (lane == 0) mbase[64] = tot;
   wave_sync();
-  uint64_t acc = 0;
   for (uint32_t t = lane; t < tot; t += 64) {
     uint32_t q = 0;
 #pragma unroll
     for (uint32_t step = 32; step; step >>= 1)
       if (mbase[q + step] <= t) q += step;
-    const uint32_t r = t - mbase[q];
-    if constexpr (PASS == PASS_PROBE) {
-      const uint32_t x = S.list[m0[q] + r];
-      const uint64_t word = a.bsw[m1[q] + ((x >> 6) - m2[q])];
-      const uint32_t h = (uint32_t)((word >> (x & 63u)) & 1ull);
-      if constexpr (W) acc += h ? (uint64_t)a.card[x] : 0ull;
-      else acc += h;
-    } else if constexpr (PASS == PASS_AND) {
-      const uint32_t w = m0[q] + r;
-      const uint32_t i2 = 2u * (w - S.aw0);
-      uint64_t v = ((uint64_t)S.bm[i2] | ((uint64_t)S.bm[i2 + 1] << 32)) & a.bsw[m1[q] + (w - m2[q])];
-      if constexpr (W) {
-        while (v) {
-          acc += a.card[64u * w + (uint32_t)__builtin_ctzll(v)];
-          v &= v - 1;
-        }
-      } else {
-        acc += (uint64_t)__popcll(v);
-      }
-    } else {
-      const uint4 x = reinterpret_cast<const uint4 *>(a.unbr)[(uint64_t)m1[q] + r];
-      acc += hit_value<KIND, W>(S, a, x.x) + hit_value<KIND, W>(S, a, x.y) + hit_value<KIND, W>(S, a, x.z) +
-             hit_value<KIND, W>(S, a, x.w);
-    }
+    acc += pass_elem<PASS, KIND, W>(a, S, m0[q], m1[q], m2[q], t - mbase[q]);
   }
   wave_sync();
   return acc;
@@ -1102,8 +1130,13 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
     if (opts->waves_per_block > 0) tp.waves_per_block = opts->waves_per_block;
     if (opts->lanes_per_row > 0) tp.lanes = opts->lanes_per_row;
   }
+  // the task kernels are built for at most 256 threads (4 waves) per block, and a
+  // wave's slice holds at most 16384 words (64 KB)
   if (tp.lds_small < 256) tp.lds_small = 256;
-  if ((uint64_t)tp.lds_small * tp.waves_per_block * 4 > 160 * 1024) tp.waves_per_block = 1;
+  if (tp.lds_small > 16384) tp.lds_small = 16384;
+  if (tp.waves_per_block < 1) tp.waves_per_block = 1;
+  if (tp.waves_per_block > 4) tp.waves_per_block = 4;
+  while (tp.waves_per_block > 1 && (uint64_t)tp.lds_small * tp.waves_per_block * 4 > 64 * 1024) tp.waves_per_block--;
   if (tp.lds_big < tp.lds_small) tp.lds_big = tp.lds_small;
   return tp;
 }
