@@ -108,7 +108,8 @@ typedef struct {
   int32_t repeat;         /* run the counting kernel this many times (>=1); stats average */
   int32_t small_degree;   /* TUNED on oriented matrices: rows with at most this many
                              entries are counted edge-centrically; 0 = default, <0 = off */
-  int32_t reserved;
+  int32_t stream_ratio;   /* TUNED: a bitset B row is streamed as its uint list instead of
+                             probed when 16 |B| < stream_ratio * probes; 0 = default, <0 = never */
 } eh_run_opts;
 
 typedef struct {

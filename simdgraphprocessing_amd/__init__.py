@@ -47,7 +47,7 @@ class MatrixInfo(C.Structure):
 class RunOpts(C.Structure):
     _fields_ = [("schedule", C.c_int32), ("lanes_per_row", C.c_int32), ("task_work", C.c_uint32),
                 ("lds_words", C.c_uint32), ("waves_per_block", C.c_int32), ("repeat", C.c_int32),
-                ("small_degree", C.c_int32), ("reserved", C.c_int32)]
+                ("small_degree", C.c_int32), ("stream_ratio", C.c_int32)]
 
 
 class RunStats(C.Structure):
@@ -167,9 +167,11 @@ def _ptr(a):
     return C.c_void_p(a.ctypes.data) if a is not None else None
 
 
-def _opts(schedule="tuned", lanes=0, task_work=0, lds_words=0, waves=0, repeat=1, small_degree=0):
+def _opts(schedule="tuned", lanes=0, task_work=0, lds_words=0, waves=0, repeat=1, small_degree=0,
+          stream_ratio=0):
     o = RunOpts()
     o.small_degree = small_degree
+    o.stream_ratio = stream_ratio
     o.schedule = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
     o.lanes_per_row = lanes
     o.task_work = task_work

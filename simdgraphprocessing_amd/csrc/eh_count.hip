@@ -50,6 +50,8 @@ struct CountArgs {
   const uint32_t *graph_of;         // per-row graph id of batch arenas (null: one graph)
   unsigned long long *task_cycles;  // optional: shader clock cycles spent per task
   uint64_t task_base;               // index of tasks[0] in the whole schedule
+  uint32_t stream_ratio;            // tuned: stream a bitset B row's uint list when
+                                    // 16 * |B| < stream_ratio * probes (0: always probe)
 };
 
 __device__ __forceinline__ uint32_t rfl(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -620,9 +622,17 @@ __device__ uint64_t sum_isect(const CountArgs &a, const AState &S, const uint32_
   for (uint32_t kc = kb; kc < ke; kc += 64) {
     const uint32_t nk = min(64u, ke - kc);
     RowDesc d = {0u, 0u, 0u, 0u};
-    if (lane < nk) d = a.desc[K[kc + lane]];
+    uint32_t key = 0;
+    if (lane < nk) {
+      key = K[kc + lane];
+      d = a.desc[key];
+    }
     const uint32_t t = desc_type(d);
-    uint32_t pc = 0, ps = 0, ac = 0, as = 0, sc = 0, seq = 0;
+    // per pair, the cheapest of: probing A's elements in B's bitset words (PROBE),
+    // AND-ing A's LDS bitmap with B's words (AND), streaming B's uint list against
+    // A's LDS structure (STREAM).  Every row keeps its uint list, so pshort rows and
+    // bitset rows whose probes would outnumber their elements are streamed.
+    uint32_t pc = 0, ps = 0, ac = 0, as = 0, sc = 0, soff = d.off;
     if (d.card) {
       if (t == EH_BS) {
         const uint32_t bw0 = d.w0, bnw = desc_aux(d);
@@ -631,31 +641,26 @@ __device__ uint64_t sum_isect(const CountArgs &a, const AState &S, const uint32_
         const uint32_t e = (xhi > 0xFFFFFFFFull) ? S.n : lds_lower_bound(S.list, S.n, (uint32_t)xhi);
         pc = e - s;
         ps = s;
+        uint64_t best = 16ull * pc;  // costs in units of 1/16 probe
+        if (a.stream_ratio && 16ull * d.card < (uint64_t)a.stream_ratio * pc) {
+          best = (16ull * d.card) / a.stream_ratio;
+          sc = (d.card + 3u) >> 2;
+          soff = a.aoff[key];
+          pc = 0;
+        }
         if constexpr (KIND == KIND_BITMAP) {
           const uint32_t os = max(S.aw0, bw0), oe = min(S.aw0 + S.anw, bw0 + bnw);
           const uint32_t ov = oe > os ? oe - os : 0u;
-          if (ov < pc) { ac = ov; as = os; pc = 0; }
+          if (16ull * ov < best) { ac = ov; as = os; pc = 0; sc = 0; }
         }
-      } else if (is_uint_type(t)) {
-        sc = (d.card + 3u) >> 2;
       } else {
-        seq = 1;
+        sc = (d.card + 3u) >> 2;
+        if (!is_uint_type(t)) soff = a.aoff[key];
       }
     }
     acc += flat_pass<PASS_PROBE, KIND, W>(a, S, pc, ps, d.off, d.w0, lane, meta);
     if constexpr (KIND == KIND_BITMAP) acc += flat_pass<PASS_AND, KIND, W>(a, S, ac, as, d.off, d.w0, lane, meta);
-    acc += flat_pass<PASS_STREAM, KIND, W>(a, S, sc, 0u, d.off, 0u, lane, meta);
-    uint64_t sm = __ballot(seq);
-    while (sm) {
-      const uint32_t q = (uint32_t)__builtin_ctzll(sm);
-      sm &= sm - 1;
-      RowDesc dq;
-      dq.card = rdl(d.card, q);
-      dq.tinfo = rdl(d.tinfo, q);
-      dq.w0 = rdl(d.w0, q);
-      dq.off = rdl(d.off, q);
-      acc += isect_b_pshort<KIND, W>(S, a, dq, lane);
-    }
+    acc += flat_pass<PASS_STREAM, KIND, W>(a, S, sc, 0u, soff, 0u, lane, meta);
   }
   return acc;
 }
@@ -1129,6 +1134,8 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
     if (opts->lds_words) tp.lds_small = opts->lds_words;
     if (opts->waves_per_block > 0) tp.waves_per_block = opts->waves_per_block;
     if (opts->lanes_per_row > 0) tp.lanes = opts->lanes_per_row;
+    if (opts->stream_ratio < 0) tp.stream_ratio = 0;
+    else if (opts->stream_ratio > 0) tp.stream_ratio = (uint32_t)opts->stream_ratio;
   }
   // the task kernels are built for at most 256 threads (4 waves) per block, and a
   // wave's slice holds at most 16384 words (64 KB)
@@ -1293,6 +1300,7 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
   a.counts = reinterpret_cast<unsigned long long *>(d_counts);
   a.tuned = sched == EH_SCHED_TUNED ? 1u : 0u;
   a.graph_of = m->dgraph_of.p ? m->dgraph_of.as<uint32_t>() : nullptr;
+  a.stream_ratio = tp.stream_ratio;
   const bool weighted = k == -3;
   if (k == 4) {
     EH_CHECK((launch_small<4, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2)));

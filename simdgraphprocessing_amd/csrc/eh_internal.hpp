@@ -107,6 +107,7 @@ struct TuneParams {
   int waves_per_block = 4;
   int big_waves = 1;       // waves per block of the large-LDS task class
   uint32_t small_deg = 0;  // rows up to this many entries use the edge-centric kernel
+  uint32_t stream_ratio = 0;  // tuned: stream bitset B rows when 16 |B| < ratio * probes (0: never)
 };
 
 struct eh_ctx {
