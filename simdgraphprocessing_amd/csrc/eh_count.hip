@@ -25,6 +25,8 @@
 //   PS x PS  prefix match, then 16-bit runs        (:829-878)
 //   U x PS   prefix-aligned run search             (:1062-1173)
 //   PS x BS  pshort elements probing bits          (:956-1017)
+#include <type_traits>
+
 #include "eh_internal.hpp"
 
 namespace {
@@ -540,13 +542,19 @@ __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t *L, uint32_t 
 //   PROBE   A's element S.list[v0 + r] tested in B's bitset words (bsw[v1 ..], first word v2)
 //   AND     word v0 + r of A's LDS bitmap AND-ed with B's word (bsw[v1 ..], first word v2)
 //   STREAM  B's elements unbr[4 (v1 + r) .. +4] tested against A's LDS structure
+// a lane's plain count in one pass adds at most 64 per element it visits, far below 2^32;
+// weighted sums (card of every hit) need 64 bits
+template <bool W>
+using PassAcc = typename std::conditional<W, uint64_t, uint32_t>::type;
+
 template <int PASS, uint32_t KIND, bool W>
-__device__ __forceinline__ uint64_t pass_elem(const CountArgs &a, const AState &S, uint32_t v0, uint32_t v1,
-                                              uint32_t v2, uint32_t r) {
+__device__ __forceinline__ PassAcc<W> pass_elem(const CountArgs &a, const AState &S, uint32_t v0, uint32_t v1,
+                                                uint32_t v2, uint32_t r) {
   if constexpr (PASS == PASS_PROBE) {
+    // the 32-bit half of B's 64-bit word that holds bit x
     const uint32_t x = S.list[v0 + r];
-    const uint64_t word = a.bsw[v1 + ((x >> 6) - v2)];
-    const uint32_t h = (uint32_t)((word >> (x & 63u)) & 1ull);
+    const uint64_t w32 = 2ull * (uint64_t)(v1 + ((x >> 6) - v2)) + ((x >> 5) & 1u);
+    const uint32_t h = (reinterpret_cast<const uint32_t *>(a.bsw)[w32] >> (x & 31u)) & 1u;
     if constexpr (W) return h ? (uint64_t)a.card[x] : 0ull;
     else return h;
   } else if constexpr (PASS == PASS_AND) {
@@ -561,12 +569,12 @@ __device__ __forceinline__ uint64_t pass_elem(const CountArgs &a, const AState &
       }
       return acc;
     } else {
-      return (uint64_t)__popcll(v);
+      return (uint32_t)__popcll(v);
     }
   } else {
     const uint4 x = reinterpret_cast<const uint4 *>(a.unbr)[(uint64_t)v1 + r];
-    return hit_value<KIND, W>(S, a, x.x) + hit_value<KIND, W>(S, a, x.y) + hit_value<KIND, W>(S, a, x.z) +
-           hit_value<KIND, W>(S, a, x.w);
+    return (PassAcc<W>)(hit_value<KIND, W>(S, a, x.x) + hit_value<KIND, W>(S, a, x.y) +
+                        hit_value<KIND, W>(S, a, x.z) + hit_value<KIND, W>(S, a, x.w));
   }
 }
 
@@ -575,17 +583,19 @@ __device__ __forceinline__ uint64_t pass_elem(const CountArgs &a, const AState &
 // lane into scalar registers); the remainders (< 64 per pair) are flattened: an
 // exclusive scan gives every remainder element a slot t, lane t % 64 takes it and
 // finds its pair by a 6-step search over the scanned bases.
-template <int PASS, uint32_t KIND, bool W>
+template <int PASS, uint32_t KIND, bool W, int UNROLL>
 __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &S, uint32_t c, uint32_t p0,
                                               uint32_t p1, uint32_t p2, uint32_t lane, uint32_t *meta) {
   if (__ballot(c != 0u) == 0ull) return 0;
-  uint64_t acc = 0;
+  PassAcc<W> acc = 0;
   const uint32_t full = c & ~63u;
   uint64_t big = __ballot(full != 0u);
   while (big) {
     const uint32_t q = (uint32_t)__builtin_ctzll(big);
     big &= big - 1;
     const uint32_t fq = rdl(full, q), q0 = rdl(p0, q), q1 = rdl(p1, q), q2 = rdl(p2, q);
+    // fq is a multiple of 64: unrolled, each lane keeps several loads in flight
+#pragma unroll UNROLL
     for (uint32_t r = lane; r < fq; r += 64) acc += pass_elem<PASS, KIND, W>(a, S, q0, q1, q2, r);
   }
   // remainders start after the whole rounds: PROBE/AND index by v0, STREAM by v1
@@ -615,7 +625,9 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
 }
 
 // sum over keys K[kb..ke) of |S & row(key)|
-template <uint32_t KIND, bool W>
+// UNROLL: depth of the whole-round loops (the triangle pass gains from 4; the
+// 4-clique level sets, with more live state, run best at 1)
+template <uint32_t KIND, bool W, int UNROLL = 1>
 __device__ uint64_t sum_isect(const CountArgs &a, const AState &S, const uint32_t *K, uint32_t kb, uint32_t ke,
                               uint32_t lane, uint32_t *meta) {
   uint64_t acc = 0;
@@ -658,9 +670,10 @@ __device__ uint64_t sum_isect(const CountArgs &a, const AState &S, const uint32_
         if (!is_uint_type(t)) soff = a.aoff[key];
       }
     }
-    acc += flat_pass<PASS_PROBE, KIND, W>(a, S, pc, ps, d.off, d.w0, lane, meta);
-    if constexpr (KIND == KIND_BITMAP) acc += flat_pass<PASS_AND, KIND, W>(a, S, ac, as, d.off, d.w0, lane, meta);
-    acc += flat_pass<PASS_STREAM, KIND, W>(a, S, sc, 0u, soff, 0u, lane, meta);
+    acc += flat_pass<PASS_PROBE, KIND, W, UNROLL>(a, S, pc, ps, d.off, d.w0, lane, meta);
+    if constexpr (KIND == KIND_BITMAP)
+      acc += flat_pass<PASS_AND, KIND, W, UNROLL>(a, S, ac, as, d.off, d.w0, lane, meta);
+    acc += flat_pass<PASS_STREAM, KIND, W, UNROLL>(a, S, sc, 0u, soff, 0u, lane, meta);
   }
   return acc;
 }
@@ -744,9 +757,9 @@ __device__ uint64_t run_task_tuned(const CountArgs &a, uint32_t kind, uint32_t i
   const uint32_t used = stage_tuned(a, kind, i, lane, Ls, A);
   if constexpr (K == 3) {
     switch (kind) {
-      case KIND_BITMAP: return sum_isect<KIND_BITMAP, W>(a, A, A.list, jb, je, lane, meta);
-      case KIND_HASH: return sum_isect<KIND_HASH, W>(a, A, A.list, jb, je, lane, meta);
-      case KIND_LIST: return sum_isect<KIND_LIST, W>(a, A, A.list, jb, je, lane, meta);
+      case KIND_BITMAP: return sum_isect<KIND_BITMAP, W, 4>(a, A, A.list, jb, je, lane, meta);
+      case KIND_HASH: return sum_isect<KIND_HASH, W, 4>(a, A, A.list, jb, je, lane, meta);
+      case KIND_LIST: return sum_isect<KIND_LIST, W, 4>(a, A, A.list, jb, je, lane, meta);
       default: return task_triangles<KIND_GLOBAL, W>(a, A, jb, je, lane);
     }
   } else {
