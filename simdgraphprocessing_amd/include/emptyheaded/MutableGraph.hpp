@@ -1,10 +1,12 @@
 // MutableGraph.hpp -- the host graph the apps start from (src/MutableGraph.hpp:49-80)
 // and its binary loader/writer (src/MutableGraph.hpp:298-360).
 //
-// File format: size_t n, then per node { uint64 external id; size_t degree;
-// uint32 neighbours[degree] }, neighbours sorted and already degree-relabelled
-// (the reference's tools do that offline).  Only undirected binary input is on
-// this path; the edge-list and attribute loaders are not provided.
+// Undirected file format: size_t n, then per node { uint64 external id; size_t
+// degree; uint32 neighbours[degree] }, neighbours sorted and already degree-
+// relabelled (the reference's tools do that offline).  Directed files
+// (src/MutableGraph.hpp:636-714) hold per node the out-neighbours and then the
+// in-neighbours, each as { size_t count; uint32 ids[count] }.  The edge-list and
+// attribute loaders are not provided.
 #pragma once
 
 #include <cstdint>
@@ -48,13 +50,16 @@ struct MutableGraph {
     delete id_map;
   }
 
-  // Host symmetric CSR of the neighbourhoods (row_off: n+1, nbrs: num_edges).
+  // Host CSR of the out-neighbourhoods, followed for a directed graph by the
+  // in-neighbourhoods as rows n..2n-1 (row_off: rows+1 entries).
   void to_csr(std::vector<uint64_t> &row_off, std::vector<uint32_t> &nbrs) const {
-    row_off.assign(num_nodes + 1, 0);
+    const size_t rows = symmetric ? num_nodes : 2 * num_nodes;
+    row_off.assign(rows + 1, 0);
     nbrs.clear();
     nbrs.reserve(num_edges);
-    for (size_t i = 0; i < num_nodes; i++) {
-      const std::vector<uint32_t> &r = *out_neighborhoods->at(i);
+    for (size_t i = 0; i < rows; i++) {
+      const std::vector<uint32_t> &r =
+          i < num_nodes ? *out_neighborhoods->at(i) : *in_neighborhoods->at(i - num_nodes);
       nbrs.insert(nbrs.end(), r.begin(), r.end());
       row_off[i + 1] = nbrs.size();
     }
@@ -75,6 +80,42 @@ struct MutableGraph {
       out.write(reinterpret_cast<const char *>(&d), sizeof(d));
       out.write(reinterpret_cast<const char *>(r.data()), (std::streamsize)(sizeof(uint32_t) * d));
     }
+  }
+
+  static MutableGraph *directedFromBinary(const std::string &path) {
+    std::ifstream in(path, std::ios::binary | std::ios::in);
+    if (!in) {
+      std::cerr << "ERROR: cannot open " << path << std::endl;
+      std::exit(1);
+    }
+    auto *id_map = new std::vector<uint64_t>();
+    auto *outs = new std::vector<std::vector<uint32_t> *>();
+    auto *ins = new std::vector<std::vector<uint32_t> *>();
+    size_t n = 0, edges = 0, maxd = 0;
+    in.read(reinterpret_cast<char *>(&n), sizeof(n));
+    for (size_t i = 0; in && i < n; i++) {
+      uint64_t ext = 0;
+      in.read(reinterpret_cast<char *>(&ext), sizeof(ext));
+      id_map->push_back(ext);
+      for (auto *side : {outs, ins}) {
+        size_t d = 0;
+        in.read(reinterpret_cast<char *>(&d), sizeof(d));
+        if (!in || d > n) {
+          std::cerr << "ERROR: malformed graph file " << path << " at node " << i << std::endl;
+          std::exit(1);
+        }
+        auto *row = new std::vector<uint32_t>(d);
+        in.read(reinterpret_cast<char *>(row->data()), (std::streamsize)(sizeof(uint32_t) * d));
+        side->push_back(row);
+        edges += d;
+        if (d > maxd) maxd = d;
+      }
+    }
+    if (!in || outs->size() != n) {
+      std::cerr << "ERROR: truncated graph file " << path << std::endl;
+      std::exit(1);
+    }
+    return new MutableGraph(n, edges, maxd, false, id_map, nullptr, outs, ins, nullptr, nullptr);
   }
 
   static MutableGraph *undirectedFromBinary(const std::string &path) {

@@ -10,8 +10,9 @@
 //
 // Rows live in HBM.  get_row(i) returns a Set<T> over a host copy of row i in the
 // reference's byte format, fetched on first use and kept for the matrix's life;
-// row_arrays[i] is that copy (null until fetched).  Unlike the reference, the
-// destructor frees everything it owns.
+// row_arrays[i] is that copy (null until fetched).  A directed graph's matrix also
+// holds the in-neighbourhoods (get_column, src/SparseMatrix.hpp:162-165) as device
+// rows n..2n-1.  Unlike the reference, the destructor frees everything it owns.
 #pragma once
 
 #include <atomic>
@@ -45,31 +46,28 @@ class SparseMatrix {
 
   static SparseMatrix *build(MutableGraph *g, const NodeSel &node_selection, const EdgeSel &edge_selection,
                              size_t num_threads) {
-    if (!g->symmetric) {
-      std::cerr << "ERROR: directed graphs are not supported by this engine" << std::endl;
-      std::exit(1);
-    }
     std::vector<uint64_t> off;
     std::vector<uint32_t> nbr;
     g->to_csr(off, nbr);
-    const size_t n = g->num_nodes;
+    const size_t n = g->num_nodes, rows = off.size() - 1;  // rows = 2n for a directed graph
     std::vector<uint8_t> mask(nbr.size());
-    std::vector<uint32_t> lengths(n);
-    std::atomic<bool> all_kept(true), lower(true);
+    std::vector<uint32_t> lengths(rows);
+    std::atomic<bool> all_kept(true), lower(g->symmetric);
     common::par_for_range(
-        num_threads ? num_threads : 1, 0, n, 256, [](size_t) {},
-        [&](size_t, size_t i) {
+        num_threads ? num_threads : 1, 0, rows, 256, [](size_t) {},
+        [&](size_t, size_t r) {
+          const uint32_t i = (uint32_t)(r < n ? r : r - n);
           uint32_t kept = 0;
           bool a = true, l = true;
-          for (uint64_t k = off[i]; k < off[i + 1]; k++) {
+          for (uint64_t k = off[r]; k < off[r + 1]; k++) {
             const uint32_t j = nbr[k];
-            const bool keep = node_selection(j, 0) && edge_selection((uint32_t)i, j, 0);
+            const bool keep = node_selection(j, 0) && edge_selection(i, j, 0);
             mask[k] = keep ? 1 : 0;
             kept += keep;
             a = a && keep;
-            l = l && (keep == (j < (uint32_t)i));
+            l = l && (keep == (j < i));
           }
-          lengths[i] = kept;
+          lengths[r] = kept;
           if (!a) all_kept.store(false, std::memory_order_relaxed);
           if (!l) lower.store(false, std::memory_order_relaxed);
         },
@@ -80,13 +78,16 @@ class SparseMatrix {
     eh_matrix_info info;
     {
       std::lock_guard<std::mutex> lock(rt.mutex());
-      emptyheaded::check(eh_matrix_build_host(rt.ctx(), (uint32_t)n, off.data(), nbr.data(), filter,
+      emptyheaded::check(eh_matrix_build_host(rt.ctx(), (uint32_t)rows, off.data(), nbr.data(), filter,
                                               filter == EH_FILTER_MASK ? mask.data() : nullptr, T::policy, &m),
                          "SparseMatrix::build");
       emptyheaded::check(eh_matrix_info_get(m, &info), "SparseMatrix::build");
     }
     auto *sm = new SparseMatrix(m, info, g);
-    std::copy(lengths.begin(), lengths.end(), sm->row_lengths);
+    std::copy(lengths.begin(), lengths.begin() + n, sm->row_lengths);
+    if (!g->symmetric) {
+      std::cout << "Number of rows + columns: " << rows << std::endl;
+    }
     std::cout << "Number of nodes: " << sm->matrix_size << std::endl;
     std::cout << "Number of edges: " << sm->cardinality << std::endl;
     std::cout << "ROW DATA SIZE (Bytes): " << sm->row_total_bytes_used << std::endl;
@@ -96,7 +97,7 @@ class SparseMatrix {
 
   ~SparseMatrix() {
     eh_matrix_free(handle);
-    for (size_t i = 0; i < matrix_size; i++) delete[] row_arrays[i];
+    for (size_t i = 0; i < device_rows_; i++) delete[] rows_[i].bytes;
     delete[] row_arrays;
     delete[] row_lengths;
   }
@@ -104,6 +105,36 @@ class SparseMatrix {
   Set<T> get_row(uint32_t row) const {
     const RowCopy &rc = fetch(row);
     return Set<T>(rc.bytes, rc.cardinality, rc.number_of_bytes, rc.type);
+  }
+
+  // in-neighbours of a directed graph; the row itself for an undirected one
+  Set<T> get_column(uint32_t column) const {
+    const RowCopy &rc = fetch(symmetric ? column : (uint32_t)(matrix_size + column));
+    return Set<T>(rc.bytes, rc.cardinality, rc.number_of_bytes, rc.type);
+  }
+
+  // the first row of maximum length (src/SparseMatrix.hpp:134-145)
+  uint32_t get_max_row_id() const {
+    size_t best = 0;
+    uint32_t id = 0;
+    for (size_t i = 0; i < matrix_size; i++)
+      if (row_lengths[i] > best) {
+        best = row_lengths[i];
+        id = (uint32_t)i;
+      }
+    std::cout << "MAXIMUM NODE: " << id_map[id] << " INTERNAL ID: " << id << std::endl;
+    return id;
+  }
+
+  // |row(a[p]) & row(b[p])| for many pairs at once on the device (columns of a
+  // directed graph are rows matrix_size + i)
+  std::vector<uint64_t> intersect_rows(const std::vector<uint32_t> &a, const std::vector<uint32_t> &b) const {
+    std::vector<uint64_t> out(a.size());
+    if (a.empty()) return out;
+    emptyheaded::Runtime &rt = emptyheaded::Runtime::get();
+    std::lock_guard<std::mutex> lock(rt.mutex());
+    emptyheaded::check(eh_intersect_rows(handle, a.data(), b.data(), a.size(), out.data()), "intersect_rows");
+    return out;
   }
 
   // with the reference's COMPRESSION 0 the decode buffer is unused (SparseMatrix.hpp:170-183)
@@ -145,10 +176,11 @@ class SparseMatrix {
   };
 
   SparseMatrix(eh_matrix *m, const eh_matrix_info &info, MutableGraph *g)
-      : matrix_size(info.n), cardinality(info.cardinality), row_total_bytes_used(info.ref_row_bytes),
-        col_total_bytes_used(0), max_nbrhood_size(g->max_nbrhood_size), symmetric(true),
-        row_lengths(new uint32_t[info.n]), row_arrays(new uint8_t *[info.n]()), id_map(g->id_map->data()),
-        handle(m), build_seconds(info.build_ms * 1e-3), rows_(new RowCopy[info.n]) {}
+      : matrix_size(g->num_nodes), cardinality(info.cardinality), row_total_bytes_used(info.ref_row_bytes),
+        col_total_bytes_used(0), max_nbrhood_size(g->max_nbrhood_size), symmetric(g->symmetric),
+        row_lengths(new uint32_t[g->num_nodes]), row_arrays(new uint8_t *[g->num_nodes]()),
+        id_map(g->id_map->data()), handle(m), build_seconds(info.build_ms * 1e-3), device_rows_(info.n),
+        rows_(new RowCopy[info.n]) {}
 
   const RowCopy &fetch(uint32_t row) const {
     RowCopy &rc = rows_[row];
@@ -165,10 +197,11 @@ class SparseMatrix {
     rc.cardinality = v.cardinality;
     rc.number_of_bytes = v.number_of_bytes;
     rc.type = (common::type)v.type;
-    row_arrays[row] = buf;
+    if (row < matrix_size) row_arrays[row] = buf;
     rc.ready.store(true, std::memory_order_release);
     return rc;
   }
 
+  size_t device_rows_;
   std::unique_ptr<RowCopy[]> rows_;
 };

@@ -62,3 +62,63 @@ def test_cli_learned_layout(fb_path):
                        capture_output=True, text=True, timeout=110, env=env)
     assert r.returncode == 0, r.stderr
     assert "Number of triangles: 1612010" in r.stdout
+
+
+# ------------------------------------------------------------------ the other set_intersect apps
+
+@pytest.fixture(scope="module")
+def dfb_path(tmp_path_factory):
+    """test/data/dfacebook.bin, the directed fixture of test/symbiosity_test.cpp:8."""
+    p = tmp_path_factory.mktemp("dfbcpp") / "dfacebook.bin"
+    with gzip.open(os.path.join(ROOT, "tests", "golden", "dfacebook.bin.gz"), "rb") as f:
+        p.write_bytes(f.read())
+    return str(p)
+
+
+@pytest.mark.gpu
+def test_similar_nodes_fixture(fb_path):
+    """test/similar_nodes_test.cpp:11 -> 904, under four layouts."""
+    rc, out, err = run(["test_similar_nodes", "similar_nodes", fb_path])
+    assert rc == 0 and "PASSED" in out, out[-2000:] + err[-2000:]
+
+
+@pytest.mark.gpu
+def test_symbiosity_fixtures(fb_path, dfb_path):
+    """test/symbiosity_test.cpp:12,19 -> 0 on the directed graph, 4039 on the undirected one."""
+    rc, out, err = run(["test_symbiosity", "symbiosity", fb_path])
+    assert rc == 0 and "PASSED" in out, out[-2000:] + err[-2000:]
+    rc, out, err = run(["test_symbiosity", "symbiosity_directed", dfb_path])
+    assert rc == 0 and "PASSED" in out, out[-2000:] + err[-2000:]
+
+
+@pytest.mark.gpu
+def test_lollipop_fixture(fb_path):
+    """test/undirected_lollipop_counting_test.cpp:11 -> 713,455,740."""
+    rc, out, err = run(["test_lollipop", "lollipop", fb_path])
+    assert rc == 0 and "PASSED" in out, out[-2000:] + err[-2000:]
+
+
+@pytest.mark.gpu
+def test_more_apps_random_graph(tmp_path):
+    """similar_nodes and lollipop on a random graph against a direct numpy count."""
+    import numpy as np
+    import oracle as O
+    n = 400
+    src, dst = O.gen_er(n, 6000, 9)
+    r = O.canonical_oriented(n, src, dst)
+    _, off, nbr = r["relabel"]
+    path = str(tmp_path / "g.bin")
+    O.write_binary(path, n, off, nbr)
+    adj = np.zeros((n, n), np.int64)
+    for i in range(n):
+        adj[i, nbr[off[i]:off[i + 1]]] = 1
+    deg = adj.sum(1)
+    a = int(np.argmax(deg))
+    similar = int(((adj @ adj[a]) > 10).sum())
+    tri_through = np.diag(adj @ adj @ adj) // 2
+    lollipops = int((deg * tri_through).sum())
+    args = [f"--graph={path}", "--input_type=binary", "--t=4", "--layout=hybrid"]
+    rc, out, err = run(["similar_nodes"] + args)
+    assert rc == 0 and f"Count: {similar}" in out, out[-1500:] + err[-1500:]
+    rc, out, err = run(["undirected_lollipop_counting"] + args)
+    assert rc == 0 and f"Count: {lollipops}" in out, out[-1500:] + err[-1500:]

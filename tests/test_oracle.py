@@ -333,3 +333,50 @@ def test_oracle_matches_reference_programs(seed):
     assert om.cliques(5, 4)[0] == O.run_ref("n_clique", 2048, off, nbr, threads=4, layout="hybrid", k=5)[0]
     for layout in ("uint", "bs", "pshort"):
         assert O.run_ref("undirected_triangle_counting", 2048, off, nbr, threads=2, layout=layout)[0] == t_ref
+
+
+# ------------------------------------------------------------------ the other set_intersect apps
+
+def test_more_app_fixtures_by_direct_count(facebook):
+    """The counts the C++ apps compute on the device follow from the graph alone; the
+    reference's fixture answers (test/similar_nodes_test.cpp:11 -> 904,
+    test/undirected_lollipop_counting_test.cpp:11 -> 713,455,740,
+    test/symbiosity_test.cpp:19 -> 4039) are reproduced here by dense counting, which
+    pins the closed form the lollipop app uses: sum over x of deg(x) * t(x)."""
+    n, off, nbr = facebook["n"], facebook["off"], facebook["nbr"]
+    adj = np.zeros((n, n), np.float64)
+    for i in range(n):
+        adj[i, nbr[off[i]:off[i + 1]]] = 1.0
+    deg = adj.sum(1)
+    a = int(np.argmax(deg))  # the first row of maximum length
+    assert int(((adj @ adj[a]) > 10).sum()) == 904
+    tri_through = ((adj @ adj) * adj).sum(1) / 2
+    assert int(round(float((deg * tri_through).sum()))) == 713455740
+    both = (adj * adj.T).sum(1)  # row & column of a symmetric matrix = the row
+    assert int((both / (deg + deg - both) > 0.5).sum()) == 4039
+
+
+def test_directed_fixture_symbiosity():
+    """test/symbiosity_test.cpp:12 -> 0 on test/data/dfacebook.bin (out- and in-rows)."""
+    import gzip
+    import os
+    raw = gzip.open(os.path.join(os.path.dirname(__file__), "golden", "dfacebook.bin.gz"), "rb").read()
+    pos = 0
+
+    def take(count, dtype):
+        nonlocal pos
+        arr = np.frombuffer(raw, dtype, count, pos)
+        pos += arr.nbytes
+        return arr
+
+    n = int(take(1, np.uint64)[0])
+    zero = 0
+    for _ in range(n):
+        take(1, np.uint64)
+        out = take(int(take(1, np.uint64)[0]), np.uint32)
+        inn = take(int(take(1, np.uint64)[0]), np.uint32)
+        both = len(np.intersect1d(out, inn))
+        union = len(out) + len(inn) - both
+        zero += union > 0 and both / union > 0.5
+    assert pos == len(raw)
+    assert zero == 0
