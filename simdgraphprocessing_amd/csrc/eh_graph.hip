@@ -413,6 +413,7 @@ extern "C" eh_status eh_graph_generate(eh_ctx *ctx, const eh_gen_spec *spec, eh_
 extern "C" eh_status eh_graph_from_edges(eh_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *src,
                                          const uint32_t *dst, int relabel, eh_graph **out) {
   if (!ctx || !out || (m && (!src || !dst))) { eh_set_error("eh_graph_from_edges: null argument"); return EH_ERR_INVALID; }
+  if (n == 0) { eh_set_error("eh_graph_from_edges: a graph needs at least one vertex"); return EH_ERR_INVALID; }
   EH_HIP(hipSetDevice(ctx->device));
   DevBuf ds, dd, keys;
   EH_CHECK(ds.alloc(4 * m + 4));
@@ -431,6 +432,7 @@ extern "C" eh_status eh_graph_from_edges(eh_ctx *ctx, uint32_t n, uint64_t m, co
 extern "C" eh_status eh_graph_from_csr(eh_ctx *ctx, uint32_t n, const uint64_t *row_off,
                                        const uint32_t *nbrs, eh_graph **out) {
   if (!ctx || !out || !row_off) { eh_set_error("eh_graph_from_csr: null argument"); return EH_ERR_INVALID; }
+  if (n == 0) { eh_set_error("eh_graph_from_csr: a graph needs at least one vertex"); return EH_ERR_INVALID; }
   if (row_off[0] != 0) { eh_set_error("row_off[0] must be 0"); return EH_ERR_INVALID; }
   for (uint32_t i = 0; i < n; i++)
     if (row_off[i + 1] < row_off[i]) { eh_set_error("row_off not monotone at %u", i); return EH_ERR_INVALID; }
