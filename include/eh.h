@@ -163,6 +163,21 @@ eh_status eh_graph_from_edges(eh_ctx *ctx, uint32_t n, uint64_t m, const uint32_
 eh_status eh_graph_generate(eh_ctx *ctx, const eh_gen_spec *spec, eh_graph **out);
 /* MutableGraph::reorder_by_degree (src/MutableGraph.hpp:20-32,100-123,268-272). */
 eh_status eh_graph_relabel_by_degree(eh_graph *g);
+/* The orderings of tools/undirectedEdgeListToBinary.cpp:26-32 (MutableGraph::reorder_*,
+ * src/MutableGraph.hpp:126-285) as device passes.  The reference draws its shuffles
+ * with std::random_shuffle; here `seed` drives counter-based permutations, so BFS,
+ * SHINGLES, RANDOM and THE_GAME are reproducible.  Ties the reference leaves to an
+ * unstable sort keep id order; STRONG_RUN places vertices that appear in no row last. */
+typedef enum {
+  EH_ORDER_DEGREE = 0,     /* reorder_by_degree: degree desc, id asc       */
+  EH_ORDER_REV_DEGREE = 1, /* reorder_by_rev_degree: degree asc            */
+  EH_ORDER_STRONG_RUN = 2, /* reorder_strong_run                           */
+  EH_ORDER_BFS = 3,        /* reorder_bfs                                  */
+  EH_ORDER_SHINGLES = 4,   /* reorder_by_shingles                          */
+  EH_ORDER_RANDOM = 5,     /* reorder_random                               */
+  EH_ORDER_THE_GAME = 6    /* reorder_by_the_game: BFS, then degree        */
+} eh_ordering;
+eh_status eh_graph_reorder(eh_graph *g, eh_ordering ordering, uint64_t seed);
 eh_status eh_graph_info_get(const eh_graph *g, eh_graph_info *out);
 /* copy the symmetric CSR to host (row_off: n+1 entries, nbrs: entries) */
 eh_status eh_graph_download(const eh_graph *g, uint64_t *row_off, uint32_t *nbrs);

@@ -57,6 +57,10 @@ def lib():
     L.orc_row_type.argtypes = [_u32p, C.c_size_t, C.c_int]
     L.orc_row_image.argtypes = [_u8p, _u32p, C.c_size_t, C.c_int]
     L.orc_row_image.restype = C.c_size_t
+    L.orc_perm.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, _u32p]
+    L.orc_perm.restype = None
+    L.orc_order.argtypes = [C.c_uint32, _u64p, _u32p, C.c_int, C.c_uint64, _u32p]
+    L.orc_order.restype = C.c_int
     L.orc_row_image_bound.argtypes = [_u32p, C.c_size_t]
     L.orc_row_image_bound.restype = C.c_size_t
     for name in ("orc_uint_build", "orc_pshort_build", "orc_bitset_build"):
@@ -473,6 +477,76 @@ def mlp_loss_grad(params, x, target, mask):
 
 
 # ---------------------------------------------------------------- reference programs
+
+ORDERINGS = {"degree": 0, "rev_degree": 1, "strong_run": 2, "bfs": 3, "shingles": 4, "random": 5, "the_game": 6}
+
+
+def perm(seed, stream, n):
+    out = np.zeros(max(n, 1), np.uint32)
+    lib().orc_perm(seed, stream, n, out)
+    return out[:n]
+
+
+def order(n, off, nbr, ordering, seed=0):
+    """new2old of one of the reference's vertex orderings (MutableGraph::reorder_*)."""
+    out = np.zeros(max(n, 1), np.uint32)
+    rc = lib().orc_order(n, _u64(off), _u32(nbr), ORDERINGS[ordering], seed, out)
+    if rc != 0:
+        raise ValueError(ordering)
+    return out[:n]
+
+
+def apply_order(n, off, nbr, new2old):
+    """Renumber a symmetric CSR by new2old; rows re-sorted."""
+    off = np.asarray(off, np.int64)
+    nbr = np.asarray(nbr, np.int64)
+    o2n = np.empty(n, np.int64)
+    o2n[np.asarray(new2old, np.int64)] = np.arange(n)
+    deg = np.diff(off)[np.asarray(new2old, np.int64)]
+    noff = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    rows = [np.sort(o2n[nbr[off[v]:off[v + 1]]]) for v in np.asarray(new2old, np.int64)]
+    nnbr = np.concatenate(rows).astype(np.uint32) if rows and len(nbr) else np.zeros(0, np.uint32)
+    return noff, nnbr
+
+
+def edge_list_graph(lines):
+    """The reference's edge-list loader (src/MutableGraph.hpp:517-598): ids in order of
+    first appearance, both directions, rows sorted and deduplicated."""
+    ext, rows = {}, []
+    for a, b in lines:
+        for x in (a, b):
+            if x not in ext:
+                ext[x] = len(ext)
+                rows.append(set())
+        rows[ext[a]].add(ext[b])
+        rows[ext[b]].add(ext[a])
+    n = len(rows)
+    off = np.zeros(n + 1, np.uint64)
+    nbr = []
+    for i, r in enumerate(rows):
+        nbr.extend(sorted(r))
+        off[i + 1] = len(nbr)
+    ids = np.zeros(n, np.uint64)
+    for x, i in ext.items():
+        ids[i] = x
+    return n, off, np.array(nbr, np.uint32), ids
+
+
+def run_ref_converter(lines, outdir):
+    """Run the reference's edge-list converter built in _ref/; returns {name: (n, off, nbr, ext)}."""
+    exe = os.path.join(REF_DIR, "undirectedEdgeListToBinary")
+    src = os.path.join(outdir, "edges.txt")
+    with open(src, "w") as f:
+        for a, b in lines:
+            f.write(f"{a} {b}\n")
+    subprocess.run([exe, src, outdir], capture_output=True, text=True, timeout=600, check=True, cwd=outdir)
+    out = {}
+    for name in ORDERINGS:
+        path = os.path.join(outdir, f"u_{name}.bin")
+        if os.path.exists(path):
+            out[name] = load_binary(path)
+    return out
+
 
 def ref_available():
     return os.path.exists(os.path.join(REF_DIR, "undirected_triangle_counting"))

@@ -267,6 +267,163 @@ int orc_relabel_by_degree(uint32_t n, const uint64_t *off, const uint32_t *nbr, 
   return 0;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Vertex orderings of tools/undirectedEdgeListToBinary.cpp:26-32             */
+/* (MutableGraph::reorder_*, src/MutableGraph.hpp:100-285), sequential.  The  */
+/* reference's std::random_shuffle is replaced by counter-based permutations: */
+/* the ids sorted by the high 32 bits of draw(stream_key(seed, stream), id),  */
+/* ties by id.  Orders are returned as new2old.                               */
+/* ------------------------------------------------------------------------- */
+
+static int cmp_u64(const void *a, const void *b) {
+  uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+void orc_perm(uint64_t seed, uint64_t stream, uint32_t n, uint32_t *new2old) {
+  uint64_t key = orc_stream_key(seed, stream);
+  uint64_t *k = (uint64_t *)malloc(8 * ((size_t)n + 1));
+  for (uint32_t i = 0; i < n; i++) k[i] = ((orc_draw(key, i) >> 32) << 32) | i;
+  qsort(k, n, 8, cmp_u64);
+  for (uint32_t r = 0; r < n; r++) new2old[r] = (uint32_t)k[r];
+  free(k);
+}
+
+/* stable order of ids by a 64-bit key per id */
+static void order_by_key(uint32_t n, const uint64_t *key, uint32_t *new2old) {
+  typedef struct { uint64_t k; uint32_t id; } kv;
+  kv *a = (kv *)malloc(sizeof(kv) * ((size_t)n + 1));
+  for (uint32_t i = 0; i < n; i++) { a[i].k = key[i]; a[i].id = i; }
+  /* bottom-up merge sort: equal keys keep id order */
+  kv *t = (kv *)malloc(sizeof(kv) * ((size_t)n + 1));
+  for (uint64_t w = 1; w < n; w *= 2) {
+    for (uint64_t lo = 0; lo < n; lo += 2 * w) {
+      uint64_t mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
+      uint64_t i = lo, j = mid, o = lo;
+      while (i < mid && j < hi) t[o++] = (a[j].k < a[i].k) ? a[j++] : a[i++];
+      while (i < mid) t[o++] = a[i++];
+      while (j < hi) t[o++] = a[j++];
+    }
+    kv *x = a; a = t; t = x;
+  }
+  for (uint32_t r = 0; r < n; r++) new2old[r] = a[r].id;
+  free(a);
+  free(t);
+}
+
+static void order_degree(uint32_t n, const uint64_t *off, uint32_t *new2old, int descending) {
+  uint64_t *key = (uint64_t *)calloc((size_t)n + 1, 8);
+  for (uint32_t i = 0; i < n; i++) {
+    uint64_t d = off[i + 1] - off[i];
+    key[i] = descending ? ~d : d;
+  }
+  order_by_key(n, key, new2old);
+  free(key);
+}
+
+/* reorder_strong_run (src/MutableGraph.hpp:179-201) */
+static void order_strong_run(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint32_t *new2old) {
+  uint32_t *rows = (uint32_t *)malloc(4 * ((size_t)n + 1));
+  order_degree(n, off, rows, 1);
+  uint8_t *seen = (uint8_t *)calloc((size_t)n + 1, 1);
+  uint32_t c = 0;
+  for (uint32_t r = 0; r < n; r++) {
+    uint32_t v = rows[r];
+    for (uint64_t k = off[v]; k < off[v + 1]; k++)
+      if (!seen[nbr[k]]) { seen[nbr[k]] = 1; new2old[c++] = nbr[k]; }
+  }
+  for (uint32_t u = 0; u < n; u++)
+    if (!seen[u]) new2old[c++] = u;
+  free(seen);
+  free(rows);
+}
+
+/* reorder_bfs (src/MutableGraph.hpp:126-172) */
+static void order_bfs(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint64_t seed, uint32_t *new2old) {
+  uint32_t *tmp = (uint32_t *)malloc(4 * ((size_t)n + 1));
+  orc_perm(seed, 102, n, tmp);
+  uint8_t *visited = (uint8_t *)calloc((size_t)n + 1, 1);
+  uint32_t *cur = (uint32_t *)malloc(4 * ((size_t)n + 1)), *next = (uint32_t *)malloc(4 * ((size_t)n + 1));
+  size_t ncur = 0, nnext = 0, c = 0, bfs_i = 0;
+  cur[ncur++] = tmp[bfs_i];
+  new2old[c++] = tmp[bfs_i];
+  visited[tmp[bfs_i]] = 1;
+  bfs_i++;
+  while (c != n) {
+    for (size_t i = 0; i < ncur; i++) {
+      uint32_t v = cur[i];
+      for (uint64_t k = off[v]; k < off[v + 1]; k++)
+        if (!visited[nbr[k]]) {
+          next[nnext++] = nbr[k];
+          new2old[c++] = nbr[k];
+          visited[nbr[k]] = 1;
+        }
+    }
+    if (nnext == 0 && c < n) {
+      while (visited[tmp[bfs_i]]) bfs_i++;
+      next[nnext++] = tmp[bfs_i];
+      new2old[c++] = tmp[bfs_i];
+      visited[tmp[bfs_i]] = 1;
+      bfs_i++;
+    }
+    uint32_t *x = cur; cur = next; next = x;
+    ncur = nnext;
+    nnext = 0;
+  }
+  free(cur);
+  free(next);
+  free(visited);
+  free(tmp);
+}
+
+/* reorder_by_shingles (src/MutableGraph.hpp:214-266) */
+static void order_shingles(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint64_t seed, uint32_t *new2old) {
+  uint32_t *ord = (uint32_t *)malloc(4 * ((size_t)n + 1));
+  uint64_t *key = (uint64_t *)calloc((size_t)n + 1, 8);
+  for (int s = 0; s < 2; s++) {
+    orc_perm(seed, 103 + (uint64_t)s, n, ord);
+    for (uint32_t j = 0; j < n; j++) {
+      uint32_t sh = 0;
+      if (off[j + 1] > off[j]) {
+        sh = nbr[off[j]];
+        uint32_t so = ord[sh];
+        for (uint64_t k = off[j] + 1; k < off[j + 1]; k++)
+          if (ord[nbr[k]] < so) { sh = nbr[k]; so = ord[sh]; }
+      }
+      key[j] |= (uint64_t)sh << (s == 0 ? 32 : 0);
+    }
+  }
+  order_by_key(n, key, new2old);
+  free(key);
+  free(ord);
+}
+
+int orc_order(uint32_t n, const uint64_t *off, const uint32_t *nbr, int ordering, uint64_t seed, uint32_t *new2old) {
+  switch (ordering) {
+    case 0: order_degree(n, off, new2old, 1); return 0;
+    case 1: order_degree(n, off, new2old, 0); return 0;
+    case 2: order_strong_run(n, off, nbr, new2old); return 0;
+    case 3: order_bfs(n, off, nbr, seed, new2old); return 0;
+    case 4: order_shingles(n, off, nbr, seed, new2old); return 0;
+    case 5: orc_perm(seed, 101, n, new2old); return 0;
+    case 6: {
+      /* reorder_by_the_game (src/MutableGraph.hpp:282-285): BFS, renumber, then degree */
+      uint32_t *b = (uint32_t *)malloc(4 * ((size_t)n + 1)), *d = (uint32_t *)malloc(4 * ((size_t)n + 1));
+      uint32_t *o2n = (uint32_t *)malloc(4 * ((size_t)n + 1));
+      uint64_t *off2 = (uint64_t *)malloc(8 * ((size_t)n + 1));
+      order_bfs(n, off, nbr, seed, b);
+      for (uint32_t r = 0; r < n; r++) o2n[b[r]] = r;
+      off2[0] = 0;
+      for (uint32_t r = 0; r < n; r++) off2[r + 1] = off2[r] + (off[b[r] + 1] - off[b[r]]);
+      order_degree(n, off2, d, 1);
+      for (uint32_t r = 0; r < n; r++) new2old[r] = b[d[r]];
+      free(b); free(d); free(o2n); free(off2);
+      return 0;
+    }
+    default: return -1;
+  }
+}
+
 /* Orientation filter nbr < node (apps/undirected_triangle_counting.cpp:20-23
  * applied by pack_data, src/SparseMatrix.hpp:249-266).  Rows are sorted, so
  * the kept entries are a prefix of each row. */

@@ -61,6 +61,11 @@ size_t orc_decode(int type, const uint8_t *payload, size_t card, size_t nbytes, 
 size_t orc_row_image_bound(const uint32_t *data, size_t len);
 size_t orc_row_image_bound_policy(const uint32_t *data, size_t len, int policy);
 
+/* vertex orderings (0 degree, 1 rev_degree, 2 strong_run, 3 bfs, 4 shingles, 5 random,
+ * 6 the_game) as new2old; seeded permutations from orc_perm */
+void orc_perm(uint64_t seed, uint64_t stream, uint32_t n, uint32_t *new2old);
+int orc_order(uint32_t n, const uint64_t *off, const uint32_t *nbr, int ordering, uint64_t seed, uint32_t *new2old);
+
 /* ---- sets and intersections ---- */
 typedef struct {
   uint8_t *data;

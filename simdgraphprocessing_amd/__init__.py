@@ -20,6 +20,7 @@ LAYOUTS = {"uint": 0, "pshort": 1, "bs": 2, "bitset": 2, "hybrid": 3, "hybrid3":
            "cost": 6}
 FILTERS = {"all": 0, "lower": 1, "mask": 2}
 SCHEDULES = {"plain": 0, "tuned": 1}
+ORDERINGS = {"degree": 0, "rev_degree": 1, "strong_run": 2, "bfs": 3, "shingles": 4, "random": 5, "the_game": 6}
 MLP_PARAMS = 5508
 
 
@@ -75,7 +76,7 @@ class GenSpec(C.Structure):
 EXPORTS = [
     "eh_abi_version", "eh_status_str", "eh_last_error", "eh_ctx_create", "eh_ctx_destroy",
     "eh_ctx_device_name", "eh_graph_from_csr", "eh_graph_from_edges", "eh_graph_generate",
-    "eh_graph_relabel_by_degree", "eh_graph_info_get", "eh_graph_download", "eh_graph_download_perm",
+    "eh_graph_relabel_by_degree", "eh_graph_reorder", "eh_graph_info_get", "eh_graph_download", "eh_graph_download_perm",
     "eh_graph_free", "eh_matrix_build", "eh_matrix_build_host", "eh_matrix_info_get", "eh_matrix_row",
     "eh_matrix_download", "eh_matrix_row_types", "eh_matrix_free", "eh_intersect", "eh_intersect_rows",
     "eh_triangle_count", "eh_clique_count", "eh_count_profile", "eh_batch_generate", "eh_batch_info",
@@ -117,6 +118,7 @@ def lib():
     L.eh_graph_from_edges.argtypes = [_vp, C.c_uint32, C.c_uint64, _vp, _vp, C.c_int, _pp]
     L.eh_graph_generate.argtypes = [_vp, C.POINTER(GenSpec), _pp]
     L.eh_graph_relabel_by_degree.argtypes = [_vp]
+    L.eh_graph_reorder.argtypes = [_vp, C.c_int, C.c_uint64]
     L.eh_graph_info_get.argtypes = [_vp, C.POINTER(GraphInfo)]
     L.eh_graph_download.argtypes = [_vp, _vp, _vp]
     L.eh_graph_download_perm.argtypes = [_vp, _vp]
@@ -348,6 +350,10 @@ class Graph:
 
     def relabel_by_degree(self):
         _check(lib().eh_graph_relabel_by_degree(self.h))
+
+    def reorder(self, ordering, seed=0):
+        """One of ORDERINGS (MutableGraph::reorder_*) on the device."""
+        _check(lib().eh_graph_reorder(self.h, ORDERINGS[ordering], seed))
 
     def build_matrix(self, filter="lower", policy="hybrid", mask=None):
         mk = None if mask is None else np.ascontiguousarray(mask, np.uint8)

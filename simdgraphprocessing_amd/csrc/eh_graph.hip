@@ -227,6 +227,16 @@ __global__ void k_iota(uint32_t *p, uint32_t n) {
     p[i] = (uint32_t)i;
 }
 
+__global__ void k_inverse_perm(const uint32_t *new2old, uint32_t n, uint32_t *old2new) {
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x)
+    old2new[new2old[r]] = (uint32_t)r;
+}
+
+__global__ void k_compose_perm(const uint32_t *outer, const uint32_t *inner, uint32_t n, uint32_t *out) {
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x)
+    out[r] = outer[inner[r]];
+}
+
 inline unsigned grid_for(uint64_t work, unsigned block = 256) {
   uint64_t g = ceil_div(work ? work : 1, block);
   return (unsigned)(g > 65536 ? 65536 : g);
@@ -303,11 +313,42 @@ eh_status graph_canonicalize(eh_ctx *ctx, uint32_t n, DevBuf &keys, uint64_t nke
   return EH_OK;
 }
 
+// Renumber the graph by new2old (device, n entries): rows and entries are remapped,
+// every row re-sorted, and the permutation composed with any earlier one.
+eh_status graph_apply_order(eh_graph *g, const uint32_t *d_new2old) {
+  eh_ctx *ctx = g->ctx;
+  hipStream_t st = ctx->stream;
+  const uint32_t n = g->n;
+  DevBuf o2n, keys, tmp, cnt;
+  EH_CHECK(o2n.alloc(4 * (size_t)n + 4));
+  k_inverse_perm<<<grid_for(n), 256, 0, st>>>(d_new2old, n, o2n.as<uint32_t>());
+  EH_CHECK(keys.alloc(8 * g->entries + 8));
+  EH_CHECK(tmp.alloc(8 * g->entries + 8));
+  k_remap_csr<<<grid_for((uint64_t)n * 64, 64), 64, 0, st>>>(g->off.as<uint64_t>(), g->nbr.as<uint32_t>(),
+                                                             n, o2n.as<uint32_t>(), keys.as<uint64_t>());
+  EH_CHECK(prim_sort_u64(ctx, keys.as<uint64_t>(), tmp.as<uint64_t>(), g->entries, 64));
+  EH_CHECK(cnt.alloc(4 * (size_t)n + 4));
+  EH_HIP(hipMemsetAsync(cnt.p, 0, 4 * (size_t)n + 4, st));
+  k_row_counts<<<grid_for(g->entries), 256, 0, st>>>(keys.as<uint64_t>(), g->entries, cnt.as<uint32_t>());
+  EH_CHECK(prim_exscan_u32_to_u64(ctx, cnt.as<uint32_t>(), g->off.as<uint64_t>(), (uint64_t)n + 1));
+  k_low_words<<<grid_for(g->entries), 256, 0, st>>>(keys.as<uint64_t>(), g->entries, g->nbr.as<uint32_t>());
+  // compose with any earlier renumbering: new2old_total[r] = old_map[new2old[r]]
+  DevBuf composed;
+  EH_CHECK(composed.alloc(4 * (size_t)n + 4));
+  k_compose_perm<<<grid_for(n), 256, 0, st>>>(g->new2old.as<uint32_t>(), d_new2old, n, composed.as<uint32_t>());
+  EH_HIP(hipGetLastError());
+  EH_HIP(hipStreamSynchronize(st));
+  g->new2old = std::move(composed);
+  return EH_OK;
+}
+
+// MutableGraph::reorder_by_degree (src/MutableGraph.hpp:20-32,268-272): degree
+// descending, old id ascending
 eh_status graph_relabel(eh_graph *g) {
   eh_ctx *ctx = g->ctx;
   hipStream_t st = ctx->stream;
   uint32_t n = g->n;
-  DevBuf deg, vk, vk2, o2n, n2o, keys, tmp;
+  DevBuf deg, vk, vk2, o2n, n2o;
   EH_CHECK(deg.alloc(4 * (size_t)n + 4));
   k_off_diff<<<grid_for(n), 256, 0, st>>>(g->off.as<uint64_t>(), n, deg.as<uint32_t>());
   EH_CHECK(vk.alloc(8 * (size_t)n + 8));
@@ -318,31 +359,7 @@ eh_status graph_relabel(eh_graph *g) {
   EH_CHECK(n2o.alloc(4 * (size_t)n + 4));
   k_perm_from_sorted<<<grid_for(n), 256, 0, st>>>(vk.as<uint64_t>(), n, n2o.as<uint32_t>(),
                                                  o2n.as<uint32_t>());
-  EH_CHECK(keys.alloc(8 * g->entries + 8));
-  EH_CHECK(tmp.alloc(8 * g->entries + 8));
-  k_remap_csr<<<grid_for((uint64_t)n * 64, 64), 64, 0, st>>>(g->off.as<uint64_t>(), g->nbr.as<uint32_t>(),
-                                                             n, o2n.as<uint32_t>(), keys.as<uint64_t>());
-  EH_CHECK(prim_sort_u64(ctx, keys.as<uint64_t>(), tmp.as<uint64_t>(), g->entries, 64));
-  DevBuf cnt;
-  EH_CHECK(cnt.alloc(4 * (size_t)n + 4));
-  EH_HIP(hipMemsetAsync(cnt.p, 0, 4 * (size_t)n + 4, st));
-  k_row_counts<<<grid_for(g->entries), 256, 0, st>>>(keys.as<uint64_t>(), g->entries, cnt.as<uint32_t>());
-  EH_CHECK(prim_exscan_u32_to_u64(ctx, cnt.as<uint32_t>(), g->off.as<uint64_t>(), (uint64_t)n + 1));
-  k_low_words<<<grid_for(g->entries), 256, 0, st>>>(keys.as<uint64_t>(), g->entries, g->nbr.as<uint32_t>());
-  // compose with any earlier relabel: new2old_total[r] = old_map[n2o[r]]
-  DevBuf composed;
-  EH_CHECK(composed.alloc(4 * (size_t)n + 4));
-  {
-    std::vector<uint32_t> a(n), b(n), c(n);
-    EH_HIP(hipMemcpyAsync(a.data(), g->new2old.p, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
-    EH_HIP(hipMemcpyAsync(b.data(), n2o.p, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
-    EH_HIP(hipStreamSynchronize(st));
-    for (uint32_t r = 0; r < n; r++) c[r] = a[b[r]];
-    EH_HIP(hipMemcpy(composed.p, c.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
-  }
-  g->new2old = std::move(composed);
-  EH_HIP(hipStreamSynchronize(st));
-  return EH_OK;
+  return graph_apply_order(g, n2o.as<uint32_t>());
 }
 
 // ---------------------------------------------------------------- C ABI

@@ -166,6 +166,9 @@ struct eh_matrix {
 
 // ---------------------------------------------------------------- primitives (eh_prim.hip)
 
+// renumber a graph by a device new2old permutation (eh_graph.hip)
+eh_status graph_apply_order(eh_graph *g, const uint32_t *d_new2old);
+
 // k-cliques for k >= 3 by the general level-set recursion (eh_clique.hip)
 eh_status clique_count_deep(eh_matrix *m, uint32_t k, uint64_t *count, double *ms);
 

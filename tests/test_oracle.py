@@ -6,6 +6,7 @@ symmetric graph, Python set intersection, hand-computed layout bytes, and --
 where oracle/_ref was built in this container -- the reference's own programs.
 """
 import itertools
+import os
 import random
 
 import numpy as np
@@ -380,3 +381,101 @@ def test_directed_fixture_symbiosity():
         zero += union > 0 and both / union > 0.5
     assert pos == len(raw)
     assert zero == 0
+
+
+# ------------------------------------------------------------------ vertex orderings
+
+def _py_orders(n, off, nbr, seed):
+    """Pure-Python restatement of MutableGraph::reorder_* (src/MutableGraph.hpp:126-285)
+    with the oracle's seeded permutations in place of std::random_shuffle."""
+    rows = [list(map(int, nbr[off[i]:off[i + 1]])) for i in range(n)]
+    deg = [len(r) for r in rows]
+    out = {"degree": sorted(range(n), key=lambda v: (-deg[v], v)),
+           "rev_degree": sorted(range(n), key=lambda v: (deg[v], v))}
+    seen, sr = set(), []
+    for v in out["degree"]:
+        for u in rows[v]:
+            if u not in seen:
+                seen.add(u)
+                sr.append(u)
+    out["strong_run"] = sr + [u for u in range(n) if u not in seen]
+    tmp = list(map(int, O.perm(seed, 102, n)))
+    visited, order, cur, bi = {tmp[0]}, [tmp[0]], [tmp[0]], 1
+    while len(order) != n:
+        nxt = []
+        for v in cur:
+            for u in rows[v]:
+                if u not in visited:
+                    visited.add(u)
+                    nxt.append(u)
+                    order.append(u)
+        if not nxt and len(order) < n:
+            while tmp[bi] in visited:
+                bi += 1
+            visited.add(tmp[bi])
+            nxt.append(tmp[bi])
+            order.append(tmp[bi])
+            bi += 1
+        cur = nxt
+    out["bfs"] = order
+    sh = []
+    for s in (0, 1):
+        rk = list(map(int, O.perm(seed, 103 + s, n)))
+        sh.append([min(r, key=lambda x: rk[x]) if r else 0 for r in rows])
+    out["shingles"] = sorted(range(n), key=lambda v: (sh[0][v], sh[1][v], v))
+    out["random"] = list(map(int, O.perm(seed, 101, n)))
+    return out
+
+
+def _small_graph(seed):
+    """ER part + a path + a star + isolated vertices: several components."""
+    n = 120
+    src, dst = O.gen_er(60, 150, seed)
+    edges = list(zip(src.tolist(), dst.tolist())) + [(70 + i, 71 + i) for i in range(10)] + \
+        [(90, 91 + i) for i in range(8)]
+    edges = [(a, b) for a, b in edges if a != b]
+    off, nbr = O.edges_to_csr(n, np.array([a for a, _ in edges]), np.array([b for _, b in edges]))
+    return n, off, nbr
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_orderings_match_python_restatement(seed):
+    n, off, nbr = _small_graph(seed)
+    want = _py_orders(n, off, nbr, seed)
+    for name, w in want.items():
+        got = O.order(n, off, nbr, name, seed).tolist()
+        assert got == w, name
+        assert sorted(got) == list(range(n))
+    # the_game = BFS, renumber, then degree
+    b = O.order(n, off, nbr, "bfs", seed)
+    boff, bnbr = O.apply_order(n, off, nbr, b)
+    d = O.order(n, boff, bnbr, "degree", seed)
+    assert O.order(n, off, nbr, "the_game", seed).tolist() == b[d].tolist()
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(O.__file__), "_ref", "undirectedEdgeListToBinary")),
+                    reason="reference converter not built (needs the reference tree)")
+def test_orderings_match_reference_converter(tmp_path):
+    """tools/undirectedEdgeListToBinary.cpp on an edge list: its deterministic orderings
+    (degree, strong_run) give the same files as the oracle's; the shuffled ones are
+    permutations of the same graph."""
+    rng = np.random.default_rng(4)
+    lines = []
+    for _ in range(2500):
+        a, b = int(rng.integers(0, 700)) * 7 + 11, int(rng.integers(0, 700)) * 7 + 11
+        if a != b:
+            lines.append((a, b))
+    ref = O.run_ref_converter(lines, str(tmp_path))
+    n, off, nbr, ids = O.edge_list_graph(lines)
+    for name in ("degree", "strong_run"):
+        rn, roff, rnbr, rext = ref[name]
+        n2o = O.order(n, off, nbr, name, 0)
+        noff, nnbr = O.apply_order(n, off, nbr, n2o)
+        assert rn == n
+        assert np.array_equal(roff, noff) and np.array_equal(rnbr, nnbr), name
+        assert np.array_equal(rext, ids[n2o]), name
+    tri = O.Matrix(n, off, nbr, "hybrid").triangles(2)[0]
+    for name, (rn, roff, rnbr, rext) in ref.items():
+        assert rn == n and len(rnbr) == len(nbr)
+        assert sorted(rext.tolist()) == sorted(ids.tolist())
+        assert O.Matrix(rn, roff, rnbr, "hybrid").triangles(2)[0] == tri, name
