@@ -1,21 +1,29 @@
-// MutableGraph.hpp -- the host graph the apps start from (src/MutableGraph.hpp:49-80)
-// and its binary loader/writer (src/MutableGraph.hpp:298-360).
+// MutableGraph.hpp -- the host graph the apps start from (src/MutableGraph.hpp:49-80),
+// its binary loader/writer (src/MutableGraph.hpp:298-360), the undirected edge-list
+// loader (src/MutableGraph.hpp:517-598) and the vertex orderings
+// (src/MutableGraph.hpp:126-285), which run on the device (eh_graph_reorder).
 //
 // Undirected file format: size_t n, then per node { uint64 external id; size_t
-// degree; uint32 neighbours[degree] }, neighbours sorted and already degree-
-// relabelled (the reference's tools do that offline).  Directed files
+// degree; uint32 neighbours[degree] }, neighbours sorted and already renumbered
+// (tools/undirectedEdgeListToBinary does that offline).  Directed files
 // (src/MutableGraph.hpp:636-714) hold per node the out-neighbours and then the
-// in-neighbours, each as { size_t count; uint32 ids[count] }.  The edge-list and
-// attribute loaders are not provided.
+// in-neighbours, each as { size_t count; uint32 ids[count] }.  The attribute
+// loaders are not provided.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <iterator>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
+
+#include "runtime.hpp"
 
 struct MutableGraph {
   size_t num_nodes;
@@ -149,5 +157,104 @@ struct MutableGraph {
       std::exit(1);
     }
     return new MutableGraph(n, edges, maxd, true, id_map, nullptr, rows, rows, nullptr, nullptr);
+  }
+
+  // Text pairs "src dst" of 64-bit external ids, separated by spaces, tabs, line
+  // breaks or 'A'.  Internal ids follow first appearance; every pair is stored in
+  // both directions; rows are sorted and deduplicated.  Unlike the reference, a
+  // self-loop is dropped (device graphs are simple) and a malformed file is an error.
+  static MutableGraph *undirectedFromEdgeList(const std::string &path) {
+    std::cout << path << std::endl;
+    std::ifstream in(path, std::ios::binary | std::ios::in);
+    if (!in) {
+      std::cerr << "ERROR: cannot open " << path << std::endl;
+      std::exit(1);
+    }
+    const std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    auto *id_map = new std::vector<uint64_t>();
+    auto *rows = new std::vector<std::vector<uint32_t> *>();
+    std::unordered_map<uint64_t, uint32_t> ids;
+    auto intern = [&](uint64_t ext) -> uint32_t {
+      auto it = ids.find(ext);
+      if (it != ids.end()) return it->second;
+      const uint32_t id = (uint32_t)rows->size();
+      ids.emplace(ext, id);
+      id_map->push_back(ext);
+      rows->push_back(new std::vector<uint32_t>());
+      return id;
+    };
+    auto is_sep = [](char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == 'A'; };
+    size_t pos = 0;
+    uint64_t pending[2];
+    int have = 0;
+    while (true) {
+      while (pos < text.size() && is_sep(text[pos])) pos++;
+      if (pos == text.size()) break;
+      const size_t start = pos;
+      uint64_t v = 0;
+      while (pos < text.size() && text[pos] >= '0' && text[pos] <= '9') v = 10 * v + (uint64_t)(text[pos++] - '0');
+      if (pos == start || (pos < text.size() && !is_sep(text[pos]))) {
+        std::cerr << "ERROR: malformed edge list " << path << " at byte " << start << std::endl;
+        std::exit(1);
+      }
+      pending[have++] = v;
+      if (have < 2) continue;
+      have = 0;
+      const uint32_t a = intern(pending[0]), b = intern(pending[1]);
+      if (a == b) continue;
+      rows->at(a)->push_back(b);
+      rows->at(b)->push_back(a);
+    }
+    if (have != 0) {
+      std::cerr << "ERROR: edge list " << path << " ends in half a pair" << std::endl;
+      std::exit(1);
+    }
+    size_t edges = 0, maxd = 0;
+    for (auto *row : *rows) {
+      std::sort(row->begin(), row->end());
+      row->erase(std::unique(row->begin(), row->end()), row->end());
+      edges += row->size();
+      maxd = std::max(maxd, row->size());
+    }
+    return new MutableGraph(rows->size(), edges, maxd, true, id_map, nullptr, rows, rows, nullptr, nullptr);
+  }
+
+  // The vertex orderings (src/MutableGraph.hpp:126-285).  The renumbering runs on
+  // the device; the rows and id_map here are then replaced by the renumbered ones.
+  // The reference shuffles with std::random_shuffle; here `seed` makes BFS,
+  // shingles, random and the_game reproducible.
+  void reorder_bfs(uint64_t seed = 0) { reorder(EH_ORDER_BFS, seed); }
+  void reorder_strong_run() { reorder(EH_ORDER_STRONG_RUN, 0); }
+  void reorder_random(uint64_t seed = 0) { reorder(EH_ORDER_RANDOM, seed); }
+  void reorder_by_shingles(uint64_t seed = 0) { reorder(EH_ORDER_SHINGLES, seed); }
+  void reorder_by_degree() { reorder(EH_ORDER_DEGREE, 0); }
+  void reorder_by_rev_degree() { reorder(EH_ORDER_REV_DEGREE, 0); }
+  void reorder_by_the_game(uint64_t seed = 0) { reorder(EH_ORDER_THE_GAME, seed); }
+
+  void reorder(eh_ordering ordering, uint64_t seed) {
+    if (!symmetric) throw emptyheaded::Error(EH_ERR_UNSUPPORTED, "MutableGraph::reorder: undirected graphs only");
+    if (num_nodes == 0) return;
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> nbr;
+    to_csr(off, nbr);
+    std::vector<uint32_t> new2old(num_nodes);
+    {
+      emptyheaded::Runtime &rt = emptyheaded::Runtime::get();
+      std::lock_guard<std::mutex> lock(rt.mutex());
+      eh_graph *g = nullptr;
+      emptyheaded::check(eh_graph_from_csr(rt.ctx(), (uint32_t)num_nodes, off.data(), nbr.data(), &g),
+                         "MutableGraph::reorder");
+      eh_status st = eh_graph_reorder(g, ordering, seed);
+      if (st == EH_OK) st = eh_graph_download(g, off.data(), nbr.data());
+      if (st == EH_OK) st = eh_graph_download_perm(g, new2old.data());
+      eh_graph_free(g);
+      emptyheaded::check(st, "MutableGraph::reorder");
+    }
+    std::vector<uint64_t> ext(num_nodes);
+    for (size_t r = 0; r < num_nodes; r++) {
+      ext[r] = id_map->at(new2old[r]);
+      out_neighborhoods->at(r)->assign(nbr.begin() + (std::ptrdiff_t)off[r], nbr.begin() + (std::ptrdiff_t)off[r + 1]);
+    }
+    id_map->swap(ext);
   }
 };

@@ -7,6 +7,7 @@ import gzip
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -122,3 +123,49 @@ def test_more_apps_random_graph(tmp_path):
     assert rc == 0 and f"Count: {similar}" in out, out[-1500:] + err[-1500:]
     rc, out, err = run(["undirected_lollipop_counting"] + args)
     assert rc == 0 and f"Count: {lollipops}" in out, out[-1500:] + err[-1500:]
+
+
+def _drop_loops(n, off, nbr):
+    rows = [nbr[off[i]:off[i + 1]] for i in range(n)]
+    rows = [r[r != i] for i, r in enumerate(rows)]
+    noff = np.zeros(n + 1, np.uint64)
+    noff[1:] = np.cumsum([len(r) for r in rows])
+    return noff, (np.concatenate(rows).astype(np.uint32) if n else np.zeros(0, np.uint32))
+
+
+@pytest.mark.gpu
+def test_cli_edge_list_converter(tmp_path):
+    """tools/undirectedEdgeListToBinary.cpp: one u_<ordering>.bin per ordering, each equal to
+    the oracle's loader (src/MutableGraph.hpp:517-598) + ordering + renumbering; the C++
+    loader drops self-loops (the vertex keeps its first-appearance id)."""
+    import oracle as O
+    rng = np.random.default_rng(9)
+    ext = rng.choice(1 << 40, size=900, replace=False).astype(np.uint64)
+    src, dst = O.gen_er(900, 2500, 4)
+    lines = [(int(ext[a]), int(ext[b])) for a, b in zip(src.tolist(), dst.tolist())]
+    lines += lines[:40]                                  # duplicates, both directions
+    lines += [(int(ext[b]), int(ext[a])) for a, b in zip(src[40:80].tolist(), dst[40:80].tolist())]
+    lines.insert(3, (7, 7))                              # a self-loop on a fresh vertex
+    seps = [" ", "\t", "A"]
+    with open(tmp_path / "edges.txt", "w") as f:
+        for i, (a, b) in enumerate(lines):
+            f.write(f"{a}{seps[i % 3]}{b}\n")
+    rc, out, err = run(["undirectedEdgeListToBinary", str(tmp_path / "edges.txt"), str(tmp_path), "5"])
+    assert rc == 0, out[-2000:] + err[-2000:]
+    n, off, nbr, ids = O.edge_list_graph(lines)
+    off, nbr = _drop_loops(n, off, nbr)
+    for name in O.ORDERINGS:
+        assert f"{name} generated" in out
+        gn, goff, gnbr, gext = O.load_binary(str(tmp_path / f"u_{name}.bin"))
+        want = O.order(n, off, nbr, name, 5)
+        woff, wnbr = O.apply_order(n, off, nbr, want)
+        assert gn == n, name
+        assert np.array_equal(goff, woff) and np.array_equal(gnbr, wnbr), name
+        assert np.array_equal(gext, ids[want]), name
+
+
+def test_cli_edge_list_converter_rejects_half_pair(tmp_path):
+    """A malformed edge list fails in the host loader, before any device call."""
+    (tmp_path / "bad.txt").write_text("1 2\n3\n")
+    rc, out, err = run(["undirectedEdgeListToBinary", str(tmp_path / "bad.txt"), str(tmp_path)])
+    assert rc != 0 and "half a pair" in err
