@@ -532,6 +532,50 @@ def edge_list_graph(lines):
     return n, off, np.array(nbr, np.uint32), ids
 
 
+def directed_edge_list_graph(lines):
+    """The reference's directed edge-list loader (src/MutableGraph.hpp:893-981): ids in order
+    of first appearance (source before destination), out-rows and in-rows sorted and
+    deduplicated.  Returns (n, out_rows, in_rows, ext_ids)."""
+    ext, outs, ins = {}, [], []
+    for a, b in lines:
+        for x in (a, b):
+            if x not in ext:
+                ext[x] = len(ext)
+                outs.append(set())
+                ins.append(set())
+        outs[ext[a]].add(ext[b])
+        ins[ext[b]].add(ext[a])
+    ids = np.zeros(len(outs), np.uint64)
+    for x, i in ext.items():
+        ids[i] = x
+    rows = lambda side: [np.array(sorted(r), np.uint32) for r in side]
+    return len(outs), rows(outs), rows(ins), ids
+
+
+def load_directed_binary(path_or_bytes):
+    """The directed binary format (src/MutableGraph.hpp:636-714): size_t n, then per node
+    {u64 external id; size_t out count; out ids; size_t in count; in ids}.
+    Returns (n, out_rows, in_rows, ext_ids)."""
+    raw = path_or_bytes if isinstance(path_or_bytes, (bytes, bytearray)) else open(path_or_bytes, "rb").read()
+    pos = 0
+
+    def take(count, dtype):
+        nonlocal pos
+        arr = np.frombuffer(raw, dtype, count, pos)
+        pos += arr.nbytes
+        return arr
+
+    n = int(take(1, np.uint64)[0])
+    ids, outs, ins = np.zeros(n, np.uint64), [], []
+    for i in range(n):
+        ids[i] = take(1, np.uint64)[0]
+        outs.append(take(int(take(1, np.uint64)[0]), np.uint32).copy())
+        ins.append(take(int(take(1, np.uint64)[0]), np.uint32).copy())
+    if pos != len(raw):
+        raise IOError("trailing bytes in directed binary file")
+    return n, outs, ins, ids
+
+
 def run_ref_converter(lines, outdir):
     """Run the reference's edge-list converter built in _ref/; returns {name: (n, off, nbr, ext)}."""
     exe = os.path.join(REF_DIR, "undirectedEdgeListToBinary")

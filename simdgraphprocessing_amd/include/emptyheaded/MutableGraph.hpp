@@ -1,6 +1,7 @@
 // MutableGraph.hpp -- the host graph the apps start from (src/MutableGraph.hpp:49-80),
 // its binary loader/writer (src/MutableGraph.hpp:298-360), the undirected edge-list
-// loader (src/MutableGraph.hpp:517-598) and the vertex orderings
+// and directed edge-list loaders (src/MutableGraph.hpp:517-598,893-981), the
+// directed writer (src/MutableGraph.hpp:636-656) and the vertex orderings
 // (src/MutableGraph.hpp:126-285), which run on the device (eh_graph_reorder).
 //
 // Undirected file format: size_t n, then per node { uint64 external id; size_t
@@ -160,29 +161,16 @@ struct MutableGraph {
   }
 
   // Text pairs "src dst" of 64-bit external ids, separated by spaces, tabs, line
-  // breaks or 'A'.  Internal ids follow first appearance; every pair is stored in
-  // both directions; rows are sorted and deduplicated.  Unlike the reference, a
-  // self-loop is dropped (device graphs are simple) and a malformed file is an error.
-  static MutableGraph *undirectedFromEdgeList(const std::string &path) {
-    std::cout << path << std::endl;
+  // breaks or 'A' (src/MutableGraph.hpp:547-556).  Calls pair(src, dst) per pair;
+  // a token that is not a number or a trailing half pair is an error.
+  template <class F>
+  static void read_edge_pairs(const std::string &path, F &&pair) {
     std::ifstream in(path, std::ios::binary | std::ios::in);
     if (!in) {
       std::cerr << "ERROR: cannot open " << path << std::endl;
       std::exit(1);
     }
     const std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
-    auto *id_map = new std::vector<uint64_t>();
-    auto *rows = new std::vector<std::vector<uint32_t> *>();
-    std::unordered_map<uint64_t, uint32_t> ids;
-    auto intern = [&](uint64_t ext) -> uint32_t {
-      auto it = ids.find(ext);
-      if (it != ids.end()) return it->second;
-      const uint32_t id = (uint32_t)rows->size();
-      ids.emplace(ext, id);
-      id_map->push_back(ext);
-      rows->push_back(new std::vector<uint32_t>());
-      return id;
-    };
     auto is_sep = [](char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == 'A'; };
     size_t pos = 0;
     uint64_t pending[2];
@@ -198,25 +186,103 @@ struct MutableGraph {
         std::exit(1);
       }
       pending[have++] = v;
-      if (have < 2) continue;
-      have = 0;
-      const uint32_t a = intern(pending[0]), b = intern(pending[1]);
-      if (a == b) continue;
-      rows->at(a)->push_back(b);
-      rows->at(b)->push_back(a);
+      if (have == 2) {
+        have = 0;
+        pair(pending[0], pending[1]);
+      }
     }
     if (have != 0) {
       std::cerr << "ERROR: edge list " << path << " ends in half a pair" << std::endl;
       std::exit(1);
     }
-    size_t edges = 0, maxd = 0;
-    for (auto *row : *rows) {
+  }
+
+  static void sort_rows(std::vector<std::vector<uint32_t> *> &rows, size_t &edges, size_t &maxd) {
+    for (auto *row : rows) {
       std::sort(row->begin(), row->end());
       row->erase(std::unique(row->begin(), row->end()), row->end());
       edges += row->size();
       maxd = std::max(maxd, row->size());
     }
+  }
+
+  // src/MutableGraph.hpp:517-598.  Internal ids follow first appearance; every pair
+  // is stored in both directions; rows are sorted and deduplicated.  Unlike the
+  // reference, a self-loop is dropped (device graphs are simple) and a malformed
+  // file is an error.
+  static MutableGraph *undirectedFromEdgeList(const std::string &path) {
+    std::cout << path << std::endl;
+    auto *id_map = new std::vector<uint64_t>();
+    auto *rows = new std::vector<std::vector<uint32_t> *>();
+    std::unordered_map<uint64_t, uint32_t> ids;
+    auto intern = [&](uint64_t ext) -> uint32_t {
+      auto it = ids.find(ext);
+      if (it != ids.end()) return it->second;
+      const uint32_t id = (uint32_t)rows->size();
+      ids.emplace(ext, id);
+      id_map->push_back(ext);
+      rows->push_back(new std::vector<uint32_t>());
+      return id;
+    };
+    read_edge_pairs(path, [&](uint64_t s, uint64_t d) {
+      const uint32_t a = intern(s), b = intern(d);
+      if (a == b) return;
+      rows->at(a)->push_back(b);
+      rows->at(b)->push_back(a);
+    });
+    size_t edges = 0, maxd = 0;
+    sort_rows(*rows, edges, maxd);
     return new MutableGraph(rows->size(), edges, maxd, true, id_map, nullptr, rows, rows, nullptr, nullptr);
+  }
+
+  // src/MutableGraph.hpp:893-981: out-rows and in-rows, ids by first appearance,
+  // both sorted and deduplicated; self-loops dropped as above.  num_edges counts
+  // distinct arcs (the reference counts input lines).
+  static MutableGraph *directedFromEdgeList(const std::string &path) {
+    auto *id_map = new std::vector<uint64_t>();
+    auto *outs = new std::vector<std::vector<uint32_t> *>();
+    auto *ins = new std::vector<std::vector<uint32_t> *>();
+    std::unordered_map<uint64_t, uint32_t> ids;
+    auto intern = [&](uint64_t ext) -> uint32_t {
+      auto it = ids.find(ext);
+      if (it != ids.end()) return it->second;
+      const uint32_t id = (uint32_t)outs->size();
+      ids.emplace(ext, id);
+      id_map->push_back(ext);
+      outs->push_back(new std::vector<uint32_t>());
+      ins->push_back(new std::vector<uint32_t>());
+      return id;
+    };
+    read_edge_pairs(path, [&](uint64_t s, uint64_t d) {
+      const uint32_t a = intern(s), b = intern(d);
+      if (a == b) return;
+      outs->at(a)->push_back(b);
+      ins->at(b)->push_back(a);
+    });
+    size_t arcs = 0, in_arcs = 0, maxd = 0;
+    sort_rows(*outs, arcs, maxd);
+    sort_rows(*ins, in_arcs, maxd);
+    return new MutableGraph(outs->size(), arcs, maxd, false, id_map, nullptr, outs, ins, nullptr, nullptr);
+  }
+
+  // src/MutableGraph.hpp:636-656: per node { uint64 external id; size_t out count;
+  // out ids; size_t in count; in ids }.
+  void writeDirectedToBinary(const std::string &path) const {
+    std::ofstream out(path, std::ios::binary | std::ios::out);
+    if (!out) {
+      std::cerr << "ERROR: cannot write " << path << std::endl;
+      std::exit(1);
+    }
+    out.write(reinterpret_cast<const char *>(&num_nodes), sizeof(num_nodes));
+    for (size_t i = 0; i < num_nodes; i++) {
+      out.write(reinterpret_cast<const char *>(&id_map->at(i)), sizeof(uint64_t));
+      for (const auto *side : {out_neighborhoods, in_neighborhoods}) {
+        const std::vector<uint32_t> &r = *side->at(i);
+        const size_t d = r.size();
+        out.write(reinterpret_cast<const char *>(&d), sizeof(d));
+        out.write(reinterpret_cast<const char *>(r.data()), (std::streamsize)(sizeof(uint32_t) * d));
+      }
+    }
   }
 
   // The vertex orderings (src/MutableGraph.hpp:126-285).  The renumbering runs on

@@ -8,6 +8,7 @@ where oracle/_ref was built in this container -- the reference's own programs.
 import itertools
 import os
 import random
+import subprocess
 
 import numpy as np
 import pytest
@@ -360,27 +361,14 @@ def test_more_app_fixtures_by_direct_count(facebook):
 def test_directed_fixture_symbiosity():
     """test/symbiosity_test.cpp:12 -> 0 on test/data/dfacebook.bin (out- and in-rows)."""
     import gzip
-    import os
     raw = gzip.open(os.path.join(os.path.dirname(__file__), "golden", "dfacebook.bin.gz"), "rb").read()
-    pos = 0
-
-    def take(count, dtype):
-        nonlocal pos
-        arr = np.frombuffer(raw, dtype, count, pos)
-        pos += arr.nbytes
-        return arr
-
-    n = int(take(1, np.uint64)[0])
+    n, outs, ins, _ = O.load_directed_binary(raw)
     zero = 0
-    for _ in range(n):
-        take(1, np.uint64)
-        out = take(int(take(1, np.uint64)[0]), np.uint32)
-        inn = take(int(take(1, np.uint64)[0]), np.uint32)
+    for out, inn in zip(outs, ins):
         both = len(np.intersect1d(out, inn))
         union = len(out) + len(inn) - both
         zero += union > 0 and both / union > 0.5
-    assert pos == len(raw)
-    assert zero == 0
+    assert n == 4039 and zero == 0
 
 
 # ------------------------------------------------------------------ vertex orderings
@@ -479,3 +467,48 @@ def test_orderings_match_reference_converter(tmp_path):
         assert rn == n and len(rnbr) == len(nbr)
         assert sorted(rext.tolist()) == sorted(ids.tolist())
         assert O.Matrix(rn, roff, rnbr, "hybrid").triangles(2)[0] == tri, name
+
+
+def _directed_lines(seed):
+    rng = np.random.default_rng(seed)
+    lines = []
+    for _ in range(3000):
+        a, b = int(rng.integers(0, 800)) * 13 + 5, int(rng.integers(0, 800)) * 13 + 5
+        if a != b:
+            lines.append((a, b))
+    return lines + lines[:50]
+
+
+def test_directed_converter_matches_restatement(tmp_path):
+    """simdgraphprocessing_amd/bin/directedEdgeListToBinary (host only) writes what the
+    restated loader + writer (src/MutableGraph.hpp:636-656,893-981) would."""
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "simdgraphprocessing_amd", "bin", "directedEdgeListToBinary")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is not built (run __graft_entry__.build())")
+    lines = _directed_lines(6)
+    src = tmp_path / "arcs.txt"
+    src.write_text("".join(f"{a}{' ' if i % 2 else chr(9)}{b}\n" for i, (a, b) in enumerate(lines)))
+    subprocess.run([exe, str(src), str(tmp_path / "d.bin")], check=True, capture_output=True, timeout=120)
+    n, outs, ins, ids = O.directed_edge_list_graph(lines)
+    gn, gouts, gins, gids = O.load_directed_binary(str(tmp_path / "d.bin"))
+    assert gn == n and np.array_equal(gids, ids)
+    assert all(np.array_equal(a, b) for a, b in zip(gouts, outs))
+    assert all(np.array_equal(a, b) for a, b in zip(gins, ins))
+    assert sum(map(len, outs)) == sum(map(len, ins)) == len(set(lines))
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(O.__file__), "_ref", "directedEdgeListToBinary")),
+                    reason="reference converter not built (needs the reference tree)")
+def test_directed_restatement_matches_reference_converter(tmp_path):
+    """tools/directedEdgeListToBinary.cpp writes the file the restatement describes."""
+    lines = _directed_lines(8)
+    src = tmp_path / "arcs.txt"
+    src.write_text("".join(f"{a} {b}\n" for a, b in lines))
+    exe = os.path.join(os.path.dirname(O.__file__), "_ref", "directedEdgeListToBinary")
+    subprocess.run([exe, str(src), str(tmp_path / "r.bin")], check=True, capture_output=True, timeout=120)
+    n, outs, ins, ids = O.directed_edge_list_graph(lines)
+    rn, routs, rins, rids = O.load_directed_binary(str(tmp_path / "r.bin"))
+    assert rn == n and np.array_equal(rids, ids)
+    assert all(np.array_equal(a, b) for a, b in zip(routs, outs))
+    assert all(np.array_equal(a, b) for a, b in zip(rins, ins))
