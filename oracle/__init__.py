@@ -21,8 +21,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_DIR = os.path.join(HERE, "_ref")
 
-BITSET, PSHORT, UINTEGER = 0, 1, 2
-POLICY = {"uint": 0, "pshort": 1, "bs": 2, "hybrid": 3, "hybrid3": 4}
+BITSET, PSHORT, UINTEGER, BITSET_NEW, NEW_TYPE = 0, 1, 2, 7, 8
+POLICY = {"uint": 0, "pshort": 1, "bs": 2, "hybrid": 3, "hybrid3": 4,
+          "bitset_new": 7, "new_type": 8, "hybrid_block": 9}
+_BUILDER = {BITSET: "bitset", PSHORT: "pshort", BITSET_NEW: "bitset_new", NEW_TYPE: "new_type"}
 ALG = {"scalar": 0, "standard": 1, "galloping": 2, "select": 3}
 
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
@@ -63,7 +65,8 @@ def lib():
     L.orc_order.restype = C.c_int
     L.orc_row_image_bound.argtypes = [_u32p, C.c_size_t]
     L.orc_row_image_bound.restype = C.c_size_t
-    for name in ("orc_uint_build", "orc_pshort_build", "orc_bitset_build"):
+    for name in ("orc_uint_build", "orc_pshort_build", "orc_bitset_build", "orc_bitset_new_build",
+                 "orc_new_type_build"):
         f = getattr(L, name)
         f.argtypes = [_u8p, _u32p, C.c_size_t]
         f.restype = C.c_size_t
@@ -263,11 +266,8 @@ def make_set(values, type_):
     if type_ == UINTEGER:
         buf = v.view(np.uint8).copy()
         nb = buf.nbytes
-    elif type_ == BITSET:
-        buf = layout_build("bitset", v)
-        nb = len(buf)
     else:
-        buf = layout_build("pshort", v)
+        buf = layout_build(_BUILDER[type_], v)
         nb = len(buf)
     buf = np.concatenate([buf, np.zeros(64, np.uint8)])
     s = OrcSet(buf.ctypes.data, len(v), nb, type_, 0)
@@ -310,7 +310,7 @@ def set_payload(values, type_):
         return type_, len(v), v.view(np.uint8).copy()
     if len(v) == 0:
         return type_, 0, np.zeros(0, np.uint8)
-    return type_, len(v), layout_build("bitset" if type_ == BITSET else "pshort", v)
+    return type_, len(v), layout_build(_BUILDER[type_], v)
 
 
 class Matrix:
@@ -596,9 +596,9 @@ def ref_available():
     return os.path.exists(os.path.join(REF_DIR, "undirected_triangle_counting"))
 
 
-def run_ref(app, n, off, nbr, threads=4, layout="hybrid", k=None, timeout=600):
+def run_ref_stats(app, n, off, nbr, threads=4, layout="hybrid", k=None, timeout=600):
     """Run a reference program built in _ref/ on a symmetric CSR (already
-    degree-relabelled); returns (count, seconds of its counting phase)."""
+    degree-relabelled); returns {count, seconds (counting phase), row_bytes}."""
     exe = os.path.join(REF_DIR, app)
     with tempfile.TemporaryDirectory() as td:
         path = os.path.join(td, "g.bin")
@@ -609,6 +609,39 @@ def run_ref(app, n, off, nbr, threads=4, layout="hybrid", k=None, timeout=600):
         out = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, cwd=td).stdout
     cnt = re.search(r"Number of (?:triangles|cliques): (\d+)", out)
     tm = re.findall(r"Time\[(?:UNDIRECTED TRIANGLE COUNTING|n_cliques)\]: ([0-9.e+-]+) s", out)
+    rb = re.search(r"ROW DATA SIZE \(Bytes\): (\d+)", out)
     if not cnt:
         raise RuntimeError(f"reference run failed: {out[-2000:]}")
-    return int(cnt.group(1)), float(tm[-1]) if tm else float("nan")
+    return {"count": int(cnt.group(1)), "seconds": float(tm[-1]) if tm else float("nan"),
+            "row_bytes": int(rb.group(1)) if rb else None}
+
+
+def run_ref(app, n, off, nbr, threads=4, layout="hybrid", k=None, timeout=600):
+    """(count, seconds of the counting phase) of a reference program; see run_ref_stats."""
+    r = run_ref_stats(app, n, off, nbr, threads, layout, k, timeout)
+    return r["count"], r["seconds"]
+
+
+def run_ref_blocks(pairs, workdir):
+    """Run _ref/ref_blocks (the reference's block-bitset builders and intersections) over
+    (a, b) pairs of sorted distinct uint32 arrays; per pair returns a list of seven
+    (card, type, bytes): bitset_new(a), bitset_new(b), new_type(a), new_type(b),
+    bn(a) & bn(b), uint(a) & bn(b), nt(a) & nt(b)."""
+    src, dst = os.path.join(workdir, "pairs.bin"), os.path.join(workdir, "blocks.bin")
+    with open(src, "wb") as f:
+        for a, b in pairs:
+            for v in (a, b):
+                v = _u32(v)
+                f.write(np.uint32(len(v)).tobytes() + v.tobytes())
+    subprocess.run([os.path.join(REF_DIR, "ref_blocks"), src, dst], check=True, capture_output=True, timeout=600)
+    raw = open(dst, "rb").read()
+    pos, out = 0, []
+    for _ in pairs:
+        rec = []
+        for _ in range(7):
+            card, typ, nb = np.frombuffer(raw, np.uint64, 3, pos).tolist()
+            pos += 24
+            rec.append((card, typ, np.frombuffer(raw, np.uint8, nb, pos).copy()))
+            pos += nb
+        out.append(rec)
+    return out

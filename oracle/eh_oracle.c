@@ -453,12 +453,19 @@ int orc_row_type(const uint32_t *data, size_t len, int policy) {
     case ORC_POLICY_BITSET: return ORC_BITSET;
     default: break;
   }
+  switch (policy) {
+    case ORC_POLICY_BITSET_NEW: return ORC_BITSET_NEW;
+    case ORC_POLICY_NEW_TYPE: return ORC_NEW_TYPE;
+    default: break;
+  }
   if (len <= 1) return ORC_UINTEGER;
   uint32_t range = data[len - 1] - data[0];
   if (range == 0) return ORC_UINTEGER;
   double density = (double)len / (double)range;
   if (policy == ORC_POLICY_HYBRID)
     return density > (1.0 / 256.0) ? ORC_BITSET : ORC_UINTEGER;
+  if (policy == ORC_POLICY_HYBRID_BLOCK)
+    return density > (1.0 / 256.0) ? ORC_BITSET_NEW : ORC_UINTEGER;
   if (density > 1.0 / 32.0) return ORC_BITSET;
   if (density > 1.0 / 256.0) return ORC_PSHORT;
   return ORC_UINTEGER;
@@ -511,8 +518,53 @@ size_t orc_bitset_build(uint8_t *out, const uint32_t *data, size_t len) {
   return 8 + 8 * (size_t)(w1 - w0 + 1);
 }
 
+/* bitset_new::build (src/set/layouts/bitset_new.hpp:66-108), BLOCK_SIZE 128
+ * (src/common.hpp:29): the u32 ids (value >> 7) of the non-empty blocks, then
+ * each block's two 64-bit words (bit v & 127 of the block; word (v >> 6) & 1). */
+size_t orc_bitset_new_build(uint8_t *out, const uint32_t *data, size_t len) {
+  if (len == 0) return 0;
+  size_t nb = 0;
+  for (size_t i = 0; i < len; i++)
+    if (i == 0 || (data[i] >> 7) != (data[i - 1] >> 7)) nb++;
+  uint8_t *W = out + 4 * nb;
+  memset(W, 0, 16 * nb);
+  size_t b = 0;
+  for (size_t i = 0; i < len; i++) {
+    if (i > 0 && (data[i] >> 7) != (data[i - 1] >> 7)) b++;
+    st32(out + 4 * b, data[i] >> 7);
+    uint8_t *w = W + 16 * b + 8 * ((data[i] >> 6) & 1u);
+    st64(w, ld64(w) | (1ull << (data[i] & 63)));
+  }
+  return 20 * nb;
+}
+
+/* new_type::build (src/set/layouts/new_type.hpp:50-88): a block whose element
+ * count c has density (c < 2 ? 0 : c / 128) above BITSET_THRESHOLD = 1/16
+ * (src/common.hpp:30) goes to a bitset_new part, every other element to a uint
+ * part: [u64 uint bytes][uint part][bitset_new part]. */
+size_t orc_new_type_build(uint8_t *out, const uint32_t *data, size_t len) {
+  if (len == 0) return 0;
+  uint32_t *dense = (uint32_t *)malloc(4 * len);
+  size_t nu = 0, nd = 0;
+  uint8_t *U = out + 8;
+  for (size_t i = 0; i < len;) {
+    size_t j = i;
+    while (j < len && (data[j] >> 7) == (data[i] >> 7)) j++;
+    const double density = (j - i) < 2 ? 0.0 : (double)(j - i) / 128.0;
+    for (size_t t = i; t < j; t++) {
+      if (density > 1.0 / 16.0) dense[nd++] = data[t];
+      else st32(U + 4 * nu++, data[t]);
+    }
+    i = j;
+  }
+  st64(out, 4 * (uint64_t)nu);
+  size_t bytes = 8 + 4 * nu + orc_bitset_new_build(out + 8 + 4 * nu, dense, nd);
+  free(dense);
+  return bytes;
+}
+
 size_t orc_row_image_bound(const uint32_t *data, size_t len) {
-  size_t b = 8 + 4 + 6 * len + 16;
+  size_t b = 8 + 4 + 8 + 20 * len + 16;
   if (len) b += 8 * (size_t)((data[len - 1] >> 6) - (data[0] >> 6) + 1) + 8;
   return b;
 }
@@ -524,6 +576,7 @@ size_t orc_row_image_bound_policy(const uint32_t *data, size_t len, int policy) 
   int t = orc_row_type(data, len, policy);
   if (t == ORC_UINTEGER) b += 4 * len;
   else if (t == ORC_PSHORT) b += 6 * len; /* worst case: one partition per element */
+  else if (t == ORC_BITSET_NEW || t == ORC_NEW_TYPE) b += 8 + 20 * len; /* a block per element */
   else if (len) b += 8 * (size_t)((data[len - 1] >> 6) - (data[0] >> 6) + 1) + 8;
   return b;
 }
@@ -534,8 +587,10 @@ size_t orc_row_image_bound_policy(const uint32_t *data, size_t len, int policy) 
 static size_t flat_build(uint8_t *out, int type, const uint32_t *data, size_t len) {
   if (type == ORC_UINTEGER) return orc_uint_build(out, data, len);
   if (len == 0) return 0;
-  size_t nb = type == ORC_BITSET ? orc_bitset_build(out + 4, data, len)
-                                 : orc_pshort_build(out + 4, data, len);
+  size_t nb = type == ORC_BITSET       ? orc_bitset_build(out + 4, data, len)
+              : type == ORC_PSHORT     ? orc_pshort_build(out + 4, data, len)
+              : type == ORC_BITSET_NEW ? orc_bitset_new_build(out + 4, data, len)
+                                       : orc_new_type_build(out + 4, data, len);
   st32(out, (uint32_t)nb);
   return nb + 4;
 }
@@ -544,7 +599,7 @@ static size_t flat_build(uint8_t *out, int type, const uint32_t *data, size_t le
  * layouts a u64 type tag precedes the sub-layout (hybrid.hpp:132-166). */
 size_t orc_row_image(uint8_t *out, const uint32_t *data, size_t len, int policy) {
   int t = orc_row_type(data, len, policy);
-  if (policy == ORC_POLICY_HYBRID || policy == ORC_POLICY_HYBRID3) {
+  if (policy == ORC_POLICY_HYBRID || policy == ORC_POLICY_HYBRID3 || policy == ORC_POLICY_HYBRID_BLOCK) {
     st64(out, (uint64_t)t);
     return 8 + flat_build(out + 8, t, data, len);
   }
@@ -556,12 +611,15 @@ size_t orc_row_image(uint8_t *out, const uint32_t *data, size_t len, int policy)
 static void view_image(const uint8_t *img, uint64_t card, int policy, orc_set *s) {
   int t;
   const uint8_t *p = img;
-  if (policy == ORC_POLICY_HYBRID || policy == ORC_POLICY_HYBRID3) {
+  if (policy == ORC_POLICY_HYBRID || policy == ORC_POLICY_HYBRID3 || policy == ORC_POLICY_HYBRID_BLOCK) {
     t = (int)ld64(p);
     p += 8;
   } else {
-    t = policy == ORC_POLICY_UINT ? ORC_UINTEGER
-        : policy == ORC_POLICY_PSHORT ? ORC_PSHORT : ORC_BITSET;
+    t = policy == ORC_POLICY_UINT         ? ORC_UINTEGER
+        : policy == ORC_POLICY_PSHORT     ? ORC_PSHORT
+        : policy == ORC_POLICY_BITSET_NEW ? ORC_BITSET_NEW
+        : policy == ORC_POLICY_NEW_TYPE   ? ORC_NEW_TYPE
+                                          : ORC_BITSET;
   }
   s->type = t;
   s->cardinality = card;
@@ -598,6 +656,29 @@ size_t orc_decode(int type, const uint8_t *payload, size_t card, size_t nbytes, 
       }
     }
     return k;
+  }
+  if (type == ORC_BITSET_NEW) {
+    /* bitset_new::foreach (src/set/layouts/bitset_new.hpp:159-191) */
+    size_t nb = nbytes / 20;
+    for (size_t b = 0; b < nb; b++) {
+      uint64_t base = 128ull * ld32(payload + 4 * b);
+      for (int h = 0; h < 2; h++) {
+        uint64_t w = ld64(payload + 4 * nb + 16 * b + 8 * h);
+        while (w) {
+          out[k++] = (uint32_t)(base + 64 * h + (uint64_t)__builtin_ctzll(w));
+          w &= w - 1;
+        }
+      }
+    }
+    return k;
+  }
+  if (type == ORC_NEW_TYPE) {
+    /* new_type::foreach (src/set/layouts/new_type.hpp:135-152): uint part, then blocks */
+    if (nbytes == 0) return 0;
+    size_t ub = (size_t)ld64(payload);
+    memcpy(out, payload + 8, ub);
+    k = ub / 4;
+    return k + orc_decode(ORC_BITSET_NEW, payload + 8 + ub, card - k, nbytes - 8 - ub, out + k);
   }
   /* pshort */
   size_t i = 0;
@@ -917,6 +998,113 @@ static size_t isect_ps_bs(const uint8_t *a, size_t a_bytes, const uint8_t *bs, s
   return c;
 }
 
+/* bitset_new x bitset_new (src/set/ops/intersection.hpp:1338-1413): the blocks
+ * present on both sides, in id order (simd_intersect_offsets, :1174-1320), are
+ * AND-ed word by word (BLOCK_SIZE 128 < 256: two 64-bit words, :1354-1362).  The
+ * result is a bitset_new set of every matched block, empty ANDs included. */
+static size_t isect_bn_bn(const uint8_t *a, size_t a_bytes, const uint8_t *b, size_t b_bytes, uint8_t *out,
+                          size_t *out_bytes) {
+  const size_t na = a_bytes / 20, nb = b_bytes / 20;
+  const uint8_t *AW = a + 4 * na, *BW = b + 4 * nb;
+  size_t k = 0;
+  for (size_t i = 0, j = 0; i < na && j < nb;) {
+    uint32_t x = ld32(a + 4 * i), y = ld32(b + 4 * j);
+    if (x < y) i++;
+    else if (x > y) j++;
+    else { k++; i++; j++; }
+  }
+  size_t c = 0, t = 0;
+  for (size_t i = 0, j = 0; i < na && j < nb;) {
+    uint32_t x = ld32(a + 4 * i), y = ld32(b + 4 * j);
+    if (x < y) { i++; continue; }
+    if (x > y) { j++; continue; }
+    for (int h = 0; h < 2; h++) {
+      uint64_t r = ld64(AW + 16 * i + 8 * h) & ld64(BW + 16 * j + 8 * h);
+      if (out) st64(out + 4 * k + 16 * t + 8 * h, r);
+      c += (size_t)__builtin_popcountll(r);
+    }
+    if (out) st32(out + 4 * t, x);
+    t++; i++; j++;
+  }
+  *out_bytes = 20 * k;
+  return c;
+}
+
+/* uint x bitset_new (src/set/ops/intersection.hpp:1425-1509): each uint value
+ * whose block (value >> 7) is present probes its bit; the hits form a uint set. */
+static size_t isect_u_bn(const uint32_t *A, size_t la, const uint8_t *b, size_t b_bytes, uint32_t *out) {
+  const size_t nb = b_bytes / 20;
+  const uint8_t *BW = b + 4 * nb;
+  size_t c = 0, j = 0;
+  for (size_t i = 0; i < la && j < nb; i++) {
+    const uint32_t blk = A[i] >> 7;
+    while (j < nb && ld32(b + 4 * j) < blk) j++;
+    if (j < nb && ld32(b + 4 * j) == blk) {
+      const uint32_t v = A[i] & 127u;
+      if ((ld64(BW + 16 * j + 8 * (v >> 6)) >> (v & 63)) & 1) {
+        if (out) out[c] = A[i];
+        c++;
+      }
+    }
+  }
+  return c;
+}
+
+/* merge of disjoint sorted lists (distinct_merge_three_way, :1511-1553) */
+static void merge3(uint32_t *out, const uint32_t *x, size_t nx, const uint32_t *y, size_t ny, const uint32_t *z,
+                   size_t nz) {
+  size_t i = 0, j = 0, k = 0, w = 0;
+  while (i < nx || j < ny || k < nz) {
+    uint32_t vx = i < nx ? x[i] : 0xFFFFFFFFu, vy = j < ny ? y[j] : 0xFFFFFFFFu, vz = k < nz ? z[k] : 0xFFFFFFFFu;
+    if (i < nx && vx <= vy && vx <= vz) out[w++] = x[i++];
+    else if (j < ny && vy <= vz) out[w++] = y[j++];
+    else out[w++] = z[k++];
+  }
+}
+
+/* new_type x new_type (src/set/ops/intersection.hpp:1555-1649): uint parts
+ * against each other and against the other side's blocks, merged into the
+ * result's uint part; the block parts AND-ed into its bitset_new part.  When
+ * neither side has blocks the result is the uint intersection alone. */
+static size_t isect_nt_nt(const uint8_t *a, uint64_t a_card, size_t a_bytes, const uint8_t *b, uint64_t b_card,
+                          size_t b_bytes, uint8_t *out, size_t *out_bytes) {
+  (void)a_card; (void)b_card;
+  const size_t aub = (size_t)ld64(a), bub = (size_t)ld64(b);
+  const uint32_t *AU = (const uint32_t *)(a + 8), *BU = (const uint32_t *)(b + 8);
+  const size_t anu = aub / 4, bnu = bub / 4;
+  const uint8_t *AB = a + 8 + aub, *BB = b + 8 + bub;
+  const size_t abb = a_bytes - 8 - aub, bbb = b_bytes - 8 - bub;
+  size_t c = 0, nu = 0;
+  uint32_t *U = out ? (uint32_t *)(out + 8) : NULL;
+  if ((aub != 0 || bub != 0) && (abb != 0 || bbb != 0)) {
+    uint32_t *s1 = NULL, *s2 = NULL, *s3 = NULL;
+    if (out) {
+      s1 = (uint32_t *)malloc(4 * (anu + 1));
+      s2 = (uint32_t *)malloc(4 * (anu + 1));
+      s3 = (uint32_t *)malloc(4 * (bnu + 1));
+    }
+    size_t c1 = isect_select(AU, anu, BU, bnu, s1);
+    size_t c2 = bbb ? isect_u_bn(AU, anu, BB, bbb, s2) : 0;
+    size_t c3 = abb ? isect_u_bn(BU, bnu, AB, abb, s3) : 0;
+    nu = c1 + c2 + c3;
+    if (out) {
+      merge3(U, s1, c1, s2, c2, s3, c3);
+      free(s1); free(s2); free(s3);
+    }
+  } else if (abb == 0 && bbb == 0) {
+    nu = isect_select(AU, anu, BU, bnu, U);
+    if (out) st64(out, 4 * (uint64_t)nu);
+    *out_bytes = 8 + 4 * nu;
+    return nu;
+  }
+  c = nu;
+  if (out) st64(out, 4 * (uint64_t)nu);
+  size_t bsb = 0;
+  if (abb != 0 && bbb != 0) c += isect_bn_bn(AB, abb, BB, bbb, out ? out + 8 + 4 * nu : NULL, &bsb);
+  *out_bytes = 8 + 4 * nu + bsb;
+  return c;
+}
+
 /* Typed-set dispatcher (ops::set_intersect(Set<hybrid>*...),
  * src/set/ops/intersection.hpp:2066-2118, widened with the pshort pairs of
  * :829-1021,1062-1173,1321-1323).  Result types follow the reference: U for
@@ -959,6 +1147,19 @@ uint64_t orc_set_intersect(orc_set *C, const orc_set *A, const orc_set *B) {
   } else if (ta == ORC_BITSET && tb == ORC_PSHORT) {
     c = isect_ps_bs(B->data, B->number_of_bytes, A->data, A->number_of_bytes, (uint16_t *)o, &ob);
     rt = ORC_PSHORT;
+  } else if (ta == ORC_BITSET_NEW && tb == ORC_BITSET_NEW) {
+    c = isect_bn_bn(A->data, A->number_of_bytes, B->data, B->number_of_bytes, o, &ob);
+    rt = ORC_BITSET_NEW;
+  } else if (ta == ORC_UINTEGER && tb == ORC_BITSET_NEW) {
+    c = isect_u_bn((const uint32_t *)A->data, A->cardinality, B->data, B->number_of_bytes, (uint32_t *)o);
+    ob = 4 * c;
+  } else if (ta == ORC_BITSET_NEW && tb == ORC_UINTEGER) {
+    c = isect_u_bn((const uint32_t *)B->data, B->cardinality, A->data, A->number_of_bytes, (uint32_t *)o);
+    ob = 4 * c;
+  } else if (ta == ORC_NEW_TYPE && tb == ORC_NEW_TYPE) {
+    c = isect_nt_nt(A->data, A->cardinality, A->number_of_bytes, B->data, B->cardinality, B->number_of_bytes, o,
+                    &ob);
+    rt = ORC_NEW_TYPE;
   } else {
     fprintf(stderr, "orc_set_intersect: unknown type pair %d/%d\n", ta, tb);
     return 0;
@@ -1098,6 +1299,7 @@ void orc_matrix_type_counts(const orc_matrix *M, uint64_t counts[3]) {
     orc_matrix_get_row(M, i, &s);
     if (s.cardinality == 0) continue;
     if (s.type >= 0 && s.type <= 2) counts[s.type]++;
+    else if (s.type == ORC_BITSET_NEW || s.type == ORC_NEW_TYPE) counts[ORC_BITSET]++;
   }
 }
 
@@ -1118,6 +1320,28 @@ static void set_foreach(const orc_set *s, elem_fn fn, void *ctx) {
         int b = __builtin_ctzll(w);
         fn(ctx, (uint32_t)(64 * (w0 + i) + (uint64_t)b));
         w &= w - 1;
+      }
+    }
+  } else if (s->type == ORC_BITSET_NEW || s->type == ORC_NEW_TYPE) {
+    /* bitset_new::foreach, new_type::foreach (bitset_new.hpp:159-191, new_type.hpp:135-152) */
+    const uint8_t *p = s->data;
+    uint64_t nbytes = s->number_of_bytes;
+    if (s->type == ORC_NEW_TYPE) {
+      if (nbytes == 0) return;
+      uint64_t ub = ld64(p);
+      for (uint64_t i = 0; i < ub / 4; i++) fn(ctx, ld32(p + 8 + 4 * i));
+      p += 8 + ub;
+      nbytes -= 8 + ub;
+    }
+    uint64_t nb = nbytes / 20;
+    for (uint64_t b = 0; b < nb; b++) {
+      uint64_t base = 128ull * ld32(p + 4 * b);
+      for (int h = 0; h < 2; h++) {
+        uint64_t w = ld64(p + 4 * nb + 16 * b + 8 * h);
+        while (w) {
+          fn(ctx, (uint32_t)(base + 64 * h + (uint64_t)__builtin_ctzll(w)));
+          w &= w - 1;
+        }
       }
     }
   } else {
@@ -1258,7 +1482,7 @@ uint64_t orc_clique_count_sample(const orc_matrix *M, int k, int threads, uint32
                                  uint32_t phase, uint64_t *edges_done, double *seconds) {
   if (k < 3 || k > 31) return 0;
   if (threads < 1) threads = 1;
-  size_t bb = 8 * ((size_t)M->n / 64 + 4) + 8 * ((size_t)M->max_row + 16) + 64;
+  size_t bb = 8 * ((size_t)M->n / 64 + 4) + 24 * ((size_t)M->max_row + 16) + 64; /* new_type: 8 + 20 per element */
   uint8_t **bufs = (uint8_t **)malloc(sizeof(uint8_t *) * (size_t)threads);
   for (int t = 0; t < threads; t++) bufs[t] = (uint8_t *)malloc(bb * (size_t)k);
   uint64_t *cnt = (uint64_t *)calloc((size_t)threads * ORC_PAD, 8);
@@ -1297,7 +1521,7 @@ uint64_t orc_pair_mix(const orc_matrix *M, uint64_t mix[4]) {
       orc_matrix_get_row(M, buf[t], &B);
       tot++;
       if (B.cardinality == 0) { mix[3]++; continue; }
-      int a = A.type, b = B.type;
+      int a = A.type == ORC_BITSET_NEW ? ORC_BITSET : A.type, b = B.type == ORC_BITSET_NEW ? ORC_BITSET : B.type;
       if (a == ORC_UINTEGER && b == ORC_UINTEGER) mix[0]++;
       else if ((a == ORC_UINTEGER && b == ORC_BITSET) || (a == ORC_BITSET && b == ORC_UINTEGER)) mix[1]++;
       else if (a == ORC_BITSET && b == ORC_BITSET) mix[2]++;
@@ -1608,7 +1832,7 @@ uint64_t orc_count_rows(const orc_matrix *M, int k, int threads, uint64_t *per_r
     tc_ctx x = {M, cnt, edg, 1, 0, 0, NULL, 0, per_row};
     par_for_range(threads, 0, M->n, 100, tc_row, &x);
   } else {
-    size_t bb = 8 * ((size_t)M->n / 64 + 4) + 8 * ((size_t)M->max_row + 16) + 64;
+    size_t bb = 8 * ((size_t)M->n / 64 + 4) + 24 * ((size_t)M->max_row + 16) + 64; /* new_type: 8 + 20 per element */
     uint8_t **bufs = (uint8_t **)malloc(sizeof(uint8_t *) * (size_t)threads);
     for (int t = 0; t < threads; t++) bufs[t] = (uint8_t *)malloc(bb * (size_t)k);
     tc_ctx x = {M, cnt, edg, 1, 0, k, bufs, bb, per_row};

@@ -20,7 +20,8 @@ extern "C" {
 #endif
 
 /* Set type tags: same values as common::type (src/common.hpp:171-181). */
-enum { ORC_BITSET = 0, ORC_PSHORT = 1, ORC_UINTEGER = 2, ORC_HYBRID = 5 };
+enum { ORC_BITSET = 0, ORC_PSHORT = 1, ORC_UINTEGER = 2, ORC_HYBRID = 5, ORC_BITSET_NEW = 7,
+       ORC_NEW_TYPE = 8 };
 
 /* Whole-matrix layout policies.  UINT/PSHORT/BITSET are the single-layout
  * --layout=uint|pshort|bs paths (src/emptyheaded.hpp:45-56); HYBRID is the
@@ -28,6 +29,11 @@ enum { ORC_BITSET = 0, ORC_PSHORT = 1, ORC_UINTEGER = 2, ORC_HYBRID = 5 };
  * pshort band of compute_type (src/set/layouts/hybrid.hpp:55-63). */
 enum { ORC_POLICY_UINT = 0, ORC_POLICY_PSHORT = 1, ORC_POLICY_BITSET = 2,
        ORC_POLICY_HYBRID = 3, ORC_POLICY_HYBRID3 = 4 };
+/* Block-bitset policies (128-value blocks, src/common.hpp:28-30): every row
+ * bitset_new (--layout=bitset_new), every row new_type (--layout=new_type)
+ * (src/emptyheaded.hpp:60-65), and the hybrid rule built with NEW_BITSET, whose
+ * dense rows are bitset_new (src/set/layouts/hybrid.hpp:83-87). */
+enum { ORC_POLICY_BITSET_NEW = 7, ORC_POLICY_NEW_TYPE = 8, ORC_POLICY_HYBRID_BLOCK = 9 };
 
 /* Intersection algorithm identifiers for orc_isect_uint (tests pin each). */
 enum { ORC_ALG_SCALAR = 0, ORC_ALG_STANDARD = 1, ORC_ALG_GALLOPING = 2, ORC_ALG_SELECT = 3 };
@@ -53,9 +59,12 @@ int orc_row_type(const uint32_t *data, size_t len, int policy);
 size_t orc_uint_build(uint8_t *out, const uint32_t *data, size_t len);
 size_t orc_pshort_build(uint8_t *out, const uint32_t *data, size_t len);
 size_t orc_bitset_build(uint8_t *out, const uint32_t *data, size_t len);
+size_t orc_bitset_new_build(uint8_t *out, const uint32_t *data, size_t len);
+size_t orc_new_type_build(uint8_t *out, const uint32_t *data, size_t len);
 /* flattened row image as stored by SparseMatrix (hybrid: u64 tag + sub-layout) */
 size_t orc_row_image(uint8_t *out, const uint32_t *data, size_t len, int policy);
-/* decode a set payload of the given type into sorted uint32 values; returns count */
+/* decode a set payload of the given type into uint32 values in the reference's
+ * foreach order (sorted, except new_type: its uint part, then its blocks); returns count */
 size_t orc_decode(int type, const uint8_t *payload, size_t card, size_t nbytes, uint32_t *out);
 /* upper bound on the bytes orc_row_image writes for a row */
 size_t orc_row_image_bound(const uint32_t *data, size_t len);

@@ -512,3 +512,132 @@ def test_directed_restatement_matches_reference_converter(tmp_path):
     assert rn == n and np.array_equal(rids, ids)
     assert all(np.array_equal(a, b) for a, b in zip(routs, outs))
     assert all(np.array_equal(a, b) for a, b in zip(rins, ins))
+
+
+# ------------------------------------------------------------------ block-bitset layouts
+
+# ROW DATA SIZE the reference's triangle program prints on facebook.bin with one thread:
+# --layout=bitset_new, --layout=new_type, and --layout=hybrid built with -DNEW_BITSET
+FB_BLOCK_ROW_BYTES = {"bitset_new": 513116, "new_type": 277200, "hybrid_block": 461324}
+
+
+def test_block_row_bytes_fixture(facebook):
+    for layout, want in FB_BLOCK_ROW_BYTES.items():
+        M = O.Matrix(facebook["n"], facebook["off"], facebook["nbr"], layout)
+        assert M.bytes() == want and M.triangles(2)[0] == FB_TRIANGLES, layout
+        assert M.cliques(4, 8)[0] == FB_4CLIQUES, layout
+
+
+def _block_pairs(seed):
+    """Set pairs over every shape the block layouts distinguish: empty, singletons,
+    blocks with 1, 2, 8, 9 and 128 elements (the new_type density cut is c > 8),
+    sparse spans, dense runs and mixtures, ids up to 2^24."""
+    rng = np.random.default_rng(seed)
+
+    def one():
+        kind = int(rng.integers(0, 7))
+        if kind == 0:
+            return np.zeros(0, np.uint32)
+        if kind == 1:
+            return np.array([int(rng.integers(0, 1 << 24))], np.uint32)
+        if kind == 2:   # one block with c elements
+            c = int(rng.choice([1, 2, 8, 9, 10, 64, 128]))
+            base = 128 * int(rng.integers(0, 4096))
+            return np.sort(rng.choice(128, c, replace=False) + base).astype(np.uint32)
+        if kind == 3:   # sparse over a wide span
+            return np.unique(rng.integers(0, 1 << 24, int(rng.integers(1, 300)))).astype(np.uint32)
+        if kind == 4:   # dense run
+            s = int(rng.integers(0, 1 << 20))
+            return np.arange(s, s + int(rng.integers(1, 700)), dtype=np.uint32)
+        hi = int(rng.integers(256, 20000))
+        v = [rng.integers(0, hi, int(rng.integers(1, 400)))]
+        for _ in range(int(rng.integers(0, 4))):   # planted dense blocks
+            b = 128 * int(rng.integers(0, hi // 128 + 1))
+            v.append(b + rng.choice(128, int(rng.integers(2, 128)), replace=False))
+        return np.unique(np.concatenate(v)).astype(np.uint32)
+
+    pairs = [(one(), one()) for _ in range(250)]
+    x = np.unique(rng.integers(0, 5000, 800)).astype(np.uint32)
+    pairs.append((x, x))
+    pairs.append((x[::2].copy(), x))
+    return pairs
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(O.__file__), "_ref", "ref_blocks")),
+                    reason="reference block-layout driver not built (needs the reference tree)")
+def test_block_layouts_match_reference(tmp_path):
+    """bitset_new / new_type images and the three block intersections equal the
+    reference's byte for byte (src/set/layouts/bitset_new.hpp:66-108,
+    new_type.hpp:50-88, src/set/ops/intersection.hpp:1338-1649)."""
+    pairs = _block_pairs(3)
+    ref = O.run_ref_blocks(pairs, str(tmp_path))
+    for (a, b), rec in zip(pairs, ref):
+        imgs = [O.layout_build("bitset_new", a), O.layout_build("bitset_new", b),
+                O.layout_build("new_type", a), O.layout_build("new_type", b)]
+        for got, (card, _, want) in zip(imgs, rec[:4]):
+            assert np.array_equal(got, want), (len(a), len(b))
+        want = np.intersect1d(a, b)
+        for (ta, tb), (card, typ, rbytes) in zip(((O.BITSET_NEW, O.BITSET_NEW), (O.UINTEGER, O.BITSET_NEW),
+                                                  (O.NEW_TYPE, O.NEW_TYPE)), rec[4:]):
+            assert card == len(want), (ta, tb)
+            if len(a) == 0 or len(b) == 0:
+                assert len(rbytes) == 0 and typ == O.UINTEGER
+                continue
+            c, t, obytes = O.set_intersect_bytes(a, ta, b, tb)
+            assert c == card and t == typ and np.array_equal(obytes, rbytes), (ta, tb, len(a), len(b))
+
+
+def test_block_layouts_decode_and_count():
+    """Round trips and |A & B| on every block pair, against numpy."""
+    for a, b in _block_pairs(5):
+        for t in (O.BITSET_NEW, O.NEW_TYPE):
+            _, c, p = O.set_payload(a, t)
+            assert sorted(O.decode(t, p, c).tolist()) == a.tolist()
+        want = np.intersect1d(a, b)
+        for ta, tb in ((O.BITSET_NEW, O.BITSET_NEW), (O.UINTEGER, O.BITSET_NEW), (O.BITSET_NEW, O.UINTEGER),
+                       (O.NEW_TYPE, O.NEW_TYPE)):
+            c, vals, _ = O.set_intersect(a, ta, b, tb)
+            assert c == len(want)
+            if c:
+                assert sorted(vals.tolist()) == want.tolist()
+
+
+def test_new_type_density_cut():
+    """A block goes to the bitset_new part iff it holds more than 8 elements
+    (density c/128 > 1/16, new_type.hpp:61-63)."""
+    for c in (1, 2, 8, 9):
+        v = np.arange(256, 256 + c, dtype=np.uint32)
+        img = O.layout_build("new_type", v)
+        uint_bytes = int(np.frombuffer(img[:8].tobytes(), np.uint64)[0])
+        assert uint_bytes == (0 if c > 8 else 4 * c), c
+        assert len(img) == 8 + uint_bytes + (20 if c > 8 else 0)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference programs not built (needs the reference tree)")
+@pytest.mark.parametrize("layout", ["bitset_new", "new_type"])
+def test_block_matrices_match_reference_programs(layout, facebook):
+    """--layout=bitset_new / new_type: the reference's triangle count and ROW DATA SIZE
+    equal the oracle matrix's, on facebook and on RMAT-12.  The reference runs with one
+    thread: its block intersections take scratch from common::scratch_space[common::tid]
+    with tid a plain static 0 (src/common.hpp:75-77), so threads share it and race."""
+    src, dst = O.gen_rmat(12, 8, 7)
+    r = O.canonical_oriented(1 << 12, src, dst)
+    _, off, nbr = r["relabel"]
+    for n, o, nb in ((facebook["n"], facebook["off"], facebook["nbr"]), (1 << 12, off, nbr)):
+        ref = O.run_ref_stats("undirected_triangle_counting", n, o, nb, threads=1, layout=layout)
+        M = O.Matrix(n, o, nb, layout)
+        assert M.triangles(2)[0] == ref["count"] == O.brute_triangles(n, o, nb)
+        assert M.bytes() == ref["row_bytes"]
+    assert O.Matrix(facebook["n"], facebook["off"], facebook["nbr"], layout).bytes() == FB_BLOCK_ROW_BYTES[layout]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(O.__file__), "_ref",
+                                                    "undirected_triangle_counting_nb")),
+                    reason="reference NEW_BITSET program not built (needs the reference tree)")
+def test_hybrid_block_matches_reference_program(facebook):
+    """hybrid built with NEW_BITSET (dense rows bitset_new, hybrid.hpp:83-87)."""
+    ref = O.run_ref_stats("undirected_triangle_counting_nb", facebook["n"], facebook["off"], facebook["nbr"],
+                          threads=1, layout="hybrid")
+    M = O.Matrix(facebook["n"], facebook["off"], facebook["nbr"], "hybrid_block")
+    assert ref["count"] == FB_TRIANGLES and M.triangles(2)[0] == FB_TRIANGLES
+    assert M.bytes() == ref["row_bytes"] == FB_BLOCK_ROW_BYTES["hybrid_block"]
