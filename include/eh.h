@@ -36,7 +36,14 @@ typedef enum {
 } eh_status;
 
 /* Set type tags, same values as common::type (src/common.hpp:171-181). */
-typedef enum { EH_BITSET = 0, EH_PSHORT = 1, EH_UINTEGER = 2, EH_HYBRID = 5 } eh_set_type;
+typedef enum {
+  EH_BITSET = 0,
+  EH_PSHORT = 1,
+  EH_UINTEGER = 2,
+  EH_HYBRID = 5,
+  EH_BITSET_NEW = 7, /* 128-value blocks (src/set/layouts/bitset_new.hpp:66-108) */
+  EH_NEW_TYPE = 8    /* uint part + dense blocks (src/set/layouts/new_type.hpp:50-88) */
+} eh_set_type;
 
 /* Matrix layout policy (src/emptyheaded.hpp:45-56 --layout, plus the
  * selectors this engine adds). */
@@ -47,7 +54,14 @@ typedef enum {
   EH_LAYOUT_HYBRID = 3,   /* hybrid::get_type, uint/bitset (--layout=hybrid) */
   EH_LAYOUT_HYBRID3 = 4,  /* hybrid with compute_type's pshort band          */
   EH_LAYOUT_LEARNED = 5,  /* per-row choice by the learned selector          */
-  EH_LAYOUT_COST = 6      /* per-row choice by the byte-cost model           */
+  EH_LAYOUT_COST = 6,     /* per-row choice by the byte-cost model, with a
+                             bitset_new arm                                   */
+  EH_LAYOUT_BITSET_NEW = 7, /* every row bitset_new        (--layout=bitset_new,
+                               src/emptyheaded.hpp:63-65)                     */
+  EH_LAYOUT_NEW_TYPE = 8,   /* every row new_type          (--layout=new_type,
+                               src/emptyheaded.hpp:60-62)                     */
+  EH_LAYOUT_HYBRID_BLOCK = 9 /* hybrid::get_type built with NEW_BITSET: dense rows
+                                bitset_new (src/set/layouts/hybrid.hpp:83-87)  */
 } eh_layout_policy;
 
 /* Edge selection applied by the matrix build (SparseMatrix.hpp:249-266). */
@@ -72,7 +86,9 @@ typedef struct eh_batch eh_batch;
 
 /* Host view of one set, in the reference's in-memory formats (Set.hpp:17-21):
  * UINTEGER data = uint32[cardinality]; BITSET data = [u64 first word][u64 words];
- * PSHORT data = u16 stream of [prefix, count-1, low16...] partitions. */
+ * PSHORT data = u16 stream of [prefix, count-1, low16...] partitions;
+ * BITSET_NEW data = [u32 block ids][2 u64 words per block], 20 bytes per block;
+ * NEW_TYPE data = [u64 uint bytes][uint32 uint part][BITSET_NEW part]. */
 typedef struct {
   const void *data;
   uint64_t cardinality;
@@ -201,17 +217,21 @@ eh_status eh_matrix_info_get(eh_matrix *m, eh_matrix_info *out);
 eh_status eh_matrix_row(eh_matrix *m, uint32_t row, void *buf, size_t cap, eh_row_view *view);
 /* kept entries as a host CSR (row_off: n+1, nbrs: cardinality) */
 eh_status eh_matrix_download(eh_matrix *m, uint64_t *row_off, uint32_t *nbrs);
-/* per-row B-side layout types (n bytes, eh_set_type values) */
+/* per-row B-side layout types (n bytes, eh_set_type values; 3 is the uint layout
+ * searched by galloping) */
 eh_status eh_matrix_row_types(eh_matrix *m, uint8_t *types);
 void eh_matrix_free(eh_matrix *m);
 
 /* ---------------------------------------------------------------- set ops */
 /* ops::set_intersect(Set<X>* C, const Set<Y>* A, const Set<Z>* B)
- * (src/set/ops/intersection.hpp:829,880,956,1019,1023,1059,1062,1321,1325,2067):
- * intersect two host sets on the device.  materialize=0 only counts (out may
- * be NULL); otherwise the result is written to out_buf in the reference's
- * result layout (U for pairs with a uint side, BS for BSxBS, PS for PSxPS and
- * PSxBS) and described by *out. */
+ * (src/set/ops/intersection.hpp:829,880,956,1019,1023,1059,1062,1321,1325,
+ * 1364,1476,1507,1555,2067): intersect two host sets on the device.
+ * materialize=0 only counts (out may be NULL); otherwise the result is written
+ * to out_buf in the reference's result layout (U for pairs with a uint side,
+ * BS for BSxBS, PS for PSxPS and PSxBS, BITSET_NEW for BITSET_NEW x BITSET_NEW,
+ * NEW_TYPE for NEW_TYPE x NEW_TYPE) and described by *out.  Block sets pair only
+ * as the reference's overloads do: BITSET_NEW with BITSET_NEW or UINTEGER,
+ * NEW_TYPE with NEW_TYPE. */
 eh_status eh_intersect(eh_ctx *ctx, const eh_row_view *a, const eh_row_view *b, int materialize,
                        void *out_buf, size_t out_cap, eh_row_view *out);
 /* Many pairs of rows of one matrix at once: counts[p] = |row(a[p]) & row(b[p])|. */

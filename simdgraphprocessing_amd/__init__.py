@@ -14,10 +14,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libeh.so")
 
-EH_BITSET, EH_PSHORT, EH_UINTEGER = 0, 1, 2
+EH_BITSET, EH_PSHORT, EH_UINTEGER, EH_BITSET_NEW, EH_NEW_TYPE = 0, 1, 2, 7, 8
 TYPE_NAMES = {0: "bitset", 1: "pshort", 2: "uint"}
 LAYOUTS = {"uint": 0, "pshort": 1, "bs": 2, "bitset": 2, "hybrid": 3, "hybrid3": 4, "learned": 5,
-           "cost": 6}
+           "cost": 6, "bitset_new": 7, "new_type": 8, "hybrid_block": 9}
 FILTERS = {"all": 0, "lower": 1, "mask": 2}
 SCHEDULES = {"plain": 0, "tuned": 1}
 ORDERINGS = {"degree": 0, "rev_degree": 1, "strong_run": 2, "bfs": 3, "shingles": 4, "random": 5, "the_game": 6}

@@ -3,12 +3,13 @@
 // Restates SparseMatrix<T,R>::build -> from_symmetric_noattribute_graph -> pack_data
 // (src/SparseMatrix.hpp:249-266,268-290,406-494) with the per-row layout choice of
 // hybrid::get_type (src/set/layouts/hybrid.hpp:73-100) and the layout encoders
-// uinteger/pshort/bitset (uinteger.hpp:49-59, pshort.hpp:46-91, bitset.hpp:65-109).
-// Passes, all on the device:
+// uinteger/pshort/bitset (uinteger.hpp:49-59, pshort.hpp:46-91, bitset.hpp:65-109)
+// and the block layouts bitset_new/new_type (bitset_new.hpp:66-118,
+// new_type.hpp:50-101).  Passes, all on the device:
 //   1. filter count   card[i] = kept entries of row i (all / nbr < node / mask)
 //   2. pack           rows copied into one 16-byte-aligned uint32 arena
-//   3. select         per-row type from (card, first, last, prefix partitions)
-//   4. encode         bitset words and pshort streams for the rows that use them
+//   3. select         per-row type from (card, first, last, prefix partitions, blocks)
+//   4. encode         bitset words, pshort streams and blocks for the rows that use them
 //   5. describe       one 16-byte RowDesc per row for the counting kernels
 #include <algorithm>
 
@@ -97,6 +98,8 @@ struct TypeArgs {
   uint32_t *parts;
   uint32_t *bs_n;   // words per BS row
   uint32_t *ps_n;   // shorts per PS row, padded to 4
+  uint32_t *blk_n;  // bid/bwd slots per BN / NT row (NT: 2 header slots + dense blocks)
+  uint32_t *ntu_n;  // ntu words per NT row (uint part padded to 4)
   uint64_t *refb;   // reference image bytes per row
   unsigned long long *type_rows;  // [3] non-empty rows per type
 };
@@ -120,6 +123,19 @@ __global__ void k_select(TypeArgs a) {
         parts = 1;
         for (uint32_t k = 1; k < c; k++) parts += (r[k] >> 16) != (r[k - 1] >> 16);
       }
+      // 128-value blocks: all of them (bitset_new), and new_type's split into blocks with
+      // more than 8 elements (density c/128 > 1/16, new_type.hpp:61-63) and a uint part
+      uint32_t nblk = 0, ndense = 0, nsparse = 0;
+      const bool need_blocks = a.policy >= EH_LAYOUT_COST;
+      if (c && need_blocks) {
+        uint32_t run = 1;
+        for (uint32_t k = 1; k <= c; k++) {
+          if (k < c && (r[k] >> 7) == (r[k - 1] >> 7)) { run++; continue; }
+          nblk++;
+          if (run > 8u) ndense++; else nsparse += run;
+          run = 1;
+        }
+      }
       uint32_t t = EH_U;
       switch (a.policy) {
         case EH_LAYOUT_UINT: t = EH_U; break;
@@ -140,14 +156,22 @@ __global__ void k_select(TypeArgs a) {
           break;
         }
         case EH_LAYOUT_COST: {
-          // fewest B-side bytes: uint 4c, pshort 2c + 4 parts, bitset 8 words
-          const uint64_t bu = 4ull * c, bp = 2ull * c + 4ull * parts, bb = 8ull * nw;
+          // fewest B-side bytes: uint 4c, pshort 2c + 4 parts, bitset 8 words,
+          // bitset_new 20 per block
+          const uint64_t bu = 4ull * c, bp = 2ull * c + 4ull * parts, bb = 8ull * nw, bn = 20ull * nblk;
           t = EH_U;
           uint64_t best = bu;
           if (c > 1 && bp < best) { t = EH_PS; best = bp; }
           if (c > 1 && bb < best) { t = EH_BS; best = bb; }
+          if (c > 1 && bn < best) { t = EH_BN; best = bn; }
           break;
         }
+        case EH_LAYOUT_BITSET_NEW: t = EH_BN; break;
+        case EH_LAYOUT_NEW_TYPE: t = EH_NT; break;
+        case EH_LAYOUT_HYBRID_BLOCK:
+          t = EH_U;
+          if (c > 1 && hi - lo > 0) t = ((double)c / (double)(hi - lo)) > (1.0 / 256.0) ? EH_BN : EH_U;
+          break;
         case EH_LAYOUT_LEARNED:
           t = a.forced ? a.forced[i] : EH_U;
           if (c <= 1) t = EH_U;
@@ -159,14 +183,22 @@ __global__ void k_select(TypeArgs a) {
       const uint32_t shorts = c + 2u * parts;
       a.bs_n[i] = (t == EH_BS && c) ? nw : 0u;
       a.ps_n[i] = (t == EH_PS && c) ? ((shorts + 3u) & ~3u) : 0u;
+      a.blk_n[i] = !c ? 0u : t == EH_BN ? nblk : t == EH_NT ? EH_NT_HEADER + ndense : 0u;
+      a.ntu_n[i] = (t == EH_NT && c) ? ((nsparse + 3u) & ~3u) : 0u;
       // reference row image: optional u64 tag, then the flattened sub-layout
-      const bool tagged = a.policy >= EH_LAYOUT_HYBRID;
+      // (hybrid.hpp:132-166; block rows: [u32 bytes][image], new_type.hpp:90-101)
+      const bool tagged = a.policy == EH_LAYOUT_HYBRID || a.policy == EH_LAYOUT_HYBRID3 ||
+                          a.policy == EH_LAYOUT_LEARNED || a.policy == EH_LAYOUT_COST ||
+                          a.policy == EH_LAYOUT_HYBRID_BLOCK;
       uint64_t bytes = tagged ? 8ull : 0ull;
-      if (is_uint_type(t)) bytes += 4ull * c;
+      if (t == EH_BN) bytes += c ? 4ull + 20ull * nblk : 0ull;
+      else if (t == EH_NT) bytes += c ? 4ull + 8ull + 4ull * nsparse + 20ull * ndense : 0ull;
+      else if (is_uint_type(t)) bytes += 4ull * c;
       else if (t == EH_BS) bytes += c ? 4ull + 8ull + 8ull * nw : 0ull;
       else bytes += c ? 4ull + 2ull * shorts : 0ull;
       a.refb[i] = bytes;
-      if (c) counted = t == EH_UG ? EH_U : t;
+      // block rows are counted with the bitsets
+      if (c) counted = t == EH_UG ? EH_U : is_block_type(t) ? EH_BS : t;
     }
     for (uint32_t q = 0; q < 3; q++) {
       const uint64_t b = __ballot(counted == q);
@@ -226,9 +258,52 @@ __global__ void k_fill_pshort(const uint32_t *card, const uint64_t *poff, const 
   }
 }
 
+// one thread per bitset_new / new_type row: the blocks in id order, each with its two
+// words (bit v & 127; word (v >> 6) & 1), and new_type's sparse elements in ntu.  A
+// new_type row's first two slots hold [uint part count, uint part offset / 4].
+__global__ void k_fill_blocks(const uint32_t *card, const uint64_t *poff, const uint32_t *unbr, const uint8_t *type,
+                              const uint64_t *blkoff, const uint64_t *ntuoff, const uint32_t *ntu_n, uint32_t n,
+                              uint32_t *bid, uint64_t *bwd, uint32_t *ntu) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t t = type[i];
+    const uint32_t c = card[i];
+    if (!is_block_type(t) || !c) continue;
+    const uint32_t *r = unbr + poff[i];
+    uint64_t slot = blkoff[i];
+    uint32_t *U = ntu + ntuoff[i];
+    uint32_t nu = 0;
+    if (t == EH_NT) slot += EH_NT_HEADER;
+    uint32_t k = 0;
+    while (k < c) {
+      const uint32_t b = r[k] >> 7;
+      uint32_t e = k;
+      while (e < c && (r[e] >> 7) == b) e++;
+      if (t == EH_BN || e - k > 8u) {
+        uint64_t w[2] = {0ull, 0ull};
+        for (uint32_t q = k; q < e; q++) w[(r[q] >> 6) & 1u] |= 1ull << (r[q] & 63u);
+        bid[slot] = b;
+        bwd[2 * slot] = w[0];
+        bwd[2 * slot + 1] = w[1];
+        slot++;
+      } else {
+        for (uint32_t q = k; q < e; q++) U[nu++] = r[q];
+      }
+      k = e;
+    }
+    if (t == EH_NT) {
+      const uint64_t h = blkoff[i];
+      bid[h] = nu;
+      bid[h + 1] = (uint32_t)(ntuoff[i] >> 2);
+      bwd[2 * h] = bwd[2 * h + 1] = bwd[2 * h + 2] = bwd[2 * h + 3] = 0ull;
+      for (uint32_t q = nu; q < ntu_n[i]; q++) U[q] = EH_PAD_VALUE;
+    }
+  }
+}
+
 __global__ void k_desc(const uint32_t *card, const uint64_t *poff, const uint32_t *unbr, const uint8_t *type,
-                       const uint32_t *parts, const uint32_t *bs_n, const uint32_t *ps_n, const uint64_t *bsoff,
-                       const uint64_t *psoff, uint32_t n, RowDesc *desc) {
+                       const uint32_t *parts, const uint32_t *bs_n, const uint32_t *blk_n, const uint64_t *bsoff,
+                       const uint64_t *psoff, const uint64_t *blkoff, uint32_t n, RowDesc *desc) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t c = card[i];
@@ -239,17 +314,20 @@ __global__ void k_desc(const uint32_t *card, const uint64_t *poff, const uint32_
     d.off = (uint32_t)(poff[i] >> 2);
     d.tinfo = t;  // empty rows keep their tag and carry no payload
     if (c && t == EH_BS) {
-      d.tinfo = EH_BS | (bs_n[i] << 2);
+      d.tinfo = make_tinfo(EH_BS, bs_n[i]);
       d.w0 = unbr[poff[i]] >> 6;
       d.off = (uint32_t)bsoff[i];
     } else if (c && t == EH_PS) {
       const uint32_t shorts = c + 2u * parts[i];
-      d.tinfo = EH_PS | (shorts << 2);
+      d.tinfo = make_tinfo(EH_PS, shorts);
       d.w0 = parts[i];
       d.off = (uint32_t)(psoff[i] >> 2);
+    } else if (c && is_block_type(t)) {
+      // the canonical list stays at off; the blocks start at slot w0
+      d.tinfo = make_tinfo(t, blk_n[i] - (t == EH_NT ? EH_NT_HEADER : 0u));
+      d.w0 = (uint32_t)blkoff[i];
     }
     desc[i] = d;
-    (void)ps_n;
   }
 }
 
@@ -281,7 +359,7 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
                            eh_edge_filter filter, const uint8_t *d_mask, eh_layout_policy policy,
                            const uint32_t *d_graph_of, eh_matrix **out) {
   (void)entries;
-  if (policy < EH_LAYOUT_UINT || policy > EH_LAYOUT_COST) { eh_set_error("unknown layout policy %d", (int)policy); return EH_ERR_INVALID; }
+  if (policy < EH_LAYOUT_UINT || policy > EH_LAYOUT_HYBRID_BLOCK) { eh_set_error("unknown layout policy %d", (int)policy); return EH_ERR_INVALID; }
   if (filter < EH_FILTER_ALL || filter > EH_FILTER_MASK) { eh_set_error("unknown edge filter %d", (int)filter); return EH_ERR_INVALID; }
   if (filter == EH_FILTER_MASK && !d_mask) { eh_set_error("EH_FILTER_MASK needs an edge mask"); return EH_ERR_INVALID; }
   hipStream_t st = ctx->stream;
@@ -292,7 +370,7 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   m->filter = filter;
   eh_status s = EH_OK;
   auto fail = [&](eh_status e) { delete m; return e; };
-  DevBuf pc, poff, type, parts, bs_n, ps_n, bsoff, psoff, refb, stats, wed;
+  DevBuf pc, poff, type, parts, bs_n, ps_n, blk_n, ntu_n, bsoff, psoff, blkoff, ntuoff, refb, stats, wed;
   if ((s = m->dcard.alloc(4ull * n + 4)) != EH_OK) return fail(s);
   if ((s = m->daoff.alloc(4ull * n + 4)) != EH_OK) return fail(s);
   if ((s = pc.alloc(4ull * n + 8)) != EH_OK) return fail(s);
@@ -321,11 +399,15 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   if ((s = parts.alloc(4ull * n + 4)) != EH_OK) return fail(s);
   if ((s = bs_n.alloc(4ull * n + 8)) != EH_OK) return fail(s);
   if ((s = ps_n.alloc(4ull * n + 8)) != EH_OK) return fail(s);
+  if ((s = blk_n.alloc(4ull * n + 8)) != EH_OK) return fail(s);
+  if ((s = ntu_n.alloc(4ull * n + 8)) != EH_OK) return fail(s);
   if ((s = refb.alloc(8ull * n + 8)) != EH_OK) return fail(s);
   if ((s = stats.alloc(64)) != EH_OK) return fail(s);
   if (hipMemsetAsync(stats.p, 0, 64, st) != hipSuccess) return fail(EH_ERR_HIP);
   if (hipMemsetAsync(bs_n.p, 0, 4ull * n + 8, st) != hipSuccess) return fail(EH_ERR_HIP);
   if (hipMemsetAsync(ps_n.p, 0, 4ull * n + 8, st) != hipSuccess) return fail(EH_ERR_HIP);
+  if (hipMemsetAsync(blk_n.p, 0, 4ull * n + 8, st) != hipSuccess) return fail(EH_ERR_HIP);
+  if (hipMemsetAsync(ntu_n.p, 0, 4ull * n + 8, st) != hipSuccess) return fail(EH_ERR_HIP);
   TypeArgs ta;
   ta.card = m->dcard.as<uint32_t>();
   ta.poff = poff.as<uint64_t>();
@@ -337,6 +419,8 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   ta.parts = parts.as<uint32_t>();
   ta.bs_n = bs_n.as<uint32_t>();
   ta.ps_n = ps_n.as<uint32_t>();
+  ta.blk_n = blk_n.as<uint32_t>();
+  ta.ntu_n = ntu_n.as<uint32_t>();
   ta.refb = refb.as<uint64_t>();
   ta.type_rows = stats.as<unsigned long long>();
   k_select<<<grid_for(n), 256, 0, st>>>(ta);
@@ -344,15 +428,26 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   if ((s = psoff.alloc(8ull * n + 16)) != EH_OK) return fail(s);
   if ((s = prim_exscan_u32_to_u64(ctx, bs_n.as<uint32_t>(), bsoff.as<uint64_t>(), (uint64_t)n + 1)) != EH_OK) return fail(s);
   if ((s = prim_exscan_u32_to_u64(ctx, ps_n.as<uint32_t>(), psoff.as<uint64_t>(), (uint64_t)n + 1)) != EH_OK) return fail(s);
-  uint64_t hb[2] = {0, 0};
+  if ((s = blkoff.alloc(8ull * n + 16)) != EH_OK) return fail(s);
+  if ((s = ntuoff.alloc(8ull * n + 16)) != EH_OK) return fail(s);
+  if ((s = prim_exscan_u32_to_u64(ctx, blk_n.as<uint32_t>(), blkoff.as<uint64_t>(), (uint64_t)n + 1)) != EH_OK) return fail(s);
+  if ((s = prim_exscan_u32_to_u64(ctx, ntu_n.as<uint32_t>(), ntuoff.as<uint64_t>(), (uint64_t)n + 1)) != EH_OK) return fail(s);
+  uint64_t hb[4] = {0, 0, 0, 0};
   if (hipMemcpyAsync(&hb[0], bsoff.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(EH_ERR_HIP);
   if (hipMemcpyAsync(&hb[1], psoff.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(EH_ERR_HIP);
+  if (hipMemcpyAsync(&hb[2], blkoff.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(EH_ERR_HIP);
+  if (hipMemcpyAsync(&hb[3], ntuoff.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(EH_ERR_HIP);
   unsigned long long trows[3];
   if (hipMemcpyAsync(trows, stats.p, 24, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(EH_ERR_HIP);
   if (hipStreamSynchronize(st) != hipSuccess) return fail(EH_ERR_HIP);
-  if (hb[0] >= (1ull << 32) || (hb[1] >> 2) >= (1ull << 32)) { eh_set_error("layout payload too large for 32-bit offsets"); return fail(EH_ERR_UNSUPPORTED); }
+  if (hb[0] >= (1ull << 32) || (hb[1] >> 2) >= (1ull << 32) || hb[2] >= (1ull << 32) || (hb[3] >> 2) >= (1ull << 32)) {
+    eh_set_error("layout payload too large for 32-bit offsets");
+    return fail(EH_ERR_UNSUPPORTED);
+  }
   m->bs_words = hb[0];
   m->ps_shorts = hb[1];
+  m->blk_slots = hb[2];
+  m->ntu_words = hb[3];
   for (int q = 0; q < 3; q++) m->rows_by_type[q] = trows[q];
   if ((s = m->dbsw.alloc(8 * hb[0] + 64)) != EH_OK) return fail(s);
   if ((s = m->dpsd.alloc(2 * hb[1] + 64)) != EH_OK) return fail(s);
@@ -366,10 +461,19 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
     k_fill_pshort<<<grid_for(n), 256, 0, st>>>(m->dcard.as<uint32_t>(), poff.as<uint64_t>(), m->dunbr.as<uint32_t>(),
                                                type.as<uint8_t>(), psoff.as<uint64_t>(), ps_n.as<uint32_t>(), n,
                                                m->dpsd.as<uint16_t>());
+  if ((s = m->dbid.alloc(4 * hb[2] + 64)) != EH_OK) return fail(s);
+  if ((s = m->dbwd.alloc(16 * hb[2] + 64)) != EH_OK) return fail(s);
+  if ((s = m->dntu.alloc(4 * hb[3] + 64)) != EH_OK) return fail(s);
+  if (hb[2])
+    k_fill_blocks<<<grid_for(n), 256, 0, st>>>(m->dcard.as<uint32_t>(), poff.as<uint64_t>(), m->dunbr.as<uint32_t>(),
+                                               type.as<uint8_t>(), blkoff.as<uint64_t>(), ntuoff.as<uint64_t>(),
+                                               ntu_n.as<uint32_t>(), n, m->dbid.as<uint32_t>(),
+                                               m->dbwd.as<uint64_t>(), m->dntu.as<uint32_t>());
   if ((s = m->ddesc.alloc(16ull * n + 16)) != EH_OK) return fail(s);
   k_desc<<<grid_for(n), 256, 0, st>>>(m->dcard.as<uint32_t>(), poff.as<uint64_t>(), m->dunbr.as<uint32_t>(),
-                                      type.as<uint8_t>(), parts.as<uint32_t>(), bs_n.as<uint32_t>(), ps_n.as<uint32_t>(),
-                                      bsoff.as<uint64_t>(), psoff.as<uint64_t>(), n, m->ddesc.as<RowDesc>());
+                                      type.as<uint8_t>(), parts.as<uint32_t>(), bs_n.as<uint32_t>(), blk_n.as<uint32_t>(),
+                                      bsoff.as<uint64_t>(), psoff.as<uint64_t>(), blkoff.as<uint64_t>(), n,
+                                      m->ddesc.as<RowDesc>());
   if (hipEventRecord(ctx->ev1, st) != hipSuccess) return fail(EH_ERR_HIP);
   if ((s = wed.alloc(8ull * n + 8)) != EH_OK) return fail(s);
   k_wedges<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(m->dcard.as<uint32_t>(), poff.as<uint64_t>(),
@@ -430,7 +534,7 @@ extern "C" eh_status eh_matrix_info_get(eh_matrix *m, eh_matrix_info *out) {
   out->rows_by_type[0] = m->rows_by_type[0];
   out->rows_by_type[1] = m->rows_by_type[1];
   out->rows_by_type[2] = m->rows_by_type[2];
-  out->payload_bytes = 8 * m->bs_words + 2 * m->ps_shorts + 4 * m->unbr_words;
+  out->payload_bytes = 8 * m->bs_words + 2 * m->ps_shorts + 4 * m->unbr_words + 20 * m->blk_slots + 4 * m->ntu_words;
   out->ref_row_bytes = m->ref_row_bytes;
   out->wedges = m->wedges;
   out->build_ms = m->build_ms;
@@ -444,19 +548,39 @@ extern "C" eh_status eh_matrix_row(eh_matrix *m, uint32_t row, void *buf, size_t
   RowDesc d;
   EH_HIP(hipMemcpy(&d, m->ddesc.as<RowDesc>() + row, sizeof(d), hipMemcpyDeviceToHost));
   const uint32_t t = desc_type(d);
+  const uint32_t nblk = is_block_type(t) ? desc_aux(d) : 0u;
+  uint32_t hdr[2] = {0u, 0u};  // new_type: uint part count, uint part offset / 4
+  if (d.card && t == EH_NT) EH_HIP(hipMemcpy(hdr, m->dbid.as<uint32_t>() + d.w0, 8, hipMemcpyDeviceToHost));
   uint64_t bytes = 0;
   if (d.card == 0) bytes = 0;
+  else if (t == EH_BN) bytes = 20ull * nblk;
+  else if (t == EH_NT) bytes = 8ull + 4ull * hdr[0] + 20ull * nblk;
   else if (is_uint_type(t)) bytes = 4ull * d.card;
   else if (t == EH_BS) bytes = 8ull + 8ull * desc_aux(d);
   else bytes = 2ull * desc_aux(d);
   view->data = buf;
   view->cardinality = d.card;
   view->number_of_bytes = bytes;
-  view->type = (int32_t)(is_uint_type(t) ? EH_U : t);
+  view->type = (int32_t)(t == EH_BN ? EH_SET_BITSET_NEW : t == EH_NT ? EH_SET_NEW_TYPE : is_uint_type(t) ? EH_U : t);
   view->reserved = 0;
   if (bytes > cap) { eh_set_error("row needs %llu bytes", (unsigned long long)bytes); return EH_ERR_CAPACITY; }
   if (!bytes) return EH_OK;
-  if (is_uint_type(t)) {
+  if (is_block_type(t)) {
+    // [new_type: u64 uint bytes, uint part][u32 block ids][2 u64 words per block]
+    uint8_t *p = (uint8_t *)buf;
+    uint64_t first = d.w0;
+    if (t == EH_NT) {
+      const uint64_t ub = 4ull * hdr[0];
+      memcpy(p, &ub, 8);
+      if (ub) EH_HIP(hipMemcpy(p + 8, m->dntu.as<uint32_t>() + 4ull * hdr[1], ub, hipMemcpyDeviceToHost));
+      p += 8 + ub;
+      first += EH_NT_HEADER;
+    }
+    if (nblk) {
+      EH_HIP(hipMemcpy(p, m->dbid.as<uint32_t>() + first, 4ull * nblk, hipMemcpyDeviceToHost));
+      EH_HIP(hipMemcpy(p + 4ull * nblk, m->dbwd.as<uint64_t>() + 2 * first, 16ull * nblk, hipMemcpyDeviceToHost));
+    }
+  } else if (is_uint_type(t)) {
     EH_HIP(hipMemcpy(buf, m->dunbr.as<uint32_t>() + 4ull * d.off, bytes, hipMemcpyDeviceToHost));
   } else if (t == EH_BS) {
     const uint64_t w0 = d.w0;
@@ -489,7 +613,10 @@ extern "C" eh_status eh_matrix_row_types(eh_matrix *m, uint8_t *types) {
   EH_HIP(hipSetDevice(m->ctx->device));
   std::vector<RowDesc> d(m->n);
   if (m->n) EH_HIP(hipMemcpy(d.data(), m->ddesc.p, 16ull * m->n, hipMemcpyDeviceToHost));
-  for (uint32_t i = 0; i < m->n; i++) types[i] = (uint8_t)desc_type(d[i]);
+  for (uint32_t i = 0; i < m->n; i++) {
+    const uint32_t t = desc_type(d[i]);
+    types[i] = (uint8_t)(t == EH_BN ? EH_SET_BITSET_NEW : t == EH_NT ? EH_SET_NEW_TYPE : t);
+  }
   return EH_OK;
 }
 

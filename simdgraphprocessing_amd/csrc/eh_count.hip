@@ -25,6 +25,10 @@
 //   PS x PS  prefix match, then 16-bit runs        (:829-878)
 //   U x PS   prefix-aligned run search             (:1062-1173)
 //   PS x BS  pshort elements probing bits          (:956-1017)
+//   BN x BN, NT x NT  blocks AND-ed with A's bitmap, new_type's uint part streamed
+//                                                  (:1338-1413,1555-1649)
+// A bitset_new / new_type B row facing a non-bitmap A is read through its
+// canonical uint list, which every row keeps.
 #include <type_traits>
 
 #include "eh_internal.hpp"
@@ -54,6 +58,11 @@ struct CountArgs {
   uint64_t task_base;               // index of tasks[0] in the whole schedule
   uint32_t stream_ratio;            // tuned: stream a bitset B row's uint list when
                                     // 16 * |B| < stream_ratio * probes (0: always probe)
+  const uint32_t *__restrict__ bid;  // block ids per slot (bitset_new / new_type rows)
+  const uint4 *__restrict__ bwd;     // the two 64-bit words of each block slot
+  const uint32_t *__restrict__ ntu;  // new_type uint parts
+  uint32_t blocks;                   // the matrix has bitset_new / new_type rows (selects
+                                     // the kernels with the block passes)
 };
 
 __device__ __forceinline__ uint32_t rfl(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -252,11 +261,60 @@ __device__ __forceinline__ uint64_t isect_b_bitset(const AState &A, const CountA
   return c;
 }
 
+// one 128-value block (slot s) AND-ed with A's LDS bitmap over their common words
+template <bool W>
+__device__ __forceinline__ uint64_t block_and(const AState &A, const CountArgs &a, uint32_t s) {
+  const uint32_t id = a.bid[s];
+  const uint4 q = a.bwd[s];
+  uint64_t c = 0;
+#pragma unroll
+  for (uint32_t h = 0; h < 2; h++) {
+    const uint32_t w = 2u * id + h;
+    if (w - A.aw0 >= A.anw) continue;
+    const uint32_t i2 = 2u * (w - A.aw0);
+    uint64_t r = ((uint64_t)A.bm[i2] | ((uint64_t)A.bm[i2 + 1] << 32)) &
+                 (h ? ((uint64_t)q.z | ((uint64_t)q.w << 32)) : ((uint64_t)q.x | ((uint64_t)q.y << 32)));
+    if constexpr (W) {
+      while (r) {
+        c += a.card[64u * w + (uint32_t)__builtin_ctzll(r)];
+        r &= r - 1;
+      }
+    } else {
+      c += (uint64_t)__popcll(r);
+    }
+  }
+  return c;
+}
+
+// |A & B| for a bitset_new / new_type B row against A's LDS bitmap: lanes over B's
+// blocks, then (new_type) over its uint part
+template <bool W>
+__device__ __forceinline__ uint64_t isect_b_blocks(const AState &A, const CountArgs &a, const RowDesc &dj,
+                                                   uint32_t lane) {
+  const uint32_t t = desc_type(dj), nb = desc_aux(dj);
+  const uint32_t first = dj.w0 + (t == EH_NT ? EH_NT_HEADER : 0u);
+  uint64_t c = 0;
+  for (uint32_t b = lane; b < nb; b += 64) c += block_and<W>(A, a, first + b);
+  if (t == EH_NT) {
+    const uint32_t nu = a.bid[dj.w0], uoff = a.bid[dj.w0 + 1];
+    const uint4 *U = reinterpret_cast<const uint4 *>(a.ntu) + uoff;
+    for (uint32_t v = lane; v < ((nu + 3u) >> 2); v += 64) {
+      const uint4 x = U[v];
+      c += hit_value<KIND_BITMAP, W>(A, a, x.x) + hit_value<KIND_BITMAP, W>(A, a, x.y) +
+           hit_value<KIND_BITMAP, W>(A, a, x.z) + hit_value<KIND_BITMAP, W>(A, a, x.w);
+    }
+  }
+  return c;
+}
+
 template <uint32_t KIND, bool W>
 __device__ __forceinline__ uint64_t isect_count(const AState &A, const CountArgs &a, const RowDesc &dj,
                                                 uint32_t lane) {
   if (dj.card == 0) return 0;
   const uint32_t t = desc_type(dj);
+  if constexpr (KIND == KIND_BITMAP) {
+    if (is_block_type(t)) return isect_b_blocks<W>(A, a, dj, lane);
+  }
   if (is_uint_type(t)) return isect_b_uint<KIND, W>(A, a, dj, lane);
   if (t == EH_PS) return isect_b_pshort<KIND, W>(A, a, dj, lane);
   return isect_b_bitset<KIND, W>(A, a, dj, lane);
@@ -524,7 +582,7 @@ __device__ uint64_t task_cliques4(const CountArgs &a, const AState &A, uint32_t 
 // wave keeps 64 independent loads in flight instead of one pair at a time.
 // pshort B rows are walked partition by partition, one pair after another.
 
-constexpr uint32_t PASS_PROBE = 0, PASS_AND = 1, PASS_STREAM = 2;
+constexpr uint32_t PASS_PROBE = 0, PASS_AND = 1, PASS_STREAM = 2, PASS_BLOCK = 3, PASS_NTSTREAM = 4;
 constexpr uint32_t META_WORDS = 260;
 constexpr uint32_t SMALL_META_WORDS = 260;
 
@@ -542,6 +600,8 @@ __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t *L, uint32_t 
 //   PROBE   A's element S.list[v0 + r] tested in B's bitset words (bsw[v1 ..], first word v2)
 //   AND     word v0 + r of A's LDS bitmap AND-ed with B's word (bsw[v1 ..], first word v2)
 //   STREAM  B's elements unbr[4 (v1 + r) .. +4] tested against A's LDS structure
+//   BLOCK   B's block slot v0 + r AND-ed with A's LDS bitmap
+//   NTSTREAM  B's new_type uint part ntu[4 (v1 + r) .. +4] tested against A
 // a lane's plain count in one pass adds at most 64 per element it visits, far below 2^32;
 // weighted sums (card of every hit) need 64 bits
 template <bool W>
@@ -571,8 +631,11 @@ __device__ __forceinline__ PassAcc<W> pass_elem(const CountArgs &a, const AState
     } else {
       return (uint32_t)__popcll(v);
     }
+  } else if constexpr (PASS == PASS_BLOCK) {
+    return (PassAcc<W>)block_and<W>(S, a, v0 + r);
   } else {
-    const uint4 x = reinterpret_cast<const uint4 *>(a.unbr)[(uint64_t)v1 + r];
+    const uint32_t *base = (PASS == PASS_NTSTREAM) ? a.ntu : a.unbr;
+    const uint4 x = reinterpret_cast<const uint4 *>(base)[(uint64_t)v1 + r];
     return (PassAcc<W>)(hit_value<KIND, W>(S, a, x.x) + hit_value<KIND, W>(S, a, x.y) +
                         hit_value<KIND, W>(S, a, x.z) + hit_value<KIND, W>(S, a, x.w));
   }
@@ -598,9 +661,9 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
 #pragma unroll UNROLL
     for (uint32_t r = lane; r < fq; r += 64) acc += pass_elem<PASS, KIND, W>(a, S, q0, q1, q2, r);
   }
-  // remainders start after the whole rounds: PROBE/AND index by v0, STREAM by v1
+  // remainders start after the whole rounds: PROBE/AND/BLOCK index by v0, the streams by v1
   c -= full;
-  if constexpr (PASS == PASS_STREAM) p1 += full;
+  if constexpr (PASS == PASS_STREAM || PASS == PASS_NTSTREAM) p1 += full;
   else p0 += full;
   if (__ballot(c != 0u) == 0ull) return acc;
   uint32_t tot;
@@ -627,7 +690,8 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
 // sum over keys K[kb..ke) of |S & row(key)|
 // UNROLL: depth of the whole-round loops (the triangle pass gains from 4; the
 // 4-clique level sets, with more live state, run best at 1)
-template <uint32_t KIND, bool W, int UNROLL = 1>
+// BLK: the matrix has bitset_new / new_type rows (their block passes are compiled in)
+template <uint32_t KIND, bool W, int UNROLL = 1, bool BLK = false>
 __device__ uint64_t sum_isect(const CountArgs &a, const AState &S, const uint32_t *K, uint32_t kb, uint32_t ke,
                               uint32_t lane, uint32_t *meta) {
   uint64_t acc = 0;
@@ -645,8 +709,23 @@ __device__ uint64_t sum_isect(const CountArgs &a, const AState &S, const uint32_
     // A's LDS structure (STREAM).  Every row keeps its uint list, so pshort rows and
     // bitset rows whose probes would outnumber their elements are streamed.
     uint32_t pc = 0, ps = 0, ac = 0, as = 0, sc = 0, soff = d.off;
+    uint32_t bc = 0, bs = 0, nc = 0, noff = 0;  // BLOCK and NTSTREAM items
     if (d.card) {
-      if (t == EH_BS) {
+      if (BLK && KIND == KIND_BITMAP && is_block_type(t)) {
+        // blocks AND-ed with the bitmap (+ new_type's uint part) unless the canonical
+        // list is shorter: a block and a 4-element vector cost about the same
+        const uint32_t nb = desc_aux(d);
+        uint32_t nu = 0, uoff = 0;
+        if (t == EH_NT) { nu = a.bid[d.w0]; uoff = a.bid[d.w0 + 1]; }
+        sc = (d.card + 3u) >> 2;
+        if (nb + ((nu + 3u) >> 2) < sc) {
+          sc = 0;
+          bc = nb;
+          bs = d.w0 + (t == EH_NT ? EH_NT_HEADER : 0u);
+          nc = (nu + 3u) >> 2;
+          noff = uoff;
+        }
+      } else if (t == EH_BS) {
         const uint32_t bw0 = d.w0, bnw = desc_aux(d);
         const uint64_t xlo = (uint64_t)bw0 * 64ull, xhi = (uint64_t)(bw0 + bnw) * 64ull;
         const uint32_t s = (xlo > S.lo + S.span) ? S.n : lds_lower_bound(S.list, S.n, (uint32_t)xlo);
@@ -674,6 +753,10 @@ __device__ uint64_t sum_isect(const CountArgs &a, const AState &S, const uint32_
     if constexpr (KIND == KIND_BITMAP)
       acc += flat_pass<PASS_AND, KIND, W, UNROLL>(a, S, ac, as, d.off, d.w0, lane, meta);
     acc += flat_pass<PASS_STREAM, KIND, W, UNROLL>(a, S, sc, 0u, soff, 0u, lane, meta);
+    if constexpr (BLK && KIND == KIND_BITMAP) {
+      acc += flat_pass<PASS_BLOCK, KIND, W, UNROLL>(a, S, bc, bs, 0u, 0u, lane, meta);
+      acc += flat_pass<PASS_NTSTREAM, KIND, W, UNROLL>(a, S, nc, 0u, noff, 0u, lane, meta);
+    }
   }
   return acc;
 }
@@ -710,7 +793,7 @@ __device__ __forceinline__ uint32_t stage_tuned(const CountArgs &a, uint32_t kin
   return used + (1u << lg);
 }
 
-template <uint32_t KIND>
+template <uint32_t KIND, bool BLK>
 __device__ uint64_t task_cliques4_tuned(const CountArgs &a, const AState &A, uint32_t jb, uint32_t je,
                                         uint32_t lane, uint32_t *free_lds, uint32_t *meta) {
   uint64_t acc = 0;
@@ -731,7 +814,7 @@ __device__ uint64_t task_cliques4_tuned(const CountArgs &a, const AState &A, uin
       AState C;
       if constexpr (KIND == KIND_BITMAP) {
         make_cstate<KIND_BITMAP>(A, clist, nc, cmemb, lane, C);
-        acc += sum_isect<KIND_BITMAP, false>(a, C, clist, 0, nc, lane, meta);
+        acc += sum_isect<KIND_BITMAP, false, 1, BLK>(a, C, clist, 0, nc, lane, meta);
       } else if constexpr (KIND == KIND_HASH) {
         make_cstate<KIND_LIST>(A, clist, nc, cmemb, lane, C);
         C.hlg = hash_lg(nc);
@@ -748,7 +831,7 @@ __device__ uint64_t task_cliques4_tuned(const CountArgs &a, const AState &A, uin
   return acc;
 }
 
-template <int K, bool W>
+template <int K, bool W, bool BLK>
 __device__ uint64_t run_task_tuned(const CountArgs &a, uint32_t kind, uint32_t i, uint32_t jb, uint32_t je,
                                    uint32_t lane, uint32_t *L) {
   uint32_t *meta = L;
@@ -757,24 +840,24 @@ __device__ uint64_t run_task_tuned(const CountArgs &a, uint32_t kind, uint32_t i
   const uint32_t used = stage_tuned(a, kind, i, lane, Ls, A);
   if constexpr (K == 3) {
     switch (kind) {
-      case KIND_BITMAP: return sum_isect<KIND_BITMAP, W, 4>(a, A, A.list, jb, je, lane, meta);
+      case KIND_BITMAP: return sum_isect<KIND_BITMAP, W, 4, BLK>(a, A, A.list, jb, je, lane, meta);
       case KIND_HASH: return sum_isect<KIND_HASH, W, 4>(a, A, A.list, jb, je, lane, meta);
       case KIND_LIST: return sum_isect<KIND_LIST, W, 4>(a, A, A.list, jb, je, lane, meta);
       default: return task_triangles<KIND_GLOBAL, W>(a, A, jb, je, lane);
     }
   } else {
     switch (kind) {
-      case KIND_BITMAP: return task_cliques4_tuned<KIND_BITMAP>(a, A, jb, je, lane, Ls + used, meta);
-      case KIND_HASH: return task_cliques4_tuned<KIND_HASH>(a, A, jb, je, lane, Ls + used, meta);
-      default: return task_cliques4_tuned<KIND_LIST>(a, A, jb, je, lane, Ls + used, meta);
+      case KIND_BITMAP: return task_cliques4_tuned<KIND_BITMAP, BLK>(a, A, jb, je, lane, Ls + used, meta);
+      case KIND_HASH: return task_cliques4_tuned<KIND_HASH, BLK>(a, A, jb, je, lane, Ls + used, meta);
+      default: return task_cliques4_tuned<KIND_LIST, BLK>(a, A, jb, je, lane, Ls + used, meta);
     }
   }
 }
 
-template <int K, bool W, bool TUNED>
+template <int K, bool W, bool TUNED, bool BLK>
 __device__ uint64_t run_task(const CountArgs &a, uint32_t kind, uint32_t i, uint32_t jb, uint32_t je,
                              uint32_t lane, uint32_t *L) {
-  if constexpr (TUNED) return run_task_tuned<K, W>(a, kind, i, jb, je, lane, L);
+  if constexpr (TUNED) return run_task_tuned<K, W, BLK>(a, kind, i, jb, je, lane, L);
   AState A;
   const uint32_t used = stage_a(a, kind, i, lane, L, A);
   if constexpr (K == 3) {
@@ -827,7 +910,7 @@ __device__ __forceinline__ uint32_t row_contains(const CountArgs &a, const RowDe
     const uint32_t w = (x >> 6) - d.w0;
     return (w < desc_aux(d)) ? (uint32_t)((a.bsw[d.off + w] >> (x & 63u)) & 1ull) : 0u;
   }
-  if (t == EH_U) return sorted_contains(a.unbr + (uint64_t)d.off * 4, d.card, x);
+  if (t == EH_U || is_block_type(t)) return sorted_contains(a.unbr + (uint64_t)d.off * 4, d.card, x);
   if (t == EH_UG) return gallop_contains(a.unbr + (uint64_t)d.off * 4, d.card, x);
   const uint16_t *S = a.psd + (uint64_t)d.off * 4;
   const uint32_t len = desc_aux(d), hx = x >> 16;
@@ -926,7 +1009,7 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
   if (lane == 0 && s) atomicAdd(&a.counts[0], (unsigned long long)s);
 }
 
-template <int K, bool W, bool TUNED>
+template <int K, bool W, bool TUNED, bool BLK>
 __global__ void __launch_bounds__(256) k_count(CountArgs a) {
   extern __shared__ uint32_t lds_all[];
   const uint32_t lane = lane_id();
@@ -949,11 +1032,11 @@ __global__ void __launch_bounds__(256) k_count(CountArgs a) {
     }
     if (a.task_cycles) {
       const uint64_t c0 = clock64();
-      acc += run_task<K, W, TUNED>(a, kind, i, jb, je, lane, L);
+      acc += run_task<K, W, TUNED, BLK>(a, kind, i, jb, je, lane, L);
       const uint64_t c1 = clock64();
       if (lane == 0) a.task_cycles[a.task_base + t] = c1 - c0;
     } else {
-      acc += run_task<K, W, TUNED>(a, kind, i, jb, je, lane, L);
+      acc += run_task<K, W, TUNED, BLK>(a, kind, i, jb, je, lane, L);
     }
   }
   const uint64_t s = wave_sum_u64(acc);
@@ -1007,6 +1090,7 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
         const uint32_t t = desc_type(d);
         uint64_t units;
         if (d.card == 0) { loc[3]++; units = 0; }
+        else if (is_block_type(t)) { loc[2]++; units = d.card; }  // block rows count with the bitsets
         else if (is_uint_type(t)) { loc[0]++; units = d.card; }
         else if (t == EH_PS) { loc[1]++; units = desc_aux(d); }
         else { loc[2]++; units = desc_aux(d); }
@@ -1054,7 +1138,7 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
       else if (nh <= s.lds_big) { kind = KIND_HASH; need = nh; }
       else { kind = KIND_LIST; need = nl; }
     } else {
-      if (ta == EH_BS) { kind = KIND_BITMAP; need = need_bitmap; }
+      if (ta == EH_BS || is_block_type(ta)) { kind = KIND_BITMAP; need = need_bitmap; }
       else if (ta == EH_PS) { kind = KIND_PSDIR; need = need_psdir; }
       else { kind = KIND_LIST; need = need_list; }
     }
@@ -1253,7 +1337,8 @@ namespace {
 template <int K, bool W>
 eh_status launch_count(eh_ctx *ctx, const CountArgs &base, const uint4 *tasks, uint64_t ntasks,
                        uint32_t lds_words, int waves, uint32_t *d_next, uint64_t task_base) {
-  auto kern = base.tuned ? k_count<K, W, true> : k_count<K, W, false>;
+  auto kern = !base.tuned ? k_count<K, W, false, false>
+                          : base.blocks ? k_count<K, W, true, true> : k_count<K, W, true, false>;
   if (ntasks == 0) return EH_OK;
   CountArgs a = base;
   a.task_base = task_base;
@@ -1310,6 +1395,10 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
   a.desc = m->ddesc.as<RowDesc>();
   a.bsw = m->dbsw.as<uint64_t>();
   a.psd = m->dpsd.as<uint16_t>();
+  a.bid = m->dbid.as<uint32_t>();
+  a.bwd = m->dbwd.as<uint4>();
+  a.ntu = m->dntu.as<uint32_t>();
+  a.blocks = m->blk_slots ? 1u : 0u;
   a.counts = reinterpret_cast<unsigned long long *>(d_counts);
   a.tuned = sched == EH_SCHED_TUNED ? 1u : 0u;
   a.graph_of = m->dgraph_of.p ? m->dgraph_of.as<uint32_t>() : nullptr;

@@ -9,8 +9,13 @@
 //   bsw[]        u64   bitset words (reference bitset: word index = value >> 6)
 //   psd[]        u16   pshort streams [prefix, count-1, low16 x count] per partition,
 //                      each row 8-byte aligned
+//   bid[], bwd[] u32 block id (value >> 7) and the block's two u64 words (uint4),
+//                      one slot per 128-value block of a bitset_new / new_type row;
+//                      a new_type row's slots start with two header slots
+//                      [uint part count, uint part offset in ntu / 4]
+//   ntu[]        u32   new_type uint parts, padded to 4 with 0xFFFFFFFF
 // The uint rows are always present (they drive the j-enumeration of a source
-// row); the pshort/bitset payloads exist for rows whose selected layout is
+// row); the pshort/bitset/block payloads exist for rows whose selected layout is
 // that type.
 #pragma once
 
@@ -28,21 +33,32 @@
 #define EH_PS 1u
 #define EH_BS 0u
 #define EH_UG 3u  // uint layout, membership by galloping (exponential) search from the front
+#define EH_BN 4u  // bitset_new: 128-value blocks (src/set/layouts/bitset_new.hpp:66-108)
+#define EH_NT 5u  // new_type: uint part + dense blocks (src/set/layouts/new_type.hpp:50-88)
+#define EH_NT_HEADER 2u  // header slots before a new_type row's blocks
 #define EH_PAD_VALUE 0xFFFFFFFFu
+// common::type values of the block layouts (src/common.hpp:179-180)
+#define EH_SET_BITSET_NEW 7
+#define EH_SET_NEW_TYPE 8
 
 struct RowDesc {
   uint32_t card;   // element count
-  uint32_t tinfo;  // bits 0-1: type (0 BS, 1 PS, 2 U, 3 U searched by galloping);
-                   // bits 2-31: BS words / PS stream shorts
-  uint32_t w0;     // BS: first u64 word index; PS: number of partitions
-  uint32_t off;    // U: element offset/4; BS: word offset; PS: short offset/4
+  uint32_t tinfo;  // bits 0-2: type (0 BS, 1 PS, 2 U, 3 U searched by galloping,
+                   // 4 bitset_new, 5 new_type);
+                   // bits 3-31: BS words / PS stream shorts / blocks
+  uint32_t w0;     // BS: first u64 word index; PS: number of partitions;
+                   // BN, NT: first slot in bid/bwd
+  uint32_t off;    // U, UG, BN, NT: canonical list offset/4; BS: word offset; PS: short offset/4
 };
 static_assert(sizeof(RowDesc) == 16, "RowDesc must be 16 bytes");
 
-__host__ __device__ inline uint32_t desc_type(const RowDesc &d) { return d.tinfo & 3u; }
-// the two uint arms share one payload (the canonical row)
+__host__ __device__ inline uint32_t desc_type(const RowDesc &d) { return d.tinfo & 7u; }
+// rows whose canonical uint list sits at `off`: the two uint arms (that list is
+// their payload) and the block layouts (kept beside their blocks)
 __host__ __device__ inline bool is_uint_type(uint32_t t) { return t >= EH_U; }
-__host__ __device__ inline uint32_t desc_aux(const RowDesc &d) { return d.tinfo >> 2; }
+__host__ __device__ inline bool is_block_type(uint32_t t) { return t >= EH_BN; }
+__host__ __device__ inline uint32_t desc_aux(const RowDesc &d) { return d.tinfo >> 3; }
+__host__ __device__ inline uint32_t make_tinfo(uint32_t t, uint32_t aux) { return t | (aux << 3); }
 
 // ---------------------------------------------------------------- errors
 
@@ -155,11 +171,13 @@ struct eh_matrix {
   uint64_t unbr_words = 0;  // u32 entries in unbr (padded)
   uint64_t bs_words = 0;
   uint64_t ps_shorts = 0;
+  uint64_t blk_slots = 0;   // bid/bwd slots
+  uint64_t ntu_words = 0;   // u32 entries in ntu
   uint64_t rows_by_type[3] = {0, 0, 0};
   uint64_t ref_row_bytes = 0;
   uint64_t wedges = 0;
   double build_ms = 0;
-  DevBuf dcard, daoff, dunbr, ddesc, dbsw, dpsd;
+  DevBuf dcard, daoff, dunbr, ddesc, dbsw, dpsd, dbid, dbwd, dntu;
   DevBuf dgraph_of;  // u32 per row: graph id (batch arenas); empty => all 0
   Schedule sched[2][2];  // [triangles, 4-cliques][plain, tuned]
 };

@@ -3,13 +3,18 @@
 // batched form over rows of one matrix.
 //
 // eh_intersect takes two host sets in the reference's byte formats
-//   uint   : u32[card]
-//   bitset : [u64 first word][u64 words...]         (src/set/layouts/bitset.hpp:65-93)
-//   pshort : u16 stream [prefix, count-1, low16...]  (src/set/layouts/pshort.hpp:46-78)
+//   uint       : u32[card]
+//   bitset     : [u64 first word][u64 words...]         (src/set/layouts/bitset.hpp:65-93)
+//   pshort     : u16 stream [prefix, count-1, low16...]  (src/set/layouts/pshort.hpp:46-78)
+//   bitset_new : [u32 block ids][2 u64 words per block]  (bitset_new.hpp:66-108)
+//   new_type   : [u64 uint bytes][uint part][bitset_new part] (new_type.hpp:50-88)
 // uploads them, decodes both to sorted u32 lists, intersects, and (when asked)
 // encodes the result in the reference's result layout: uint for any pair with a
 // uint side, the AND over the overlapping word span for bitset x bitset, a
-// pshort stream (empty partitions dropped) for pshort x pshort and pshort x bitset.
+// pshort stream (empty partitions dropped) for pshort x pshort and pshort x bitset,
+// the matched blocks for bitset_new x bitset_new, and for new_type x new_type the
+// reference's split (uint parts against each other and the other side's blocks,
+// merged; blocks AND-ed).
 // Each call is one small chain of single-workgroup kernels; it serves wrappers and
 // tests, not the counting pass.
 //
@@ -88,6 +93,35 @@ k_decode_set(int type, const uint8_t *payload, uint64_t bytes, uint64_t card, ui
     }
     return;
   }
+  if (type == EH_SET_BITSET_NEW) {
+    // blocks in id order; a block's elements follow its popcount prefix
+    const uint64_t nb = bytes / 20;
+    const uint8_t *W = payload + 4 * nb;
+    uint32_t base = 0;
+    for (uint64_t c0 = 0; c0 < nb; c0 += blockDim.x) {
+      const uint64_t b = c0 + threadIdx.x;
+      uint64_t w0 = 0, w1 = 0;
+      uint32_t id = 0;
+      if (b < nb) {
+        id = reinterpret_cast<const uint32_t *>(payload)[b];
+        w0 = ld64u(W + 16 * b);
+        w1 = ld64u(W + 16 * b + 8);
+      }
+      uint32_t tot;
+      uint32_t pos = base + block_scan_excl((uint32_t)(__popcll(w0) + __popcll(w1)), sh, &tot);
+      for (uint32_t h = 0; h < 2; h++) {
+        uint64_t word = h ? w1 : w0;
+        while (word) {
+          const uint32_t bit = (uint32_t)__builtin_ctzll(word);
+          if (pos < card) out[pos] = 128u * id + 64u * h + bit;
+          pos++;
+          word &= word - 1;
+        }
+      }
+      base += tot;
+    }
+    return;
+  }
   // pshort: pscratch holds per partition {first short, element count, output base}
   const uint16_t *S = reinterpret_cast<const uint16_t *>(payload);
   const uint64_t ns = bytes / 2;
@@ -140,6 +174,93 @@ k_isect_lists(const uint32_t *S, uint32_t ns, const uint32_t *L, uint32_t nl, ui
     base += tot;
   }
   if (threadIdx.x == 0) res[0] = base;
+}
+
+// bitset_new x bitset_new (src/set/ops/intersection.hpp:1364-1413): the blocks present
+// on both sides, in id order, each AND-ed; out = [u32 ids x K][2 u64 words x K] with
+// K matched blocks (empty ANDs kept); res[0] = |A & B|, res[1] = K
+__global__ void __launch_bounds__(ISECT_THREADS)
+k_isect_bn_bn(const uint8_t *A, uint32_t na, const uint8_t *B, uint32_t nb, uint8_t *out, unsigned long long *res) {
+  __shared__ uint32_t sh[32];
+  const uint32_t *Aid = reinterpret_cast<const uint32_t *>(A), *Bid = reinterpret_cast<const uint32_t *>(B);
+  const uint8_t *AW = A + 4ull * na, *BW = B + 4ull * nb;
+  // pass 1: K
+  uint32_t K = 0;
+  for (uint32_t c0 = 0; c0 < na; c0 += blockDim.x) {
+    const uint32_t k = c0 + threadIdx.x;
+    const uint32_t hit = (k < na && list_contains(Bid, nb, Aid[k])) ? 1u : 0u;
+    uint32_t tot;
+    (void)block_scan_excl(hit, sh, &tot);
+    K += tot;
+  }
+  // pass 2: ids, words and the count
+  uint32_t base = 0, count = 0;
+  for (uint32_t c0 = 0; c0 < na; c0 += blockDim.x) {
+    const uint32_t k = c0 + threadIdx.x;
+    uint32_t hit = 0, j = 0;
+    if (k < na) {
+      const uint32_t x = Aid[k];
+      uint32_t lo = 0, len = nb;
+      while (len > 1) {
+        const uint32_t half = len >> 1;
+        lo = (Bid[lo + half] <= x) ? lo + half : lo;
+        len -= half;
+      }
+      hit = (nb && Bid[lo] == x) ? 1u : 0u;
+      j = lo;
+    }
+    uint32_t tot;
+    const uint32_t pos = base + block_scan_excl(hit, sh, &tot);
+    if (hit) {
+      const uint64_t r0 = ld64u(AW + 16ull * k) & ld64u(BW + 16ull * j);
+      const uint64_t r1 = ld64u(AW + 16ull * k + 8) & ld64u(BW + 16ull * j + 8);
+      count += (uint32_t)(__popcll(r0) + __popcll(r1));
+      if (out) {
+        uint32_t *o = reinterpret_cast<uint32_t *>(out);
+        o[pos] = Aid[k];
+        uint32_t *w = reinterpret_cast<uint32_t *>(out + 4ull * K + 16ull * pos);
+        w[0] = (uint32_t)r0;
+        w[1] = (uint32_t)(r0 >> 32);
+        w[2] = (uint32_t)r1;
+        w[3] = (uint32_t)(r1 >> 32);
+      }
+    }
+    base += tot;
+  }
+  uint32_t total;
+  (void)block_scan_excl(count, sh, &total);
+  if (threadIdx.x == 0) {
+    res[0] = total;
+    res[1] = K;
+  }
+}
+
+// merge of three disjoint sorted lists (distinct_merge_three_way,
+// src/set/ops/intersection.hpp:1511-1553): every element lands at its index plus the
+// number of smaller elements in the other two lists
+__device__ __forceinline__ uint32_t count_below(const uint32_t *L, uint32_t n, uint32_t x) {
+  uint32_t lo = 0, len = n;
+  while (len > 0) {
+    const uint32_t half = len >> 1;
+    if (L[lo + half] < x) { lo += half + 1; len -= half + 1; }
+    else len = half;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(ISECT_THREADS)
+k_merge3(const uint32_t *X, const uint32_t *Y, const uint32_t *Z, const unsigned long long *nx,
+         const unsigned long long *ny, const unsigned long long *nz, uint32_t *out) {
+  const uint32_t n[3] = {(uint32_t)nx[0], (uint32_t)ny[0], (uint32_t)nz[0]};
+  const uint32_t *L[3] = {X, Y, Z};
+  for (uint32_t q = 0; q < 3; q++)
+    for (uint32_t k = threadIdx.x; k < n[q]; k += blockDim.x) {
+      const uint32_t x = L[q][k];
+      uint32_t pos = k;
+      for (uint32_t o = 0; o < 3; o++)
+        if (o != q) pos += count_below(L[o], n[o], x);
+      out[pos] = x;
+    }
 }
 
 // bitset x bitset over the overlapping word span [start, start+total)
@@ -236,17 +357,207 @@ bool pshort_ok(const eh_row_view *v) {
   return i == ns && total == v->cardinality;
 }
 
+inline uint32_t ld32h(const uint8_t *p) { uint32_t v; memcpy(&v, p, 4); return v; }
+inline uint64_t ld64h(const uint8_t *p) { uint64_t v; memcpy(&v, p, 8); return v; }
+
+// a bitset_new image: whole 20-byte blocks, ids strictly increasing below 2^25, card bits
+bool bn_ok(const uint8_t *p, uint64_t bytes, uint64_t card) {
+  if (bytes % 20) return false;
+  const uint64_t nb = bytes / 20;
+  uint64_t total = 0;
+  for (uint64_t k = 0; k < nb; k++) {
+    const uint32_t id = ld32h(p + 4 * k);
+    if (id >= (1u << 25) || (k && id <= ld32h(p + 4 * (k - 1)))) return false;
+    total += (uint64_t)__builtin_popcountll(ld64h(p + 4 * nb + 16 * k)) +
+             (uint64_t)__builtin_popcountll(ld64h(p + 4 * nb + 16 * k + 8));
+  }
+  return total == card;
+}
+
+// a new_type image: [u64 uint bytes][strictly increasing uint part outside every block][bitset_new]
+bool nt_ok(const eh_row_view *v) {
+  const uint8_t *p = static_cast<const uint8_t *>(v->data);
+  const uint64_t bytes = v->number_of_bytes;
+  if (bytes < 8) return false;
+  const uint64_t ub = ld64h(p);
+  if (ub % 4 || ub > bytes - 8 || ub / 4 > v->cardinality) return false;
+  const uint64_t nu = ub / 4, bb = bytes - 8 - ub;
+  const uint8_t *B = p + 8 + ub;
+  if (!bn_ok(B, bb, v->cardinality - nu)) return false;
+  const uint64_t nb = bb / 20;
+  for (uint64_t k = 0; k < nu; k++) {
+    const uint32_t x = ld32h(p + 8 + 4 * k);
+    if (k && x <= ld32h(p + 8 + 4 * (k - 1))) return false;
+    for (uint64_t lo = 0, hi = nb; lo < hi;) {  // x's block must not be one of the blocks
+      const uint64_t mid = (lo + hi) / 2;
+      const uint32_t id = ld32h(B + 4 * mid);
+      if (id == (x >> 7)) return false;
+      if (id < (x >> 7)) lo = mid + 1; else hi = mid;
+    }
+  }
+  return true;
+}
+
 bool view_ok(const eh_row_view *v) {
   if (v->cardinality == 0) return true;
   if (!v->data || v->cardinality > 0xFFFFFFFFull) return false;
   if (v->type == EH_U) return true;
   if (v->type == EH_BS) return v->number_of_bytes >= 16 && (v->number_of_bytes - 8) % 8 == 0;
   if (v->type == EH_PS) return v->number_of_bytes >= 6 && v->number_of_bytes % 2 == 0 && pshort_ok(v);
+  if (v->type == EH_SET_BITSET_NEW)
+    return v->number_of_bytes >= 20 && bn_ok(static_cast<const uint8_t *>(v->data), v->number_of_bytes, v->cardinality);
+  if (v->type == EH_SET_NEW_TYPE) return nt_ok(v);
   return false;
 }
 
+bool is_block_set(int t) { return t == EH_SET_BITSET_NEW || t == EH_SET_NEW_TYPE; }
+
 uint64_t view_bytes(const eh_row_view *v) {
   return v->type == EH_U ? 4ull * v->cardinality : v->number_of_bytes;
+}
+
+// S & L of two device lists (S the shorter one keeps its order): R (capacity |S|), *res = count
+void isect_lists(hipStream_t st, const uint32_t *x, uint32_t nx, const uint32_t *y, uint32_t ny, uint32_t *R,
+                 unsigned long long *res) {
+  const bool xs = nx <= ny;
+  k_isect_lists<<<1, ISECT_THREADS, 0, st>>>(xs ? x : y, xs ? nx : ny, xs ? y : x, xs ? ny : nx, R, res);
+}
+
+// the block pairs the reference overloads: bitset_new x bitset_new
+// (intersection.hpp:1364-1413), uint x bitset_new (:1476-1509), new_type x new_type
+// (:1555-1649)
+eh_status intersect_blocks(eh_ctx *ctx, const eh_row_view *a, const eh_row_view *b, int materialize, void *out_buf,
+                           size_t out_cap, eh_row_view *out) {
+  hipStream_t st = ctx->stream;
+  const int ta = a->type, tb = b->type;
+  const bool bnbn = ta == EH_SET_BITSET_NEW && tb == EH_SET_BITSET_NEW;
+  const bool ubn = (ta == EH_U && tb == EH_SET_BITSET_NEW) || (ta == EH_SET_BITSET_NEW && tb == EH_U);
+  const bool ntnt = ta == EH_SET_NEW_TYPE && tb == EH_SET_NEW_TYPE;
+  if (!bnbn && !ubn && !ntnt) {
+    eh_set_error("no set_intersect overload for set types %d x %d", ta, tb);
+    return EH_ERR_INVALID;
+  }
+  const int rt = bnbn ? EH_SET_BITSET_NEW : ntnt ? EH_SET_NEW_TYPE : EH_U;
+  if (out) {
+    out->data = out_buf;
+    out->cardinality = 0;
+    out->number_of_bytes = 0;
+    out->type = rt;
+    out->reserved = 0;
+  }
+  // these overloads return an empty uint set when either side has no bytes
+  if (a->cardinality == 0 || b->cardinality == 0) {
+    if (out) out->type = EH_U;
+    return EH_OK;
+  }
+  const uint64_t ba = view_bytes(a), bb = view_bytes(b);
+  DevBuf pa, pb, res;
+  EH_CHECK(pa.alloc(ba + 16));
+  EH_CHECK(pb.alloc(bb + 16));
+  EH_CHECK(res.alloc(128));
+  EH_HIP(hipMemsetAsync(res.p, 0, 128, st));
+  EH_HIP(hipMemcpyAsync(pa.p, a->data, ba, hipMemcpyHostToDevice, st));
+  EH_HIP(hipMemcpyAsync(pb.p, b->data, bb, hipMemcpyHostToDevice, st));
+  unsigned long long *R = res.as<unsigned long long>();
+  unsigned long long h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+  if (bnbn) {
+    const uint32_t na = (uint32_t)(ba / 20), nb = (uint32_t)(bb / 20);
+    DevBuf o;
+    if (materialize) EH_CHECK(o.alloc(20ull * (na < nb ? na : nb) + 16));
+    k_isect_bn_bn<<<1, ISECT_THREADS, 0, st>>>(pa.as<uint8_t>(), na, pb.as<uint8_t>(), nb,
+                                               materialize ? o.as<uint8_t>() : nullptr, R);
+    EH_HIP(hipGetLastError());
+    EH_HIP(hipMemcpyAsync(h, R, 16, hipMemcpyDeviceToHost, st));
+    EH_HIP(hipStreamSynchronize(st));
+    if (out) out->cardinality = h[0];
+    if (!materialize) return EH_OK;
+    const uint64_t need = 20ull * h[1];
+    if (need > out_cap) { eh_set_error("result needs %llu bytes", (unsigned long long)need); return EH_ERR_CAPACITY; }
+    if (need) EH_HIP(hipMemcpy(out_buf, o.p, need, hipMemcpyDeviceToHost));
+    out->number_of_bytes = need;
+    return EH_OK;
+  }
+
+  if (ubn) {
+    const bool a_bn = ta == EH_SET_BITSET_NEW;
+    const uint64_t nbn = a_bn ? a->cardinality : b->cardinality, nu = a_bn ? b->cardinality : a->cardinality;
+    DevBuf lbn, r;
+    EH_CHECK(lbn.alloc(4 * nbn + 16));
+    EH_CHECK(r.alloc(4 * (nu < nbn ? nu : nbn) + 16));
+    k_decode_set<<<1, ISECT_THREADS, 0, st>>>(EH_SET_BITSET_NEW, a_bn ? pa.as<uint8_t>() : pb.as<uint8_t>(),
+                                              a_bn ? ba : bb, nbn, lbn.as<uint32_t>(), nullptr);
+    EH_HIP(hipGetLastError());
+    isect_lists(st, a_bn ? pb.as<uint32_t>() : pa.as<uint32_t>(), (uint32_t)nu, lbn.as<uint32_t>(), (uint32_t)nbn,
+                r.as<uint32_t>(), R);
+    EH_HIP(hipGetLastError());
+    EH_HIP(hipMemcpyAsync(h, R, 8, hipMemcpyDeviceToHost, st));
+    EH_HIP(hipStreamSynchronize(st));
+    if (out) out->cardinality = h[0];
+    if (!materialize) return EH_OK;
+    const uint64_t need = 4ull * h[0];
+    if (need > out_cap) { eh_set_error("result needs %llu bytes", (unsigned long long)need); return EH_ERR_CAPACITY; }
+    if (need) EH_HIP(hipMemcpy(out_buf, r.p, need, hipMemcpyDeviceToHost));
+    out->number_of_bytes = need;
+    return EH_OK;
+  }
+
+  // new_type x new_type: [u64 uint bytes][uint part][bitset_new part] on both sides
+  const uint8_t *ha = static_cast<const uint8_t *>(a->data), *hb = static_cast<const uint8_t *>(b->data);
+  const uint64_t uba = ld64h(ha), ubb = ld64h(hb);
+  const uint32_t nua = (uint32_t)(uba / 4), nub = (uint32_t)(ubb / 4);
+  const uint64_t bba = ba - 8 - uba, bbb = bb - 8 - ubb;
+  const uint32_t nba = (uint32_t)(bba / 20), nbb = (uint32_t)(bbb / 20);
+  const uint64_t cba = a->cardinality - nua, cbb = b->cardinality - nub;  // elements in blocks
+  const uint32_t *AI = reinterpret_cast<const uint32_t *>(pa.as<uint8_t>() + 8);
+  const uint32_t *BI = reinterpret_cast<const uint32_t *>(pb.as<uint8_t>() + 8);
+  const uint8_t *ABS = pa.as<uint8_t>() + 8 + uba, *BBS = pb.as<uint8_t>() + 8 + ubb;
+  const bool all = (uba || ubb) && (bba || bbb);
+  const bool uint_only = !bba && !bbb;
+  DevBuf r1, r2, r3, da, db, u, o;
+  EH_CHECK(r1.alloc(4ull * (nua < nub ? nua : nub) + 16));
+  EH_CHECK(r2.alloc(4ull * nua + 16));
+  EH_CHECK(r3.alloc(4ull * nub + 16));
+  EH_CHECK(u.alloc(4ull * (nua + nub) + 16));
+  isect_lists(st, AI, nua, BI, nub, r1.as<uint32_t>(), R + 0);
+  EH_HIP(hipGetLastError());
+  if (all) {
+    // A's uint part against B's blocks, B's uint part against A's blocks
+    if (nua && bbb) {
+      EH_CHECK(db.alloc(4 * cbb + 16));
+      k_decode_set<<<1, ISECT_THREADS, 0, st>>>(EH_SET_BITSET_NEW, BBS, bbb, cbb, db.as<uint32_t>(), nullptr);
+      isect_lists(st, AI, nua, db.as<uint32_t>(), (uint32_t)cbb, r2.as<uint32_t>(), R + 2);
+    }
+    if (nub && bba) {
+      EH_CHECK(da.alloc(4 * cba + 16));
+      k_decode_set<<<1, ISECT_THREADS, 0, st>>>(EH_SET_BITSET_NEW, ABS, bba, cba, da.as<uint32_t>(), nullptr);
+      isect_lists(st, BI, nub, da.as<uint32_t>(), (uint32_t)cba, r3.as<uint32_t>(), R + 4);
+    }
+    EH_HIP(hipGetLastError());
+    if (materialize)
+      k_merge3<<<1, ISECT_THREADS, 0, st>>>(r1.as<uint32_t>(), r2.as<uint32_t>(), r3.as<uint32_t>(), R + 0, R + 2,
+                                            R + 4, u.as<uint32_t>());
+    EH_HIP(hipGetLastError());
+  }
+  if (!uint_only && bba && bbb) {
+    if (materialize) EH_CHECK(o.alloc(20ull * (nba < nbb ? nba : nbb) + 16));
+    k_isect_bn_bn<<<1, ISECT_THREADS, 0, st>>>(ABS, nba, BBS, nbb, materialize ? o.as<uint8_t>() : nullptr, R + 6);
+    EH_HIP(hipGetLastError());
+  }
+  EH_HIP(hipMemcpyAsync(h, R, 64, hipMemcpyDeviceToHost, st));
+  EH_HIP(hipStreamSynchronize(st));
+  const uint64_t nu = h[0] + h[2] + h[4];
+  if (out) out->cardinality = nu + h[6];
+  if (!materialize) return EH_OK;
+  const uint64_t need = 8 + 4 * nu + 20 * h[7];
+  if (need > out_cap) { eh_set_error("result needs %llu bytes", (unsigned long long)need); return EH_ERR_CAPACITY; }
+  uint8_t *ob = static_cast<uint8_t *>(out_buf);
+  const uint64_t hdr = 4 * nu;
+  memcpy(ob, &hdr, 8);
+  if (nu) EH_HIP(hipMemcpy(ob + 8, all ? u.p : r1.p, 4 * nu, hipMemcpyDeviceToHost));
+  if (h[7]) EH_HIP(hipMemcpy(ob + 8 + 4 * nu, o.p, 20 * h[7], hipMemcpyDeviceToHost));
+  out->number_of_bytes = need;
+  return EH_OK;
 }
 
 }  // namespace
@@ -256,6 +567,7 @@ extern "C" eh_status eh_intersect(eh_ctx *ctx, const eh_row_view *a, const eh_ro
   if (!ctx || !a || !b || (materialize && (!out_buf || !out))) { eh_set_error("null argument"); return EH_ERR_INVALID; }
   if (!view_ok(a) || !view_ok(b)) { eh_set_error("malformed set view (type or byte count)"); return EH_ERR_INVALID; }
   EH_HIP(hipSetDevice(ctx->device));
+  if (is_block_set(a->type) || is_block_set(b->type)) return intersect_blocks(ctx, a, b, materialize, out_buf, out_cap, out);
   hipStream_t st = ctx->stream;
   const int ta = a->type, tb = b->type;
   int rt = EH_U;
