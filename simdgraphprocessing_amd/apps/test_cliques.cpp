@@ -27,6 +27,8 @@ int main(int argc, char **argv) {
     run_case<pshort>(inputGraph, "pshort", 4, 30004668L, schedule);
     run_case<bitset>(inputGraph, "bs", 4, 30004668L, schedule);
     run_case<hybrid>(inputGraph, "hybrid", 3, 1612010L, schedule);
+    run_case<new_type>(inputGraph, "new_type", 4, 30004668L, schedule);
+    run_case<bitset_new>(inputGraph, "bitset_new", 4, 30004668L, schedule);
   }
   delete inputGraph;
   return TEST_RESULT();

@@ -1,6 +1,7 @@
 // SparseMatrix::get_row and ops::set_intersect through the C++ surface, the way
 // the reference's triangle loop uses them (apps/undirected_triangle_counting.cpp:45-56),
-// checked against std::set_intersection over the host neighbourhoods:
+// checked against std::set_intersection over the host neighbourhoods, for every
+// layout including the block layouts bitset_new / new_type:
 //     test_set_ops <facebook.bin> [rows]
 #define GOOGLE_TEST
 #include <algorithm>
@@ -26,6 +27,7 @@ static void check_layout(MutableGraph *g, uint32_t rows) {
     Set<T> A = m->get_row(i);
     std::vector<uint32_t> a;
     A.foreach([&](uint32_t x) { a.push_back(x); });
+    std::sort(a.begin(), a.end());  // new_type visits its uint part before its blocks
     const std::vector<uint32_t> ka = kept(i);
     EXPECT_EQ(ka.size(), a.size());
     EXPECT_EQ(true, a == ka);
@@ -38,6 +40,7 @@ static void check_layout(MutableGraph *g, uint32_t rows) {
       std::set_intersection(ka.begin(), ka.end(), kb.begin(), kb.end(), std::back_inserter(want));
       std::vector<uint32_t> c;
       C.foreach([&](uint32_t x) { c.push_back(x); });
+      std::sort(c.begin(), c.end());
       if (c != want) EXPECT_EQ(want.size(), c.size());
       got_total += C.cardinality;
       want_total += want.size();
@@ -60,6 +63,9 @@ int main(int argc, char **argv) {
   check_layout<bitset>(g, rows);
   check_layout<uinteger>(g, rows);
   check_layout<hybrid3>(g, rows);
+  check_layout<bitset_new>(g, rows);
+  check_layout<new_type>(g, rows);
+  check_layout<hybrid_block>(g, rows);
   delete g;
   return TEST_RESULT();
 }

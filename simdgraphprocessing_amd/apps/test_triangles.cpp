@@ -28,6 +28,9 @@ int main(int argc, char **argv) {
     run_case<pshort>(inputGraph, "pshort", schedule);
     run_case<bitset>(inputGraph, "bs", schedule);
     run_case<hybrid3>(inputGraph, "hybrid3", schedule);
+    run_case<new_type>(inputGraph, "new_type", schedule);
+    run_case<bitset_new>(inputGraph, "bitset_new", schedule);
+    run_case<hybrid_block>(inputGraph, "hybrid_block", schedule);
   }
   delete inputGraph;
   return TEST_RESULT();

@@ -1,7 +1,8 @@
 // Parser.hpp -- Parser and input_parser::parse (src/Parser.hpp:10-180): the app
 // command line.  Flags as in the reference: --graph=<path> --input_type=binary
 // --t=<threads> --layout=<uint|pshort|bs|hybrid> [--n=<clique size, default 4>]
-// [--start_node=<id>]; this engine adds --layout=hybrid3|cost|learned and
+// [--start_node=<id>]; the reference's main also takes new_type and bitset_new
+// (src/emptyheaded.hpp:60-65); this engine adds --layout=hybrid3|cost|learned|hybrid_block and
 // --schedule=<tuned|plain>.  Only undirected binary input is on this path.
 #pragma once
 
@@ -36,7 +37,7 @@ inline void printUsage(const std::string &app) {
   std::cout << "USAGE: ./application <OPTIONS>" << std::endl;
   std::cout << "OPTIONS: " << std::endl;
   std::cout << "\tREQUIRED: --graph=<path to graph> --input_type=<'binary'> --t=<# of threads>" << std::endl;
-  std::cout << "\t\t--layout=<uint,pshort,bs,hybrid,hybrid3,cost,learned>" << std::endl;
+  std::cout << "\t\t--layout=<uint,pshort,bs,hybrid,new_type,bitset_new,hybrid3,cost,learned,hybrid_block>" << std::endl;
   std::cout << "\tOPTIONAL: --schedule=<tuned,plain>" << std::endl;
   if (app == "n_clique") {
     std::cout << "\tOPTIONAL: --n=<size of cliques to find, 3 to 16, default is 4>" << std::endl;

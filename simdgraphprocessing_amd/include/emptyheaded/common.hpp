@@ -26,7 +26,9 @@ enum type : uint8_t {
   UINTEGER = EH_UINTEGER,
   BITPACKED = 3,
   VARIANT = 4,
-  HYBRID = EH_HYBRID
+  HYBRID = EH_HYBRID,
+  BITSET_NEW = EH_BITSET_NEW,
+  NEW_TYPE = EH_NEW_TYPE
 };
 
 enum graph_type { DIRECTED, UNDIRECTED };

@@ -90,6 +90,9 @@ int main(int argc, char *argv[]) {
   if (l == "hybrid") return emptyheaded::run_layout<hybrid>(input_data);
   if (l == "hybrid3") return emptyheaded::run_layout<hybrid3>(input_data);
   if (l == "cost") return emptyheaded::run_layout<cost_hybrid>(input_data);
+  if (l == "new_type") return emptyheaded::run_layout<new_type>(input_data);
+  if (l == "bitset_new") return emptyheaded::run_layout<bitset_new>(input_data);
+  if (l == "hybrid_block") return emptyheaded::run_layout<hybrid_block>(input_data);
   if (l == "learned") {
     emptyheaded::load_selector_params();
     return emptyheaded::run_layout<learned_hybrid>(input_data);

@@ -1,5 +1,5 @@
 // layouts.hpp -- the layout tag classes of src/set/layouts/ (uinteger.hpp,
-// pshort.hpp, bitset.hpp, hybrid.hpp).  In the reference they hold the
+// pshort.hpp, bitset.hpp, hybrid.hpp, bitset_new.hpp, new_type.hpp).  In the reference they hold the
 // encoders and the per-layout loops; here the encoders run on the device, so a
 // tag carries its type value, the matrix layout policy it selects, and the
 // host iteration over a set's bytes that Set<T>::foreach needs.
@@ -12,13 +12,45 @@
 
 namespace emptyheaded {
 
+// Iterate the blocks of a bitset_new image: [u32 ids][2 u64 words per block]
+// (bitset_new.hpp:159-191).
+template <typename F>
+inline void foreach_blocks(F f, const uint8_t *data, size_t number_of_bytes) {
+  const size_t nb = number_of_bytes / 20;
+  for (size_t b = 0; b < nb; b++) {
+    uint32_t id;
+    std::memcpy(&id, data + 4 * b, 4);
+    for (size_t h = 0; h < 2; h++) {
+      uint64_t word;
+      std::memcpy(&word, data + 4 * nb + 16 * b + 8 * h, 8);
+      while (word) {
+        f((uint32_t)(128u * id + 64u * h + (unsigned)__builtin_ctzll(word)));
+        word &= word - 1;
+      }
+    }
+  }
+}
+
 // Iterate the elements of a set held in one of the reference byte formats:
 // uint u32[card]; bitset [u64 first word][u64 words]; pshort u16 partitions
-// [prefix, count-1, low16...] (bitset.hpp:65-96, pshort.hpp:46-78).
+// [prefix, count-1, low16...] (bitset.hpp:65-96, pshort.hpp:46-78); bitset_new
+// blocks; new_type [u64 uint bytes][uint part][bitset_new part], uint part first
+// (new_type.hpp:135-152).
 template <typename F>
 inline void foreach_bytes(F f, const uint8_t *data, size_t cardinality, size_t number_of_bytes, common::type t) {
   if (cardinality == 0 || data == nullptr) return;
-  if (t == common::BITSET) {
+  if (t == common::BITSET_NEW) {
+    foreach_blocks(f, data, number_of_bytes);
+  } else if (t == common::NEW_TYPE) {
+    uint64_t ub;
+    std::memcpy(&ub, data, 8);
+    for (size_t k = 0; k < ub / 4; k++) {
+      uint32_t v;
+      std::memcpy(&v, data + 8 + 4 * k, 4);
+      f(v);
+    }
+    foreach_blocks(f, data + 8 + ub, number_of_bytes - 8 - ub);
+  } else if (t == common::BITSET) {
     uint64_t w0;
     std::memcpy(&w0, data, 8);
     const size_t nw = (number_of_bytes - 8) / 8;
@@ -78,5 +110,10 @@ EH_LAYOUT_TAG(hybrid, common::HYBRID, EH_LAYOUT_HYBRID)
 EH_LAYOUT_TAG(hybrid3, common::HYBRID, EH_LAYOUT_HYBRID3)
 EH_LAYOUT_TAG(cost_hybrid, common::HYBRID, EH_LAYOUT_COST)
 EH_LAYOUT_TAG(learned_hybrid, common::HYBRID, EH_LAYOUT_LEARNED)
+// the block-bitset layouts (--layout=bitset_new / new_type, src/emptyheaded.hpp:60-65)
+// and hybrid built with NEW_BITSET (src/set/layouts/hybrid.hpp:83-87)
+EH_LAYOUT_TAG(bitset_new, common::BITSET_NEW, EH_LAYOUT_BITSET_NEW)
+EH_LAYOUT_TAG(new_type, common::NEW_TYPE, EH_LAYOUT_NEW_TYPE)
+EH_LAYOUT_TAG(hybrid_block, common::HYBRID, EH_LAYOUT_HYBRID_BLOCK)
 
 #undef EH_LAYOUT_TAG

@@ -5,10 +5,12 @@
 // B->type at run time by the device (eh_intersect).  As in the reference:
 //   * an empty side gives an empty C;
 //   * C's type follows the reference's result layouts (uint whenever a side is
-//     uint, bitset for bitset x bitset, pshort for pshort x pshort/bitset);
+//     uint, bitset for bitset x bitset, pshort for pshort x pshort/bitset,
+//     bitset_new for bitset_new x bitset_new, new_type for new_type x new_type);
 //   * C->data is caller-owned and must hold the result: at most
 //     4*min(|A|,|B|) bytes for a uint result, 8 + 8*min(words) for a bitset
-//     result, 6*min(|A|,|B|) for a pshort result;
+//     result, 6*min(|A|,|B|) for a pshort result, 8 + 20*min(|A|,|B|) for a
+//     block result;
 //   * WRITE_VECTOR, when defined to 0 before the include, counts without
 //     writing C->data (apps/undirected_triangle_counting.cpp:1).
 // Device failures raise emptyheaded::Error.
@@ -50,7 +52,8 @@ template <class X, class Y, class Z>
 inline Set<X> *set_intersect(Set<X> *C, const Set<Y> *A, const Set<Z> *B) {
   const eh_row_view a = detail::view_of(A), b = detail::view_of(B);
   const uint64_t mc = std::min<uint64_t>(A->cardinality, B->cardinality);
-  const uint64_t cap = std::max<uint64_t>({4 * mc, 8 + 8 * std::min(detail::words_of(A), detail::words_of(B)), 6 * mc});
+  const uint64_t cap =
+      std::max<uint64_t>({4 * mc, 8 + 8 * std::min(detail::words_of(A), detail::words_of(B)), 6 * mc, 8 + 20 * mc});
   eh_row_view out;
   emptyheaded::Runtime &rt = emptyheaded::Runtime::get();
   {

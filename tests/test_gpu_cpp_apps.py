@@ -55,6 +55,18 @@ def test_cli_cliques(fb_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("layout,row_bytes", [("bitset_new", 513116), ("new_type", 277200)])
+def test_cli_block_layouts(fb_path, layout, row_bytes):
+    """--layout=bitset_new / new_type (src/emptyheaded.hpp:60-65): the count and the ROW DATA
+    SIZE the reference's program prints on facebook.bin with one thread."""
+    rc, out, err = run(["undirected_triangle_counting", f"--graph={fb_path}", "--input_type=binary", "--t=4",
+                        f"--layout={layout}"])
+    assert rc == 0, err
+    assert "Number of triangles: 1612010" in out
+    assert f"ROW DATA SIZE (Bytes): {row_bytes}" in out
+
+
+@pytest.mark.gpu
 def test_cli_learned_layout(fb_path):
     params = os.path.join(ROOT, "simdgraphprocessing_amd", "selector_params.npy")
     env = dict(os.environ, EH_SELECTOR_PARAMS=params)
