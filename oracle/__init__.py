@@ -64,6 +64,11 @@ def lib():
     L.orc_order.argtypes = [C.c_uint32, _u64p, _u32p, C.c_int, C.c_uint64, _u32p]
     L.orc_order.restype = C.c_int
     L.orc_row_image_bound.argtypes = [_u32p, C.c_size_t]
+    L.orc_bfs_path.argtypes = [C.c_uint32, _u64p, _u32p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                               C.c_uint32, C.c_void_p]
+    L.orc_bfs_path.restype = C.c_uint32
+    L.orc_bitset_difference.argtypes = [_u8p, C.c_size_t, _u8p, C.c_size_t, _u32p]
+    L.orc_bitset_difference.restype = C.c_size_t
     L.orc_row_image_bound.restype = C.c_size_t
     for name in ("orc_uint_build", "orc_pshort_build", "orc_bitset_build", "orc_bitset_new_build",
                  "orc_new_type_build"):
@@ -576,6 +581,27 @@ def load_directed_binary(path_or_bytes):
     return n, outs, ins, ids
 
 
+def bfs_path(n, off, nbr, start, max_depth=0xEFFFFFF, cap=64):
+    """n_path's BFS (apps/n_path.cpp:41-85) over rows [0, n): returns
+    {path_length, rounds, level_sizes, reached}."""
+    rounds, reached = C.c_uint32(), C.c_uint64()
+    sizes = np.zeros(cap, np.uint64)
+    p = lib().orc_bfs_path(n, _u64(off), _u32(nbr), start, max_depth, C.addressof(rounds), sizes.ctypes.data,
+                           cap, C.addressof(reached))
+    return {"path_length": int(p), "rounds": rounds.value, "level_sizes": sizes[:min(rounds.value, cap)].tolist(),
+            "reached": reached.value}
+
+
+def bitset_difference(a_bytes, b_bytes):
+    """set_difference of two bitset images (difference.hpp:5-84) as a sorted uint32 array."""
+    a = np.ascontiguousarray(a_bytes, np.uint8)
+    b = np.ascontiguousarray(b_bytes, np.uint8)
+    out = np.zeros(max(8 * len(a), 8), np.uint32)
+    k = lib().orc_bitset_difference(a if len(a) else np.zeros(8, np.uint8), len(a),
+                                    b if len(b) else np.zeros(8, np.uint8), len(b), out)
+    return out[:k].copy()
+
+
 def run_ref_converter(lines, outdir):
     """Run the reference's edge-list converter built in _ref/; returns {name: (n, off, nbr, ext)}."""
     exe = os.path.join(REF_DIR, "undirectedEdgeListToBinary")
@@ -607,7 +633,7 @@ def run_ref_stats(app, n, off, nbr, threads=4, layout="hybrid", k=None, timeout=
         if k is not None:
             cmd.append(f"--n={k}")
         out = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, cwd=td).stdout
-    cnt = re.search(r"Number of (?:triangles|cliques): (\d+)", out)
+    cnt = re.search(r"Number of (?:triangles|cliques): (\d+)", out) or re.search(r"path length: (\d+)", out)
     tm = re.findall(r"Time\[(?:UNDIRECTED TRIANGLE COUNTING|n_cliques)\]: ([0-9.e+-]+) s", out)
     rb = re.search(r"ROW DATA SIZE \(Bytes\): (\d+)", out)
     if not cnt:

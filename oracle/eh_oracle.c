@@ -1533,6 +1533,70 @@ uint64_t orc_pair_mix(const orc_matrix *M, uint64_t mix[4]) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* BFS path length (apps/n_path.cpp:41-85)                                    */
+/* ------------------------------------------------------------------------- */
+
+uint32_t orc_bfs_path(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint32_t start, uint32_t max_depth,
+                      uint32_t *rounds, uint64_t *level_sizes, uint32_t cap, uint64_t *reached) {
+  const size_t words = (size_t)n / 64 + 2;
+  uint64_t *visited = (uint64_t *)calloc(words, 8), *old = (uint64_t *)calloc(words, 8);
+  uint32_t *frontier = (uint32_t *)malloc(4 * ((size_t)n + 1));
+  size_t nf = 1;
+  uint32_t path = 0, r = 0;
+  frontier[0] = start;
+  visited[start >> 6] |= 1ull << (start & 63);
+  for (;;) {
+    memcpy(old, visited, 8 * words);
+    /* set_union of every frontier row into the visited bitset */
+    for (size_t f = 0; f < nf; f++)
+      for (uint64_t k = off[frontier[f]]; k < off[frontier[f] + 1]; k++)
+        visited[nbr[k] >> 6] |= 1ull << (nbr[k] & 63);
+    /* set_difference(visited, old): the new frontier in id order */
+    nf = 0;
+    for (size_t w = 0; w < words; w++) {
+      uint64_t d = visited[w] & ~old[w];
+      while (d) {
+        frontier[nf++] = (uint32_t)(64 * w + (size_t)__builtin_ctzll(d));
+        d &= d - 1;
+      }
+    }
+    if (r < cap && level_sizes) level_sizes[r] = nf;
+    r++;
+    if (nf == 0 || path >= max_depth) break;
+    path++;
+  }
+  if (rounds) *rounds = r;
+  if (reached) {
+    uint64_t c = 0;
+    for (size_t w = 0; w < words; w++) c += (uint64_t)__builtin_popcountll(visited[w]);
+    *reached = c;
+  }
+  free(visited);
+  free(old);
+  free(frontier);
+  return path;
+}
+
+/* set_difference (src/set/ops/difference.hpp:5-84) on bitset images: A's words AND NOT
+ * B's over their common span, A's words outside it as they are */
+size_t orc_bitset_difference(const uint8_t *a, size_t a_bytes, const uint8_t *b, size_t b_bytes, uint32_t *out) {
+  if (a_bytes < 8) return 0;
+  const uint64_t a0 = ld64(a), na = (a_bytes - 8) / 8;
+  const uint64_t b0 = b_bytes >= 8 ? ld64(b) : 0, nb = b_bytes >= 8 ? (b_bytes - 8) / 8 : 0;
+  size_t c = 0;
+  for (uint64_t i = 0; i < na; i++) {
+    const uint64_t w = a0 + i;
+    uint64_t r = ld64(a + 8 + 8 * i);
+    if (w >= b0 && w < b0 + nb) r &= ~ld64(b + 8 + 8 * (w - b0));
+    while (r) {
+      out[c++] = (uint32_t)(64 * w + (uint64_t)__builtin_ctzll(r));
+      r &= r - 1;
+    }
+  }
+  return c;
+}
+
+/* ------------------------------------------------------------------------- */
 /* independent brute force on the symmetric graph (no orientation, no        */
 /* layouts): marker arrays over vertex ids.                                   */
 /* ------------------------------------------------------------------------- */

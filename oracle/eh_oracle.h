@@ -110,6 +110,20 @@ uint64_t orc_clique_count_sample(const orc_matrix *M, int k, int threads, uint32
 uint64_t orc_pair_mix(const orc_matrix *M, uint64_t mix[4]);
 uint64_t orc_count_rows(const orc_matrix *M, int k, int threads, uint64_t *per_row);
 
+/* ---- BFS path length (apps/n_path.cpp:27-87) ---- */
+/* Level-synchronous BFS over rows [0, n) of a CSR from `start`: each round ORs the
+ * frontier's rows into the visited bitset (set_union, src/set/ops/union.hpp:6-127) and
+ * takes the new frontier as visited minus the previous visited (set_difference,
+ * src/set/ops/difference.hpp:5-84), in id order.  Stops when the frontier is empty or
+ * path_length reached max_depth.  Returns path_length; *rounds = frontier expansions
+ * run, level_sizes[r] (r < cap) = size of the frontier produced by round r, *reached =
+ * visited vertices. */
+uint32_t orc_bfs_path(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint32_t start, uint32_t max_depth,
+                      uint32_t *rounds, uint64_t *level_sizes, uint32_t cap, uint64_t *reached);
+/* set_difference(C, A, B) of two bitset images [u64 first word][words] into a sorted
+ * uint list: the elements of A not in B.  Returns the count. */
+size_t orc_bitset_difference(const uint8_t *a, size_t a_bytes, const uint8_t *b, size_t b_bytes, uint32_t *out);
+
 /* ---- independent brute force (symmetric CSR) ---- */
 uint64_t orc_brute_triangles(uint32_t n, const uint64_t *off, const uint32_t *nbr);
 uint64_t orc_brute_cliques(uint32_t n, const uint64_t *off, const uint32_t *nbr, int k);

@@ -8,6 +8,7 @@ where oracle/_ref was built in this container -- the reference's own programs.
 import itertools
 import os
 import random
+import re
 import subprocess
 
 import numpy as np
@@ -641,3 +642,106 @@ def test_hybrid_block_matches_reference_program(facebook):
     M = O.Matrix(facebook["n"], facebook["off"], facebook["nbr"], "hybrid_block")
     assert ref["count"] == FB_TRIANGLES and M.triangles(2)[0] == FB_TRIANGLES
     assert M.bytes() == ref["row_bytes"] == FB_BLOCK_ROW_BYTES["hybrid_block"]
+
+
+# ------------------------------------------------------------------ BFS path length (n_path)
+
+def _out_csr(outs):
+    off = np.zeros(len(outs) + 1, np.uint64)
+    off[1:] = np.cumsum([len(r) for r in outs])
+    return off, (np.concatenate(outs).astype(np.uint32) if len(outs) else np.zeros(0, np.uint32))
+
+
+def _dfacebook():
+    import gzip
+    raw = gzip.open(os.path.join(os.path.dirname(__file__), "golden", "dfacebook.bin.gz"), "rb").read()
+    n, outs, ins, ext = O.load_directed_binary(raw)
+    off, nbr = _out_csr(outs)
+    return n, off, nbr, ext
+
+
+def test_bfs_fixture_dfacebook():
+    """test/n_path_test.cpp:7-11: dfacebook.bin, start node 0, depth 4 -> path length 4."""
+    n, off, nbr, ext = _dfacebook()
+    start = int(np.nonzero(ext == 0)[0][0]) if (ext == 0).any() else 0
+    assert O.bfs_path(n, off, nbr, start, 4)["path_length"] == 4
+
+
+def _bfs_levels(n, off, nbr, start):
+    """Independent BFS distances (scipy) -> sizes of levels 1, 2, ..."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import shortest_path
+    A = sp.csr_matrix((np.ones(len(nbr)), nbr.astype(np.int64), off.astype(np.int64)), shape=(n, n))
+    d = shortest_path(A, directed=True, unweighted=True, indices=start)
+    d = d[np.isfinite(d)].astype(np.int64)
+    return np.bincount(d)[1:].tolist(), int(d.max()), int(len(d))
+
+
+def test_bfs_levels_match_independent_bfs():
+    for seed in (1, 2, 3):
+        for n, (off, nbr) in ((3000, O.edges_to_csr(3000, *O.gen_er(3000, 4000, seed))),
+                              (1 << 11, O.canonical_oriented(1 << 11, *O.gen_rmat(11, 4, seed))["sym"])):
+            rng = np.random.default_rng(seed)
+            for start in rng.integers(0, n, 4).tolist():
+                levels, ecc, reach = _bfs_levels(n, off, nbr, start)
+                r = O.bfs_path(n, off, nbr, start)
+                assert r["path_length"] == ecc and r["reached"] == reach
+                assert r["level_sizes"] == levels + [0]
+                for depth in (0, 1, 2, ecc + 3):
+                    assert O.bfs_path(n, off, nbr, start, depth)["path_length"] == min(ecc, depth)
+
+
+def test_bfs_closed_forms():
+    """A path of k vertices from one end has eccentricity k-1; a star's centre 1, a leaf 2;
+    an isolated start 0."""
+    k = 40
+    off, nbr = O.edges_to_csr(k, np.arange(k - 1, dtype=np.uint32), np.arange(1, k, dtype=np.uint32))
+    assert O.bfs_path(k, off, nbr, 0)["path_length"] == k - 1
+    assert O.bfs_path(k, off, nbr, k // 2)["path_length"] == k // 2
+    off, nbr = O.edges_to_csr(50, np.zeros(49, np.uint32), np.arange(1, 50, dtype=np.uint32))
+    assert O.bfs_path(50, off, nbr, 0)["path_length"] == 1
+    assert O.bfs_path(50, off, nbr, 7)["path_length"] == 2
+    off, nbr = O.edges_to_csr(60, np.zeros(49, np.uint32), np.arange(1, 50, dtype=np.uint32))
+    assert O.bfs_path(60, off, nbr, 55) == {"path_length": 0, "rounds": 1, "level_sizes": [0], "reached": 1}
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(O.__file__), "_ref", "n_path")),
+                    reason="reference n_path not built (needs the reference tree)")
+def test_bfs_matches_reference_program(facebook, tmp_path):
+    """apps/n_path.cpp: path lengths printed by the reference program."""
+    # facebook, default start (the first row of maximum length) and depth
+    start = int(np.argmax(np.diff(facebook["off"])))
+    ref = O.run_ref_stats("n_path", facebook["n"], facebook["off"], facebook["nbr"], layout="hybrid")
+    assert ref["count"] == O.bfs_path(facebook["n"], facebook["off"], facebook["nbr"], start)["path_length"] == 5
+    # RMAT-12 from several starts and depths (ids are their own external ids)
+    n = 1 << 12
+    _, off, nbr = O.canonical_oriented(n, *O.gen_rmat(12, 4, 5))["relabel"]
+    path = tmp_path / "g.bin"
+    O.write_binary(str(path), n, off, nbr)
+    for start, depth in ((0, 100), (17, 100), (4000, 2), (4095, 1000), (300, 3)):
+        out = subprocess.run([os.path.join(os.path.dirname(O.__file__), "_ref", "n_path"), f"--graph={path}",
+                              "--input_type=binary", "--t=2", "--layout=uint", f"--n={depth}",
+                              f"--start_node={start}"], capture_output=True, text=True, timeout=120).stdout
+        want = int(re.search(r"path length: (\d+)", out).group(1))
+        assert O.bfs_path(n, off, nbr, start, depth)["path_length"] == want, (start, depth)
+    # the directed fixture, out-rows, several starts (external ids = internal ids here)
+    import gzip
+    dpath = tmp_path / "dfacebook.bin"
+    dpath.write_bytes(gzip.open(os.path.join(os.path.dirname(__file__), "golden", "dfacebook.bin.gz")).read())
+    dn, doff, dnbr, dext = _dfacebook()
+    for start in (0, 5, 1000, 3000):
+        out = subprocess.run([os.path.join(os.path.dirname(O.__file__), "_ref", "n_path"), f"--graph={dpath}",
+                              "--input_type=binary", "--t=2", "--layout=uint", "--n=4", f"--start_node={start}",
+                              "--asymmetric"], capture_output=True, text=True, timeout=120).stdout
+        want = int(re.search(r"path length: (\d+)", out).group(1))
+        internal = int(np.nonzero(dext == start)[0][0])
+        assert O.bfs_path(dn, doff, dnbr, internal, 4)["path_length"] == want, start
+
+
+def test_bitset_difference_is_set_difference():
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        a = np.unique(rng.integers(0, 5000, int(rng.integers(1, 300)))).astype(np.uint32)
+        b = np.unique(rng.integers(0, 5000, int(rng.integers(1, 300)))).astype(np.uint32)
+        got = O.bitset_difference(O.layout_build("bitset", a), O.layout_build("bitset", b))
+        assert got.tolist() == np.setdiff1d(a, b).tolist()
