@@ -234,6 +234,17 @@ void eh_matrix_free(eh_matrix *m);
  * NEW_TYPE with NEW_TYPE. */
 eh_status eh_intersect(eh_ctx *ctx, const eh_row_view *a, const eh_row_view *b, int materialize,
                        void *out_buf, size_t out_cap, eh_row_view *out);
+/* ops::set_union(Set<bitset>* A, Set<X>* B) (src/set/ops/union.hpp:6-127): *out = A's
+ * bitset with B's elements added, in A's word span (the reference writes in place; pass
+ * A's own buffer as out_buf to do the same).  B is BITSET (OR over the common words),
+ * UINTEGER or PSHORT; B's elements outside A's span are dropped.  out->cardinality is
+ * the result's element count. */
+eh_status eh_union(eh_ctx *ctx, const eh_row_view *a, const eh_row_view *b, void *out_buf, size_t out_cap,
+                   eh_row_view *out);
+/* ops::set_difference(Set<uinteger>* C, const Set<bitset>* A, const Set<bitset>* B)
+ * (src/set/ops/difference.hpp:5-84): the elements of A not in B, as a sorted uint set. */
+eh_status eh_difference(eh_ctx *ctx, const eh_row_view *a, const eh_row_view *b, void *out_buf, size_t out_cap,
+                        eh_row_view *out);
 /* Many pairs of rows of one matrix at once: counts[p] = |row(a[p]) & row(b[p])|. */
 eh_status eh_intersect_rows(eh_matrix *m, const uint32_t *a_rows, const uint32_t *b_rows,
                             uint64_t npairs, uint64_t *counts);
@@ -248,6 +259,24 @@ eh_status eh_triangle_count(eh_matrix *m, const eh_run_opts *opts, uint64_t *cou
  * level-set recursion (opts are not used). */
 eh_status eh_clique_count(eh_matrix *m, uint32_t k, const eh_run_opts *opts, uint64_t *count,
                           eh_run_stats *stats);
+
+/* n_path<T,R>::run (apps/n_path.cpp:27-87): breadth-first search from `start` over
+ * the rows [0, n_vertices) of a matrix built with EH_FILTER_ALL (0: all rows; a
+ * directed matrix's out-rows are rows [0, n)).  Each round ORs the frontier's rows
+ * into the visited set (set_union, src/set/ops/union.hpp:6-127) and takes the new
+ * frontier as the vertices not visited before (set_difference,
+ * src/set/ops/difference.hpp:5-84); the search stops when the frontier is empty or
+ * path_length reached max_depth (query_depth; n_path's default is 0xEFFFFFF).
+ * level_sizes[r] (r < level_cap, may be NULL with level_cap 0) receives the size of the
+ * frontier round r produced. */
+typedef struct {
+  uint32_t path_length; /* n_path::path_length */
+  uint32_t rounds;      /* frontier expansions run */
+  uint64_t reached;     /* visited vertices, the start included */
+  double kernel_ms;     /* device time of the search */
+} eh_bfs_info;
+eh_status eh_bfs_path(eh_matrix *m, uint32_t n_vertices, uint32_t start, uint32_t max_depth,
+                      uint64_t *level_sizes, uint32_t level_cap, eh_bfs_info *info);
 /* No reference counterpart (the reference times whole phases only,
  * src/common.hpp:87-99): one counting pass with the shader-clock cycles of
  * every task recorded; task_rows/task_kinds/task_cycles get ntasks entries

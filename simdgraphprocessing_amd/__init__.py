@@ -51,6 +51,11 @@ class RunOpts(C.Structure):
                 ("small_degree", C.c_int32), ("stream_ratio", C.c_int32)]
 
 
+class BfsInfo(C.Structure):
+    _fields_ = [("path_length", C.c_uint32), ("rounds", C.c_uint32), ("reached", C.c_uint64),
+                ("kernel_ms", C.c_double)]
+
+
 class RunStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("sched_ms", C.c_double), ("tasks", C.c_uint64),
                 ("alg_bytes", C.c_uint64), ("pairs", C.c_uint64 * 4), ("work_units", C.c_uint64),
@@ -79,7 +84,7 @@ EXPORTS = [
     "eh_graph_relabel_by_degree", "eh_graph_reorder", "eh_graph_info_get", "eh_graph_download", "eh_graph_download_perm",
     "eh_graph_free", "eh_matrix_build", "eh_matrix_build_host", "eh_matrix_info_get", "eh_matrix_row",
     "eh_matrix_download", "eh_matrix_row_types", "eh_matrix_free", "eh_intersect", "eh_intersect_rows",
-    "eh_triangle_count", "eh_clique_count", "eh_count_profile", "eh_batch_generate", "eh_batch_info",
+    "eh_union", "eh_difference", "eh_triangle_count", "eh_clique_count", "eh_bfs_path", "eh_count_profile", "eh_batch_generate", "eh_batch_info",
     "eh_batch_graph_params", "eh_batch_graph_download", "eh_batch_count", "eh_batch_free",
     "eh_selector_set_params", "eh_selector_get_params", "eh_matrix_features",
     "eh_matrix_profile_layouts", "eh_mlp_forward", "eh_mlp_loss_grad", "eh_selector_train",
@@ -135,6 +140,10 @@ def lib():
     L.eh_intersect.argtypes = [_vp, C.POINTER(RowView), C.POINTER(RowView), C.c_int, _vp, C.c_size_t,
                                C.POINTER(RowView)]
     L.eh_intersect_rows.argtypes = [_vp, _vp, _vp, C.c_uint64, _vp]
+    for name in ("eh_union", "eh_difference"):
+        getattr(L, name).argtypes = [_vp, C.POINTER(RowView), C.POINTER(RowView), _vp, C.c_size_t,
+                                     C.POINTER(RowView)]
+    L.eh_bfs_path.argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_uint32, _vp, C.c_uint32, C.POINTER(BfsInfo)]
     L.eh_triangle_count.argtypes = [_vp, C.POINTER(RunOpts), C.POINTER(C.c_uint64), C.POINTER(RunStats)]
     L.eh_clique_count.argtypes = [_vp, C.c_uint32, C.POINTER(RunOpts), C.POINTER(C.c_uint64),
                                   C.POINTER(RunStats)]
@@ -275,6 +284,29 @@ class Context:
         ov = RowView()
         _check(L.eh_intersect(self.h, C.byref(va), C.byref(vb), int(materialize), _ptr(out), cap, C.byref(ov)))
         return int(ov.cardinality), int(ov.type), out[:ov.number_of_bytes].copy()
+
+    def _setop(self, fn, a, b, cap):
+        keep = []
+
+        def view(s):
+            t, card, payload = s
+            p = np.ascontiguousarray(payload, np.uint8)
+            keep.append(p)
+            return RowView(p.ctypes.data if len(p) else None, card, len(p), t, 0)
+
+        va, vb = view(a), view(b)
+        out = np.zeros(max(cap, 8), np.uint8)
+        ov = RowView()
+        _check(getattr(lib(), fn)(self.h, C.byref(va), C.byref(vb), _ptr(out), len(out), C.byref(ov)))
+        return int(ov.cardinality), int(ov.type), out[:ov.number_of_bytes].copy()
+
+    def union(self, a, b):
+        """ops::set_union(Set<bitset>* A, Set<X>* B): A's bitset with B's elements added."""
+        return self._setop("eh_union", a, b, len(a[2]) + 8)
+
+    def difference(self, a, b):
+        """ops::set_difference(C, bitset A, bitset B): the elements of A not in B (uint)."""
+        return self._setop("eh_difference", a, b, 256 * len(a[2]) + 8)
 
     # ---- learned selector
     def set_selector(self, params):
@@ -440,6 +472,14 @@ class SparseMatrix:
         cyc = np.zeros(max(n, 1), np.uint64)
         _check(lib().eh_count_profile(self.h, k, C.byref(o), _ptr(rows), _ptr(kinds), _ptr(cyc), n, C.byref(nt)))
         return rows[:n], kinds[:n], cyc[:n]
+
+    def bfs_path(self, start, max_depth=0xEFFFFFF, n_vertices=0, level_cap=256):
+        """n_path's BFS (apps/n_path.cpp:27-87) from row `start`."""
+        sizes = np.zeros(max(level_cap, 1), np.uint64)
+        bi = BfsInfo()
+        _check(lib().eh_bfs_path(self.h, n_vertices, start, max_depth, _ptr(sizes), level_cap, C.byref(bi)))
+        return {"path_length": bi.path_length, "rounds": bi.rounds, "reached": bi.reached,
+                "level_sizes": sizes[:min(bi.rounds, level_cap)].tolist(), "kernel_ms": bi.kernel_ms}
 
     def intersect_rows(self, a_rows, b_rows):
         a = np.ascontiguousarray(a_rows, np.uint32)

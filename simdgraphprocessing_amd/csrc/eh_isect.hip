@@ -263,6 +263,56 @@ k_merge3(const uint32_t *X, const uint32_t *Y, const uint32_t *Z, const unsigned
     }
 }
 
+// set_union(bitset A, bitset B) over their common words (union.hpp:7-57): A |= B
+__global__ void __launch_bounds__(ISECT_THREADS)
+k_union_words(uint8_t *A, uint64_t a_off, const uint8_t *B, uint64_t b_off, uint64_t total) {
+  for (uint64_t i = threadIdx.x; i < total; i += blockDim.x) {
+    uint32_t *a = reinterpret_cast<uint32_t *>(A + 8 * (a_off + i));
+    const uint32_t *b = reinterpret_cast<const uint32_t *>(B + 8 * (b_off + i));
+    a[0] |= b[0];
+    a[1] |= b[1];
+  }
+}
+
+// set_union(bitset A, uint/pshort B) (union.hpp:58-80): B's elements inside A's words set
+// their bits (A: words of the image after its u64 start, first word a0, na words)
+__global__ void __launch_bounds__(ISECT_THREADS)
+k_union_elems(uint8_t *A, uint64_t a0, uint64_t na, const uint32_t *list, uint64_t n) {
+  uint32_t *W = reinterpret_cast<uint32_t *>(A);
+  for (uint64_t k = threadIdx.x; k < n; k += blockDim.x) {
+    const uint32_t x = list[k];
+    const uint64_t w = (uint64_t)(x >> 6) - a0;
+    if ((x >> 6) >= a0 && w < na) atomicOr(&W[2 * w + ((x >> 5) & 1u)], 1u << (x & 31u));
+  }
+}
+
+// set_difference(C, bitset A, bitset B) (difference.hpp:5-84): A's words AND NOT B's over
+// their common span, A's other words as they are, expanded in id order; res[0] = |C|
+__global__ void __launch_bounds__(ISECT_THREADS)
+k_difference(const uint8_t *A, uint64_t a0, uint64_t na, const uint8_t *B, uint64_t b0, uint64_t nb, uint32_t *out,
+             unsigned long long *res) {
+  __shared__ uint32_t sh[32];
+  uint32_t base = 0;
+  for (uint64_t c0 = 0; c0 < na; c0 += blockDim.x) {
+    const uint64_t i = c0 + threadIdx.x;
+    uint64_t r = 0;
+    if (i < na) {
+      r = ld64u(A + 8 * i);
+      const uint64_t w = a0 + i;
+      if (w >= b0 && w - b0 < nb) r &= ~ld64u(B + 8 * (w - b0));
+    }
+    uint32_t tot;
+    uint32_t pos = base + block_scan_excl((uint32_t)__popcll(r), sh, &tot);
+    while (r) {
+      if (out) out[pos] = (uint32_t)(64ull * (a0 + i) + (uint64_t)__builtin_ctzll(r));
+      pos++;
+      r &= r - 1;
+    }
+    base += tot;
+  }
+  if (threadIdx.x == 0) res[0] = base;
+}
+
 // bitset x bitset over the overlapping word span [start, start+total)
 __global__ void __launch_bounds__(ISECT_THREADS)
 k_isect_bs_bs(const uint8_t *A, uint64_t a_off, const uint8_t *B, uint64_t b_off, uint64_t total, uint8_t *out,
@@ -672,6 +722,107 @@ extern "C" eh_status eh_intersect(eh_ctx *ctx, const eh_row_view *a, const eh_ro
   if (need > out_cap) { eh_set_error("result needs %llu bytes", (unsigned long long)need); return EH_ERR_CAPACITY; }
   if (need) EH_HIP(hipMemcpy(out_buf, o.p, need, hipMemcpyDeviceToHost));
   out->number_of_bytes = need;
+  return EH_OK;
+}
+
+extern "C" eh_status eh_union(eh_ctx *ctx, const eh_row_view *a, const eh_row_view *b, void *out_buf, size_t out_cap,
+                              eh_row_view *out) {
+  if (!ctx || !a || !b || !out || !out_buf) { eh_set_error("null argument"); return EH_ERR_INVALID; }
+  if (a->type != EH_BS || a->number_of_bytes < 16 || (a->number_of_bytes - 8) % 8 || !a->data) {
+    eh_set_error("eh_union: A must be a non-empty bitset image");
+    return EH_ERR_INVALID;
+  }
+  if (b->type != EH_BS && b->type != EH_U && b->type != EH_PS) {
+    eh_set_error("eh_union: no set_union overload for B of type %d", b->type);
+    return EH_ERR_INVALID;
+  }
+  if (b->cardinality && !view_ok(b)) { eh_set_error("malformed set view (type or byte count)"); return EH_ERR_INVALID; }
+  const uint64_t ba = a->number_of_bytes;
+  if (ba > out_cap) { eh_set_error("result needs %llu bytes", (unsigned long long)ba); return EH_ERR_CAPACITY; }
+  EH_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  uint64_t a0 = 0;
+  memcpy(&a0, a->data, 8);
+  const uint64_t na = (ba - 8) / 8;
+  DevBuf pa, pb, lb, ps, res;
+  EH_CHECK(pa.alloc(ba + 16));
+  EH_CHECK(res.alloc(64));
+  EH_HIP(hipMemcpyAsync(pa.p, a->data, ba, hipMemcpyHostToDevice, st));
+  if (b->cardinality) {
+    const uint64_t bb = view_bytes(b);
+    EH_CHECK(pb.alloc(bb + 16));
+    EH_HIP(hipMemcpyAsync(pb.p, b->data, bb, hipMemcpyHostToDevice, st));
+    if (b->type == EH_BS) {
+      uint64_t b0 = 0;
+      memcpy(&b0, b->data, 8);
+      const uint64_t nbw = (bb - 8) / 8;
+      const uint64_t start = a0 > b0 ? a0 : b0, end = (a0 + na) < (b0 + nbw) ? (a0 + na) : (b0 + nbw);
+      if (start < end)
+        k_union_words<<<1, ISECT_THREADS, 0, st>>>(pa.as<uint8_t>() + 8, start - a0, pb.as<uint8_t>() + 8, start - b0,
+                                                   end - start);
+    } else {
+      EH_CHECK(lb.alloc(4 * b->cardinality + 16));
+      EH_CHECK(ps.alloc(4 * (3 * (bb / 6 + 2) + 2)));
+      k_decode_set<<<1, ISECT_THREADS, 0, st>>>(b->type, pb.as<uint8_t>(), bb, b->cardinality, lb.as<uint32_t>(),
+                                                ps.as<uint32_t>());
+      k_union_elems<<<1, ISECT_THREADS, 0, st>>>(pa.as<uint8_t>() + 8, a0, na, lb.as<uint32_t>(), b->cardinality);
+    }
+    EH_HIP(hipGetLastError());
+  }
+  // the result's element count: A \ {} over A's own words
+  k_difference<<<1, ISECT_THREADS, 0, st>>>(pa.as<uint8_t>() + 8, a0, na, nullptr, 0, 0, nullptr,
+                                            res.as<unsigned long long>());
+  EH_HIP(hipGetLastError());
+  unsigned long long c = 0;
+  EH_HIP(hipMemcpyAsync(&c, res.p, 8, hipMemcpyDeviceToHost, st));
+  EH_HIP(hipMemcpyAsync(out_buf, pa.p, ba, hipMemcpyDeviceToHost, st));
+  EH_HIP(hipStreamSynchronize(st));
+  out->data = out_buf;
+  out->cardinality = c;
+  out->number_of_bytes = ba;
+  out->type = EH_BS;
+  out->reserved = 0;
+  return EH_OK;
+}
+
+extern "C" eh_status eh_difference(eh_ctx *ctx, const eh_row_view *a, const eh_row_view *b, void *out_buf,
+                                   size_t out_cap, eh_row_view *out) {
+  if (!ctx || !a || !b || !out || (out_cap && !out_buf)) { eh_set_error("null argument"); return EH_ERR_INVALID; }
+  for (const eh_row_view *v : {a, b})
+    if (v->type != EH_BS || (v->number_of_bytes && (v->number_of_bytes < 16 || (v->number_of_bytes - 8) % 8 || !v->data))) {
+      eh_set_error("eh_difference: both sets must be bitset images");
+      return EH_ERR_INVALID;
+    }
+  out->data = out_buf;
+  out->cardinality = 0;
+  out->number_of_bytes = 0;
+  out->type = EH_U;
+  out->reserved = 0;
+  // the reference leaves C untouched unless both sides have bytes (difference.hpp:7)
+  if (a->number_of_bytes == 0 || b->number_of_bytes == 0) return EH_OK;
+  EH_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  uint64_t a0 = 0, b0 = 0;
+  memcpy(&a0, a->data, 8);
+  memcpy(&b0, b->data, 8);
+  const uint64_t na = (a->number_of_bytes - 8) / 8, nb = (b->number_of_bytes - 8) / 8;
+  DevBuf pa, pb, o, res;
+  EH_CHECK(pa.alloc(a->number_of_bytes + 16));
+  EH_CHECK(pb.alloc(b->number_of_bytes + 16));
+  EH_CHECK(o.alloc(256 * na + 16));
+  EH_CHECK(res.alloc(64));
+  EH_HIP(hipMemcpyAsync(pa.p, a->data, a->number_of_bytes, hipMemcpyHostToDevice, st));
+  EH_HIP(hipMemcpyAsync(pb.p, b->data, b->number_of_bytes, hipMemcpyHostToDevice, st));
+  k_difference<<<1, ISECT_THREADS, 0, st>>>(pa.as<uint8_t>() + 8, a0, na, pb.as<uint8_t>() + 8, b0, nb,
+                                            o.as<uint32_t>(), res.as<unsigned long long>());
+  EH_HIP(hipGetLastError());
+  unsigned long long c = 0;
+  EH_HIP(hipMemcpyAsync(&c, res.p, 8, hipMemcpyDeviceToHost, st));
+  EH_HIP(hipStreamSynchronize(st));
+  out->cardinality = c;
+  if (4 * c > out_cap) { eh_set_error("result needs %llu bytes", (unsigned long long)(4 * c)); return EH_ERR_CAPACITY; }
+  if (c) EH_HIP(hipMemcpy(out_buf, o.p, 4 * c, hipMemcpyDeviceToHost));
+  out->number_of_bytes = 4 * c;
   return EH_OK;
 }
 
