@@ -1,9 +1,10 @@
 // Parser.hpp -- Parser and input_parser::parse (src/Parser.hpp:10-180): the app
-// command line.  Flags as in the reference: --graph=<path> --input_type=binary
-// --t=<threads> --layout=<uint|pshort|bs|hybrid> [--n=<clique size, default 4>]
-// [--start_node=<id>]; the reference's main also takes new_type and bitset_new
-// (src/emptyheaded.hpp:60-65); this engine adds --layout=hybrid3|cost|learned|hybrid_block and
-// --schedule=<tuned|plain>.  Only undirected binary input is on this path.
+// command line.  Flags as in the reference: --graph=<path> --input_type=<binary|text>
+// --t=<threads> --layout=<uint|pshort|bs|hybrid> [--n=<clique size / path depth>]
+// [--start_node=<id>] [--asymmetric] (a directed graph); the reference's main also
+// takes new_type and bitset_new (src/emptyheaded.hpp:60-65); this engine adds
+// --layout=hybrid3|cost|learned|hybrid_block and --schedule=<tuned|plain>.  Attribute
+// files are not read.
 #pragma once
 
 #include <getopt.h>
@@ -36,12 +37,17 @@ namespace input_parser {
 inline void printUsage(const std::string &app) {
   std::cout << "USAGE: ./application <OPTIONS>" << std::endl;
   std::cout << "OPTIONS: " << std::endl;
-  std::cout << "\tREQUIRED: --graph=<path to graph> --input_type=<'binary'> --t=<# of threads>" << std::endl;
+  std::cout << "\tREQUIRED: --graph=<path to graph> --input_type=<'binary','text'> --t=<# of threads>" << std::endl;
   std::cout << "\t\t--layout=<uint,pshort,bs,hybrid,new_type,bitset_new,hybrid3,cost,learned,hybrid_block>" << std::endl;
   std::cout << "\tOPTIONAL: --schedule=<tuned,plain>" << std::endl;
+  std::cout << "\tOPTIONAL: --asymmetric (directed input), --input_type=text (edge list)" << std::endl;
   if (app == "n_clique") {
     std::cout << "\tOPTIONAL: --n=<size of cliques to find, 3 to 16, default is 4>" << std::endl;
     std::cout << "*takes a undirected graph as input*" << std::endl;
+  }
+  if (app == "n_path") {
+    std::cout << "\tOPTIONAL: --n=<maximum path length> --start_node=<external id, default: the largest row>"
+              << std::endl;
   }
   std::exit(0);
 }
@@ -82,13 +88,22 @@ inline Parser parse(int argc, char *argv[], const std::string &app) {
     }
   }
   if (num_threads == 0 || !graph_path || !input_type || help || !layout_type) printUsage(app);
-  if (n == -1 && app == "n_clique") n = 4;
-  if (attributes || directed || std::string(input_type) != "binary") {
-    std::cerr << "ERROR: this engine reads undirected binary graphs only" << std::endl;
+  if (n == -1 && (app == "n_clique" || app == "n_cycle")) n = 4;
+  if (n == -1 && app == "n_path") n = 0xeffffff;
+  const std::string kind(input_type);
+  if (attributes || (kind != "binary" && kind != "text")) {
+    std::cerr << "ERROR: this engine reads binary or text edge-list graphs without attributes" << std::endl;
     std::exit(1);
   }
-  std::cout << "Loading a undirected graph" << std::endl;
-  MutableGraph *g = MutableGraph::undirectedFromBinary(graph_path);
+  MutableGraph *g;
+  if (directed) {
+    std::cout << "Loading a directed graph" << std::endl;
+    g = kind == "text" ? MutableGraph::directedFromEdgeList(graph_path) : MutableGraph::directedFromBinary(graph_path);
+  } else {
+    std::cout << "Loading a undirected graph" << std::endl;
+    g = kind == "text" ? MutableGraph::undirectedFromEdgeList(graph_path)
+                       : MutableGraph::undirectedFromBinary(graph_path);
+  }
   Parser p(num_threads, false, n, (size_t)start_node, g, layout_type);
   if (schedule) {
     const std::string s(schedule);

@@ -62,6 +62,39 @@ class Set {
                                type);
   }
 
+  // Set<T>::from_array (src/set/Set.hpp:161-165) for the layouts built on the host:
+  // uinteger (a copy) and bitset ([u64 first word][words], bitset.hpp:65-96)
+  static Set<T> from_array(uint8_t *set_data, const uint32_t *array_data, size_t data_size) {
+    const double density = data_size ? (double)((array_data[data_size - 1] - array_data[0]) / data_size) : 0.0;
+    size_t bytes = 0;
+    if (T::get_type() == common::UINTEGER) {
+      std::memcpy(set_data, array_data, 4 * data_size);
+      bytes = 4 * data_size;
+    } else if (T::get_type() == common::BITSET && data_size) {
+      const uint64_t w0 = array_data[0] >> 6, nw = (array_data[data_size - 1] >> 6) - w0 + 1;
+      std::memcpy(set_data, &w0, 8);
+      std::memset(set_data + 8, 0, 8 * nw);
+      for (size_t k = 0; k < data_size; k++) {
+        uint64_t w;
+        uint8_t *p = set_data + 8 + 8 * ((array_data[k] >> 6) - w0);
+        std::memcpy(&w, p, 8);
+        w |= 1ull << (array_data[k] & 63u);
+        std::memcpy(p, &w, 8);
+      }
+      bytes = 8 + 8 * nw;
+    }
+    return Set<T>(set_data, data_size, bytes, density, T::get_type());
+  }
+
+  // T::par_foreach (e.g. uinteger.hpp:101-118): f(tid, x) for every element; the host
+  // walk here is one thread, so tid is 0 and 1 thread is reported
+  template <typename F>
+  size_t par_foreach(size_t num_threads, F f) const {
+    (void)num_threads;
+    foreach([&](uint32_t x) { f((size_t)0, x); });
+    return 1;
+  }
+
   void copy_from(const Set<T> &src) {
     std::memcpy(data, src.data, src.number_of_bytes);
     cardinality = src.cardinality;

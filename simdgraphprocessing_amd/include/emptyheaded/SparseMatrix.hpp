@@ -126,6 +126,23 @@ class SparseMatrix {
     return id;
   }
 
+  // the row of an external id, by linear search; 0 when absent (src/SparseMatrix.hpp:147-154)
+  uint32_t get_internal_id(uint64_t external_id) const {
+    for (size_t i = 0; i < matrix_size; i++)
+      if (id_map[i] == external_id) return (uint32_t)i;
+    return 0;
+  }
+
+  // n_path's search (apps/n_path.cpp:41-85) over the rows [0, matrix_size) on the device
+  eh_bfs_info bfs_path(uint32_t start, uint32_t max_depth) const {
+    eh_bfs_info info;
+    emptyheaded::Runtime &rt = emptyheaded::Runtime::get();
+    std::lock_guard<std::mutex> lock(rt.mutex());
+    emptyheaded::check(eh_bfs_path(handle, (uint32_t)matrix_size, start, max_depth, nullptr, 0, &info),
+                       "SparseMatrix::bfs_path");
+    return info;
+  }
+
   // |row(a[p]) & row(b[p])| for many pairs at once on the device (columns of a
   // directed graph are rows matrix_size + i)
   std::vector<uint64_t> intersect_rows(const std::vector<uint32_t> &a, const std::vector<uint32_t> &b) const {

@@ -98,6 +98,14 @@ inline void foreach_bytes(F f, const uint8_t *data, size_t cardinality, size_t n
     static void foreach(F f, const uint8_t *data, size_t card, size_t bytes, common::type t) {   \
       emptyheaded::foreach_bytes(f, data, card, bytes, t);                                       \
     }                                                                                            \
+    /* the bit helpers of bitset.hpp:50-63 */                                                    \
+    static size_t word_index(uint32_t bit_index) { return bit_index >> 6; }                      \
+    static bool is_set(uint32_t index, const uint64_t *in_array, uint64_t start_index) {         \
+      return (in_array[word_index(index) - start_index] >> (index & 63u)) & 1ull;                \
+    }                                                                                            \
+    static void set(uint32_t index, uint64_t *in_array, uint64_t start_index) {                  \
+      in_array[word_index(index) - start_index] |= 1ull << (index & 63u);                        \
+    }                                                                                            \
   };
 
 // src/set/layouts/uinteger.hpp, pshort.hpp, bitset.hpp, hybrid.hpp

@@ -1,5 +1,6 @@
 // ops.hpp -- ops::set_intersect(Set<X>* C, const Set<Y>* A, const Set<Z>* B)
-// (src/set/ops/intersection.hpp:829,880,956,1019,1023,1059,1062,1321,1325,2067).
+// (src/set/ops/intersection.hpp:829,880,956,1019,1023,1059,1062,1321,1325,2067),
+// ops::set_union (union.hpp:6-127) and ops::set_difference (difference.hpp:5-84).
 //
 // One overload covers every (X, Y, Z): the pair is dispatched on A->type and
 // B->type at run time by the device (eh_intersect).  As in the reference:
@@ -66,6 +67,39 @@ inline Set<X> *set_intersect(Set<X> *C, const Set<Y> *A, const Set<Z> *B) {
   C->number_of_bytes = WRITE_VECTOR ? out.number_of_bytes : 0;
   C->density = 0.0;
   C->type = (common::type)out.type;
+  return C;
+}
+
+// ops::set_union(Set<bitset>* A, Set<X>* B) (src/set/ops/union.hpp:6-127): B's
+// elements OR-ed into A's words in place (B bitset, uinteger or pshort; elements
+// outside A's word span are dropped).  Like the reference, A's cardinality is left as
+// it was.
+template <class X>
+inline void set_union(Set<bitset> *A, const Set<X> *B) {
+  const eh_row_view a = detail::view_of(A), b = detail::view_of(B);
+  eh_row_view out;
+  emptyheaded::Runtime &rt = emptyheaded::Runtime::get();
+  std::lock_guard<std::mutex> lock(rt.mutex());
+  emptyheaded::check(eh_union(rt.ctx(), &a, &b, A->data, A->number_of_bytes, &out), "ops::set_union");
+}
+
+// ops::set_difference(Set<uinteger>* C, const Set<bitset>* A, const Set<bitset>* B)
+// (src/set/ops/difference.hpp:5-84): C = the elements of A not in B, sorted; C->data
+// must hold 4 bytes per element of A.  C is left as it was unless both sides have bytes.
+inline Set<uinteger> *set_difference(Set<uinteger> *C, const Set<bitset> *A, const Set<bitset> *B) {
+  const eh_row_view a = detail::view_of(A), b = detail::view_of(B);
+  eh_row_view out;
+  emptyheaded::Runtime &rt = emptyheaded::Runtime::get();
+  {
+    std::lock_guard<std::mutex> lock(rt.mutex());
+    const uint64_t cap = A->number_of_bytes >= 8 ? 256 * ((A->number_of_bytes - 8) / 8) : 0;
+    emptyheaded::check(eh_difference(rt.ctx(), &a, &b, C->data, cap, &out), "ops::set_difference");
+  }
+  if (A->number_of_bytes == 0 || B->number_of_bytes == 0) return C;
+  C->cardinality = out.cardinality;
+  C->number_of_bytes = out.number_of_bytes;
+  C->density = 0.0;
+  C->type = common::UINTEGER;
   return C;
 }
 

@@ -54,6 +54,46 @@ def test_cli_cliques(fb_path):
     assert "Number of cliques: 30004668" in out
 
 
+@pytest.fixture(scope="module")
+def dfb_path(tmp_path_factory):
+    p = tmp_path_factory.mktemp("dfbcpp") / "dfacebook.bin"
+    with gzip.open(os.path.join(ROOT, "tests", "golden", "dfacebook.bin.gz"), "rb") as f:
+        p.write_bytes(f.read())
+    return str(p)
+
+
+@pytest.mark.gpu
+def test_cpp_n_path(fb_path, dfb_path):
+    """test/n_path_test.cpp: dfacebook.bin directed, start 0, depth 4 -> 4; facebook from its
+    largest row -> 5, the reference program's output."""
+    rc, out, err = run(["test_n_path", dfb_path, fb_path])
+    assert rc == 0 and "PASSED" in out, out[-2000:] + err[-2000:]
+
+
+@pytest.mark.gpu
+def test_cli_n_path(fb_path, dfb_path):
+    rc, out, err = run(["n_path", f"--graph={dfb_path}", "--input_type=binary", "--t=4", "--layout=uint", "--n=4",
+                        "--start_node=0", "--asymmetric"])
+    assert rc == 0 and "Loading a directed graph" in out and "path length: 4" in out, out[-2000:] + err
+    rc, out, err = run(["n_path", f"--graph={fb_path}", "--input_type=binary", "--t=4", "--layout=hybrid"])
+    assert rc == 0 and "Start node - External: 107 Internal: 0" in out and "path length: 5" in out, out + err
+
+
+@pytest.mark.gpu
+def test_cli_text_input(fb_path, tmp_path):
+    """--input_type=text: the edge-list loader (src/MutableGraph.hpp:517-598) feeds the apps."""
+    import oracle as O
+    n, off, nbr, ext = O.load_binary(fb_path)
+    with open(tmp_path / "fb.txt", "w") as f:
+        for i in range(n):
+            for j in nbr[off[i]:off[i + 1]].tolist():
+                if j > i:
+                    f.write(f"{ext[i]} {ext[j]}\n")
+    rc, out, err = run(["undirected_triangle_counting", f"--graph={tmp_path / 'fb.txt'}", "--input_type=text",
+                        "--t=4", "--layout=hybrid"])
+    assert rc == 0 and "Number of triangles: 1612010" in out, out[-2000:] + err
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("layout,row_bytes", [("bitset_new", 513116), ("new_type", 277200)])
 def test_cli_block_layouts(fb_path, layout, row_bytes):
