@@ -114,6 +114,31 @@ def config_c1(ctx, with_cpu):
     return out
 
 
+def config_bfs(ctx, with_cpu, scale=22, ef=16, seed=2):
+    """n_path's BFS (apps/n_path.cpp:27-87) on the RMAT graph of the headline, every entry kept,
+    from its largest row (get_max_row_id), no depth limit."""
+    g = ctx.generate_rmat(scale, ef, seed)
+    M = g.build_matrix("all", "hybrid")
+    i = M.info()
+    start = 0  # rows are degree-relabelled: row 0 is the first row of maximum length
+    runs = [M.bfs_path(start) for _ in range(5)]
+    r = sorted(runs, key=lambda x: x["kernel_ms"])[len(runs) // 2]
+    # edges traversed: every entry of every reached row is scanned once
+    out = {"workload": f"BFS path length, RMAT scale {scale} ef {ef} seed {seed}, all entries, start = largest row",
+           "path_length": r["path_length"], "rounds": r["rounds"], "reached": r["reached"],
+           "level_sizes": r["level_sizes"], "kernel_ms": r["kernel_ms"], "entries": i["cardinality"],
+           "teps": i["cardinality"] / (r["kernel_ms"] * 1e-3)}
+    if with_cpu:
+        import oracle as O
+        off, nbr = g.download()
+        t0 = time.perf_counter()
+        ref = O.bfs_path(i["n"], off, nbr, start, cap=256)
+        secs = time.perf_counter() - t0
+        assert ref["path_length"] == r["path_length"] and ref["level_sizes"] == r["level_sizes"]
+        out["cpu"] = {"seconds": secs, "teps": i["cardinality"] / secs, "cores": 1, "kind": "port"}
+    return out
+
+
 def config_c3(ctx, with_cpu):
     """Chung-Lu n=1e6, gamma 2.1, mean degree 32 + 200 planted 256-vertex communities (p=.5), seed 3: 4-cliques."""
     g = ctx.generate_chung_lu(1000000, 32, 2.1, 3, communities=200, community_size=256)
@@ -216,7 +241,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--policy", default="auto", help="layouts of the timed pass: auto = learned when the "
                     "trained selector parameters exist, else hybrid")
-    ap.add_argument("--configs", default="c1,c3,c4,c5", help="extra configs to report (c1,c3,c4,c5)")
+    ap.add_argument("--configs", default="c1,c3,c4,c5,bfs", help="extra configs to report (c1,c3,c4,c5,bfs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--params", default="", help='fixed launch shape instead of autotuning, e.g. '
@@ -315,6 +340,8 @@ def main():
         extra["c3"] = config_c3(ctx, not args.no_cpu)
     if "c4" in wanted:
         extra["c4"] = config_c4(ctx, not args.no_cpu)
+    if "bfs" in wanted:
+        extra["bfs"] = config_bfs(ctx, not args.no_cpu, args.scale, args.edge_factor, args.seed)
     if "c5" in wanted and args.policy == "learned":
         extra["c5"] = config_c5(ctx, not args.no_cpu)
 
