@@ -121,12 +121,13 @@ def config_bfs(ctx, with_cpu, scale=22, ef=16, seed=2):
     M = g.build_matrix("all", "hybrid")
     i = M.info()
     start = 0  # rows are degree-relabelled: row 0 is the first row of maximum length
-    runs = [M.bfs_path(start) for _ in range(5)]
+    runs = [M.bfs_path(start, mode="symmetric") for _ in range(5)]
     r = sorted(runs, key=lambda x: x["kernel_ms"])[len(runs) // 2]
     # edges traversed: every entry of every reached row is scanned once
     out = {"workload": f"BFS path length, RMAT scale {scale} ef {ef} seed {seed}, all entries, start = largest row",
            "path_length": r["path_length"], "rounds": r["rounds"], "reached": r["reached"],
-           "level_sizes": r["level_sizes"], "kernel_ms": r["kernel_ms"], "entries": i["cardinality"],
+           "level_sizes": r["level_sizes"], "kernel_ms": r["kernel_ms"], "bottom_up_rounds": r["bottom_up"],
+           "top_down_kernel_ms": M.bfs_path(start)["kernel_ms"], "entries": i["cardinality"],
            "teps": i["cardinality"] / (r["kernel_ms"] * 1e-3)}
     if with_cpu:
         import oracle as O

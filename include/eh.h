@@ -269,13 +269,24 @@ eh_status eh_clique_count(eh_matrix *m, uint32_t k, const eh_run_opts *opts, uin
  * path_length reached max_depth (query_depth; n_path's default is 0xEFFFFFF).
  * level_sizes[r] (r < level_cap, may be NULL with level_cap 0) receives the size of the
  * frontier round r produced. */
+typedef enum {
+  EH_BFS_TOP_DOWN = 0,  /* every round expands the frontier's rows */
+  EH_BFS_SYMMETRIC = 1, /* rows are symmetric: wide rounds may run bottom-up over the rows */
+  EH_BFS_IN_ROWS = 2    /* rows [n_vertices, 2 n_vertices) hold in-neighbourhoods (a directed
+                           matrix as the C++ surface builds it): wide rounds run bottom-up over them */
+} eh_bfs_mode;
 typedef struct {
-  uint32_t path_length; /* n_path::path_length */
-  uint32_t rounds;      /* frontier expansions run */
-  uint64_t reached;     /* visited vertices, the start included */
-  double kernel_ms;     /* device time of the search */
+  uint32_t path_length;  /* n_path::path_length */
+  uint32_t rounds;       /* frontier expansions run */
+  uint64_t reached;      /* visited vertices, the start included */
+  double kernel_ms;      /* device time of the search */
+  uint32_t bottom_up;    /* rounds run bottom-up */
+  uint32_t reserved;
 } eh_bfs_info;
-eh_status eh_bfs_path(eh_matrix *m, uint32_t n_vertices, uint32_t start, uint32_t max_depth,
+/* mode: how wide rounds may be run; every mode gives the same frontiers, path length and
+ * level sizes (a round's new frontier is the set of unvisited vertices adjacent to the
+ * current one, whichever side enumerates the adjacency). */
+eh_status eh_bfs_path(eh_matrix *m, uint32_t n_vertices, uint32_t start, uint32_t max_depth, eh_bfs_mode mode,
                       uint64_t *level_sizes, uint32_t level_cap, eh_bfs_info *info);
 /* No reference counterpart (the reference times whole phases only,
  * src/common.hpp:87-99): one counting pass with the shader-clock cycles of

@@ -53,7 +53,10 @@ class RunOpts(C.Structure):
 
 class BfsInfo(C.Structure):
     _fields_ = [("path_length", C.c_uint32), ("rounds", C.c_uint32), ("reached", C.c_uint64),
-                ("kernel_ms", C.c_double)]
+                ("kernel_ms", C.c_double), ("bottom_up", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+BFS_MODES = {"top_down": 0, "symmetric": 1, "in_rows": 2}
 
 
 class RunStats(C.Structure):
@@ -143,7 +146,8 @@ def lib():
     for name in ("eh_union", "eh_difference"):
         getattr(L, name).argtypes = [_vp, C.POINTER(RowView), C.POINTER(RowView), _vp, C.c_size_t,
                                      C.POINTER(RowView)]
-    L.eh_bfs_path.argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_uint32, _vp, C.c_uint32, C.POINTER(BfsInfo)]
+    L.eh_bfs_path.argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, _vp, C.c_uint32,
+                              C.POINTER(BfsInfo)]
     L.eh_triangle_count.argtypes = [_vp, C.POINTER(RunOpts), C.POINTER(C.c_uint64), C.POINTER(RunStats)]
     L.eh_clique_count.argtypes = [_vp, C.c_uint32, C.POINTER(RunOpts), C.POINTER(C.c_uint64),
                                   C.POINTER(RunStats)]
@@ -473,13 +477,16 @@ class SparseMatrix:
         _check(lib().eh_count_profile(self.h, k, C.byref(o), _ptr(rows), _ptr(kinds), _ptr(cyc), n, C.byref(nt)))
         return rows[:n], kinds[:n], cyc[:n]
 
-    def bfs_path(self, start, max_depth=0xEFFFFFF, n_vertices=0, level_cap=256):
-        """n_path's BFS (apps/n_path.cpp:27-87) from row `start`."""
+    def bfs_path(self, start, max_depth=0xEFFFFFF, n_vertices=0, mode="top_down", level_cap=256):
+        """n_path's BFS (apps/n_path.cpp:27-87) from row `start`; mode: top_down, symmetric
+        (rows are symmetric) or in_rows (rows [n, 2n) are in-neighbourhoods)."""
         sizes = np.zeros(max(level_cap, 1), np.uint64)
         bi = BfsInfo()
-        _check(lib().eh_bfs_path(self.h, n_vertices, start, max_depth, _ptr(sizes), level_cap, C.byref(bi)))
+        _check(lib().eh_bfs_path(self.h, n_vertices, start, max_depth, BFS_MODES[mode], _ptr(sizes), level_cap,
+                                 C.byref(bi)))
         return {"path_length": bi.path_length, "rounds": bi.rounds, "reached": bi.reached,
-                "level_sizes": sizes[:min(bi.rounds, level_cap)].tolist(), "kernel_ms": bi.kernel_ms}
+                "level_sizes": sizes[:min(bi.rounds, level_cap)].tolist(), "kernel_ms": bi.kernel_ms,
+                "bottom_up": bi.bottom_up}
 
     def intersect_rows(self, a_rows, b_rows):
         a = np.ascontiguousarray(a_rows, np.uint32)

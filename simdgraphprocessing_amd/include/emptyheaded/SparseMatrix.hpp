@@ -138,7 +138,9 @@ class SparseMatrix {
     eh_bfs_info info;
     emptyheaded::Runtime &rt = emptyheaded::Runtime::get();
     std::lock_guard<std::mutex> lock(rt.mutex());
-    emptyheaded::check(eh_bfs_path(handle, (uint32_t)matrix_size, start, max_depth, nullptr, 0, &info),
+    // a directed matrix holds its in-neighbourhoods as rows matrix_size .. 2 matrix_size - 1
+    emptyheaded::check(eh_bfs_path(handle, (uint32_t)matrix_size, start, max_depth,
+                                   symmetric ? EH_BFS_SYMMETRIC : EH_BFS_IN_ROWS, nullptr, 0, &info),
                        "SparseMatrix::bfs_path");
     return info;
   }

@@ -23,11 +23,14 @@ def _dfacebook():
     return n, off, cat(outs + ins), ext
 
 
-def check(M, n, off, nbr, start, depth, n_vertices=0):
-    got = M.bfs_path(start, depth, n_vertices=n_vertices)
+def check(M, n, off, nbr, start, depth, n_vertices=0, modes=("top_down", "symmetric")):
+    """Every mode (top-down only, or bottom-up rounds where they pay) gives the oracle's
+    path length, rounds, level sizes and reach."""
     want = O.bfs_path(n, off, nbr, start, depth, cap=256)
-    for k in ("path_length", "rounds", "reached", "level_sizes"):
-        assert got[k] == want[k], (k, start, depth, got[k], want[k])
+    for mode in modes:
+        got = M.bfs_path(start, depth, n_vertices=n_vertices, mode=mode)
+        for k in ("path_length", "rounds", "reached", "level_sizes"):
+            assert got[k] == want[k], (mode, k, start, depth, got[k], want[k])
     return got
 
 
@@ -37,10 +40,25 @@ def test_bfs_fixture_dfacebook(ctx):
     M = ctx.matrix_from_csr(2 * n, off, nbr, "all", "uint")
     start = int(np.nonzero(ext == 0)[0][0])
     out_off, out_nbr = off[:n + 1], nbr[:int(off[n])]
-    got = check(M, n, out_off, out_nbr, start, 4, n_vertices=n)
+    got = check(M, n, out_off, out_nbr, start, 4, n_vertices=n, modes=("top_down", "in_rows"))
     assert got["path_length"] == 4
     for s in (5, 1000, 3000):
-        check(M, n, out_off, out_nbr, s, 4, n_vertices=n)
+        check(M, n, out_off, out_nbr, s, 4, n_vertices=n, modes=("top_down", "in_rows"))
+    for s in (0, 77):   # no depth limit
+        check(M, n, out_off, out_nbr, s, 0xEFFFFFF, n_vertices=n, modes=("top_down", "in_rows"))
+
+
+def test_bfs_in_rows_bottom_up(ctx):
+    """A symmetric RMAT-12 given as out-rows + in-rows (2n rows): wide rounds run bottom-up
+    over rows [n, 2n) and still give the oracle's levels."""
+    n = 1 << 12
+    _, off, nbr = O.canonical_oriented(n, *O.gen_rmat(12, 8, 4))["relabel"]
+    off2 = np.concatenate([off, off[1:] + off[-1]]).astype(np.uint64)
+    nbr2 = np.concatenate([nbr, nbr]).astype(np.uint32)
+    M = ctx.matrix_from_csr(2 * n, off2, nbr2, "all", "hybrid")
+    for s in (0, 5, 3000):
+        check(M, n, off, nbr, s, 0xEFFFFFF, n_vertices=n, modes=("top_down", "in_rows"))
+    assert M.bfs_path(0, n_vertices=n, mode="in_rows")["bottom_up"] > 0
 
 
 @pytest.mark.parametrize("policy", ["uint", "hybrid", "bs", "pshort", "bitset_new"])
@@ -63,6 +81,7 @@ def test_bfs_rmat(ctx, scale):
         for s in rng.integers(0, n, 6).tolist() + [0, n - 1]:
             for d in (0xEFFFFFF, 3):
                 g = check(M, n, off, nbr, s, d)
+        assert M.bfs_path(0, mode="symmetric")["bottom_up"] > 0
         print(f"RMAT-{scale} {policy}: last BFS {g['kernel_ms']:.3f} ms, reached {g['reached']}", flush=True)
 
 
@@ -80,6 +99,8 @@ def test_bfs_edges(ctx):
         M.bfs_path(100)
     with pytest.raises(eh.EHError):
         M.bfs_path(0, n_vertices=101)
+    with pytest.raises(eh.EHError):
+        M.bfs_path(0, n_vertices=60, mode="in_rows")   # in-rows need 2 x 60 rows
 
 
 def _bitset(vals):
