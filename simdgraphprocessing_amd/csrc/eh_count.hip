@@ -934,6 +934,61 @@ __device__ __forceinline__ void tri_decode(uint32_t r, uint32_t *p, uint32_t *q)
   *p = r - (b * (b - 1u)) / 2u;
 }
 
+// One membership query x in row(j) of the two-query interleave below: a bitset row or an
+// empty row answers at once, a pshort row walks its partitions, every other row (uint,
+// and the canonical list of a block row) is searched in the sorted list.
+struct MemberQuery {
+  const uint32_t *L;
+  uint32_t base, len, x, hit;
+};
+
+__device__ __forceinline__ void mq_start(const CountArgs &a, const RowDesc &d, uint32_t x, MemberQuery &q) {
+  q.x = x;
+  q.base = 0;
+  q.len = 0;
+  q.hit = 0;
+  q.L = a.unbr;
+  if (d.card == 0) return;
+  const uint32_t t = desc_type(d);
+  if (t == EH_BS || t == EH_PS) {
+    q.hit = row_contains(a, d, x);
+    return;
+  }
+  q.L = a.unbr + (uint64_t)d.off * 4;
+  q.len = d.card;
+}
+
+// advance NQ searches together: every probe is issued before any is used, so a lane keeps
+// NQ dependent load chains in flight; lanes whose search is over probe their base
+template <int NQ>
+__device__ __forceinline__ void mq_finish(MemberQuery (&q)[NQ]) {
+  for (;;) {
+    bool more = false;
+#pragma unroll
+    for (int k = 0; k < NQ; k++) more |= q[k].len > 1;
+    if (!more) break;
+    uint32_t v[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; k++) v[k] = q[k].L[q[k].base + (q[k].len >> 1)];
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+      const uint32_t h = q[k].len >> 1;
+      if (q[k].len > 1) {
+        q[k].base = (v[k] <= q[k].x) ? q[k].base + h : q[k].base;
+        q[k].len -= h;
+      }
+    }
+  }
+  uint32_t v[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; k++) v[k] = q[k].L[q[k].base];
+#pragma unroll
+  for (int k = 0; k < NQ; k++)
+    if (q[k].len == 1) q[k].hit = v[k] == q[k].x;
+}
+
+constexpr int SMALL_NQ = 4;  // pairs per lane whose searches run interleaved
+
 template <int K, bool W>
 __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t *__restrict__ rows, uint32_t nrows) {
   extern __shared__ uint32_t lds_all[];
@@ -965,6 +1020,47 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
     rcnt[lane] = 0;
     if (lane == 0) mbase[64] = tot;
     wave_sync();
+    if constexpr (K == 3) {
+      // SMALL_NQ pairs per lane, t + 64 k, their searches interleaved
+      for (uint32_t t = lane; t < tot; t += 64 * SMALL_NQ) {
+        uint32_t q[SMALL_NQ], x[SMALL_NQ], j[SMALL_NQ];
+#pragma unroll
+        for (int k = 0; k < SMALL_NQ; k++) {
+          const uint32_t tk = t + 64u * k;
+          q[k] = 0;
+#pragma unroll
+          for (uint32_t step = 32; step; step >>= 1)
+            if (mbase[q[k] + step] <= tk) q[k] += step;
+          x[k] = 0;
+          j[k] = 0;
+          if (tk < tot) {
+            uint32_t p, sq;
+            tri_decode(tk - mbase[q[k]], &p, &sq);
+            const uint32_t *R = a.unbr + (uint64_t)moff[q[k]] * 4;
+            x[k] = R[p];
+            j[k] = R[sq];
+          }
+        }
+        MemberQuery m[SMALL_NQ];
+#pragma unroll
+        for (int k = 0; k < SMALL_NQ; k++) {
+          const RowDesc d = (t + 64u * k < tot) ? a.desc[j[k]] : RowDesc{0u, 0u, 0u, 0u};
+          mq_start(a, d, x[k], m[k]);
+        }
+        mq_finish<SMALL_NQ>(m);
+#pragma unroll
+        for (int k = 0; k < SMALL_NQ; k++) {
+          uint64_t h;
+          if constexpr (W) h = m[k].hit ? (uint64_t)a.card[x[k]] : 0ull;
+          else h = m[k].hit;
+          if (per_graph) {
+            if (h) atomicAdd(&rcnt[q[k]], (uint32_t)h);
+          } else {
+            acc += h;
+          }
+        }
+      }
+    } else {
     for (uint32_t t = lane; t < tot; t += 64) {
       uint32_t q = 0;
 #pragma unroll
@@ -973,25 +1069,26 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
       const uint32_t r = t - mbase[q];
       const uint32_t *R = a.unbr + (uint64_t)moff[q] * 4;
       uint64_t h;
-      if constexpr (K == 3) {
-        uint32_t p, s;
-        tri_decode(r, &p, &s);
-        const uint32_t x = R[p], j = R[s];
-        const uint32_t m = row_contains(a, a.desc[j], x);
-        if constexpr (W) h = m ? (uint64_t)a.card[x] : 0ull;
-        else h = m;
-      } else {
+      {
         uint32_t p, s;
         tri_decode(r, &p, &s);
         const uint32_t x = R[p], y = R[s];
         h = 0;
         if (row_contains(a, a.desc[y], x)) {
-          // (x, y) closes a triangle with the source: every later z adjacent to both closes a 4-clique
+          // (x, y) closes a triangle with the source: every later z adjacent to both closes a
+          // 4-clique; two z's at a time, their four searches interleaved
           const uint32_t d = mdeg[q];
-          for (uint32_t u = s + 1u; u < d; u++) {
-            const uint32_t z = R[u];
-            const RowDesc dz = a.desc[z];
-            if (row_contains(a, dz, y) && row_contains(a, dz, x)) h++;
+          for (uint32_t u = s + 1u; u < d; u += 2) {
+            const bool two = u + 1u < d;
+            const RowDesc dz0 = a.desc[R[u]];
+            const RowDesc dz1 = two ? a.desc[R[u + 1u]] : RowDesc{0u, 0u, 0u, 0u};
+            MemberQuery m[4];
+            mq_start(a, dz0, y, m[0]);
+            mq_start(a, dz0, x, m[1]);
+            mq_start(a, dz1, y, m[2]);
+            mq_start(a, dz1, x, m[3]);
+            mq_finish<4>(m);
+            h += (m[0].hit & m[1].hit) + (m[2].hit & m[3].hit);
           }
         }
       }
@@ -1000,6 +1097,7 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
       } else {
         acc += h;
       }
+    }
     }
     wave_sync();
     if (per_graph && lane < nr && rcnt[lane]) atomicAdd(&a.counts[gid], (unsigned long long)rcnt[lane]);
