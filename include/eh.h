@@ -309,9 +309,11 @@ eh_status eh_batch_graph_download(eh_batch *b, uint32_t g, uint64_t *row_off, ui
                                   uint64_t cap);
 /* mode 0: one persistent launch over every graph, replayed from a captured
  * hipGraph; mode 1: one launch sequence per graph.  tri/k4 get ngraphs counts
- * each (either may be NULL). */
-eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *opts, uint64_t *tri,
-                         uint64_t *k4, double *ms);
+ * each (either may be NULL: that pass is not run).  tri_opts / k4_opts set the
+ * schedule and launch shape of the triangle and 4-clique passes (NULL: defaults);
+ * the graph is recorded again whenever either pass changes. */
+eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tri_opts,
+                         const eh_run_opts *k4_opts, uint64_t *tri, uint64_t *k4, double *ms);
 void eh_batch_free(eh_batch *b);
 
 /* ---------------------------------------------------------------- learned selector */

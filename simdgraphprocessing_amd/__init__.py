@@ -157,7 +157,8 @@ def lib():
     L.eh_batch_info.argtypes = [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.eh_batch_graph_params.argtypes = [_vp, _vp, _vp]
     L.eh_batch_graph_download.argtypes = [_vp, C.c_uint32, _vp, _vp, C.c_uint64]
-    L.eh_batch_count.argtypes = [_vp, C.c_int, C.POINTER(RunOpts), _vp, _vp, C.POINTER(C.c_double)]
+    L.eh_batch_count.argtypes = [_vp, C.c_int, C.POINTER(RunOpts), C.POINTER(RunOpts), _vp, _vp,
+                                 C.POINTER(C.c_double)]
     L.eh_batch_free.argtypes = [_vp]
     L.eh_batch_free.restype = None
     L.eh_selector_set_params.argtypes = [_vp, _vp]
@@ -528,12 +529,16 @@ class Batch:
         _check(lib().eh_batch_graph_download(self.h, g, _ptr(off), _ptr(nbr), cap))
         return n, off, nbr[:int(off[-1])].copy()
 
-    def count(self, mode="persistent", schedule="tuned", triangles=True, cliques=True, **kw):
+    def count(self, mode="persistent", schedule="tuned", triangles=True, cliques=True, tri_opts=None,
+              k4_opts=None, **kw):
+        """Per-graph triangle and 4-clique counts.  ``kw`` sets both passes' options;
+        ``tri_opts`` / ``k4_opts`` (dicts) override them for one pass."""
         ng = self.info()["graphs"]
         tri = np.zeros(ng, np.uint64) if triangles else None
         k4 = np.zeros(ng, np.uint64) if cliques else None
         ms = C.c_double()
-        o = _opts(schedule, **kw)
-        _check(lib().eh_batch_count(self.h, {"persistent": 0, "per_graph": 1}[mode], C.byref(o), _ptr(tri),
-                                    _ptr(k4), C.byref(ms)))
+        o3 = _opts(schedule, **{**kw, **(tri_opts or {})})
+        o4 = _opts(schedule, **{**kw, **(k4_opts or {})})
+        _check(lib().eh_batch_count(self.h, {"persistent": 0, "per_graph": 1}[mode], C.byref(o3), C.byref(o4),
+                                    _ptr(tri), _ptr(k4), C.byref(ms)))
         return tri, k4, ms.value

@@ -41,7 +41,7 @@ def _store(db):
 
 def candidates(M, k=3):
     """Launch/schedule grid: edge-centric row threshold x LDS budget x waves per block."""
-    smalls = (16, 24, 32, 48, 64) if k == 3 else (4, 8, 12, 16, 24)
+    smalls = (16, 24, 32, 48, 64) if k == 3 else (8, 16, 32, 48, 64)
     for small in smalls:
         for lds, waves in ((2048, 2), (2048, 4), (1536, 4), (3072, 2)):
             yield {"small_degree": small, "lds_words": lds, "waves": waves}
@@ -75,3 +75,60 @@ def tune(M, k=3, schedule="tuned", repeat=3, budget_s=60.0, use_cache=True, verb
     if use_cache:
         _store(db)
     return best[1], best[0], trials
+
+
+def _batch_signature(b, k, schedule):
+    i = b.info()
+    n, m = b.params()
+    key = f"batch:{i['graphs']}:{i['vertices']}:{i['edges']}:{int(n.sum())}:{int(m.sum())}:{k}:{schedule}"
+    return hashlib.sha1(key.encode()).hexdigest()[:16]
+
+
+def batch_candidates(k):
+    """Grid for one pass of a batch: edge-centric row threshold x LDS budget x waves."""
+    smalls = (32, 48, 64, 96) if k == 3 else (16, 32, 48, 64)
+    for small in smalls:
+        for lds, waves in ((2048, 2), (2048, 4), (1536, 4), (3072, 2)):
+            yield {"small_degree": small, "lds_words": lds, "waves": waves}
+
+
+def tune_batch(b, mode="persistent", schedule="tuned", repeat=2, budget_s=30.0, use_cache=True, verbose=False):
+    """Tune the triangle and the 4-clique pass of a batch separately (each timed alone in
+    `mode`); return ({"tri": params, "k4": params}, {"tri": ms, "k4": ms})."""
+    dev = b.ctx.device_name
+    db = _load() if use_cache else {}
+    params, best_ms = {}, {}
+    for name, k in (("tri", 3), ("k4", 4)):
+        sig = f"{dev}|{mode}|{_batch_signature(b, k, schedule)}"
+        if use_cache and sig in db:
+            params[name], best_ms[name] = db[sig]["params"], db[sig]["ms"]
+            continue
+        want = dict(triangles=(k == 3), cliques=(k == 4))
+
+        def run(p):
+            opt = {"tri_opts": p} if k == 3 else {"k4_opts": p}
+            return b.count(mode, schedule, **want, **opt)
+
+        ref = run({})
+        ref = ref[0] if k == 3 else ref[1]
+        best = None
+        t0 = time.time()
+        for p in batch_candidates(k):
+            if time.time() - t0 > budget_s:
+                break
+            ms = []
+            for _ in range(repeat):
+                r = run(p)
+                got = r[0] if k == 3 else r[1]
+                if not (got == ref).all():
+                    raise AssertionError(f"batch autotune candidate {p} changed the counts")
+                ms.append(r[2])
+            if verbose:
+                print(f"  batch autotune k={k} {p} -> {min(ms):.3f} ms", flush=True)
+            if best is None or min(ms) < best[0]:
+                best = (min(ms), p)
+        params[name], best_ms[name] = best[1], best[0]
+        db[sig] = {"params": best[1], "ms": best[0], "time": time.time()}
+    if use_cache:
+        _store(db)
+    return params, best_ms

@@ -63,6 +63,31 @@ __global__ void k_zero64(unsigned long long *p, uint32_t n) {
 
 }  // namespace
 
+// what a captured graph or a per-graph plan bakes in: the schedule buffers it reads
+// and the launch shape it was recorded with
+struct PassKey {
+  bool want = false;
+  int sched = -1;
+  TuneParams tp;
+  const void *tasks = nullptr, *small_rows = nullptr;
+  uint64_t n_big = 0, n_small = 0, n_small_rows = 0;
+  uint32_t lds_small = 0, lds_big = 0;
+};
+
+static bool same_tp(const TuneParams &x, const TuneParams &y) {
+  return x.lanes == y.lanes && x.task_work == y.task_work && x.lds_small == y.lds_small && x.lds_big == y.lds_big &&
+         x.waves_per_block == y.waves_per_block && x.big_waves == y.big_waves && x.small_deg == y.small_deg &&
+         x.stream_ratio == y.stream_ratio;
+}
+
+static bool same_key(const PassKey &x, const PassKey &y) {
+  if (x.want != y.want) return false;
+  if (!x.want) return true;
+  return x.sched == y.sched && same_tp(x.tp, y.tp) && x.tasks == y.tasks && x.small_rows == y.small_rows &&
+         x.n_big == y.n_big && x.n_small == y.n_small && x.n_small_rows == y.n_small_rows &&
+         x.lds_small == y.lds_small && x.lds_big == y.lds_big;
+}
+
 struct eh_batch {
   eh_ctx *ctx = nullptr;
   uint32_t ng = 0;
@@ -75,13 +100,13 @@ struct eh_batch {
   // per-graph launch plans (tasks grouped by graph, per class)
   struct Plan {
     bool valid = false;
-    int sched = -1;
+    PassKey key;
     DevBuf small_rows, tasks_big, tasks_small;
     std::vector<uint64_t> sr_off, tb_off, ts_off;  // ng + 1 offsets
   } plan[2];  // [0] triangles, [1] 4-cliques
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int graph_sched = -1;
+  PassKey graph_key[2];
   ~eh_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -173,11 +198,12 @@ extern "C" eh_status eh_batch_graph_download(eh_batch *b, uint32_t g, uint64_t *
 }
 
 // group a union schedule's work by graph for the per-graph launches
-static eh_status build_plan(eh_batch *b, int kidx, int sched) {
+static eh_status build_plan(eh_batch *b, int kidx, const PassKey &key) {
   eh_batch::Plan &P = b->plan[kidx];
-  if (P.valid && P.sched == sched) return EH_OK;
+  if (P.valid && same_key(P.key, key)) return EH_OK;
+  P.valid = false;
   eh_matrix *M = b->M;
-  Schedule &s = M->sched[kidx][sched];
+  Schedule &s = M->sched[kidx][key.sched];
   const uint32_t ng = b->ng;
   std::vector<uint32_t> sr(s.n_small_rows);
   std::vector<uint4> tasks(s.n_big + s.n_small);
@@ -208,34 +234,47 @@ static eh_status build_plan(eh_batch *b, int kidx, int sched) {
   if (!tb.empty()) EH_HIP(hipMemcpy(P.tasks_big.p, tb.data(), 16 * tb.size(), hipMemcpyHostToDevice));
   if (!ts.empty()) EH_HIP(hipMemcpy(P.tasks_small.p, ts.data(), 16 * ts.size(), hipMemcpyHostToDevice));
   P.valid = true;
-  P.sched = sched;
+  P.key = key;
   return EH_OK;
 }
 
-extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *opts, uint64_t *tri, uint64_t *k4,
-                                    double *ms) {
+extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tri_opts,
+                                    const eh_run_opts *k4_opts, uint64_t *tri, uint64_t *k4, double *ms) {
   if (!b || (mode != 0 && mode != 1)) { eh_set_error("eh_batch_count: bad argument"); return EH_ERR_INVALID; }
   eh_ctx *ctx = b->ctx;
   EH_HIP(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   eh_matrix *M = b->M;
   const uint32_t ng = b->ng;
-  const int sched = opts ? opts->schedule : EH_SCHED_TUNED;
+  const eh_run_opts *opts[2] = {tri_opts, k4_opts};
   const bool want[2] = {tri != nullptr, k4 != nullptr};
   const int kk[2] = {3, 4};
-  TuneParams tp[2];
-  // make sure both schedules exist (count_run builds them; its result is discarded)
+  PassKey key[2];
   for (int q = 0; q < 2; q++) {
+    key[q].want = want[q];
     if (!want[q]) continue;
-    tp[q] = tune_defaults(M, sched, kk[q], opts);
-    EH_CHECK(count_run(M, kk[q], opts, b->counts.as<uint64_t>() + (uint64_t)q * ng, ng, nullptr, nullptr, nullptr));
+    const int sched = opts[q] ? opts[q]->schedule : EH_SCHED_TUNED;
+    if (sched != EH_SCHED_PLAIN && sched != EH_SCHED_TUNED) { eh_set_error("bad schedule %d", sched); return EH_ERR_INVALID; }
+    // count_run builds (or reuses) the pass's schedule; its result is discarded
+    EH_CHECK(count_run(M, kk[q], opts[q], b->counts.as<uint64_t>() + (uint64_t)q * ng, ng, nullptr, nullptr, nullptr));
+    const Schedule &S = M->sched[q][sched];
+    key[q].sched = sched;
+    key[q].tp = tune_defaults(M, sched, kk[q], opts[q]);
+    key[q].tasks = S.tasks.p;
+    key[q].small_rows = S.small_rows.p;
+    key[q].n_big = S.n_big;
+    key[q].n_small = S.n_small;
+    key[q].n_small_rows = S.n_small_rows;
+    key[q].lds_small = S.lds_small;
+    key[q].lds_big = S.lds_big;
   }
   unsigned long long *cnt = b->counts.as<unsigned long long>();
   uint32_t *nx = b->next.as<uint32_t>();
   float elapsed = 0;
   if (mode == 0) {
-    // one captured graph: zero the counters, then the triangle and 4-clique passes
-    if (!b->exec || b->graph_sched != sched) {
+    // one captured graph: zero the counters, then the triangle and 4-clique passes; it is
+    // recorded again whenever a pass's schedule buffers or launch shape changed
+    if (!b->exec || !same_key(b->graph_key[0], key[0]) || !same_key(b->graph_key[1], key[1])) {
       if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
       if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
       EH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -243,7 +282,7 @@ extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *op
       eh_status s = EH_OK;
       for (int q = 0; q < 2 && s == EH_OK; q++)
         if (want[q])
-          s = schedule_launch(M, kk[q], sched, tp[q], M->sched[q][sched],
+          s = schedule_launch(M, kk[q], key[q].sched, key[q].tp, M->sched[q][key[q].sched],
                               b->counts.as<uint64_t>() + (uint64_t)q * ng, nx + 4 * q, nullptr);
       hipGraph_t gr = nullptr;
       hipError_t e = hipStreamEndCapture(st, &gr);
@@ -251,7 +290,8 @@ extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *op
       if (e != hipSuccess) { eh_set_error("graph capture: %s", hipGetErrorString(e)); return EH_ERR_HIP; }
       b->graph = gr;
       EH_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-      b->graph_sched = sched;
+      b->graph_key[0] = key[0];
+      b->graph_key[1] = key[1];
     }
     EH_HIP(hipEventRecord(ctx->ev0, st));
     EH_HIP(hipGraphLaunch(b->exec, st));
@@ -260,18 +300,17 @@ extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *op
     EH_HIP(hipEventElapsedTime(&elapsed, ctx->ev0, ctx->ev1));
   } else {
     for (int q = 0; q < 2; q++)
-      if (want[q]) EH_CHECK(build_plan(b, q, sched));
+      if (want[q]) EH_CHECK(build_plan(b, q, key[q]));
     EH_HIP(hipEventRecord(ctx->ev0, st));
     k_zero64<<<grid_for(2ull * ng), 256, 0, st>>>(cnt, 2 * ng);
     for (uint32_t g = 0; g < ng; g++) {
       for (int q = 0; q < 2; q++) {
         if (!want[q]) continue;
         const eh_batch::Plan &P = b->plan[q];
-        const Schedule &S = M->sched[q][sched];
-        EH_CHECK(launch_parts(M, kk[q], sched, tp[q], P.small_rows.as<uint32_t>() + P.sr_off[g],
+        EH_CHECK(launch_parts(M, kk[q], key[q].sched, key[q].tp, P.small_rows.as<uint32_t>() + P.sr_off[g],
                               P.sr_off[g + 1] - P.sr_off[g], P.tasks_big.as<uint4>() + P.tb_off[g],
                               P.tb_off[g + 1] - P.tb_off[g], P.tasks_small.as<uint4>() + P.ts_off[g],
-                              P.ts_off[g + 1] - P.ts_off[g], S.lds_small, S.lds_big,
+                              P.ts_off[g + 1] - P.ts_off[g], key[q].lds_small, key[q].lds_big,
                               b->counts.as<uint64_t>() + (uint64_t)q * ng, nx + 4 * q, nullptr));
       }
     }

@@ -29,6 +29,7 @@
 //                                                  (:1338-1413,1555-1649)
 // A bitset_new / new_type B row facing a non-bitmap A is read through its
 // canonical uint list, which every row keeps.
+#include <mutex>
 #include <type_traits>
 
 #include "eh_internal.hpp"
@@ -991,6 +992,7 @@ constexpr int SMALL_NQ = 4;  // pairs per lane whose searches run interleaved
 
 template <int K, bool W>
 __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t *__restrict__ rows, uint32_t nrows) {
+  static_assert(K == 3, "4-cliques of short rows run in k_small_cliques4");
   extern __shared__ uint32_t lds_all[];
   const uint32_t lane = lane_id();
   uint32_t *meta = lds_all + (threadIdx.x >> 6) * SMALL_META_WORDS;
@@ -1020,7 +1022,7 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
     rcnt[lane] = 0;
     if (lane == 0) mbase[64] = tot;
     wave_sync();
-    if constexpr (K == 3) {
+    {
       // SMALL_NQ pairs per lane, t + 64 k, their searches interleaved
       for (uint32_t t = lane; t < tot; t += 64 * SMALL_NQ) {
         uint32_t q[SMALL_NQ], x[SMALL_NQ], j[SMALL_NQ];
@@ -1060,44 +1062,131 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
           }
         }
       }
-    } else {
-    for (uint32_t t = lane; t < tot; t += 64) {
-      uint32_t q = 0;
+    }
+    wave_sync();
+    if (per_graph && lane < nr && rcnt[lane]) atomicAdd(&a.counts[gid], (unsigned long long)rcnt[lane]);
+    wave_sync();
+  }
+  const uint64_t s = wave_sum_u64(acc);
+  if (lane == 0 && s) atomicAdd(&a.counts[0], (unsigned long long)s);
+}
+
+// 4-cliques of short rows through each row's local graph.  For a source row i with
+// sorted entries R[0..d), the pairs p < c with R[p] in row(R[c]) are the edges of
+// the graph induced on row(i); a 4-clique {i, R[a], R[b], R[c]} (a < b < c) is a
+// triangle of that local graph.  Pass 1 runs the triangle pass's membership tests
+// (four interleaved per lane) and records each hit as bit p of the LDS bit-row
+// adj(c) (ceil(d/32) words per local vertex); pass 2 visits the pairs again and, for
+// every local edge (b, c), adds popcount(adj(b) & adj(c)) -- the a < b adjacent to
+// both.  The extension over later entries is then LDS work, not more HBM searches.
+// A wave takes 64 rows at a time and runs them in sub-batches whose bit-rows fit
+// its SMALL_ADJ_WORDS (a row of up to 64 entries needs at most 128 words).
+constexpr uint32_t SMALL4_META_WORDS = 324;
+constexpr uint32_t SMALL_ADJ_WORDS = 1024;
+
+__global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint32_t *__restrict__ rows,
+                                                        uint32_t nrows) {
+  extern __shared__ uint32_t lds_all[];
+  const uint32_t lane = lane_id();
+  uint32_t *meta = lds_all + (threadIdx.x >> 6) * (SMALL4_META_WORDS + SMALL_ADJ_WORDS);
+  uint32_t *mbase = meta, *moff = meta + 65, *rcnt = meta + 130, *mdeg = meta + 194, *mwb = meta + 258;
+  uint32_t *adj = meta + SMALL4_META_WORDS;
+  const bool per_graph = a.graph_of != nullptr;
+  uint64_t acc = 0;
+  for (;;) {
+    uint32_t c0 = 0;
+    if (lane == 0) c0 = atomicAdd(a.next, 64u);
+    c0 = rfl(c0);
+    if (c0 >= nrows) break;
+    const uint32_t nr = min(64u, nrows - c0);
+    uint32_t off = 0, gid = 0, d = 0;
+    if (lane < nr) {
+      const uint32_t i = rows[c0 + lane];
+      d = a.card[i];
+      off = a.aoff[i];
+      if (per_graph) gid = a.graph_of[i];
+    }
+    rcnt[lane] = 0;
+    bool pend = lane < nr && d >= 3u;  // fewer than 3 entries close no 4-clique
+    while (__ballot(pend)) {
+      // the pending rows whose bit-rows fit form a prefix of the lanes (at least one row)
+      const uint32_t wpr = (d + 31u) >> 5;
+      const uint32_t words = pend ? d * wpr : 0u;
+      uint32_t wtot;
+      const uint32_t wb = wave_excl_scan(words, lane, &wtot);
+      const bool sel = pend && wb + words <= SMALL_ADJ_WORDS;
+      const uint32_t items = sel ? (d * (d - 1u)) / 2u : 0u;
+      uint32_t tot;
+      const uint32_t base = wave_excl_scan(items, lane, &tot);
+      tot = rfl(tot);
+      mbase[lane] = base;
+      moff[lane] = off;
+      mdeg[lane] = sel ? d : 0u;
+      mwb[lane] = wb;
+      if (lane == 0) mbase[64] = tot;
+      for (uint32_t w = lane; w < SMALL_ADJ_WORDS; w += 64) adj[w] = 0u;
+      wave_sync();
+      // pass 1: local edges, SMALL_NQ membership tests per lane in flight
+      for (uint32_t t = lane; t < tot; t += 64 * SMALL_NQ) {
+        uint32_t q[SMALL_NQ], x[SMALL_NQ], j[SMALL_NQ], pp[SMALL_NQ], cc[SMALL_NQ];
 #pragma unroll
-      for (uint32_t step = 32; step; step >>= 1)
-        if (mbase[q + step] <= t) q += step;
-      const uint32_t r = t - mbase[q];
-      const uint32_t *R = a.unbr + (uint64_t)moff[q] * 4;
-      uint64_t h;
-      {
-        uint32_t p, s;
-        tri_decode(r, &p, &s);
-        const uint32_t x = R[p], y = R[s];
-        h = 0;
-        if (row_contains(a, a.desc[y], x)) {
-          // (x, y) closes a triangle with the source: every later z adjacent to both closes a
-          // 4-clique; two z's at a time, their four searches interleaved
-          const uint32_t d = mdeg[q];
-          for (uint32_t u = s + 1u; u < d; u += 2) {
-            const bool two = u + 1u < d;
-            const RowDesc dz0 = a.desc[R[u]];
-            const RowDesc dz1 = two ? a.desc[R[u + 1u]] : RowDesc{0u, 0u, 0u, 0u};
-            MemberQuery m[4];
-            mq_start(a, dz0, y, m[0]);
-            mq_start(a, dz0, x, m[1]);
-            mq_start(a, dz1, y, m[2]);
-            mq_start(a, dz1, x, m[3]);
-            mq_finish<4>(m);
-            h += (m[0].hit & m[1].hit) + (m[2].hit & m[3].hit);
+        for (int k = 0; k < SMALL_NQ; k++) {
+          const uint32_t tk = t + 64u * k;
+          q[k] = 0;
+#pragma unroll
+          for (uint32_t step = 32; step; step >>= 1)
+            if (mbase[q[k] + step] <= tk) q[k] += step;
+          x[k] = 0;
+          j[k] = 0;
+          pp[k] = 0;
+          cc[k] = 0;
+          if (tk < tot) {
+            tri_decode(tk - mbase[q[k]], &pp[k], &cc[k]);
+            const uint32_t *R = a.unbr + (uint64_t)moff[q[k]] * 4;
+            x[k] = R[pp[k]];
+            j[k] = R[cc[k]];
+          }
+        }
+        MemberQuery m[SMALL_NQ];
+#pragma unroll
+        for (int k = 0; k < SMALL_NQ; k++) {
+          const RowDesc dd = (t + 64u * k < tot) ? a.desc[j[k]] : RowDesc{0u, 0u, 0u, 0u};
+          mq_start(a, dd, x[k], m[k]);
+        }
+        mq_finish<SMALL_NQ>(m);
+#pragma unroll
+        for (int k = 0; k < SMALL_NQ; k++) {
+          if (m[k].hit) {
+            const uint32_t wq = (mdeg[q[k]] + 31u) >> 5;
+            atomicOr(&adj[mwb[q[k]] + cc[k] * wq + (pp[k] >> 5)], 1u << (pp[k] & 31u));
           }
         }
       }
-      if (per_graph) {
-        if (h) atomicAdd(&rcnt[q], (uint32_t)h);
-      } else {
-        acc += h;
+      wave_sync();
+      // pass 2: every local edge (b, c) adds |adj(b) & adj(c)|
+      for (uint32_t t = lane; t < tot; t += 64) {
+        uint32_t q = 0;
+#pragma unroll
+        for (uint32_t step = 32; step; step >>= 1)
+          if (mbase[q + step] <= t) q += step;
+        uint32_t b, c;
+        tri_decode(t - mbase[q], &b, &c);
+        const uint32_t wq = (mdeg[q] + 31u) >> 5;
+        const uint32_t *ac = adj + mwb[q] + c * wq;
+        uint32_t h = 0;
+        if ((ac[b >> 5] >> (b & 31u)) & 1u) {
+          const uint32_t *ab = adj + mwb[q] + b * wq;
+          // adj(b) only holds bits below b: words past b's word are zero
+          for (uint32_t w = 0; w <= (b >> 5); w++) h += __popc(ab[w] & ac[w]);
+        }
+        if (per_graph) {
+          if (h) atomicAdd(&rcnt[q], h);
+        } else {
+          acc += h;
+        }
       }
-    }
+      wave_sync();
+      pend = pend && !sel;
     }
     wave_sync();
     if (per_graph && lane < nr && rcnt[lane]) atomicAdd(&a.counts[gid], (unsigned long long)rcnt[lane]);
@@ -1321,7 +1410,7 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
     tp.task_work = 0;
   }
   // rows this short are cheaper as explicit wedges (triangles) or triples (4-cliques)
-  tp.small_deg = (sched == EH_SCHED_TUNED && m->filter == EH_FILTER_LOWER) ? (k == 4 ? 8u : 32u) : 0u;
+  tp.small_deg = (sched == EH_SCHED_TUNED && m->filter == EH_FILTER_LOWER) ? 32u : 0u;
   if (opts) {
     if (opts->small_degree < 0) tp.small_deg = 0;
     else if (opts->small_degree > 0 && tp.small_deg) tp.small_deg = (uint32_t)std::min(opts->small_degree, k == 4 ? 64 : 256);
@@ -1432,6 +1521,27 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
 
 namespace {
 
+// blocks of `kern` one CU holds at this shape; the query is cached per (kernel, shape)
+// because per-graph batch launches would otherwise repeat it thousands of times
+int blocks_per_cu(const void *kern, int threads, size_t lds_bytes) {
+  struct Entry {
+    const void *k;
+    int threads;
+    size_t lds;
+    int per_cu;
+  };
+  static std::mutex mu;
+  static std::vector<Entry> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  for (const Entry &e : cache)
+    if (e.k == kern && e.threads == threads && e.lds == lds_bytes) return e.per_cu;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds_bytes) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  cache.push_back({kern, threads, lds_bytes, per_cu});
+  return per_cu;
+}
+
 template <int K, bool W>
 eh_status launch_count(eh_ctx *ctx, const CountArgs &base, const uint4 *tasks, uint64_t ntasks,
                        uint32_t lds_words, int waves, uint32_t *d_next, uint64_t task_base) {
@@ -1446,9 +1556,7 @@ eh_status launch_count(eh_ctx *ctx, const CountArgs &base, const uint4 *tasks, u
   a.lds_words = lds_words;
   const size_t lds_bytes = (size_t)lds_words * 4 * waves;
   const int threads = 64 * waves;
-  int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds_bytes);
-  if (e != hipSuccess || per_cu < 1) per_cu = 1;
+  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(kern), threads, lds_bytes);
   uint64_t blocks = (uint64_t)per_cu * (ctx->num_cus ? ctx->num_cus : 256);
   const uint64_t need = ceil_div(ntasks, (uint64_t)waves);
   if (blocks > need) blocks = need;
@@ -1464,16 +1572,15 @@ eh_status launch_small(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows,
   CountArgs a = base;
   a.next = d_next;
   const int waves = 4, threads = 256;
-  const size_t lds_bytes = (size_t)SMALL_META_WORDS * 4 * waves;
-  int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_small_rows<K, W>, threads, lds_bytes);
-  if (e != hipSuccess || per_cu < 1) per_cu = 1;
+  const size_t lds_bytes =
+      (size_t)(K == 4 ? SMALL4_META_WORDS + SMALL_ADJ_WORDS : SMALL_META_WORDS) * 4 * waves;
+  auto kern = (K == 4) ? k_small_cliques4 : k_small_rows<3, W>;
+  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(kern), threads, lds_bytes);
   uint64_t blocks = (uint64_t)per_cu * (ctx->num_cus ? ctx->num_cus : 256);
   const uint64_t need = ceil_div(ceil_div(nrows, 64), (uint64_t)waves);
   if (blocks > need) blocks = need;
   k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
-  hipLaunchKernelGGL((k_small_rows<K, W>), dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a, rows,
-                     (uint32_t)nrows);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a, rows, (uint32_t)nrows);
   EH_HIP(hipGetLastError());
   return EH_OK;
 }

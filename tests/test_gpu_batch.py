@@ -40,3 +40,25 @@ def test_batch_counts(ctx, schedule):
         assert tri_p[g] == t and tri_g[g] == t, (g, tri_p[g], tri_g[g], t)
         assert k4_p[g] == k and k4_g[g] == k, (g, k4_p[g], k4_g[g], k)
     print(f"batch {ng} graphs {schedule}: persistent {ms_p:.2f} ms, per-graph {ms_g:.2f} ms", flush=True)
+
+
+def test_batch_pass_options_and_recapture(ctx):
+    """Each pass takes its own launch shape; changing either one rebuilds its schedule,
+    and the captured graph and the per-graph plans must follow (never replaying freed
+    schedule buffers).  Counts stay those of the default shape in every combination."""
+    b = ctx.generate_batch(4, 24)
+    tri, k4, _ = b.count("persistent", "tuned")
+    shapes = (({"small_degree": 64}, {"small_degree": 8}),
+              ({"small_degree": 16}, {"small_degree": 64}),
+              ({"small_degree": -1}, {"small_degree": -1, "lds_words": 3072, "waves": 2}),
+              ({"small_degree": 64}, {"small_degree": 48}))
+    for t_o, k_o in shapes:
+        for mode in ("persistent", "per_graph", "persistent"):
+            t, k, _ = b.count(mode, "tuned", tri_opts=t_o, k4_opts=k_o)
+            assert np.array_equal(t, tri) and np.array_equal(k, k4), (mode, t_o, k_o)
+    t, k, _ = b.count("persistent", "tuned", cliques=False, tri_opts={"small_degree": 32})
+    assert k is None and np.array_equal(t, tri)
+    t, k, _ = b.count("persistent", "tuned", triangles=False)
+    assert t is None and np.array_equal(k, k4)
+    t, k, _ = b.count("persistent", "plain")
+    assert np.array_equal(t, tri) and np.array_equal(k, k4)

@@ -73,7 +73,7 @@ def test_rmat_parity(ctx, scale):
             assert c == exp, (policy, sched)
 
 
-@pytest.mark.parametrize("small", [-1, 2, 8, 16, 32])
+@pytest.mark.parametrize("small", [-1, 2, 3, 8, 16, 32, 48, 64])
 def test_small_row_kernel(ctx, facebook, small):
     """The edge-centric path for short rows must not change any count."""
     M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "lower", "hybrid")
