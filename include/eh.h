@@ -126,6 +126,11 @@ typedef struct {
                              entries are counted edge-centrically; 0 = default, <0 = off */
   int32_t stream_ratio;   /* TUNED: a bitset B row is streamed as its uint list instead of
                              probed when 16 |B| < stream_ratio * probes; 0 = default, <0 = never */
+  int32_t bstat_split;    /* TUNED triangles on oriented (EH_FILTER_LOWER) matrices: pairs (i,j)
+                             of rows above small_degree whose B row id j is below this bound are
+                             counted B-stationary (row(j) staged once as an LDS bitmap, the part
+                             of every row(i) below j streamed through it); 0 = default, <0 = off,
+                             else 64..524288 */
 } eh_run_opts;
 
 typedef struct {

@@ -40,11 +40,17 @@ def _store(db):
 
 
 def candidates(M, k=3):
-    """Launch/schedule grid: edge-centric row threshold x LDS budget x waves per block."""
+    """Launch/schedule grid: edge-centric row threshold x LDS budget x waves per block, and for
+    triangles the B-stationary split (off, or B rows below 32Ki / 64Ki ids)."""
     smalls = (16, 24, 32, 48, 64) if k == 3 else (8, 16, 32, 48, 64)
-    for small in smalls:
-        for lds, waves in ((2048, 2), (2048, 4), (1536, 4), (3072, 2)):
-            yield {"small_degree": small, "lds_words": lds, "waves": waves}
+    splits = (-1, 32768, 65536) if k == 3 else (-1,)
+    for bstat in splits:
+        for small in smalls:
+            for lds, waves in ((2048, 2), (2048, 4), (1536, 4), (3072, 2)):
+                p = {"small_degree": small, "lds_words": lds, "waves": waves}
+                if k == 3:
+                    p["bstat"] = bstat
+                yield p
 
 
 def tune(M, k=3, schedule="tuned", repeat=3, budget_s=60.0, use_cache=True, verbose=False):

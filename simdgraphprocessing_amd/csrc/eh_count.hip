@@ -64,6 +64,8 @@ struct CountArgs {
   const uint32_t *__restrict__ ntu;  // new_type uint parts
   uint32_t blocks;                   // the matrix has bitset_new / new_type rows (selects
                                      // the kernels with the block passes)
+  uint32_t grab;                     // tasks a wave takes per queue atomic (one address is
+                                     // shared by every wave: one task per atomic serialises)
 };
 
 __device__ __forceinline__ uint32_t rfl(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -1196,6 +1198,70 @@ __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint3
   if (lane == 0 && s) atomicAdd(&a.counts[0], (unsigned long long)s);
 }
 
+// B-stationary triangle pairs.  On an oriented matrix row(j) only holds ids below j,
+// so for a pair (i, j) with j at position p of row(i), |row(i) & row(j)| =
+// |row(i)[0, p) & row(j)|.  A task {j, b, e} stages row(j) once as an LDS bitmap over
+// its word span and streams the prefixes row(i)[0, p) of its in-pairs [b, e) through
+// it, 16-byte vectors flattened over the lanes (elements at p and beyond are >= j and
+// fall outside the bitmap's span).  Each prefix is read contiguously, where probing
+// every element of it into row(j)'s bitset in HBM touches about one cache line per
+// element: the hub rows with low ids are probed by most rows.
+__global__ void __launch_bounds__(256) k_bstat(CountArgs a, const uint4 *__restrict__ btasks, uint64_t nbt,
+                                               const uint2 *__restrict__ in_pairs) {
+  extern __shared__ uint32_t lds_all[];
+  const uint32_t lane = lane_id();
+  uint32_t *meta = lds_all + (threadIdx.x >> 6) * a.lds_words;
+  uint32_t *bm = meta + META_WORDS;
+  uint64_t acc = 0;
+  uint64_t t = 0, tend = 0;
+  for (;;) {
+    if (t >= tend) {
+      uint32_t t0 = 0;
+      if (lane == 0) t0 = atomicAdd(a.next, a.grab);
+      t = rfl(t0);
+      if (t >= nbt) break;
+      tend = min(t + a.grab, nbt);
+    }
+    const uint4 task = btasks[t++];
+    const uint32_t j = rfl(task.x), b = rfl(task.y), e = rfl(task.z);
+    const uint32_t dj = rfl(a.card[j]);
+    const uint32_t *G = a.unbr + (uint64_t)a.aoff[j] * 4;
+    AState S;
+    S.lo = rfl(G[0]);
+    const uint32_t hi = rfl(G[dj - 1]);
+    S.span = hi - S.lo;
+    S.aw0 = S.lo >> 6;
+    S.anw = (hi >> 6) - S.aw0 + 1u;
+    S.base32 = 2u * S.aw0;
+    S.bm = bm;
+    S.list = nullptr;
+    S.n = dj;
+    S.P = 0;
+    S.dpre = S.dstart = nullptr;
+    S.ht = nullptr;
+    S.hlg = 0;
+    for (uint32_t w = lane; w < 2u * S.anw; w += 64) bm[w] = 0u;
+    wave_sync();
+    for (uint32_t k = lane; k < dj; k += 64) {
+      const uint32_t x = G[k];
+      atomicOr(&bm[(x >> 5) - S.base32], 1u << (x & 31u));
+    }
+    wave_sync();
+    for (uint32_t c0 = b; c0 < e; c0 += 64) {
+      uint32_t nv = 0, off = 0;
+      if (c0 + lane < e) {
+        const uint2 pr = in_pairs[c0 + lane];
+        off = pr.x;
+        nv = (pr.y + 3u) >> 2;
+      }
+      acc += flat_pass<PASS_STREAM, KIND_BITMAP, false, 4>(a, S, nv, 0u, off, 0u, lane, meta);
+    }
+    wave_sync();
+  }
+  const uint64_t sum = wave_sum_u64(acc);
+  if (lane == 0 && sum) atomicAdd(&a.counts[0], (unsigned long long)sum);
+}
+
 template <int K, bool W, bool TUNED, bool BLK>
 __global__ void __launch_bounds__(256) k_count(CountArgs a) {
   extern __shared__ uint32_t lds_all[];
@@ -1203,11 +1269,15 @@ __global__ void __launch_bounds__(256) k_count(CountArgs a) {
   uint32_t *L = lds_all + (threadIdx.x >> 6) * a.lds_words;
   uint64_t acc = 0;
   uint32_t cur = 0;
+  uint64_t t = 0, tend = 0;
   for (;;) {
-    uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(a.next, 1u);
-    t = rfl(t);
-    if (t >= a.ntasks) break;
+    if (t >= tend) {
+      uint32_t t0 = 0;
+      if (lane == 0) t0 = atomicAdd(a.next, a.grab);
+      t = rfl(t0);
+      if (t >= a.ntasks) break;
+      tend = min(t + a.grab, a.ntasks);
+    }
     const uint4 task = a.tasks[t];
     const uint32_t i = rfl(task.x), jb = rfl(task.y), je = rfl(task.z), w = rfl(task.w);
     const uint32_t gid = w & GID_MASK, kind = w >> KIND_SHIFT;
@@ -1225,6 +1295,7 @@ __global__ void __launch_bounds__(256) k_count(CountArgs a) {
     } else {
       acc += run_task<K, W, TUNED, BLK>(a, kind, i, jb, je, lane, L);
     }
+    t++;
   }
   const uint64_t s = wave_sum_u64(acc);
   if (lane == 0 && s) atomicAdd(&a.counts[cur], (unsigned long long)s);
@@ -1251,6 +1322,9 @@ struct SchedArgs {
   uint32_t lds_small, lds_big;
   uint64_t task_work;
   uint32_t small_deg;     // rows with 1..small_deg entries go to the edge-centric kernel
+  uint32_t bstat_j;       // task rows hand their pairs with j < bstat_j to the B-stationary pass
+  uint32_t *incount;      // out (bstat): task rows containing j, per j < bstat_j
+  uint32_t *jstart;       // out: first entry of row i its A-side tasks cover
   uint32_t *is_small;     // out: 1 for those rows
   uint64_t *sortkey;      // out: (class << 62) | (order << 32) | i
   uint32_t *ntask;        // out: tasks of row i
@@ -1267,13 +1341,17 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
   for (uint32_t i = wid; i < s.n; i += nw) {
     const uint32_t dA = s.card[i];
     uint64_t work = 0;
-    uint32_t lo = 0, hi = 0;
+    uint32_t lo = 0, hi = 0, below = 0;
+    // rows above the edge-centric threshold are task rows; with the B-stationary pass on,
+    // their pairs with j < bstat_j (a prefix of the sorted row) leave the A-side tasks
+    const bool task_row = dA > s.small_deg;
     if (dA) {
       const uint32_t *G = s.unbr + (uint64_t)s.aoff[i] * 4;
       lo = G[0];
       hi = G[dA - 1];
       for (uint32_t k = lane; k < dA; k += 64) {
-        const RowDesc d = s.desc[G[k]];
+        const uint32_t jv = G[k];
+        const RowDesc d = s.desc[jv];
         const uint32_t t = desc_type(d);
         uint64_t units;
         if (d.card == 0) { loc[3]++; units = 0; }
@@ -1282,15 +1360,22 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
         else if (t == EH_PS) { loc[1]++; units = desc_aux(d); }
         else { loc[2]++; units = desc_aux(d); }
         loc[5] += d.card;
+        if (task_row && jv < s.bstat_j) {
+          below++;
+          if (d.card) atomicAdd(&s.incount[jv], 1u);
+          continue;
+        }
         work += units + 16u;
       }
       work = wave_sum_u64(work);
-      loc[4] += (lane == 0) ? work - 16ull * dA : 0ull;
+      below = (uint32_t)wave_sum_u64(below);
+      loc[4] += (lane == 0) ? work - 16ull * (dA - below) : 0ull;
     }
     if (lane != 0) continue;
+    s.jstart[i] = below;
     const bool small = dA > 0 && dA <= s.small_deg;
     s.is_small[i] = small ? 1u : 0u;
-    if (dA == 0 || small) {
+    if (dA == 0 || small || below == dA) {
       s.sortkey[i] = (3ull << 62) | i;
       s.ntask[i] = 0;
       s.rowinfo[i] = 0;
@@ -1338,7 +1423,7 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
     uint32_t nt = 1;
     if (s.tuned && s.task_work) {
       uint64_t q = (work + s.task_work - 1) / s.task_work;
-      nt = (uint32_t)min<uint64_t>(max<uint64_t>(q, 1ull), (uint64_t)dA);
+      nt = (uint32_t)min<uint64_t>(max<uint64_t>(q, 1ull), (uint64_t)(dA - below));
     }
     s.ntask[i] = nt;
     const uint64_t order = s.tuned ? (0x3FFFFFFFull - min<uint64_t>(work, 0x3FFFFFFFull)) : 0ull;
@@ -1362,17 +1447,59 @@ __global__ void k_gather_ntask(const uint64_t *sorted, const uint32_t *ntask, ui
 }
 
 __global__ void k_expand_tasks(const uint64_t *sorted, const uint32_t *nt_sorted, const uint64_t *toff,
-                               const uint32_t *card, const uint32_t *rowinfo, uint32_t n, uint4 *tasks) {
+                               const uint32_t *card, const uint32_t *jstart, const uint32_t *rowinfo, uint32_t n,
+                               uint4 *tasks) {
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
     const uint32_t nt = nt_sorted[q];
     if (!nt) continue;
     const uint32_t i = (uint32_t)sorted[q];
-    const uint32_t d = card[i];
+    const uint32_t js = jstart[i];
+    const uint32_t d = card[i] - js;
     const uint64_t base = toff[q];
     for (uint32_t p = 0; p < nt; p++) {
-      const uint32_t jb = (uint32_t)(((uint64_t)p * d) / nt);
-      const uint32_t je = (uint32_t)(((uint64_t)(p + 1) * d) / nt);
+      const uint32_t jb = js + (uint32_t)(((uint64_t)p * d) / nt);
+      const uint32_t je = js + (uint32_t)(((uint64_t)(p + 1) * d) / nt);
       tasks[base + p] = make_uint4(i, jb, je, rowinfo[i]);
+    }
+  }
+}
+
+// B-stationary in-pairs: every task row writes (its offset / 4, position) into the list
+// of each non-empty B row j < bstat_j it holds (one wave per row, positions [0, jstart))
+__global__ void __launch_bounds__(256) k_bstat_fill(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr,
+                                                    const RowDesc *desc, const uint32_t *jstart, uint32_t n,
+                                                    const uint64_t *inoff, uint32_t *infill, uint2 *in_pairs) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t i = wid; i < n; i += nw) {
+    const uint32_t js = jstart[i];
+    if (js == 0) continue;
+    const uint32_t *G = unbr + (uint64_t)aoff[i] * 4;
+    for (uint32_t k = lane; k < js; k += 64) {
+      const uint32_t j = G[k];
+      if (desc[j].card == 0) continue;
+      const uint32_t slot = atomicAdd(&infill[j], 1u);
+      in_pairs[inoff[j] + slot] = make_uint2(aoff[i], k);
+    }
+  }
+}
+
+// B-stationary tasks: the in-list of row j cut into pieces of at most `chunk` pairs
+__global__ void k_bstat_ntask(const uint32_t *incount, uint32_t nj, uint32_t chunk, uint32_t *ntb) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nj; j += gridDim.x * blockDim.x)
+    ntb[j] = (incount[j] + chunk - 1u) / chunk;
+}
+
+__global__ void k_bstat_expand(const uint32_t *incount, const uint64_t *inoff, const uint64_t *boff, uint32_t nj,
+                               uint32_t chunk, uint4 *btasks) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nj; j += gridDim.x * blockDim.x) {
+    const uint32_t c = incount[j];
+    const uint64_t b0 = inoff[j], t0 = boff[j];
+    for (uint32_t p = 0; p * chunk < c; p++) {
+      const uint32_t b = (uint32_t)(b0 + (uint64_t)p * chunk);
+      const uint32_t e = (uint32_t)(b0 + min<uint64_t>((uint64_t)(p + 1) * chunk, c));
+      btasks[t0 + p] = make_uint4(j, b, e, 0u);
     }
   }
 }
@@ -1411,6 +1538,10 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
   }
   // rows this short are cheaper as explicit wedges (triangles) or triples (4-cliques)
   tp.small_deg = (sched == EH_SCHED_TUNED && m->filter == EH_FILTER_LOWER) ? 32u : 0u;
+  tp.bstat_j = 0;
+  // triangle tasks are many and light: two per queue atomic; 4-clique tasks are few and
+  // heavy, where pulling two at once unbalances the end of the pass
+  tp.grab = (k == 4) ? 1u : 2u;
   if (opts) {
     if (opts->small_degree < 0) tp.small_deg = 0;
     else if (opts->small_degree > 0 && tp.small_deg) tp.small_deg = (uint32_t)std::min(opts->small_degree, k == 4 ? 64 : 256);
@@ -1420,6 +1551,8 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
     if (opts->lanes_per_row > 0) tp.lanes = opts->lanes_per_row;
     if (opts->stream_ratio < 0) tp.stream_ratio = 0;
     else if (opts->stream_ratio > 0) tp.stream_ratio = (uint32_t)opts->stream_ratio;
+    if (opts->bstat_split < 0) tp.bstat_j = 0;
+    else if (opts->bstat_split > 0) tp.bstat_j = (uint32_t)std::min(std::max(opts->bstat_split, 64), 524288);
   }
   // the task kernels are built for at most 256 threads (4 waves) per block, and a
   // wave's slice holds at most 16384 words (64 KB)
@@ -1436,7 +1569,16 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   eh_ctx *ctx = m->ctx;
   hipStream_t st = ctx->stream;
   const uint32_t n = m->n;
-  DevBuf key, key2, ntask, rowinfo, stats, ntsorted, toff, cnt2, is_small, spos;
+  DevBuf key, key2, ntask, rowinfo, stats, ntsorted, toff, cnt2, is_small, spos, jstart, incount;
+  // B-stationary pairs: tuned triangles on an oriented single-graph matrix only (the
+  // prefix identity needs every row(j) below j)
+  const uint32_t bstat_j = (sched == EH_SCHED_TUNED && k == 3 && m->filter == EH_FILTER_LOWER &&
+                            !m->dgraph_of.p) ? std::min(tp.bstat_j, n) : 0u;
+  EH_CHECK(jstart.alloc(4ull * n + 8));
+  if (bstat_j) {
+    EH_CHECK(incount.alloc(4ull * bstat_j + 8));
+    k_zero_u32<<<grid_for(bstat_j), 256, 0, st>>>(incount.as<uint32_t>(), bstat_j);
+  }
   EH_CHECK(is_small.alloc(4ull * n + 8));
   EH_CHECK(spos.alloc(8ull * n + 16));
   k_zero_u32<<<grid_for((uint64_t)n + 1), 256, 0, st>>>(is_small.as<uint32_t>(), n + 1);
@@ -1462,6 +1604,9 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   sa.lds_big = tp.lds_big;
   sa.task_work = tp.task_work;
   sa.small_deg = tp.small_deg;
+  sa.bstat_j = bstat_j;
+  sa.incount = bstat_j ? incount.as<uint32_t>() : nullptr;
+  sa.jstart = jstart.as<uint32_t>();
   sa.is_small = is_small.as<uint32_t>();
   sa.sortkey = key.as<uint64_t>();
   sa.ntask = ntask.as<uint32_t>();
@@ -1500,9 +1645,48 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   if (hcnt[0]) EH_HIP(hipMemcpy(&nbig, toff.as<uint64_t>() + hcnt[0], 8, hipMemcpyDeviceToHost));
   EH_CHECK(s.tasks.alloc(16 * ntasks + 16));
   k_expand_tasks<<<grid_for(n), 256, 0, st>>>(key.as<uint64_t>(), ntsorted.as<uint32_t>(), toff.as<uint64_t>(),
-                                              m->dcard.as<uint32_t>(), rowinfo.as<uint32_t>(), n,
-                                              s.tasks.as<uint4>());
+                                              m->dcard.as<uint32_t>(), jstart.as<uint32_t>(),
+                                              rowinfo.as<uint32_t>(), n, s.tasks.as<uint4>());
   EH_HIP(hipGetLastError());
+  s.bstat_j = 0;
+  s.n_btasks = s.n_in_pairs = 0;
+  if (bstat_j) {
+    // in-lists of the B rows below bstat_j, then tasks of at most BSTAT_CHUNK pairs each
+    constexpr uint32_t BSTAT_CHUNK = 256;
+    DevBuf inoff, infill, ntb, boff;
+    EH_CHECK(inoff.alloc(8ull * bstat_j + 16));
+    EH_CHECK(infill.alloc(4ull * bstat_j + 8));
+    EH_CHECK(ntb.alloc(4ull * bstat_j + 8));
+    EH_CHECK(boff.alloc(8ull * bstat_j + 16));
+    k_zero_u32<<<grid_for(bstat_j + 1), 256, 0, st>>>(infill.as<uint32_t>(), bstat_j + 1);
+    k_zero_u32<<<1, 64, 0, st>>>(incount.as<uint32_t>() + bstat_j, 1);
+    EH_CHECK(prim_exscan_u32_to_u64(ctx, incount.as<uint32_t>(), inoff.as<uint64_t>(), (uint64_t)bstat_j + 1));
+    uint64_t npairs = 0;
+    EH_HIP(hipMemcpyAsync(&npairs, inoff.as<uint64_t>() + bstat_j, 8, hipMemcpyDeviceToHost, st));
+    EH_HIP(hipStreamSynchronize(st));
+    if (npairs >= (1ull << 32)) {
+      eh_set_error("%llu B-stationary pairs exceed the 32-bit pair index", (unsigned long long)npairs);
+      return EH_ERR_UNSUPPORTED;
+    }
+    EH_CHECK(s.in_pairs.alloc(8 * npairs + 16));
+    k_bstat_fill<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(
+        m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(), m->dunbr.as<uint32_t>(), m->ddesc.as<RowDesc>(),
+        jstart.as<uint32_t>(), n, inoff.as<uint64_t>(), infill.as<uint32_t>(), s.in_pairs.as<uint2>());
+    k_bstat_ntask<<<grid_for(bstat_j + 1), 256, 0, st>>>(incount.as<uint32_t>(), bstat_j + 1, BSTAT_CHUNK,
+                                                        ntb.as<uint32_t>());
+    EH_CHECK(prim_exscan_u32_to_u64(ctx, ntb.as<uint32_t>(), boff.as<uint64_t>(), (uint64_t)bstat_j + 1));
+    uint64_t nbt = 0;
+    EH_HIP(hipMemcpyAsync(&nbt, boff.as<uint64_t>() + bstat_j, 8, hipMemcpyDeviceToHost, st));
+    EH_HIP(hipStreamSynchronize(st));
+    EH_CHECK(s.btasks.alloc(16 * nbt + 16));
+    k_bstat_expand<<<grid_for(bstat_j), 256, 0, st>>>(incount.as<uint32_t>(), inoff.as<uint64_t>(),
+                                                     boff.as<uint64_t>(), bstat_j, BSTAT_CHUNK,
+                                                     s.btasks.as<uint4>());
+    EH_HIP(hipGetLastError());
+    s.bstat_j = bstat_j;
+    s.n_btasks = nbt;
+    s.n_in_pairs = npairs;
+  }
   EH_HIP(hipEventRecord(ctx->ev1, st));
   EH_HIP(hipEventSynchronize(ctx->ev1));
   float ms = 0;
@@ -1585,13 +1769,37 @@ eh_status launch_small(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows,
   return EH_OK;
 }
 
+eh_status launch_bstat(eh_ctx *ctx, const CountArgs &base, const uint4 *btasks, uint64_t nbt,
+                       const uint2 *in_pairs, uint32_t bstat_j, uint32_t *d_next) {
+  if (nbt == 0) return EH_OK;
+  CountArgs a = base;
+  a.next = d_next;
+  // a wave's slice: the pass metadata and a bitmap over the word span of any row below bstat_j
+  a.lds_words = META_WORDS + 2u * (((bstat_j - 1u) >> 6) + 1u);
+  int waves = 4;
+  while (waves > 1 && (size_t)a.lds_words * 4 * waves > 64 * 1024) waves--;
+  const size_t lds_bytes = (size_t)a.lds_words * 4 * waves;
+  const int threads = 64 * waves;
+  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(k_bstat), threads, lds_bytes);
+  uint64_t blocks = (uint64_t)per_cu * (ctx->num_cus ? ctx->num_cus : 256);
+  const uint64_t need = ceil_div(ceil_div(nbt, (uint64_t)a.grab), (uint64_t)waves);
+  if (blocks > need) blocks = need;
+  k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
+  hipLaunchKernelGGL(k_bstat, dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a, btasks, nbt,
+                     in_pairs);
+  EH_HIP(hipGetLastError());
+  return EH_OK;
+}
+
 }  // namespace
 
 eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, const uint32_t *small_rows,
                        uint64_t n_small_rows, const uint4 *tasks_big, uint64_t n_big, const uint4 *tasks_small,
                        uint64_t n_small, uint32_t lds_small, uint32_t lds_big, uint64_t *d_counts, uint32_t *d_next,
-                       uint64_t *d_task_cycles) {
+                       uint64_t *d_task_cycles, const uint4 *btasks, uint64_t n_btasks, const uint2 *in_pairs,
+                       uint32_t bstat_j) {
   CountArgs a;
+  a.grab = tp.grab ? tp.grab : 1u;
   a.task_cycles = reinterpret_cast<unsigned long long *>(d_task_cycles);
   a.task_base = 0;
   a.card = m->dcard.as<uint32_t>();
@@ -1609,6 +1817,10 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
   a.graph_of = m->dgraph_of.p ? m->dgraph_of.as<uint32_t>() : nullptr;
   a.stream_ratio = tp.stream_ratio;
   const bool weighted = k == -3;
+  if (n_btasks && k != 3) {
+    eh_set_error("B-stationary schedules serve the plain triangle count only");
+    return EH_ERR_INVALID;
+  }
   if (k == 4) {
     EH_CHECK((launch_small<4, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2)));
     EH_CHECK((launch_count<4, false>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0)));
@@ -1621,6 +1833,7 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
                                     n_big)));
   } else {
     EH_CHECK((launch_small<3, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2)));
+    EH_CHECK(launch_bstat(m->ctx, a, btasks, n_btasks, in_pairs, bstat_j, d_next + 3));
     EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0)));
     EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,
                                      n_big)));
@@ -1632,7 +1845,8 @@ eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, 
                           uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles) {
   const uint4 *t = s.tasks.as<uint4>();
   return launch_parts(m, k, sched, tp, s.small_rows.as<uint32_t>(), s.n_small_rows, t, s.n_big, t + s.n_big,
-                      s.n_small, s.lds_small, s.lds_big, d_counts, d_next, d_task_cycles);
+                      s.n_small, s.lds_small, s.lds_big, d_counts, d_next, d_task_cycles, s.btasks.as<uint4>(),
+                      s.n_btasks, s.in_pairs.as<uint2>(), s.bstat_j);
 }
 
 // k: 3 triangles, 4 four-cliques, -3 weighted triangle pass (sum of card(x) over
@@ -1649,10 +1863,11 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
   const int skey = kk * 8 + sched;
   double sched_ms = 0;
   if (!s.valid || s.task_work != tp.task_work || s.lds_small != tp.lds_small || s.lds_big != tp.lds_big ||
-      (int)s.sched != skey || s.small_deg != tp.small_deg) {
+      (int)s.sched != skey || s.small_deg != tp.small_deg || s.bstat_req != tp.bstat_j) {
     s = Schedule();
     EH_CHECK(schedule_build(m, sched, kk, tp, s));
     s.sched = skey;
+    s.bstat_req = tp.bstat_j;
     sched_ms = s.ms;
   }
   DevBuf own_counts, next;

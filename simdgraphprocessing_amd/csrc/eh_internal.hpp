@@ -124,6 +124,8 @@ struct TuneParams {
   int big_waves = 1;       // waves per block of the large-LDS task class
   uint32_t small_deg = 0;  // rows up to this many entries use the edge-centric kernel
   uint32_t stream_ratio = 0;  // tuned: stream bitset B rows when 16 |B| < ratio * probes (0: never)
+  uint32_t bstat_j = 0;       // tuned triangles: pairs with B row id below this run B-stationary (0: off)
+  uint32_t grab = 2;          // tasks a wave takes from the queue per atomic
 };
 
 struct eh_ctx {
@@ -159,6 +161,12 @@ struct Schedule {
   DevBuf small_rows;        // rows counted by the edge-centric kernel
   uint64_t n_small_rows = 0;
   uint32_t small_deg = 0;
+  // B-stationary part (tuned triangles): in_pairs holds, per B row j < bstat_j, the
+  // (row offset / 4, position of j) of every task row containing j; btasks are
+  // {j, first pair, end pair, 0}
+  uint32_t bstat_j = 0, bstat_req = 0;  // in effect / as asked (0 where it does not apply)
+  DevBuf in_pairs, btasks;
+  uint64_t n_btasks = 0, n_in_pairs = 0;
 };
 
 struct eh_matrix {
@@ -218,7 +226,8 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
 eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, const uint32_t *small_rows,
                        uint64_t n_small_rows, const uint4 *tasks_big, uint64_t n_big, const uint4 *tasks_small,
                        uint64_t n_small, uint32_t lds_small, uint32_t lds_big, uint64_t *d_counts, uint32_t *d_next,
-                       uint64_t *d_task_cycles);
+                       uint64_t *d_task_cycles, const uint4 *btasks = nullptr, uint64_t n_btasks = 0,
+                       const uint2 *in_pairs = nullptr, uint32_t bstat_j = 0);
 
 // timing helper: device ms between two events recorded on ctx->stream
 double ctx_elapsed_ms(eh_ctx *ctx);

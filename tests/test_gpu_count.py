@@ -88,6 +88,28 @@ def test_small_row_kernel(ctx, facebook, small):
         assert M.clique_count(4, "tuned", small_degree=small)[0] == om.cliques(4, 8)[0], pol
 
 
+@pytest.mark.parametrize("split", [-1, 64, 100, 1024, 4096, 524288])
+def test_b_stationary_pairs(ctx, facebook, split):
+    """Pairs whose B row id is below the split are counted B-stationary (row(j) as an LDS
+    bitmap, the prefixes of the rows holding j streamed through it); any split, with or
+    without the edge-centric short rows, and under every layout, gives the same count."""
+    M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "lower", "hybrid")
+    for small in (-1, 16, 48):
+        assert M.triangle_count("tuned", small_degree=small, bstat=split)[0] == FB_TRIANGLES, small
+    # B-stationary applies to triangles only: the 4-clique pass ignores the option
+    assert M.clique_count(4, "tuned", bstat=split)[0] == FB_4CLIQUES
+    g = ctx.generate_rmat(14, 16, 5)
+    off, nbr = g.download()
+    exp = O.Matrix(1 << 14, off, nbr, "hybrid").triangles(8)[0]
+    for pol in ("hybrid", "cost", "uint", "pshort", "bs", "bitset_new"):
+        M = g.build_matrix("lower", pol)
+        for small in (-1, 32):
+            assert M.triangle_count("tuned", small_degree=small, bstat=split)[0] == exp, (pol, small)
+    # unoriented matrices keep every pair A-side (the prefix identity needs row(j) below j)
+    M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "all", "hybrid")
+    assert M.triangle_count("tuned", bstat=split)[0] == 6 * FB_TRIANGLES
+
+
 def test_unoriented_matrix(ctx, facebook):
     """EH_FILTER_ALL keeps both directions: every triangle is counted 6 times
     (sum over ordered (i,j) of |N(i) & N(j)|), and the small-row path stays off."""
