@@ -46,7 +46,7 @@ def candidates(M, k=3):
     splits = (-1, 32768, 65536) if k == 3 else (-1,)
     for bstat in splits:
         for small in smalls:
-            for lds, waves in ((2048, 2), (2048, 4), (1536, 4), (3072, 2)):
+            for lds, waves in ((2048, 2), (2048, 4), (1536, 4), (1024, 4), (3072, 2)):
                 p = {"small_degree": small, "lds_words": lds, "waves": waves}
                 if k == 3:
                     p["bstat"] = bstat

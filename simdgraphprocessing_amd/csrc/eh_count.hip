@@ -66,6 +66,8 @@ struct CountArgs {
                                      // the kernels with the block passes)
   uint32_t grab;                     // tasks a wave takes per queue atomic (one address is
                                      // shared by every wave: one task per atomic serialises)
+  const uint32_t *jstart;            // B-stationary on: per row, the entries before this
+                                     // position are counted by k_bstat (null: none)
 };
 
 __device__ __forceinline__ uint32_t rfl(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -1007,20 +1009,23 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
     c0 = rfl(c0);
     if (c0 >= nrows) break;
     const uint32_t nr = min(64u, nrows - c0);
-    uint32_t items = 0, off = 0, gid = 0, d = 0;
+    uint32_t items = 0, off = 0, gid = 0, d = 0, skip = 0;
     if (lane < nr) {
       const uint32_t i = rows[c0 + lane];
       d = a.card[i];
       off = a.aoff[i];
       if (per_graph) gid = a.graph_of[i];
-      items = (d * (d - 1u)) / 2u;  // pairs; 4-cliques extend the pairs that are edges
+      // wedges (p, q), p < q, of the row; those with q below jstart belong to k_bstat
+      const uint32_t js = a.jstart ? a.jstart[i] : 0u;
+      skip = (js * (js - (js ? 1u : 0u))) / 2u;
+      items = (d * (d - 1u)) / 2u - skip;
     }
     uint32_t tot;
     const uint32_t base = wave_excl_scan(items, lane, &tot);
     tot = rfl(tot);
     mbase[lane] = base;
     moff[lane] = off;
-    mdeg[lane] = d;
+    mdeg[lane] = skip;
     rcnt[lane] = 0;
     if (lane == 0) mbase[64] = tot;
     wave_sync();
@@ -1039,7 +1044,7 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
           j[k] = 0;
           if (tk < tot) {
             uint32_t p, sq;
-            tri_decode(tk - mbase[q[k]], &p, &sq);
+            tri_decode(tk - mbase[q[k]] + mdeg[q[k]], &p, &sq);
             const uint32_t *R = a.unbr + (uint64_t)moff[q[k]] * 4;
             x[k] = R[p];
             j[k] = R[sq];
@@ -1342,9 +1347,8 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
     const uint32_t dA = s.card[i];
     uint64_t work = 0;
     uint32_t lo = 0, hi = 0, below = 0;
-    // rows above the edge-centric threshold are task rows; with the B-stationary pass on,
-    // their pairs with j < bstat_j (a prefix of the sorted row) leave the A-side tasks
-    const bool task_row = dA > s.small_deg;
+    // with the B-stationary pass on, every row's pairs with j < bstat_j (a prefix of the
+    // sorted row) leave its A-side tasks or its edge-centric wedges
     if (dA) {
       const uint32_t *G = s.unbr + (uint64_t)s.aoff[i] * 4;
       lo = G[0];
@@ -1360,9 +1364,10 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
         else if (t == EH_PS) { loc[1]++; units = desc_aux(d); }
         else { loc[2]++; units = desc_aux(d); }
         loc[5] += d.card;
-        if (task_row && jv < s.bstat_j) {
+        if (jv < s.bstat_j) {
           below++;
-          if (d.card) atomicAdd(&s.incount[jv], 1u);
+          // the row's first entry has an empty prefix: nothing to count
+          if (d.card && k > 0) atomicAdd(&s.incount[jv], 1u);
           continue;
         }
         work += units + 16u;
@@ -1373,7 +1378,7 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
     }
     if (lane != 0) continue;
     s.jstart[i] = below;
-    const bool small = dA > 0 && dA <= s.small_deg;
+    const bool small = dA > 0 && dA <= s.small_deg && below < dA;
     s.is_small[i] = small ? 1u : 0u;
     if (dA == 0 || small || below == dA) {
       s.sortkey[i] = (3ull << 62) | i;
@@ -1478,7 +1483,7 @@ __global__ void __launch_bounds__(256) k_bstat_fill(const uint32_t *card, const 
     const uint32_t *G = unbr + (uint64_t)aoff[i] * 4;
     for (uint32_t k = lane; k < js; k += 64) {
       const uint32_t j = G[k];
-      if (desc[j].card == 0) continue;
+      if (desc[j].card == 0 || k == 0) continue;
       const uint32_t slot = atomicAdd(&infill[j], 1u);
       in_pairs[inoff[j] + slot] = make_uint2(aoff[i], k);
     }
@@ -1569,7 +1574,8 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   eh_ctx *ctx = m->ctx;
   hipStream_t st = ctx->stream;
   const uint32_t n = m->n;
-  DevBuf key, key2, ntask, rowinfo, stats, ntsorted, toff, cnt2, is_small, spos, jstart, incount;
+  DevBuf key, key2, ntask, rowinfo, stats, ntsorted, toff, cnt2, is_small, spos, incount;
+  DevBuf &jstart = s.jstart;
   // B-stationary pairs: tuned triangles on an oriented single-graph matrix only (the
   // prefix identity needs every row(j) below j)
   const uint32_t bstat_j = (sched == EH_SCHED_TUNED && k == 3 && m->filter == EH_FILTER_LOWER &&
@@ -1797,9 +1803,10 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
                        uint64_t n_small_rows, const uint4 *tasks_big, uint64_t n_big, const uint4 *tasks_small,
                        uint64_t n_small, uint32_t lds_small, uint32_t lds_big, uint64_t *d_counts, uint32_t *d_next,
                        uint64_t *d_task_cycles, const uint4 *btasks, uint64_t n_btasks, const uint2 *in_pairs,
-                       uint32_t bstat_j) {
+                       uint32_t bstat_j, const uint32_t *jstart) {
   CountArgs a;
   a.grab = tp.grab ? tp.grab : 1u;
+  a.jstart = jstart;
   a.task_cycles = reinterpret_cast<unsigned long long *>(d_task_cycles);
   a.task_base = 0;
   a.card = m->dcard.as<uint32_t>();
@@ -1846,7 +1853,8 @@ eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, 
   const uint4 *t = s.tasks.as<uint4>();
   return launch_parts(m, k, sched, tp, s.small_rows.as<uint32_t>(), s.n_small_rows, t, s.n_big, t + s.n_big,
                       s.n_small, s.lds_small, s.lds_big, d_counts, d_next, d_task_cycles, s.btasks.as<uint4>(),
-                      s.n_btasks, s.in_pairs.as<uint2>(), s.bstat_j);
+                      s.n_btasks, s.in_pairs.as<uint2>(), s.bstat_j,
+                      s.bstat_j ? s.jstart.as<uint32_t>() : nullptr);
 }
 
 // k: 3 triangles, 4 four-cliques, -3 weighted triangle pass (sum of card(x) over
