@@ -307,7 +307,7 @@ def main():
             prm = json.loads(args.params)
             best = Mc.triangle_count("tuned", repeat=5, **prm)[1]["kernel_ms"]
         elif not args.no_tune:
-            prm, best, _ = autotune.tune(Mc, 3, "tuned", repeat=2, budget_s=30, use_cache=True)
+            prm, best, _ = autotune.tune(Mc, 3, "tuned", repeat=2, budget_s=45, use_cache=True)
             best = min(best, Mc.triangle_count("tuned", repeat=5, **prm)[1]["kernel_ms"])
         cands[pol] = {"matrix": Mc, "params": prm, "default_ms": stc["kernel_ms"], "tuned_ms": best}
         log(f"tuned {pol}: default shape {stc['kernel_ms']:.3f} ms, autotuned {prm} {best:.3f} ms")

@@ -118,7 +118,7 @@ typedef struct {
   int32_t schedule;       /* eh_schedule */
   int32_t lanes_per_row;  /* 0 = autotuned/default; else 4,8,16,32,64 */
   uint32_t task_work;     /* 0 = default; target work units per task */
-  uint32_t lds_words;     /* 0 = default; per-wave LDS budget in 32-bit words (256..16384) */
+  uint32_t lds_words;     /* 0 = default; per-wave LDS budget in 32-bit words (512..16384) */
   int32_t waves_per_block;/* 0 = default (tuned 2, plain 4); at most 4, and at most
                              64 KB of LDS slices per block */
   int32_t repeat;         /* run the counting kernel this many times (>=1); stats average */
@@ -129,8 +129,8 @@ typedef struct {
   int32_t bstat_split;    /* TUNED triangles on oriented (EH_FILTER_LOWER) matrices: pairs (i,j)
                              of rows above small_degree whose B row id j is below this bound are
                              counted B-stationary (row(j) staged once as an LDS bitmap, the part
-                             of every row(i) below j streamed through it); 0 = default, <0 = off,
-                             else 64..524288 */
+                             of every row(i) below j streamed through it); 0 = default (49152),
+                             <0 = off, else 64..65536 */
 } eh_run_opts;
 
 typedef struct {

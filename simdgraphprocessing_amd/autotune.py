@@ -41,12 +41,13 @@ def _store(db):
 
 def candidates(M, k=3):
     """Launch/schedule grid: edge-centric row threshold x LDS budget x waves per block, and for
-    triangles the B-stationary split (off, or B rows below 32Ki / 64Ki ids)."""
-    smalls = (16, 24, 32, 48, 64) if k == 3 else (8, 16, 32, 48, 64)
-    splits = (-1, 32768, 65536) if k == 3 else (-1,)
+    triangles the B-stationary split (B rows below 48Ki / 64Ki / 32Ki ids, or off).  The
+    likelier winners come first, so a tight time budget still reaches them."""
+    smalls = (32, 48, 16, 24, 64) if k == 3 else (8, 16, 32, 48, 64)
+    splits = (49152, 65536, 32768, -1) if k == 3 else (-1,)
     for bstat in splits:
         for small in smalls:
-            for lds, waves in ((2048, 2), (2048, 4), (1536, 4), (1024, 4), (3072, 2)):
+            for lds, waves in ((512, 4), (768, 4), (1024, 4), (1536, 4), (2048, 4), (2048, 2), (3072, 2)):
                 p = {"small_degree": small, "lds_words": lds, "waves": waves}
                 if k == 3:
                     p["bstat"] = bstat

@@ -68,6 +68,8 @@ struct CountArgs {
                                      // shared by every wave: one task per atomic serialises)
   const uint32_t *jstart;            // B-stationary on: per row, the entries before this
                                      // position are counted by k_bstat (null: none)
+  const uint4 *lo16;                 // B-stationary: those entries as 16-bit ids (they are all
+                                     // below the split, at most 65536), padded to 8 per vector
 };
 
 __device__ __forceinline__ uint32_t rfl(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -587,7 +589,8 @@ __device__ uint64_t task_cliques4(const CountArgs &a, const AState &A, uint32_t 
 // wave keeps 64 independent loads in flight instead of one pair at a time.
 // pshort B rows are walked partition by partition, one pair after another.
 
-constexpr uint32_t PASS_PROBE = 0, PASS_AND = 1, PASS_STREAM = 2, PASS_BLOCK = 3, PASS_NTSTREAM = 4;
+constexpr uint32_t PASS_PROBE = 0, PASS_AND = 1, PASS_STREAM = 2, PASS_BLOCK = 3, PASS_NTSTREAM = 4,
+                   PASS_STREAM16 = 5;
 constexpr uint32_t META_WORDS = 260;
 constexpr uint32_t SMALL_META_WORDS = 260;
 
@@ -638,6 +641,19 @@ __device__ __forceinline__ PassAcc<W> pass_elem(const CountArgs &a, const AState
     }
   } else if constexpr (PASS == PASS_BLOCK) {
     return (PassAcc<W>)block_and<W>(S, a, v0 + r);
+  } else if constexpr (PASS == PASS_STREAM16) {
+    // eight 16-bit ids against a bitmap that covers every id below the split from bit 0:
+    // no span test (ids at or past the pair's B row, and the padding, find zero bits)
+    const uint4 x = a.lo16[(uint64_t)v1 + r];
+    const uint32_t *bm = S.bm;
+    uint32_t h = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t w = q == 0 ? x.x : q == 1 ? x.y : q == 2 ? x.z : x.w;
+      const uint32_t e0 = w & 0xFFFFu, e1 = w >> 16;
+      h += ((bm[e0 >> 5] >> (e0 & 31u)) & 1u) + ((bm[e1 >> 5] >> (e1 & 31u)) & 1u);
+    }
+    return (PassAcc<W>)h;
   } else {
     const uint32_t *base = (PASS == PASS_NTSTREAM) ? a.ntu : a.unbr;
     const uint4 x = reinterpret_cast<const uint4 *>(base)[(uint64_t)v1 + r];
@@ -668,7 +684,7 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
   }
   // remainders start after the whole rounds: PROBE/AND/BLOCK index by v0, the streams by v1
   c -= full;
-  if constexpr (PASS == PASS_STREAM || PASS == PASS_NTSTREAM) p1 += full;
+  if constexpr (PASS == PASS_STREAM || PASS == PASS_NTSTREAM || PASS == PASS_STREAM16) p1 += full;
   else p0 += full;
   if (__ballot(c != 0u) == 0ull) return acc;
   uint32_t tot;
@@ -848,7 +864,9 @@ __device__ uint64_t run_task_tuned(const CountArgs &a, uint32_t kind, uint32_t i
       case KIND_BITMAP: return sum_isect<KIND_BITMAP, W, 4, BLK>(a, A, A.list, jb, je, lane, meta);
       case KIND_HASH: return sum_isect<KIND_HASH, W, 4>(a, A, A.list, jb, je, lane, meta);
       case KIND_LIST: return sum_isect<KIND_LIST, W, 4>(a, A, A.list, jb, je, lane, meta);
-      default: return task_triangles<KIND_GLOBAL, W>(a, A, jb, je, lane);
+      // nothing staged: B's elements are searched in row(i) in HBM, the pairs flattened
+      // like the staged kinds (the wave's slice holds the pass metadata only)
+      default: return sum_isect<KIND_GLOBAL, W, 4>(a, A, A.list, jb, je, lane, meta);
     }
   } else {
     switch (kind) {
@@ -1206,19 +1224,32 @@ __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint3
 // B-stationary triangle pairs.  On an oriented matrix row(j) only holds ids below j,
 // so for a pair (i, j) with j at position p of row(i), |row(i) & row(j)| =
 // |row(i)[0, p) & row(j)|.  A task {j, b, e} stages row(j) once as an LDS bitmap over
-// its word span and streams the prefixes row(i)[0, p) of its in-pairs [b, e) through
-// it, 16-byte vectors flattened over the lanes (elements at p and beyond are >= j and
-// fall outside the bitmap's span).  Each prefix is read contiguously, where probing
-// every element of it into row(j)'s bitset in HBM touches about one cache line per
-// element: the hub rows with low ids are probed by most rows.
+// the ids [0, split) and streams the prefixes row(i)[0, p) of its in-pairs [b, e)
+// through it, 16-byte vectors of eight 16-bit ids flattened over the lanes (every id of
+// such a prefix is below j < split <= 65536; the vector's tail holds ids >= j or the
+// padding split - 1, whose bits are zero in any row(j) with j < split).  Each prefix is
+// read contiguously at two bytes an id, where probing every element of it into row(j)'s
+// bitset in HBM touches about one cache line per element: the hub rows with low ids are
+// probed by most rows.
 __global__ void __launch_bounds__(256) k_bstat(CountArgs a, const uint4 *__restrict__ btasks, uint64_t nbt,
-                                               const uint2 *__restrict__ in_pairs) {
+                                               const uint2 *__restrict__ in_pairs, uint32_t bm_words) {
   extern __shared__ uint32_t lds_all[];
   const uint32_t lane = lane_id();
   uint32_t *meta = lds_all + (threadIdx.x >> 6) * a.lds_words;
   uint32_t *bm = meta + META_WORDS;
   uint64_t acc = 0;
   uint64_t t = 0, tend = 0;
+  AState S;
+  S.bm = bm;
+  S.base32 = S.aw0 = S.lo = 0;
+  S.anw = bm_words >> 1;
+  S.span = 0xFFFFFFFFu;
+  S.list = nullptr;
+  S.n = 0;
+  S.P = 0;
+  S.dpre = S.dstart = nullptr;
+  S.ht = nullptr;
+  S.hlg = 0;
   for (;;) {
     if (t >= tend) {
       uint32_t t0 = 0;
@@ -1231,25 +1262,12 @@ __global__ void __launch_bounds__(256) k_bstat(CountArgs a, const uint4 *__restr
     const uint32_t j = rfl(task.x), b = rfl(task.y), e = rfl(task.z);
     const uint32_t dj = rfl(a.card[j]);
     const uint32_t *G = a.unbr + (uint64_t)a.aoff[j] * 4;
-    AState S;
-    S.lo = rfl(G[0]);
-    const uint32_t hi = rfl(G[dj - 1]);
-    S.span = hi - S.lo;
-    S.aw0 = S.lo >> 6;
-    S.anw = (hi >> 6) - S.aw0 + 1u;
-    S.base32 = 2u * S.aw0;
-    S.bm = bm;
-    S.list = nullptr;
-    S.n = dj;
-    S.P = 0;
-    S.dpre = S.dstart = nullptr;
-    S.ht = nullptr;
-    S.hlg = 0;
-    for (uint32_t w = lane; w < 2u * S.anw; w += 64) bm[w] = 0u;
+    // the whole bitmap is cleared: a prefix vector's tail may test ids up to split - 1
+    for (uint32_t w = lane; w < bm_words; w += 64) bm[w] = 0u;
     wave_sync();
     for (uint32_t k = lane; k < dj; k += 64) {
       const uint32_t x = G[k];
-      atomicOr(&bm[(x >> 5) - S.base32], 1u << (x & 31u));
+      atomicOr(&bm[x >> 5], 1u << (x & 31u));
     }
     wave_sync();
     for (uint32_t c0 = b; c0 < e; c0 += 64) {
@@ -1257,9 +1275,9 @@ __global__ void __launch_bounds__(256) k_bstat(CountArgs a, const uint4 *__restr
       if (c0 + lane < e) {
         const uint2 pr = in_pairs[c0 + lane];
         off = pr.x;
-        nv = (pr.y + 3u) >> 2;
+        nv = (pr.y + 7u) >> 3;
       }
-      acc += flat_pass<PASS_STREAM, KIND_BITMAP, false, 4>(a, S, nv, 0u, off, 0u, lane, meta);
+      acc += flat_pass<PASS_STREAM16, KIND_BITMAP, false, 4>(a, S, nv, 0u, off, 0u, lane, meta);
     }
     wave_sync();
   }
@@ -1473,7 +1491,8 @@ __global__ void k_expand_tasks(const uint64_t *sorted, const uint32_t *nt_sorted
 // of each non-empty B row j < bstat_j it holds (one wave per row, positions [0, jstart))
 __global__ void __launch_bounds__(256) k_bstat_fill(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr,
                                                     const RowDesc *desc, const uint32_t *jstart, uint32_t n,
-                                                    const uint64_t *inoff, uint32_t *infill, uint2 *in_pairs) {
+                                                    const uint64_t *inoff, uint32_t *infill, const uint64_t *lo16off,
+                                                    uint2 *in_pairs) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
@@ -1485,8 +1504,31 @@ __global__ void __launch_bounds__(256) k_bstat_fill(const uint32_t *card, const 
       const uint32_t j = G[k];
       if (desc[j].card == 0 || k == 0) continue;
       const uint32_t slot = atomicAdd(&infill[j], 1u);
-      in_pairs[inoff[j] + slot] = make_uint2(aoff[i], k);
+      in_pairs[inoff[j] + slot] = make_uint2((uint32_t)lo16off[i], k);
     }
+  }
+}
+
+// 16-bit copies of the B-stationary prefixes: row i's first jstart[i] entries (all below
+// the split) in ceil(jstart / 8) 16-byte vectors, the tail padded with split - 1
+__global__ void k_lo16_count(const uint32_t *jstart, uint32_t n, uint32_t *nvec) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    nvec[i] = (jstart[i] + 7u) >> 3;
+}
+
+__global__ void __launch_bounds__(256) k_lo16_fill(const uint32_t *aoff, const uint32_t *unbr, const uint32_t *jstart,
+                                                   uint32_t n, const uint64_t *lo16off, uint32_t pad,
+                                                   uint16_t *lo16) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t i = wid; i < n; i += nw) {
+    const uint32_t js = jstart[i];
+    if (js == 0) continue;
+    const uint32_t *G = unbr + (uint64_t)aoff[i] * 4;
+    uint16_t *D = lo16 + lo16off[i] * 8;
+    const uint32_t len = ((js + 7u) >> 3) * 8u;
+    for (uint32_t k = lane; k < len; k += 64) D[k] = (uint16_t)(k < js ? G[k] : pad);
   }
 }
 
@@ -1543,7 +1585,9 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
   }
   // rows this short are cheaper as explicit wedges (triangles) or triples (4-cliques)
   tp.small_deg = (sched == EH_SCHED_TUNED && m->filter == EH_FILTER_LOWER) ? 32u : 0u;
-  tp.bstat_j = 0;
+  // B rows below 48Ki ids B-stationary (schedule_build applies it to tuned triangle passes
+  // over oriented single-graph matrices only)
+  tp.bstat_j = (sched == EH_SCHED_TUNED && k == 3) ? 49152u : 0u;
   // triangle tasks are many and light: two per queue atomic; 4-clique tasks are few and
   // heavy, where pulling two at once unbalances the end of the pass
   tp.grab = (k == 4) ? 1u : 2u;
@@ -1557,11 +1601,12 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
     if (opts->stream_ratio < 0) tp.stream_ratio = 0;
     else if (opts->stream_ratio > 0) tp.stream_ratio = (uint32_t)opts->stream_ratio;
     if (opts->bstat_split < 0) tp.bstat_j = 0;
-    else if (opts->bstat_split > 0) tp.bstat_j = (uint32_t)std::min(std::max(opts->bstat_split, 64), 524288);
+    else if (opts->bstat_split > 0) tp.bstat_j = (uint32_t)std::min(std::max(opts->bstat_split, 64), 65536);
   }
   // the task kernels are built for at most 256 threads (4 waves) per block, and a
   // wave's slice holds at most 16384 words (64 KB)
-  if (tp.lds_small < 256) tp.lds_small = 256;
+  // the flattened passes keep META_WORDS of pair metadata in every slice
+  if (tp.lds_small < 512) tp.lds_small = 512;
   if (tp.lds_small > 16384) tp.lds_small = 16384;
   if (tp.waves_per_block < 1) tp.waves_per_block = 1;
   if (tp.waves_per_block > 4) tp.waves_per_block = 4;
@@ -1675,9 +1720,28 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
       return EH_ERR_UNSUPPORTED;
     }
     EH_CHECK(s.in_pairs.alloc(8 * npairs + 16));
+    // 16-bit prefixes
+    DevBuf nvec, lo16off;
+    EH_CHECK(nvec.alloc(4ull * n + 8));
+    EH_CHECK(lo16off.alloc(8ull * n + 16));
+    k_lo16_count<<<grid_for(n), 256, 0, st>>>(jstart.as<uint32_t>(), n, nvec.as<uint32_t>());
+    k_zero_u32<<<1, 64, 0, st>>>(nvec.as<uint32_t>() + n, 1);
+    EH_CHECK(prim_exscan_u32_to_u64(ctx, nvec.as<uint32_t>(), lo16off.as<uint64_t>(), (uint64_t)n + 1));
+    uint64_t nv16 = 0;
+    EH_HIP(hipMemcpyAsync(&nv16, lo16off.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, st));
+    EH_HIP(hipStreamSynchronize(st));
+    if (nv16 >= (1ull << 32)) {
+      eh_set_error("B-stationary prefixes exceed the 32-bit vector index");
+      return EH_ERR_UNSUPPORTED;
+    }
+    EH_CHECK(s.lo16.alloc(16 * nv16 + 16));
+    k_lo16_fill<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(m->daoff.as<uint32_t>(), m->dunbr.as<uint32_t>(),
+                                                          jstart.as<uint32_t>(), n, lo16off.as<uint64_t>(),
+                                                          bstat_j - 1u, s.lo16.as<uint16_t>());
     k_bstat_fill<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(
         m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(), m->dunbr.as<uint32_t>(), m->ddesc.as<RowDesc>(),
-        jstart.as<uint32_t>(), n, inoff.as<uint64_t>(), infill.as<uint32_t>(), s.in_pairs.as<uint2>());
+        jstart.as<uint32_t>(), n, inoff.as<uint64_t>(), infill.as<uint32_t>(), lo16off.as<uint64_t>(),
+        s.in_pairs.as<uint2>());
     k_bstat_ntask<<<grid_for(bstat_j + 1), 256, 0, st>>>(incount.as<uint32_t>(), bstat_j + 1, BSTAT_CHUNK,
                                                         ntb.as<uint32_t>());
     EH_CHECK(prim_exscan_u32_to_u64(ctx, ntb.as<uint32_t>(), boff.as<uint64_t>(), (uint64_t)bstat_j + 1));
@@ -1780,8 +1844,9 @@ eh_status launch_bstat(eh_ctx *ctx, const CountArgs &base, const uint4 *btasks, 
   if (nbt == 0) return EH_OK;
   CountArgs a = base;
   a.next = d_next;
-  // a wave's slice: the pass metadata and a bitmap over the word span of any row below bstat_j
-  a.lds_words = META_WORDS + 2u * (((bstat_j - 1u) >> 6) + 1u);
+  // a wave's slice: the pass metadata and a bitmap over the ids [0, bstat_j)
+  const uint32_t bm_words = 2u * (((bstat_j - 1u) >> 6) + 1u);
+  a.lds_words = META_WORDS + bm_words;
   int waves = 4;
   while (waves > 1 && (size_t)a.lds_words * 4 * waves > 64 * 1024) waves--;
   const size_t lds_bytes = (size_t)a.lds_words * 4 * waves;
@@ -1792,7 +1857,7 @@ eh_status launch_bstat(eh_ctx *ctx, const CountArgs &base, const uint4 *btasks, 
   if (blocks > need) blocks = need;
   k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
   hipLaunchKernelGGL(k_bstat, dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a, btasks, nbt,
-                     in_pairs);
+                     in_pairs, bm_words);
   EH_HIP(hipGetLastError());
   return EH_OK;
 }
@@ -1803,10 +1868,11 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
                        uint64_t n_small_rows, const uint4 *tasks_big, uint64_t n_big, const uint4 *tasks_small,
                        uint64_t n_small, uint32_t lds_small, uint32_t lds_big, uint64_t *d_counts, uint32_t *d_next,
                        uint64_t *d_task_cycles, const uint4 *btasks, uint64_t n_btasks, const uint2 *in_pairs,
-                       uint32_t bstat_j, const uint32_t *jstart) {
+                       uint32_t bstat_j, const uint32_t *jstart, const uint4 *lo16) {
   CountArgs a;
   a.grab = tp.grab ? tp.grab : 1u;
   a.jstart = jstart;
+  a.lo16 = lo16;
   a.task_cycles = reinterpret_cast<unsigned long long *>(d_task_cycles);
   a.task_base = 0;
   a.card = m->dcard.as<uint32_t>();
@@ -1854,7 +1920,7 @@ eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, 
   return launch_parts(m, k, sched, tp, s.small_rows.as<uint32_t>(), s.n_small_rows, t, s.n_big, t + s.n_big,
                       s.n_small, s.lds_small, s.lds_big, d_counts, d_next, d_task_cycles, s.btasks.as<uint4>(),
                       s.n_btasks, s.in_pairs.as<uint2>(), s.bstat_j,
-                      s.bstat_j ? s.jstart.as<uint32_t>() : nullptr);
+                      s.bstat_j ? s.jstart.as<uint32_t>() : nullptr, s.bstat_j ? s.lo16.as<uint4>() : nullptr);
 }
 
 // k: 3 triangles, 4 four-cliques, -3 weighted triangle pass (sum of card(x) over

@@ -166,6 +166,7 @@ struct Schedule {
   // {j, first pair, end pair, 0}
   uint32_t bstat_j = 0, bstat_req = 0;  // in effect / as asked (0 where it does not apply)
   DevBuf in_pairs, btasks, jstart;  // jstart: per row, entries counted B-stationary
+  DevBuf lo16;                      // those entries as 16-bit ids, 8 per 16-byte vector
   uint64_t n_btasks = 0, n_in_pairs = 0;
 };
 
@@ -227,7 +228,8 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
                        uint64_t n_small_rows, const uint4 *tasks_big, uint64_t n_big, const uint4 *tasks_small,
                        uint64_t n_small, uint32_t lds_small, uint32_t lds_big, uint64_t *d_counts, uint32_t *d_next,
                        uint64_t *d_task_cycles, const uint4 *btasks = nullptr, uint64_t n_btasks = 0,
-                       const uint2 *in_pairs = nullptr, uint32_t bstat_j = 0, const uint32_t *jstart = nullptr);
+                       const uint2 *in_pairs = nullptr, uint32_t bstat_j = 0, const uint32_t *jstart = nullptr,
+                       const uint4 *lo16 = nullptr);
 
 // timing helper: device ms between two events recorded on ctx->stream
 double ctx_elapsed_ms(eh_ctx *ctx);

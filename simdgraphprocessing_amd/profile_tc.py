@@ -26,7 +26,7 @@ import sys
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
 COPY_BYTES = 1 << 30
-COUNT_KERNELS = ("k_count", "k_small_rows")
+COUNT_KERNELS = ("k_count", "k_small_rows", "k_bstat")
 
 
 def run(a):
@@ -44,6 +44,8 @@ def run(a):
         kw["lds_words"] = a.lds_words
     if a.waves:
         kw["waves"] = a.waves
+    if a.bstat:
+        kw["bstat"] = a.bstat
     c, st = M.triangle_count("tuned", **kw)
     for _ in range(a.passes):
         c, st = M.triangle_count("tuned", **kw)
@@ -116,6 +118,7 @@ def main():
     r.add_argument("--small-degree", type=int, default=0)
     r.add_argument("--lds-words", type=int, default=0)
     r.add_argument("--waves", type=int, default=0)
+    r.add_argument("--bstat", type=int, default=0, help="B-stationary split (0: default, <0: off)")
     s = sub.add_parser("summarize")
     s.add_argument("--scale", type=int, default=22)
     s.add_argument("--policy", default="hybrid")
