@@ -67,10 +67,16 @@ def _rows(path, counter):
                 if r["Counter_Name"] == counter]
 
 
+def _is_kernel(name, base):
+    """name is kernel `base` itself (template arguments or a signature may follow), not a
+    longer name that starts with it (k_bstat vs k_bstat_fill)."""
+    return name == base or name.startswith(base + "<") or name.startswith(base + "(")
+
+
 def _per_kernel_mean(rows):
     out = {}
     for base in COUNT_KERNELS:
-        v = [x for name, x in rows if name.startswith(base)]
+        v = [x for name, x in rows if _is_kernel(name, base)]
         if v:
             out[base] = sum(v) / len(v)
     return out
