@@ -129,7 +129,7 @@ typedef struct {
   int32_t bstat_split;    /* TUNED triangles on oriented (EH_FILTER_LOWER) matrices: pairs (i,j)
                              of rows above small_degree whose B row id j is below this bound are
                              counted B-stationary (row(j) staged once as an LDS bitmap, the part
-                             of every row(i) below j streamed through it); 0 = default (49152),
+                             of every row(i) below j streamed through it); 0 = default (65536),
                              <0 = off, else 64..65536 */
 } eh_run_opts;
 

@@ -41,10 +41,10 @@ def _store(db):
 
 def candidates(M, k=3):
     """Launch/schedule grid: edge-centric row threshold x LDS budget x waves per block, and for
-    triangles the B-stationary split (B rows below 48Ki / 64Ki / 32Ki ids, or off).  The
+    triangles the B-stationary split (B rows below 64Ki / 48Ki / 32Ki ids, or off).  The
     likelier winners come first, so a tight time budget still reaches them."""
     smalls = (32, 48, 16, 24, 64) if k == 3 else (8, 16, 32, 48, 64)
-    splits = (49152, 65536, 32768, -1) if k == 3 else (-1,)
+    splits = (65536, 49152, 32768, -1) if k == 3 else (-1,)
     for bstat in splits:
         for small in smalls:
             for lds, waves in ((512, 4), (768, 4), (1024, 4), (1536, 4), (2048, 4), (2048, 2), (3072, 2)):

@@ -615,7 +615,46 @@ __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t *L, uint32_t 
 template <bool W>
 using PassAcc = typename std::conditional<W, uint64_t, uint32_t>::type;
 
-template <int PASS, uint32_t KIND, bool W>
+// the four ids of a streamed B vector tested against S; with LOCKSTEP a sorted list (in LDS,
+// or in HBM for an unstaged row) is searched for all four together, so each step's four loads
+// are in flight at once (B's padding 0xFFFFFFFF never equals an id)
+template <uint32_t KIND, bool W, bool LOCKSTEP>
+__device__ __forceinline__ PassAcc<W> stream_hits(const AState &S, const CountArgs &a, const uint4 x) {
+  if constexpr ((KIND == KIND_LIST || KIND == KIND_GLOBAL) && !W && LOCKSTEP) {
+    uint32_t len = S.n;
+    if (len == 0) return 0;
+    uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+    while (len > 1) {
+      const uint32_t half = len >> 1;
+      const uint32_t v0 = S.list[b0 + half], v1 = S.list[b1 + half], v2 = S.list[b2 + half],
+                     v3 = S.list[b3 + half];
+      b0 = (v0 <= x.x) ? b0 + half : b0;
+      b1 = (v1 <= x.y) ? b1 + half : b1;
+      b2 = (v2 <= x.z) ? b2 + half : b2;
+      b3 = (v3 <= x.w) ? b3 + half : b3;
+      len -= half;
+    }
+    return (PassAcc<W>)((S.list[b0] == x.x) + (S.list[b1] == x.y) + (S.list[b2] == x.z) + (S.list[b3] == x.w));
+  } else {
+    return (PassAcc<W>)(hit_value<KIND, W>(S, a, x.x) + hit_value<KIND, W>(S, a, x.y) +
+                        hit_value<KIND, W>(S, a, x.z) + hit_value<KIND, W>(S, a, x.w));
+  }
+}
+
+// eight 16-bit ids against a bitmap that covers every id below the split from bit 0: no span
+// test (ids at or past the pair's B row, and the padding, find zero bits)
+__device__ __forceinline__ uint32_t bitmap16_hits(const uint32_t *bm, const uint4 x) {
+  uint32_t h = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const uint32_t w = q == 0 ? x.x : q == 1 ? x.y : q == 2 ? x.z : x.w;
+    const uint32_t e0 = w & 0xFFFFu, e1 = w >> 16;
+    h += ((bm[e0 >> 5] >> (e0 & 31u)) & 1u) + ((bm[e1 >> 5] >> (e1 & 31u)) & 1u);
+  }
+  return h;
+}
+
+template <int PASS, uint32_t KIND, bool W, bool LOCKSTEP = false>
 __device__ __forceinline__ PassAcc<W> pass_elem(const CountArgs &a, const AState &S, uint32_t v0, uint32_t v1,
                                                 uint32_t v2, uint32_t r) {
   if constexpr (PASS == PASS_PROBE) {
@@ -642,23 +681,10 @@ __device__ __forceinline__ PassAcc<W> pass_elem(const CountArgs &a, const AState
   } else if constexpr (PASS == PASS_BLOCK) {
     return (PassAcc<W>)block_and<W>(S, a, v0 + r);
   } else if constexpr (PASS == PASS_STREAM16) {
-    // eight 16-bit ids against a bitmap that covers every id below the split from bit 0:
-    // no span test (ids at or past the pair's B row, and the padding, find zero bits)
-    const uint4 x = a.lo16[(uint64_t)v1 + r];
-    const uint32_t *bm = S.bm;
-    uint32_t h = 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const uint32_t w = q == 0 ? x.x : q == 1 ? x.y : q == 2 ? x.z : x.w;
-      const uint32_t e0 = w & 0xFFFFu, e1 = w >> 16;
-      h += ((bm[e0 >> 5] >> (e0 & 31u)) & 1u) + ((bm[e1 >> 5] >> (e1 & 31u)) & 1u);
-    }
-    return (PassAcc<W>)h;
+    return (PassAcc<W>)bitmap16_hits(S.bm, a.lo16[(uint64_t)v1 + r]);
   } else {
     const uint32_t *base = (PASS == PASS_NTSTREAM) ? a.ntu : a.unbr;
-    const uint4 x = reinterpret_cast<const uint4 *>(base)[(uint64_t)v1 + r];
-    return (PassAcc<W>)(hit_value<KIND, W>(S, a, x.x) + hit_value<KIND, W>(S, a, x.y) +
-                        hit_value<KIND, W>(S, a, x.z) + hit_value<KIND, W>(S, a, x.w));
+    return stream_hits<KIND, W, LOCKSTEP>(S, a, reinterpret_cast<const uint4 *>(base)[(uint64_t)v1 + r]);
   }
 }
 
@@ -680,7 +706,7 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
     const uint32_t fq = rdl(full, q), q0 = rdl(p0, q), q1 = rdl(p1, q), q2 = rdl(p2, q);
     // fq is a multiple of 64: unrolled, each lane keeps several loads in flight
 #pragma unroll UNROLL
-    for (uint32_t r = lane; r < fq; r += 64) acc += pass_elem<PASS, KIND, W>(a, S, q0, q1, q2, r);
+    for (uint32_t r = lane; r < fq; r += 64) acc += pass_elem<PASS, KIND, W, (UNROLL > 1)>(a, S, q0, q1, q2, r);
   }
   // remainders start after the whole rounds: PROBE/AND/BLOCK index by v0, the streams by v1
   c -= full;
@@ -697,12 +723,37 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
   m2[lane] = p2;
   if (lane == 0) mbase[64] = tot;
   wave_sync();
-  for (uint32_t t = lane; t < tot; t += 64) {
-    uint32_t q = 0;
+  if constexpr ((PASS == PASS_STREAM || PASS == PASS_STREAM16) && UNROLL > 1) {
+    // UNROLL items per lane with their vector loads issued before any is tested; an item
+    // past the end re-reads the last one (always a valid address) and is not counted
+    const uint4 *vb = (PASS == PASS_STREAM16) ? a.lo16 : reinterpret_cast<const uint4 *>(a.unbr);
+    for (uint32_t t0 = lane; t0 < tot; t0 += 64u * UNROLL) {
+      uint4 v[UNROLL];
 #pragma unroll
-    for (uint32_t step = 32; step; step >>= 1)
-      if (mbase[q + step] <= t) q += step;
-    acc += pass_elem<PASS, KIND, W>(a, S, m0[q], m1[q], m2[q], t - mbase[q]);
+      for (int u = 0; u < UNROLL; u++) {
+        const uint32_t t = min(t0 + 64u * u, tot - 1u);
+        uint32_t q = 0;
+#pragma unroll
+        for (uint32_t step = 32; step; step >>= 1)
+          if (mbase[q + step] <= t) q += step;
+        v[u] = vb[(uint64_t)m1[q] + (t - mbase[q])];
+      }
+#pragma unroll
+      for (int u = 0; u < UNROLL; u++) {
+        if (t0 + 64u * u < tot) {
+          if constexpr (PASS == PASS_STREAM16) acc += (PassAcc<W>)bitmap16_hits(S.bm, v[u]);
+          else acc += stream_hits<KIND, W, true>(S, a, v[u]);
+        }
+      }
+    }
+  } else {
+    for (uint32_t t = lane; t < tot; t += 64) {
+      uint32_t q = 0;
+#pragma unroll
+      for (uint32_t step = 32; step; step >>= 1)
+        if (mbase[q + step] <= t) q += step;
+      acc += pass_elem<PASS, KIND, W>(a, S, m0[q], m1[q], m2[q], t - mbase[q]);
+    }
   }
   wave_sync();
   return acc;
@@ -1585,9 +1636,9 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
   }
   // rows this short are cheaper as explicit wedges (triangles) or triples (4-cliques)
   tp.small_deg = (sched == EH_SCHED_TUNED && m->filter == EH_FILTER_LOWER) ? 32u : 0u;
-  // B rows below 48Ki ids B-stationary (schedule_build applies it to tuned triangle passes
+  // B rows below 64Ki ids B-stationary (schedule_build applies it to tuned triangle passes
   // over oriented single-graph matrices only)
-  tp.bstat_j = (sched == EH_SCHED_TUNED && k == 3) ? 49152u : 0u;
+  tp.bstat_j = (sched == EH_SCHED_TUNED && k == 3) ? 65536u : 0u;
   // triangle tasks are many and light: two per queue atomic; 4-clique tasks are few and
   // heavy, where pulling two at once unbalances the end of the pass
   tp.grab = (k == 4) ? 1u : 2u;
