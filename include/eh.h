@@ -131,6 +131,9 @@ typedef struct {
                              counted B-stationary (row(j) staged once as an LDS bitmap, the part
                              of every row(i) below j streamed through it); 0 = default (65536),
                              <0 = off, else 64..65536 */
+  int32_t local_max;      /* TUNED 4-cliques on oriented matrices: rows with more than small_degree
+                             and at most this many entries count the triangles of their local
+                             graph (bit-rows in LDS); 0 = default (512), <0 = off, at most 512 */
 } eh_run_opts;
 
 typedef struct {

@@ -48,7 +48,8 @@ class MatrixInfo(C.Structure):
 class RunOpts(C.Structure):
     _fields_ = [("schedule", C.c_int32), ("lanes_per_row", C.c_int32), ("task_work", C.c_uint32),
                 ("lds_words", C.c_uint32), ("waves_per_block", C.c_int32), ("repeat", C.c_int32),
-                ("small_degree", C.c_int32), ("stream_ratio", C.c_int32), ("bstat_split", C.c_int32)]
+                ("small_degree", C.c_int32), ("stream_ratio", C.c_int32), ("bstat_split", C.c_int32),
+                ("local_max", C.c_int32)]
 
 
 class BfsInfo(C.Structure):
@@ -184,9 +185,10 @@ def _ptr(a):
 
 
 def _opts(schedule="tuned", lanes=0, task_work=0, lds_words=0, waves=0, repeat=1, small_degree=0,
-          stream_ratio=0, bstat=0):
+          stream_ratio=0, bstat=0, local_max=0):
     o = RunOpts()
     o.bstat_split = bstat
+    o.local_max = local_max
     o.small_degree = small_degree
     o.stream_ratio = stream_ratio
     o.schedule = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)

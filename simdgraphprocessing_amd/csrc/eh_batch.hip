@@ -77,7 +77,8 @@ struct PassKey {
 static bool same_tp(const TuneParams &x, const TuneParams &y) {
   return x.lanes == y.lanes && x.task_work == y.task_work && x.lds_small == y.lds_small && x.lds_big == y.lds_big &&
          x.waves_per_block == y.waves_per_block && x.big_waves == y.big_waves && x.small_deg == y.small_deg &&
-         x.stream_ratio == y.stream_ratio && x.bstat_j == y.bstat_j && x.grab == y.grab;
+         x.stream_ratio == y.stream_ratio && x.bstat_j == y.bstat_j && x.grab == y.grab &&
+         x.local_max == y.local_max;
 }
 
 static bool same_key(const PassKey &x, const PassKey &y) {
@@ -101,8 +102,8 @@ struct eh_batch {
   struct Plan {
     bool valid = false;
     PassKey key;
-    DevBuf small_rows, tasks_big, tasks_small;
-    std::vector<uint64_t> sr_off, tb_off, ts_off;  // ng + 1 offsets
+    DevBuf small_rows, tasks_big, tasks_small, local_rows[2];
+    std::vector<uint64_t> sr_off, tb_off, ts_off, lr_off[2];  // ng + 1 offsets
   } plan[2];  // [0] triangles, [1] 4-cliques
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -227,6 +228,16 @@ static eh_status build_plan(eh_batch *b, int kidx, const PassKey &key) {
   P.sr_off.assign(ng + 1, 0);
   for (uint32_t r : sr) P.sr_off[gid_row(r) + 1]++;
   for (uint32_t g = 0; g < ng; g++) P.sr_off[g + 1] += P.sr_off[g];
+  // the local-graph row classes (4-cliques), in row order and so already grouped by graph
+  for (int c = 0; c < 2; c++) {
+    std::vector<uint32_t> lr(s.n_local[c]);
+    if (!lr.empty()) EH_HIP(hipMemcpy(lr.data(), s.local_rows[c].p, 4 * lr.size(), hipMemcpyDeviceToHost));
+    P.lr_off[c].assign(ng + 1, 0);
+    for (uint32_t r : lr) P.lr_off[c][gid_row(r) + 1]++;
+    for (uint32_t g = 0; g < ng; g++) P.lr_off[c][g + 1] += P.lr_off[c][g];
+    EH_CHECK(P.local_rows[c].alloc(4 * lr.size() + 16));
+    if (!lr.empty()) EH_HIP(hipMemcpy(P.local_rows[c].p, lr.data(), 4 * lr.size(), hipMemcpyHostToDevice));
+  }
   EH_CHECK(P.small_rows.alloc(4 * sr.size() + 16));
   EH_CHECK(P.tasks_big.alloc(16 * tb.size() + 16));
   EH_CHECK(P.tasks_small.alloc(16 * ts.size() + 16));
@@ -283,7 +294,7 @@ extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tr
       for (int q = 0; q < 2 && s == EH_OK; q++)
         if (want[q])
           s = schedule_launch(M, kk[q], key[q].sched, key[q].tp, M->sched[q][key[q].sched],
-                              b->counts.as<uint64_t>() + (uint64_t)q * ng, nx + 4 * q, nullptr);
+                              b->counts.as<uint64_t>() + (uint64_t)q * ng, nx + 8 * q, nullptr);
       hipGraph_t gr = nullptr;
       hipError_t e = hipStreamEndCapture(st, &gr);
       if (s != EH_OK) { if (gr) (void)hipGraphDestroy(gr); return s; }
@@ -311,7 +322,10 @@ extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tr
                               P.sr_off[g + 1] - P.sr_off[g], P.tasks_big.as<uint4>() + P.tb_off[g],
                               P.tb_off[g + 1] - P.tb_off[g], P.tasks_small.as<uint4>() + P.ts_off[g],
                               P.ts_off[g + 1] - P.ts_off[g], key[q].lds_small, key[q].lds_big,
-                              b->counts.as<uint64_t>() + (uint64_t)q * ng, nx + 4 * q, nullptr));
+                              b->counts.as<uint64_t>() + (uint64_t)q * ng, nx + 8 * q, nullptr, nullptr, 0, nullptr,
+                              0, nullptr, nullptr, P.local_rows[0].as<uint32_t>() + P.lr_off[0][g],
+                              P.lr_off[0][g + 1] - P.lr_off[0][g], P.local_rows[1].as<uint32_t>() + P.lr_off[1][g],
+                              P.lr_off[1][g + 1] - P.lr_off[1][g]));
       }
     }
     EH_HIP(hipEventRecord(ctx->ev1, st));

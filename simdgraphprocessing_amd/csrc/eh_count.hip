@@ -1158,6 +1158,8 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
 // A wave takes 64 rows at a time and runs them in sub-batches whose bit-rows fit
 // its SMALL_ADJ_WORDS (a row of up to 64 entries needs at most 128 words).
 constexpr uint32_t SMALL4_META_WORDS = 324;
+constexpr uint32_t LOCAL4_META = 196;  // k_local4: pair scan bases (65), offsets (64), columns (64)
+constexpr uint32_t LOCAL4_MID = 128, LOCAL4_MAX = 512;  // the two k_local4 row classes
 constexpr uint32_t SMALL_ADJ_WORDS = 1024;
 
 __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint32_t *__restrict__ rows,
@@ -1336,6 +1338,146 @@ __global__ void __launch_bounds__(256) k_bstat(CountArgs a, const uint4 *__restr
   if (lane == 0 && sum) atomicAdd(&a.counts[0], (unsigned long long)sum);
 }
 
+// 4-cliques of mid-size rows through each row's local graph (the short-row form of
+// k_small_cliques4 for rows of up to LMAX entries, one row per block).  Row i's list R is
+// staged in LDS with a hash table from id to position; for every c the B row row(R[c]) is
+// streamed (16-byte vectors; each wave flattens the items of its own 64 c's over its lanes,
+// four loads per lane in flight) and each of its ids found in R at position p (always p < c:
+// row(R[c]) lies below R[c]) sets bit p of the bit-row adj(c).  Then every local edge (b, c)
+// adds popcount(adj(b) & adj(c)).  This replaces, per pair (i, j), the materialised
+// C = A & row(j) and the stream of row(k) for every k in C: on rows inside dense communities
+// that is |C| rows per pair instead of one.  GROUP_BLOCK: the block's four waves share one
+// row and its LDS (long rows); otherwise each wave takes its own row (rows up to 128).
+template <uint32_t LMAX>
+constexpr uint32_t local4_slice(bool group_block) {
+  // the task slot, the scan metadata of each wave of the group, the row's list, hash keys and
+  // positions (2 * LMAX each) and its bit-rows (LMAX x LMAX / 32 words)
+  return 4 + LOCAL4_META * (group_block ? 4 : 1) + 5 * LMAX + LMAX * (LMAX / 32);
+}
+
+template <uint32_t LMAX, bool GROUP_BLOCK>
+__global__ void __launch_bounds__(256) k_local4(CountArgs a, const uint32_t *__restrict__ rows, uint32_t nrows) {
+  constexpr uint32_t CAP = 2 * LMAX;
+  extern __shared__ uint32_t lds_all[];
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  // the group that shares one row: the block, or one wave
+  const uint32_t gwave = GROUP_BLOCK ? wave : 0u, gwaves = GROUP_BLOCK ? (blockDim.x >> 6) : 1u;
+  const uint32_t gtid = GROUP_BLOCK ? threadIdx.x : lane, gsize = GROUP_BLOCK ? blockDim.x : 64u;
+  uint32_t *gbase = lds_all + (GROUP_BLOCK ? 0u : wave * local4_slice<LMAX>(false));
+  uint32_t *slot = gbase;
+  uint32_t *mbase = gbase + 4 + gwave * LOCAL4_META, *moff = mbase + 65, *mcol = mbase + 130;
+  uint32_t *list = gbase + 4 + gwaves * LOCAL4_META, *hkey = list + LMAX, *hpos = hkey + CAP, *adj = hpos + CAP;
+  auto gsync = [] {
+    if constexpr (GROUP_BLOCK) __syncthreads();
+    else wave_sync();
+  };
+  const bool per_graph = a.graph_of != nullptr;
+  uint64_t acc = 0;
+  for (;;) {
+    if (gtid == 0) slot[0] = atomicAdd(a.next, 1u);
+    gsync();
+    const uint32_t r = rfl(slot[0]);
+    if (r >= nrows) break;
+    const uint32_t i = rows[r];
+    const uint32_t d = rfl(a.card[i]);
+    const uint32_t *R = a.unbr + (uint64_t)a.aoff[i] * 4;
+    const uint32_t wr = (d + 31u) >> 5;  // bit-row stride in words
+    uint32_t lg = 4;
+    while ((1u << lg) < 2u * d) lg++;
+    const uint32_t cap = 1u << lg, mask = cap - 1u;
+    for (uint32_t k = gtid; k < d; k += gsize) list[k] = R[k];
+    for (uint32_t w = gtid; w < cap; w += gsize) hkey[w] = HASH_EMPTY;
+    for (uint32_t w = gtid; w < d * wr; w += gsize) adj[w] = 0u;
+    gsync();
+    for (uint32_t k = gtid; k < d; k += gsize) {
+      const uint32_t x = list[k];
+      uint32_t h = (x * 2654435761u) >> (32u - lg);
+      while (atomicCAS(&hkey[h], HASH_EMPTY, x) != HASH_EMPTY) h = (h + 1u) & mask;
+      hpos[h] = k;
+    }
+    gsync();
+    // local edges: stream row(R[c]) for c in [1, d), wave w of the group taking every
+    // gwaves-th run of 64
+    for (uint32_t c0 = 1 + 64u * gwave; c0 < d; c0 += 64u * gwaves) {
+      const uint32_t c = c0 + lane;
+      uint32_t items = 0, off = 0;
+      if (c < d) {
+        const uint32_t j = list[c];
+        items = (a.card[j] + 3u) >> 2;
+        off = a.aoff[j];
+      }
+      uint32_t tot;
+      const uint32_t bse = wave_excl_scan(items, lane, &tot);
+      tot = rfl(tot);
+      mbase[lane] = bse;
+      moff[lane] = off;
+      mcol[lane] = c;
+      if (lane == 0) mbase[64] = tot;
+      wave_sync();
+      for (uint32_t t0 = lane; t0 < tot; t0 += 64u * 4u) {
+        uint4 v[4];
+        uint32_t col[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const uint32_t t = min(t0 + 64u * u, tot - 1u);
+          uint32_t q = 0;
+#pragma unroll
+          for (uint32_t step = 32; step; step >>= 1)
+            if (mbase[q + step] <= t) q += step;
+          v[u] = reinterpret_cast<const uint4 *>(a.unbr)[(uint64_t)moff[q] + (t - mbase[q])];
+          col[u] = mcol[q];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (t0 + 64u * u >= tot) continue;
+          const uint32_t xs[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+            const uint32_t x = xs[e];
+            if (x == HASH_EMPTY) continue;  // the list padding
+            uint32_t h = (x * 2654435761u) >> (32u - lg);
+            for (;;) {
+              const uint32_t key = hkey[h];
+              if (key == x) {
+                const uint32_t pp = hpos[h];
+                atomicOr(&adj[col[u] * wr + (pp >> 5)], 1u << (pp & 31u));
+                break;
+              }
+              if (key == HASH_EMPTY) break;
+              h = (h + 1u) & mask;
+            }
+          }
+        }
+      }
+      wave_sync();
+    }
+    gsync();
+    // triangles of the local graph: every (c, word of adj(c)) unit walks its bits b
+    uint64_t rc = 0;
+    for (uint32_t u = gtid; u < d * wr; u += gsize) {
+      const uint32_t c = u / wr, w = u - c * wr;
+      uint32_t bits = adj[c * wr + w];
+      const uint32_t *ac = adj + c * wr;
+      while (bits) {
+        const uint32_t b = w * 32u + (uint32_t)__builtin_ctz(bits);
+        bits &= bits - 1u;
+        const uint32_t *ab = adj + b * wr;
+        for (uint32_t ww = 0; ww <= (b >> 5); ww++) rc += (uint32_t)__popc(ab[ww] & ac[ww]);
+      }
+    }
+    if (per_graph) {
+      const uint64_t sr = wave_sum_u64(rc);
+      if (lane == 0 && sr) atomicAdd(&a.counts[a.graph_of[i]], (unsigned long long)sr);
+    } else {
+      acc += rc;
+    }
+    // every wave is done with the row's LDS (and has read the slot) before the next row
+    gsync();
+  }
+  const uint64_t sum = wave_sum_u64(acc);
+  if (lane == 0 && sum) atomicAdd(&a.counts[0], (unsigned long long)sum);
+}
+
 template <int K, bool W, bool TUNED, bool BLK>
 __global__ void __launch_bounds__(256) k_count(CountArgs a) {
   extern __shared__ uint32_t lds_all[];
@@ -1396,10 +1538,11 @@ struct SchedArgs {
   uint32_t lds_small, lds_big;
   uint64_t task_work;
   uint32_t small_deg;     // rows with 1..small_deg entries go to the edge-centric kernel
+  uint32_t local_max;     // 4-cliques: rows above small_deg and up to this go to k_local4
   uint32_t bstat_j;       // task rows hand their pairs with j < bstat_j to the B-stationary pass
   uint32_t *incount;      // out (bstat): task rows containing j, per j < bstat_j
   uint32_t *jstart;       // out: first entry of row i its A-side tasks cover
-  uint32_t *is_small;     // out: 1 for those rows
+  uint32_t *is_small;     // out: 1 for those rows, 2 / 3 for the k_local4 classes
   uint64_t *sortkey;      // out: (class << 62) | (order << 32) | i
   uint32_t *ntask;        // out: tasks of row i
   uint32_t *rowinfo;      // out: kind << 28 | graph id
@@ -1448,8 +1591,9 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
     if (lane != 0) continue;
     s.jstart[i] = below;
     const bool small = dA > 0 && dA <= s.small_deg && below < dA;
-    s.is_small[i] = small ? 1u : 0u;
-    if (dA == 0 || small || below == dA) {
+    const bool local = !small && dA > 2 && dA <= s.local_max;
+    s.is_small[i] = small ? 1u : local ? (dA <= LOCAL4_MID ? 2u : 3u) : 0u;
+    if (dA == 0 || small || local || below == dA) {
       s.sortkey[i] = (3ull << 62) | i;
       s.ntask[i] = 0;
       s.rowinfo[i] = 0;
@@ -1602,6 +1746,11 @@ __global__ void k_bstat_expand(const uint32_t *incount, const uint64_t *inoff, c
   }
 }
 
+__global__ void k_class_flag(const uint32_t *cls, uint32_t n, uint32_t c, uint32_t *flag) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    flag[i] = cls[i] == c ? 1u : 0u;
+}
+
 __global__ void k_scatter_small(const uint32_t *is_small, const uint64_t *pos, uint32_t n, uint32_t *rows) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     if (is_small[i]) rows[pos[i]] = i;
@@ -1642,6 +1791,7 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
   // triangle tasks are many and light: two per queue atomic; 4-clique tasks are few and
   // heavy, where pulling two at once unbalances the end of the pass
   tp.grab = (k == 4) ? 1u : 2u;
+  tp.local_max = (sched == EH_SCHED_TUNED && k == 4 && m->filter == EH_FILTER_LOWER) ? LOCAL4_MAX : 0u;
   if (opts) {
     if (opts->small_degree < 0) tp.small_deg = 0;
     else if (opts->small_degree > 0 && tp.small_deg) tp.small_deg = (uint32_t)std::min(opts->small_degree, k == 4 ? 64 : 256);
@@ -1652,6 +1802,9 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
     if (opts->stream_ratio < 0) tp.stream_ratio = 0;
     else if (opts->stream_ratio > 0) tp.stream_ratio = (uint32_t)opts->stream_ratio;
     if (opts->bstat_split < 0) tp.bstat_j = 0;
+    if (opts->local_max < 0) tp.local_max = 0;
+    else if (opts->local_max > 0 && tp.local_max)
+      tp.local_max = (uint32_t)std::min<int32_t>(opts->local_max, (int32_t)LOCAL4_MAX);
     else if (opts->bstat_split > 0) tp.bstat_j = (uint32_t)std::min(std::max(opts->bstat_split, 64), 65536);
   }
   // the task kernels are built for at most 256 threads (4 waves) per block, and a
@@ -1706,6 +1859,8 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   sa.lds_big = tp.lds_big;
   sa.task_work = tp.task_work;
   sa.small_deg = tp.small_deg;
+  const uint32_t local_max = (sched == EH_SCHED_TUNED && k == 4) ? tp.local_max : 0u;
+  sa.local_max = local_max;
   sa.bstat_j = bstat_j;
   sa.incount = bstat_j ? incount.as<uint32_t>() : nullptr;
   sa.jstart = jstart.as<uint32_t>();
@@ -1716,6 +1871,25 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   sa.stats = stats.as<unsigned long long>();
   k_row_work<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(sa);
   EH_HIP(hipGetLastError());
+  // rows of each k_local4 class, in row order
+  for (int c = 0; c < 2; c++) {
+    s.n_local[c] = 0;
+    if (!local_max) continue;
+    DevBuf flag;
+    EH_CHECK(flag.alloc(4ull * n + 8));
+    k_class_flag<<<grid_for(n), 256, 0, st>>>(is_small.as<uint32_t>(), n, 2u + c, flag.as<uint32_t>());
+    k_zero_u32<<<1, 64, 0, st>>>(flag.as<uint32_t>() + n, 1);
+    EH_CHECK(prim_exscan_u32_to_u64(ctx, flag.as<uint32_t>(), spos.as<uint64_t>(), (uint64_t)n + 1));
+    uint64_t nl = 0;
+    EH_HIP(hipMemcpyAsync(&nl, spos.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, st));
+    EH_HIP(hipStreamSynchronize(st));
+    EH_CHECK(s.local_rows[c].alloc(4 * nl + 16));
+    if (nl)
+      k_scatter_small<<<grid_for(n), 256, 0, st>>>(flag.as<uint32_t>(), spos.as<uint64_t>(), n,
+                                                  s.local_rows[c].as<uint32_t>());
+    s.n_local[c] = nl;
+  }
+  k_class_flag<<<grid_for(n), 256, 0, st>>>(is_small.as<uint32_t>(), n, 1u, is_small.as<uint32_t>());
   EH_CHECK(prim_exscan_u32_to_u64(ctx, is_small.as<uint32_t>(), spos.as<uint64_t>(), (uint64_t)n + 1));
   uint64_t nsmall = 0;
   EH_HIP(hipMemcpyAsync(&nsmall, spos.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, st));
@@ -1913,13 +2087,32 @@ eh_status launch_bstat(eh_ctx *ctx, const CountArgs &base, const uint4 *btasks, 
   return EH_OK;
 }
 
+template <uint32_t LMAX, bool GROUP_BLOCK>
+eh_status launch_local(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows, uint64_t nrows, uint32_t *d_next) {
+  if (nrows == 0) return EH_OK;
+  CountArgs a = base;
+  a.next = d_next;
+  const int waves = 4, threads = 256;
+  const size_t lds_bytes = (size_t)local4_slice<LMAX>(GROUP_BLOCK) * 4 * (GROUP_BLOCK ? 1 : waves);
+  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(k_local4<LMAX, GROUP_BLOCK>), threads, lds_bytes);
+  uint64_t blocks = (uint64_t)per_cu * (ctx->num_cus ? ctx->num_cus : 256);
+  const uint64_t need = GROUP_BLOCK ? nrows : ceil_div(nrows, (uint64_t)waves);
+  if (blocks > need) blocks = need;
+  k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
+  hipLaunchKernelGGL((k_local4<LMAX, GROUP_BLOCK>), dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a,
+                     rows, (uint32_t)nrows);
+  EH_HIP(hipGetLastError());
+  return EH_OK;
+}
+
 }  // namespace
 
 eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, const uint32_t *small_rows,
                        uint64_t n_small_rows, const uint4 *tasks_big, uint64_t n_big, const uint4 *tasks_small,
                        uint64_t n_small, uint32_t lds_small, uint32_t lds_big, uint64_t *d_counts, uint32_t *d_next,
                        uint64_t *d_task_cycles, const uint4 *btasks, uint64_t n_btasks, const uint2 *in_pairs,
-                       uint32_t bstat_j, const uint32_t *jstart, const uint4 *lo16) {
+                       uint32_t bstat_j, const uint32_t *jstart, const uint4 *lo16, const uint32_t *local_rows0,
+                       uint64_t n_local0, const uint32_t *local_rows1, uint64_t n_local1) {
   CountArgs a;
   a.grab = tp.grab ? tp.grab : 1u;
   a.jstart = jstart;
@@ -1947,6 +2140,8 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
   }
   if (k == 4) {
     EH_CHECK((launch_small<4, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2)));
+    EH_CHECK((launch_local<LOCAL4_MID, false>(m->ctx, a, local_rows0, n_local0, d_next + 3)));
+    EH_CHECK((launch_local<LOCAL4_MAX, true>(m->ctx, a, local_rows1, n_local1, d_next + 4)));
     EH_CHECK((launch_count<4, false>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0)));
     EH_CHECK((launch_count<4, false>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,
                                      n_big)));
@@ -1971,7 +2166,8 @@ eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, 
   return launch_parts(m, k, sched, tp, s.small_rows.as<uint32_t>(), s.n_small_rows, t, s.n_big, t + s.n_big,
                       s.n_small, s.lds_small, s.lds_big, d_counts, d_next, d_task_cycles, s.btasks.as<uint4>(),
                       s.n_btasks, s.in_pairs.as<uint2>(), s.bstat_j,
-                      s.bstat_j ? s.jstart.as<uint32_t>() : nullptr, s.bstat_j ? s.lo16.as<uint4>() : nullptr);
+                      s.bstat_j ? s.jstart.as<uint32_t>() : nullptr, s.bstat_j ? s.lo16.as<uint4>() : nullptr,
+                      s.local_rows[0].as<uint32_t>(), s.n_local[0], s.local_rows[1].as<uint32_t>(), s.n_local[1]);
 }
 
 // k: 3 triangles, 4 four-cliques, -3 weighted triangle pass (sum of card(x) over
@@ -1988,11 +2184,13 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
   const int skey = kk * 8 + sched;
   double sched_ms = 0;
   if (!s.valid || s.task_work != tp.task_work || s.lds_small != tp.lds_small || s.lds_big != tp.lds_big ||
-      (int)s.sched != skey || s.small_deg != tp.small_deg || s.bstat_req != tp.bstat_j) {
+      (int)s.sched != skey || s.small_deg != tp.small_deg || s.bstat_req != tp.bstat_j ||
+      s.local_req != tp.local_max) {
     s = Schedule();
     EH_CHECK(schedule_build(m, sched, kk, tp, s));
     s.sched = skey;
     s.bstat_req = tp.bstat_j;
+    s.local_req = tp.local_max;
     sched_ms = s.ms;
   }
   DevBuf own_counts, next;
@@ -2001,7 +2199,7 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
     EH_CHECK(own_counts.alloc(8ull * (ncounts ? ncounts : 1)));
     dc = own_counts.as<unsigned long long>();
   }
-  EH_CHECK(next.alloc(16));
+  EH_CHECK(next.alloc(32));
   const int repeat = (opts && opts->repeat > 1) ? opts->repeat : 1;
   float total_ms = 0;
   for (int r = 0; r < repeat; r++) {

@@ -110,6 +110,26 @@ def test_b_stationary_pairs(ctx, facebook, split):
     assert M.triangle_count("tuned", bstat=split)[0] == 6 * FB_TRIANGLES
 
 
+@pytest.mark.parametrize("local_max", [-1, 64, 128, 200, 512])
+def test_local_graph_cliques(ctx, facebook, local_max):
+    """4-cliques of rows above small_degree counted as triangles of each row's local graph
+    (LDS bit-rows built from streamed B rows, one wave per row up to 128 entries, one block
+    per row up to 512): any bound and threshold gives the oracle's count."""
+    M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "lower", "hybrid")
+    for small in (-1, 8, 32):
+        assert M.clique_count(4, "tuned", local_max=local_max, small_degree=small)[0] == FB_4CLIQUES, small
+    # a dense graph: rows of 100-300 entries and many local edges per row
+    g = ctx.generate_er(1500, 150000, 7)
+    off, nbr = g.download()
+    exp = O.Matrix(1500, off, nbr, "hybrid").cliques(4, 8)[0]
+    for pol in ("hybrid", "uint", "pshort"):
+        M = g.build_matrix("lower", pol)
+        for small in (-1, 16):
+            assert M.clique_count(4, "tuned", local_max=local_max, small_degree=small)[0] == exp, (pol, small)
+    # triangle passes ignore the option
+    assert M.triangle_count("tuned", local_max=local_max)[0] == O.Matrix(1500, off, nbr, "hybrid").triangles(8)[0]
+
+
 def test_unoriented_matrix(ctx, facebook):
     """EH_FILTER_ALL keeps both directions: every triangle is counted 6 times
     (sum over ordered (i,j) of |N(i) & N(j)|), and the small-row path stays off."""
