@@ -590,7 +590,10 @@ __device__ uint64_t task_cliques4(const CountArgs &a, const AState &A, uint32_t 
 // pshort B rows are walked partition by partition, one pair after another.
 
 constexpr uint32_t PASS_PROBE = 0, PASS_AND = 1, PASS_STREAM = 2, PASS_BLOCK = 3, PASS_NTSTREAM = 4,
-                   PASS_STREAM16 = 5;
+                   PASS_STREAM16 = 5, PASS_STREAMH = 6;
+// hashed B-stationary: longest B row staged, rows per bin, table words per bin (plus one table)
+constexpr uint32_t BSH_MAX_D = 256, BSH_GROUP = 16, BSH_TABLE_WORDS = 1024;
+constexpr uint32_t BSH_MAX_TABLE = 512;  // 2^hash_lg(BSH_MAX_D)
 constexpr uint32_t META_WORDS = 260;
 constexpr uint32_t SMALL_META_WORDS = 260;
 
@@ -654,6 +657,28 @@ __device__ __forceinline__ uint32_t bitmap16_hits(const uint32_t *bm, const uint
   return h;
 }
 
+// four ids of a streamed prefix vector against one B row's LDS hash table (`meta`: table
+// offset | log2 size << 16); ids at or above the row's id `bound` (the prefix tail, the
+// padding) cannot be in it
+__device__ __forceinline__ uint32_t hash_hits(const uint32_t *ht, uint32_t meta, uint32_t bound, const uint4 x) {
+  const uint32_t *T = ht + (meta & 0xFFFFu);
+  const uint32_t lg = meta >> 16, mask = (1u << lg) - 1u;
+  uint32_t h = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const uint32_t e = q == 0 ? x.x : q == 1 ? x.y : q == 2 ? x.z : x.w;
+    if (e >= bound) continue;
+    uint32_t s = (e * 2654435761u) >> (32u - lg);
+    for (;;) {
+      const uint32_t v = T[s];
+      if (v == e) { h++; break; }
+      if (v == HASH_EMPTY) break;
+      s = (s + 1u) & mask;
+    }
+  }
+  return h;
+}
+
 template <int PASS, uint32_t KIND, bool W, bool LOCKSTEP = false>
 __device__ __forceinline__ PassAcc<W> pass_elem(const CountArgs &a, const AState &S, uint32_t v0, uint32_t v1,
                                                 uint32_t v2, uint32_t r) {
@@ -682,6 +707,8 @@ __device__ __forceinline__ PassAcc<W> pass_elem(const CountArgs &a, const AState
     return (PassAcc<W>)block_and<W>(S, a, v0 + r);
   } else if constexpr (PASS == PASS_STREAM16) {
     return (PassAcc<W>)bitmap16_hits(S.bm, a.lo16[(uint64_t)v1 + r]);
+  } else if constexpr (PASS == PASS_STREAMH) {
+    return (PassAcc<W>)hash_hits(S.ht, v0, v2, reinterpret_cast<const uint4 *>(a.unbr)[(uint64_t)v1 + r]);
   } else {
     const uint32_t *base = (PASS == PASS_NTSTREAM) ? a.ntu : a.unbr;
     return stream_hits<KIND, W, LOCKSTEP>(S, a, reinterpret_cast<const uint4 *>(base)[(uint64_t)v1 + r]);
@@ -710,7 +737,8 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
   }
   // remainders start after the whole rounds: PROBE/AND/BLOCK index by v0, the streams by v1
   c -= full;
-  if constexpr (PASS == PASS_STREAM || PASS == PASS_NTSTREAM || PASS == PASS_STREAM16) p1 += full;
+  if constexpr (PASS == PASS_STREAM || PASS == PASS_NTSTREAM || PASS == PASS_STREAM16 || PASS == PASS_STREAMH)
+    p1 += full;
   else p0 += full;
   if (__ballot(c != 0u) == 0ull) return acc;
   uint32_t tot;
@@ -723,12 +751,13 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
   m2[lane] = p2;
   if (lane == 0) mbase[64] = tot;
   wave_sync();
-  if constexpr ((PASS == PASS_STREAM || PASS == PASS_STREAM16) && UNROLL > 1) {
+  if constexpr ((PASS == PASS_STREAM || PASS == PASS_STREAM16 || PASS == PASS_STREAMH) && UNROLL > 1) {
     // UNROLL items per lane with their vector loads issued before any is tested; an item
     // past the end re-reads the last one (always a valid address) and is not counted
     const uint4 *vb = (PASS == PASS_STREAM16) ? a.lo16 : reinterpret_cast<const uint4 *>(a.unbr);
     for (uint32_t t0 = lane; t0 < tot; t0 += 64u * UNROLL) {
       uint4 v[UNROLL];
+      uint32_t hm[UNROLL], hb[UNROLL];
 #pragma unroll
       for (int u = 0; u < UNROLL; u++) {
         const uint32_t t = min(t0 + 64u * u, tot - 1u);
@@ -737,11 +766,14 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
         for (uint32_t step = 32; step; step >>= 1)
           if (mbase[q + step] <= t) q += step;
         v[u] = vb[(uint64_t)m1[q] + (t - mbase[q])];
+        hm[u] = m0[q];
+        hb[u] = m2[q];
       }
 #pragma unroll
       for (int u = 0; u < UNROLL; u++) {
         if (t0 + 64u * u < tot) {
           if constexpr (PASS == PASS_STREAM16) acc += (PassAcc<W>)bitmap16_hits(S.bm, v[u]);
+          else if constexpr (PASS == PASS_STREAMH) acc += (PassAcc<W>)hash_hits(S.ht, hm[u], hb[u], v[u]);
           else acc += stream_hits<KIND, W, true>(S, a, v[u]);
         }
       }
@@ -1478,6 +1510,109 @@ __global__ void __launch_bounds__(256) k_local4(CountArgs a, const uint32_t *__r
   if (lane == 0 && sum) atomicAdd(&a.counts[0], (unsigned long long)sum);
 }
 
+// Hashed B-stationary pairs: the pairs (i, j) with j at or above the split, for rows i whose
+// B rows there all have at most BSH_MAX_D entries.  B rows are grouped: a bin holds up to
+// BSH_GROUP consecutive such rows whose LDS hash tables (2^hash_lg(|row(j)|) slots each) fit
+// in BSH_TABLE_WORDS plus one table.  A task {bin, b, e} stages every table of its bin and
+// streams the prefixes row(i)[0, p) of the in-pairs [b, e) of the bin's rows (listed in row
+// order, so a pair's B row follows from its index), 16-byte vectors of the canonical list
+// tested in the pair's table (ids at or above j cannot be in row(j)).  Grouping the B rows
+// amortises the chain of dependent loads that staging one short row per task would cost.
+__global__ void __launch_bounds__(256) k_bshg(CountArgs a, const uint4 *__restrict__ htasks, uint64_t nht,
+                                              const uint2 *__restrict__ hpairs, const uint32_t *__restrict__ hj,
+                                              const uint64_t *__restrict__ hoff, const uint32_t *__restrict__ bstart) {
+  extern __shared__ uint32_t lds_all[];
+  const uint32_t lane = lane_id();
+  uint32_t *meta = lds_all + (threadIdx.x >> 6) * a.lds_words;
+  uint32_t *gj = meta + META_WORDS, *gtab = gj + BSH_GROUP, *gerow = gtab + BSH_GROUP,
+           *gebase = gerow + BSH_GROUP, *gpbase = gebase + BSH_GROUP + 1;
+  uint32_t *tables = gpbase + BSH_GROUP + 1;
+  AState S;
+  S.ht = tables;
+  S.bm = nullptr;
+  S.list = nullptr;
+  S.n = S.lo = S.span = S.aw0 = S.anw = S.base32 = S.P = S.hlg = 0;
+  S.dpre = S.dstart = nullptr;
+  uint64_t acc = 0;
+  uint64_t t = 0, tend = 0;
+  for (;;) {
+    if (t >= tend) {
+      uint32_t t0 = 0;
+      if (lane == 0) t0 = atomicAdd(a.next, a.grab);
+      t = rfl(t0);
+      if (t >= nht) break;
+      tend = min(t + a.grab, nht);
+    }
+    const uint4 task = htasks[t++];
+    const uint32_t bin = rfl(task.x), pb = rfl(task.y), pe = rfl(task.z);
+    const uint32_t h0 = rfl(bstart[bin]), nj = rfl(bstart[bin + 1]) - h0;
+    const uint64_t p0 = hoff[h0];
+    // the bin's rows: tables and element ranges
+    uint32_t j = 0, d = 0, sz = 0, lg = 0, eoff = 0, poff = 0;
+    if (lane < nj) {
+      j = hj[h0 + lane];
+      d = a.card[j];
+      eoff = a.aoff[j];
+      lg = 4;
+      while ((1u << lg) < 2u * d) lg++;
+      sz = 1u << lg;
+      poff = (uint32_t)(hoff[h0 + lane] - p0);
+    }
+    uint32_t tsz, tde;
+    const uint32_t tb = wave_excl_scan(sz, lane, &tsz);
+    const uint32_t eb = wave_excl_scan(d, lane, &tde);
+    tsz = rfl(tsz);
+    tde = rfl(tde);
+    if (lane < nj) {
+      gj[lane] = j;
+      gtab[lane] = tb | (lg << 16);
+      gerow[lane] = eoff;
+      gebase[lane] = eb;
+      gpbase[lane] = poff;
+    }
+    if (lane == 0) {
+      gebase[nj] = tde;
+      gpbase[nj] = (uint32_t)(hoff[h0 + nj] - p0);
+    }
+    for (uint32_t w = lane; w < tsz; w += 64) tables[w] = HASH_EMPTY;
+    wave_sync();
+    // insert every element of the bin's rows into its row's table
+    for (uint32_t e = lane; e < tde; e += 64) {
+      uint32_t k = 0;
+#pragma unroll
+      for (uint32_t step = BSH_GROUP / 2; step; step >>= 1)
+        if (k + step < nj && gebase[k + step] <= e) k += step;
+      const uint32_t x = a.unbr[(uint64_t)gerow[k] * 4 + (e - gebase[k])];
+      const uint32_t m = gtab[k];
+      uint32_t *T = tables + (m & 0xFFFFu);
+      const uint32_t tl = m >> 16, mask = (1u << tl) - 1u;
+      uint32_t hsh = (x * 2654435761u) >> (32u - tl);
+      while (atomicCAS(&T[hsh], HASH_EMPTY, x) != HASH_EMPTY) hsh = (hsh + 1u) & mask;
+    }
+    wave_sync();
+    for (uint32_t c0 = pb; c0 < pe; c0 += 64) {
+      uint32_t nv = 0, off = 0, tm = 0, bound = 0;
+      const uint32_t pi = c0 + lane;
+      if (pi < pe) {
+        const uint32_t rel = (uint32_t)(pi - p0);
+        uint32_t k = 0;
+#pragma unroll
+        for (uint32_t step = BSH_GROUP / 2; step; step >>= 1)
+          if (k + step < nj && gpbase[k + step] <= rel) k += step;
+        const uint2 pr = hpairs[pi];
+        off = pr.x;
+        nv = (pr.y + 3u) >> 2;
+        tm = gtab[k];
+        bound = gj[k];
+      }
+      acc += flat_pass<PASS_STREAMH, KIND_HASH, false, 4>(a, S, nv, tm, off, bound, lane, meta);
+    }
+    wave_sync();
+  }
+  const uint64_t sum = wave_sum_u64(acc);
+  if (lane == 0 && sum) atomicAdd(&a.counts[0], (unsigned long long)sum);
+}
+
 template <int K, bool W, bool TUNED, bool BLK>
 __global__ void __launch_bounds__(256) k_count(CountArgs a) {
   extern __shared__ uint32_t lds_all[];
@@ -1540,8 +1675,12 @@ struct SchedArgs {
   uint32_t small_deg;     // rows with 1..small_deg entries go to the edge-centric kernel
   uint32_t local_max;     // 4-cliques: rows above small_deg and up to this go to k_local4
   uint32_t bstat_j;       // task rows hand their pairs with j < bstat_j to the B-stationary pass
+  uint32_t bsh_max;       // rows whose B rows at or above bstat_j all have at most this many
+                          // entries hand those pairs to the hashed B-stationary pass (0: off)
   uint32_t *incount;      // out (bstat): task rows containing j, per j < bstat_j
+  uint32_t *hcount;       // out (bsh): such rows containing j, per j >= bstat_j (index j - bstat_j)
   uint32_t *jstart;       // out: first entry of row i its A-side tasks cover
+  uint32_t *jbelow;       // out: entries of row i below bstat_j
   uint32_t *is_small;     // out: 1 for those rows, 2 / 3 for the k_local4 classes
   uint64_t *sortkey;      // out: (class << 62) | (order << 32) | i
   uint32_t *ntask;        // out: tasks of row i
@@ -1558,7 +1697,8 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
   for (uint32_t i = wid; i < s.n; i += nw) {
     const uint32_t dA = s.card[i];
     uint64_t work = 0;
-    uint32_t lo = 0, hi = 0, below = 0;
+    uint32_t lo = 0, hi = 0, below = 0, bad = 0;
+    bool full = false;
     // with the B-stationary pass on, every row's pairs with j < bstat_j (a prefix of the
     // sorted row) leave its A-side tasks or its edge-centric wedges
     if (dA) {
@@ -1582,14 +1722,27 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
           if (d.card && k > 0) atomicAdd(&s.incount[jv], 1u);
           continue;
         }
+        if (d.card > s.bsh_max) bad = 1u;
         work += units + 16u;
       }
       work = wave_sum_u64(work);
       below = (uint32_t)wave_sum_u64(below);
       loc[4] += (lane == 0) ? work - 16ull * (dA - below) : 0ull;
+      // every pair at or above the split goes hashed B-stationary when no B row there is longer
+      // than bsh_max: the row then has no A-side or edge-centric work left
+      full = s.bstat_j && s.bsh_max && __ballot(bad != 0u) == 0ull;
+      if (full && below < dA) {
+        for (uint32_t k = max(below, 1u) + lane; k < dA; k += 64) {
+          const uint32_t jv = G[k];
+          if (s.desc[jv].card) atomicAdd(&s.hcount[jv - s.bstat_j], 1u);
+        }
+      }
     }
     if (lane != 0) continue;
-    s.jstart[i] = below;
+    const uint32_t js = full ? dA : below;
+    s.jstart[i] = js;
+    s.jbelow[i] = below;
+    below = js;  // from here on: the entries before the A-side / edge-centric part
     const bool small = dA > 0 && dA <= s.small_deg && below < dA;
     const bool local = !small && dA > 2 && dA <= s.local_max;
     s.is_small[i] = small ? 1u : local ? (dA <= LOCAL4_MID ? 2u : 3u) : 0u;
@@ -1704,6 +1857,81 @@ __global__ void __launch_bounds__(256) k_bstat_fill(const uint32_t *card, const 
   }
 }
 
+// ---- hashed B-stationary schedule: B rows with in-pairs, their bins, in-pairs, tasks
+__global__ void k_nonzero_flag(const uint32_t *c, uint32_t n, uint32_t *flag) {
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+    flag[q] = c[q] ? 1u : 0u;
+}
+
+// compacted B rows (in row order), their pair counts and table-word weights
+__global__ void k_bsh_rows(const uint32_t *hcount, const uint64_t *hpos, uint32_t nh, uint32_t split,
+                           const uint32_t *card, uint32_t *hj, uint32_t *hc, uint32_t *hw) {
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += gridDim.x * blockDim.x) {
+    if (!hcount[q]) continue;
+    const uint64_t h = hpos[q];
+    const uint32_t j = split + q, d = card[j];
+    uint32_t lg = 4;
+    while ((1u << lg) < 2u * d) lg++;
+    hj[h] = j;
+    hc[h] = hcount[q];
+    // a bin spans BSH_TABLE_WORDS of these weights: at most BSH_GROUP - 3 rows each
+    hw[h] = (1u << lg) + BSH_TABLE_WORDS / BSH_GROUP;
+  }
+}
+
+__global__ void k_bsh_bin_flag(const uint64_t *pw, uint32_t nhr, uint32_t *flag) {
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nhr; h += gridDim.x * blockDim.x)
+    flag[h] = (h == 0 || pw[h] / BSH_TABLE_WORDS != pw[h - 1] / BSH_TABLE_WORDS) ? 1u : 0u;
+}
+
+__global__ void k_bsh_bin_start(const uint32_t *flag, const uint64_t *bpos, uint32_t nhr, uint32_t *bstart) {
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nhr; h += gridDim.x * blockDim.x)
+    if (flag[h]) bstart[bpos[h]] = h;
+}
+
+// every row whose pairs at or above the split all go hashed B-stationary (jstart == card)
+// writes (its offset / 16 bytes, position) into the list of each non-empty B row there
+__global__ void __launch_bounds__(256) k_bsh_fill(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr,
+                                                  const RowDesc *desc, const uint32_t *jstart,
+                                                  const uint32_t *jbelow, uint32_t n, uint32_t split,
+                                                  const uint64_t *hpos, const uint64_t *hoff, uint32_t *hfill,
+                                                  uint2 *hpairs) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t i = wid; i < n; i += nw) {
+    const uint32_t d = card[i], jb = jbelow[i];
+    if (jstart[i] != d || jb >= d) continue;
+    const uint32_t *G = unbr + (uint64_t)aoff[i] * 4;
+    for (uint32_t k = max(jb, 1u) + lane; k < d; k += 64) {
+      const uint32_t j = G[k];
+      if (desc[j].card == 0) continue;
+      const uint64_t h = hpos[j - split];
+      const uint32_t slot = atomicAdd(&hfill[h], 1u);
+      hpairs[hoff[h] + slot] = make_uint2(aoff[i], k);
+    }
+  }
+}
+
+__global__ void k_bsh_bin_ntask(const uint32_t *bstart, const uint64_t *hoff, uint32_t nb, uint32_t chunk,
+                                uint32_t *ntb) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+    const uint64_t np = hoff[bstart[b + 1]] - hoff[bstart[b]];
+    ntb[b] = (uint32_t)((np + chunk - 1) / chunk);
+  }
+}
+
+__global__ void k_bsh_bin_expand(const uint32_t *bstart, const uint64_t *hoff, const uint64_t *boff, uint32_t nb,
+                                 uint32_t chunk, uint4 *htasks) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+    const uint64_t p0 = hoff[bstart[b]], p1 = hoff[bstart[b + 1]], t0 = boff[b];
+    for (uint64_t q = 0; p0 + q * chunk < p1; q++) {
+      const uint32_t pb = (uint32_t)(p0 + q * chunk), pe = (uint32_t)min(p1, p0 + (q + 1) * chunk);
+      htasks[t0 + q] = make_uint4(b, pb, pe, 0u);
+    }
+  }
+}
+
 // 16-bit copies of the B-stationary prefixes: row i's first jstart[i] entries (all below
 // the split) in ceil(jstart / 8) 16-byte vectors, the tail padded with split - 1
 __global__ void k_lo16_count(const uint32_t *jstart, uint32_t n, uint32_t *nvec) {
@@ -1788,6 +2016,7 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
   // B rows below 64Ki ids B-stationary (schedule_build applies it to tuned triangle passes
   // over oriented single-graph matrices only)
   tp.bstat_j = (sched == EH_SCHED_TUNED && k == 3) ? 65536u : 0u;
+  tp.bsh_max = tp.bstat_j ? BSH_MAX_D : 0u;
   // triangle tasks are many and light: two per queue atomic; 4-clique tasks are few and
   // heavy, where pulling two at once unbalances the end of the pass
   tp.grab = (k == 4) ? 1u : 2u;
@@ -1801,7 +2030,7 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
     if (opts->lanes_per_row > 0) tp.lanes = opts->lanes_per_row;
     if (opts->stream_ratio < 0) tp.stream_ratio = 0;
     else if (opts->stream_ratio > 0) tp.stream_ratio = (uint32_t)opts->stream_ratio;
-    if (opts->bstat_split < 0) tp.bstat_j = 0;
+    if (opts->bstat_split < 0) tp.bstat_j = tp.bsh_max = 0;
     if (opts->local_max < 0) tp.local_max = 0;
     else if (opts->local_max > 0 && tp.local_max)
       tp.local_max = (uint32_t)std::min<int32_t>(opts->local_max, (int32_t)LOCAL4_MAX);
@@ -1830,9 +2059,18 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   const uint32_t bstat_j = (sched == EH_SCHED_TUNED && k == 3 && m->filter == EH_FILTER_LOWER &&
                             !m->dgraph_of.p) ? std::min(tp.bstat_j, n) : 0u;
   EH_CHECK(jstart.alloc(4ull * n + 8));
+  DevBuf jbelow, hcount;
+  EH_CHECK(jbelow.alloc(4ull * n + 8));
+  // rows whose B rows at or above the split are all short go wholly B-stationary (hashed)
+  const uint32_t bsh_max = (bstat_j && bstat_j < n) ? tp.bsh_max : 0u;
+  const uint32_t nh = bsh_max ? n - bstat_j : 0u;
   if (bstat_j) {
     EH_CHECK(incount.alloc(4ull * bstat_j + 8));
     k_zero_u32<<<grid_for(bstat_j), 256, 0, st>>>(incount.as<uint32_t>(), bstat_j);
+  }
+  if (nh) {
+    EH_CHECK(hcount.alloc(4ull * nh + 8));
+    k_zero_u32<<<grid_for((uint64_t)nh + 1), 256, 0, st>>>(hcount.as<uint32_t>(), nh + 1);
   }
   EH_CHECK(is_small.alloc(4ull * n + 8));
   EH_CHECK(spos.alloc(8ull * n + 16));
@@ -1862,8 +2100,11 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   const uint32_t local_max = (sched == EH_SCHED_TUNED && k == 4) ? tp.local_max : 0u;
   sa.local_max = local_max;
   sa.bstat_j = bstat_j;
+  sa.bsh_max = bsh_max;
   sa.incount = bstat_j ? incount.as<uint32_t>() : nullptr;
+  sa.hcount = nh ? hcount.as<uint32_t>() : nullptr;
   sa.jstart = jstart.as<uint32_t>();
+  sa.jbelow = jbelow.as<uint32_t>();
   sa.is_small = is_small.as<uint32_t>();
   sa.sortkey = key.as<uint64_t>();
   sa.ntask = ntask.as<uint32_t>();
@@ -1949,7 +2190,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     DevBuf nvec, lo16off;
     EH_CHECK(nvec.alloc(4ull * n + 8));
     EH_CHECK(lo16off.alloc(8ull * n + 16));
-    k_lo16_count<<<grid_for(n), 256, 0, st>>>(jstart.as<uint32_t>(), n, nvec.as<uint32_t>());
+    k_lo16_count<<<grid_for(n), 256, 0, st>>>(jbelow.as<uint32_t>(), n, nvec.as<uint32_t>());
     k_zero_u32<<<1, 64, 0, st>>>(nvec.as<uint32_t>() + n, 1);
     EH_CHECK(prim_exscan_u32_to_u64(ctx, nvec.as<uint32_t>(), lo16off.as<uint64_t>(), (uint64_t)n + 1));
     uint64_t nv16 = 0;
@@ -1961,11 +2202,11 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     }
     EH_CHECK(s.lo16.alloc(16 * nv16 + 16));
     k_lo16_fill<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(m->daoff.as<uint32_t>(), m->dunbr.as<uint32_t>(),
-                                                          jstart.as<uint32_t>(), n, lo16off.as<uint64_t>(),
+                                                          jbelow.as<uint32_t>(), n, lo16off.as<uint64_t>(),
                                                           bstat_j - 1u, s.lo16.as<uint16_t>());
     k_bstat_fill<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(
         m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(), m->dunbr.as<uint32_t>(), m->ddesc.as<RowDesc>(),
-        jstart.as<uint32_t>(), n, inoff.as<uint64_t>(), infill.as<uint32_t>(), lo16off.as<uint64_t>(),
+        jbelow.as<uint32_t>(), n, inoff.as<uint64_t>(), infill.as<uint32_t>(), lo16off.as<uint64_t>(),
         s.in_pairs.as<uint2>());
     k_bstat_ntask<<<grid_for(bstat_j + 1), 256, 0, st>>>(incount.as<uint32_t>(), bstat_j + 1, BSTAT_CHUNK,
                                                         ntb.as<uint32_t>());
@@ -1981,6 +2222,75 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     s.bstat_j = bstat_j;
     s.n_btasks = nbt;
     s.n_in_pairs = npairs;
+  }
+  s.n_htasks = 0;
+  if (nh) {
+    // hashed part: the B rows at or above the split with in-pairs (compacted, row order),
+    // bins of them by table words, the in-pairs of each, tasks of at most BSH_CHUNK pairs
+    constexpr uint32_t BSH_CHUNK = 256;
+    DevBuf hflag, hpos, hc, hw, pw, bflag, bpos, hfill, ntb, boff;
+    EH_CHECK(hflag.alloc(4ull * nh + 8));
+    EH_CHECK(hpos.alloc(8ull * nh + 16));
+    k_nonzero_flag<<<grid_for(nh), 256, 0, st>>>(hcount.as<uint32_t>(), nh, hflag.as<uint32_t>());
+    k_zero_u32<<<1, 64, 0, st>>>(hflag.as<uint32_t>() + nh, 1);
+    EH_CHECK(prim_exscan_u32_to_u64(ctx, hflag.as<uint32_t>(), hpos.as<uint64_t>(), (uint64_t)nh + 1));
+    uint64_t nhr = 0;
+    EH_HIP(hipMemcpyAsync(&nhr, hpos.as<uint64_t>() + nh, 8, hipMemcpyDeviceToHost, st));
+    EH_HIP(hipStreamSynchronize(st));
+    if (nhr) {
+      EH_CHECK(s.hj.alloc(4 * nhr + 16));
+      EH_CHECK(hc.alloc(4 * nhr + 16));
+      EH_CHECK(hw.alloc(4 * nhr + 16));
+      EH_CHECK(s.hoff.alloc(8 * nhr + 16));
+      EH_CHECK(pw.alloc(8 * nhr + 16));
+      EH_CHECK(bflag.alloc(4 * nhr + 16));
+      EH_CHECK(bpos.alloc(8 * nhr + 16));
+      EH_CHECK(hfill.alloc(4 * nhr + 16));
+      k_bsh_rows<<<grid_for(nh), 256, 0, st>>>(hcount.as<uint32_t>(), hpos.as<uint64_t>(), nh, bstat_j,
+                                               m->dcard.as<uint32_t>(), s.hj.as<uint32_t>(), hc.as<uint32_t>(),
+                                               hw.as<uint32_t>());
+      k_zero_u32<<<1, 64, 0, st>>>(hc.as<uint32_t>() + nhr, 1);
+      EH_CHECK(prim_exscan_u32_to_u64(ctx, hc.as<uint32_t>(), s.hoff.as<uint64_t>(), nhr + 1));
+      EH_CHECK(prim_exscan_u32_to_u64(ctx, hw.as<uint32_t>(), pw.as<uint64_t>(), nhr));
+      k_bsh_bin_flag<<<grid_for(nhr), 256, 0, st>>>(pw.as<uint64_t>(), (uint32_t)nhr, bflag.as<uint32_t>());
+      k_zero_u32<<<1, 64, 0, st>>>(bflag.as<uint32_t>() + nhr, 1);
+      EH_CHECK(prim_exscan_u32_to_u64(ctx, bflag.as<uint32_t>(), bpos.as<uint64_t>(), nhr + 1));
+      uint64_t nb = 0, nhp = 0;
+      EH_HIP(hipMemcpyAsync(&nb, bpos.as<uint64_t>() + nhr, 8, hipMemcpyDeviceToHost, st));
+      EH_HIP(hipMemcpyAsync(&nhp, s.hoff.as<uint64_t>() + nhr, 8, hipMemcpyDeviceToHost, st));
+      EH_HIP(hipStreamSynchronize(st));
+      if (nhp >= (1ull << 32)) {
+        eh_set_error("%llu hashed B-stationary pairs exceed the 32-bit pair index", (unsigned long long)nhp);
+        return EH_ERR_UNSUPPORTED;
+      }
+      EH_CHECK(s.bstart.alloc(4 * nb + 16));
+      k_bsh_bin_start<<<grid_for(nhr), 256, 0, st>>>(bflag.as<uint32_t>(), bpos.as<uint64_t>(), (uint32_t)nhr,
+                                                     s.bstart.as<uint32_t>());
+      const uint32_t last = (uint32_t)nhr;
+      EH_HIP(hipStreamSynchronize(st));
+      EH_HIP(hipMemcpy(s.bstart.as<uint32_t>() + nb, &last, 4, hipMemcpyHostToDevice));
+      EH_CHECK(s.hpairs.alloc(8 * nhp + 16));
+      k_zero_u32<<<grid_for(nhr + 1), 256, 0, st>>>(hfill.as<uint32_t>(), (uint32_t)(nhr + 1));
+      k_bsh_fill<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(
+          m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(), m->dunbr.as<uint32_t>(), m->ddesc.as<RowDesc>(),
+          jstart.as<uint32_t>(), jbelow.as<uint32_t>(), n, bstat_j, hpos.as<uint64_t>(), s.hoff.as<uint64_t>(),
+          hfill.as<uint32_t>(), s.hpairs.as<uint2>());
+      EH_CHECK(ntb.alloc(4 * nb + 16));
+      EH_CHECK(boff.alloc(8 * nb + 16));
+      k_bsh_bin_ntask<<<grid_for(nb), 256, 0, st>>>(s.bstart.as<uint32_t>(), s.hoff.as<uint64_t>(), (uint32_t)nb,
+                                                    BSH_CHUNK, ntb.as<uint32_t>());
+      k_zero_u32<<<1, 64, 0, st>>>(ntb.as<uint32_t>() + nb, 1);
+      EH_CHECK(prim_exscan_u32_to_u64(ctx, ntb.as<uint32_t>(), boff.as<uint64_t>(), nb + 1));
+      uint64_t nht = 0;
+      EH_HIP(hipMemcpyAsync(&nht, boff.as<uint64_t>() + nb, 8, hipMemcpyDeviceToHost, st));
+      EH_HIP(hipStreamSynchronize(st));
+      EH_CHECK(s.htasks.alloc(16 * nht + 16));
+      k_bsh_bin_expand<<<grid_for(nb), 256, 0, st>>>(s.bstart.as<uint32_t>(), s.hoff.as<uint64_t>(),
+                                                     boff.as<uint64_t>(), (uint32_t)nb, BSH_CHUNK,
+                                                     s.htasks.as<uint4>());
+      EH_HIP(hipGetLastError());
+      s.n_htasks = nht;
+    }
   }
   EH_HIP(hipEventRecord(ctx->ev1, st));
   EH_HIP(hipEventSynchronize(ctx->ev1));
@@ -2105,6 +2415,27 @@ eh_status launch_local(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows,
   return EH_OK;
 }
 
+eh_status launch_bshg(eh_ctx *ctx, const CountArgs &base, const uint4 *htasks, uint64_t nht, const uint2 *hpairs,
+                      const uint32_t *hj, const uint64_t *hoff, const uint32_t *bstart, uint32_t *d_next) {
+  if (nht == 0) return EH_OK;
+  CountArgs a = base;
+  a.next = d_next;
+  // a wave's slice: the pass metadata, the bin's rows (j, table, element offset, element and
+  // pair bases) and its tables (at most BSH_TABLE_WORDS + one largest table)
+  a.lds_words = META_WORDS + 5 * BSH_GROUP + 2 + BSH_TABLE_WORDS + BSH_MAX_TABLE + BSH_TABLE_WORDS / BSH_GROUP;
+  const int waves = 4, threads = 256;
+  const size_t lds_bytes = (size_t)a.lds_words * 4 * waves;
+  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(k_bshg), threads, lds_bytes);
+  uint64_t blocks = (uint64_t)per_cu * (ctx->num_cus ? ctx->num_cus : 256);
+  const uint64_t need = ceil_div(ceil_div(nht, (uint64_t)a.grab), (uint64_t)waves);
+  if (blocks > need) blocks = need;
+  k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
+  hipLaunchKernelGGL(k_bshg, dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a, htasks, nht, hpairs,
+                     hj, hoff, bstart);
+  EH_HIP(hipGetLastError());
+  return EH_OK;
+}
+
 }  // namespace
 
 eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, const uint32_t *small_rows,
@@ -2112,7 +2443,7 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
                        uint64_t n_small, uint32_t lds_small, uint32_t lds_big, uint64_t *d_counts, uint32_t *d_next,
                        uint64_t *d_task_cycles, const uint4 *btasks, uint64_t n_btasks, const uint2 *in_pairs,
                        uint32_t bstat_j, const uint32_t *jstart, const uint4 *lo16, const uint32_t *local_rows0,
-                       uint64_t n_local0, const uint32_t *local_rows1, uint64_t n_local1) {
+                       uint64_t n_local0, const uint32_t *local_rows1, uint64_t n_local1, const Schedule *hs) {
   CountArgs a;
   a.grab = tp.grab ? tp.grab : 1u;
   a.jstart = jstart;
@@ -2134,7 +2465,7 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
   a.graph_of = m->dgraph_of.p ? m->dgraph_of.as<uint32_t>() : nullptr;
   a.stream_ratio = tp.stream_ratio;
   const bool weighted = k == -3;
-  if (n_btasks && k != 3) {
+  if ((n_btasks || (hs && hs->n_htasks)) && k != 3) {
     eh_set_error("B-stationary schedules serve the plain triangle count only");
     return EH_ERR_INVALID;
   }
@@ -2153,6 +2484,9 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
   } else {
     EH_CHECK((launch_small<3, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2)));
     EH_CHECK(launch_bstat(m->ctx, a, btasks, n_btasks, in_pairs, bstat_j, d_next + 3));
+    if (hs && hs->n_htasks)
+      EH_CHECK(launch_bshg(m->ctx, a, hs->htasks.as<uint4>(), hs->n_htasks, hs->hpairs.as<uint2>(),
+                           hs->hj.as<uint32_t>(), hs->hoff.as<uint64_t>(), hs->bstart.as<uint32_t>(), d_next + 4));
     EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0)));
     EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,
                                      n_big)));
@@ -2167,7 +2501,8 @@ eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, 
                       s.n_small, s.lds_small, s.lds_big, d_counts, d_next, d_task_cycles, s.btasks.as<uint4>(),
                       s.n_btasks, s.in_pairs.as<uint2>(), s.bstat_j,
                       s.bstat_j ? s.jstart.as<uint32_t>() : nullptr, s.bstat_j ? s.lo16.as<uint4>() : nullptr,
-                      s.local_rows[0].as<uint32_t>(), s.n_local[0], s.local_rows[1].as<uint32_t>(), s.n_local[1]);
+                      s.local_rows[0].as<uint32_t>(), s.n_local[0], s.local_rows[1].as<uint32_t>(), s.n_local[1],
+                      &s);
 }
 
 // k: 3 triangles, 4 four-cliques, -3 weighted triangle pass (sum of card(x) over

@@ -125,6 +125,8 @@ struct TuneParams {
   uint32_t small_deg = 0;  // rows up to this many entries use the edge-centric kernel
   uint32_t stream_ratio = 0;  // tuned: stream bitset B rows when 16 |B| < ratio * probes (0: never)
   uint32_t bstat_j = 0;       // tuned triangles: pairs with B row id below this run B-stationary (0: off)
+  uint32_t bsh_max = 0;       // and rows whose B rows above it have at most this many entries run
+                              // those pairs hashed B-stationary (0: off)
   uint32_t grab = 2;          // tasks a wave takes from the queue per atomic
   uint32_t local_max = 0;     // 4-cliques: rows above small_deg and up to this run k_local4 (0: off)
 };
@@ -169,6 +171,10 @@ struct Schedule {
   DevBuf in_pairs, btasks, jstart;  // jstart: per row, entries counted B-stationary
   DevBuf lo16;                      // those entries as 16-bit ids, 8 per 16-byte vector
   uint32_t local_req = 0;           // 4-cliques: the local_max the schedule was built for
+  // hashed B-stationary part: B rows at or above the split with in-pairs (row order), the
+  // offset of each one's in-pairs, the first row of each bin, the pairs and the tasks
+  DevBuf hj, hoff, bstart, hpairs, htasks;
+  uint64_t n_htasks = 0;
   DevBuf local_rows[2];             // rows of the two k_local4 classes (<= 128, <= 512 entries)
   uint64_t n_local[2] = {0, 0};
   uint64_t n_btasks = 0, n_in_pairs = 0;
@@ -234,7 +240,8 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
                        uint64_t *d_task_cycles, const uint4 *btasks = nullptr, uint64_t n_btasks = 0,
                        const uint2 *in_pairs = nullptr, uint32_t bstat_j = 0, const uint32_t *jstart = nullptr,
                        const uint4 *lo16 = nullptr, const uint32_t *local_rows0 = nullptr, uint64_t n_local0 = 0,
-                       const uint32_t *local_rows1 = nullptr, uint64_t n_local1 = 0);
+                       const uint32_t *local_rows1 = nullptr, uint64_t n_local1 = 0,
+                       const Schedule *hs = nullptr);
 
 // timing helper: device ms between two events recorded on ctx->stream
 double ctx_elapsed_ms(eh_ctx *ctx);
