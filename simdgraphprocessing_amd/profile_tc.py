@@ -26,7 +26,7 @@ import sys
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
 COPY_BYTES = 1 << 30
-COUNT_KERNELS = ("k_count", "k_small_rows", "k_bstat")
+COUNT_KERNELS = ("k_count", "k_small_rows", "k_bstat", "k_bshg")
 
 
 def run(a):
