@@ -594,7 +594,8 @@ constexpr uint32_t PASS_PROBE = 0, PASS_AND = 1, PASS_STREAM = 2, PASS_BLOCK = 3
 // hashed B-stationary: longest B row staged, rows per bin, table words per bin (plus one table)
 constexpr uint32_t BSH_MAX_D = 256, BSH_GROUP = 16, BSH_TABLE_WORDS = 1024;
 constexpr uint32_t BSH_MAX_TABLE = 512;  // 2^hash_lg(BSH_MAX_D)
-constexpr uint32_t META_WORDS = 260;
+// flat_pass metadata: four rank-ordered arrays of 64 words and 64 slot-block start masks
+constexpr uint32_t META_WORDS = 384;
 constexpr uint32_t SMALL_META_WORDS = 260;
 
 __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t *L, uint32_t n, uint32_t x) {
@@ -650,30 +651,54 @@ __device__ __forceinline__ uint32_t bitmap16_hits(const uint32_t *bm, const uint
   uint32_t h = 0;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
+    // id w & 0xFFFF: word bits 5..15, bit w & 31; id w >> 16: word w >> 21, bit (w >> 16) & 31
+    // (the bit-field extract reads only the low five bits of its offset)
     const uint32_t w = q == 0 ? x.x : q == 1 ? x.y : q == 2 ? x.z : x.w;
-    const uint32_t e0 = w & 0xFFFFu, e1 = w >> 16;
-    h += ((bm[e0 >> 5] >> (e0 & 31u)) & 1u) + ((bm[e1 >> 5] >> (e1 & 31u)) & 1u);
+    h += __builtin_amdgcn_ubfe(bm[__builtin_amdgcn_ubfe(w, 5u, 11u)], w, 1u) +
+         __builtin_amdgcn_ubfe(bm[w >> 21], w >> 16, 1u);
   }
   return h;
 }
 
 // four ids of a streamed prefix vector against one B row's LDS hash table (`meta`: table
-// offset | log2 size << 16); ids at or above the row's id `bound` (the prefix tail, the
-// padding) cannot be in it
+// offset | log2 size << 16).  The table is 2^lg words in buckets of two: an id hashes to a
+// bucket, an insert takes the first free slot from there on (slot 0 before slot 1), so a
+// lookup stops at the first bucket that holds the id or a free slot.  At most half the
+// slots are used, so few lookups go past their first bucket: the four first buckets are
+// read together without branching, and a masked loop follows the rest.  Ids at or above
+// the row's id `bound` (the prefix tail, the padding) cannot be in it.
+__device__ __forceinline__ uint32_t hash_bucket(uint32_t e, uint32_t lg) { return (e * 2654435761u) >> (33u - lg); }
+
 __device__ __forceinline__ uint32_t hash_hits(const uint32_t *ht, uint32_t meta, uint32_t bound, const uint4 x) {
   const uint32_t *T = ht + (meta & 0xFFFFu);
-  const uint32_t lg = meta >> 16, mask = (1u << lg) - 1u;
-  uint32_t h = 0;
+  const uint32_t lg = meta >> 16, bmask = (1u << (lg - 1u)) - 1u;
+  const uint32_t e[4] = {x.x, x.y, x.z, x.w};
+  uint32_t b[4], v0[4], v1[4];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    const uint32_t e = q == 0 ? x.x : q == 1 ? x.y : q == 2 ? x.z : x.w;
-    if (e >= bound) continue;
-    uint32_t s = (e * 2654435761u) >> (32u - lg);
-    for (;;) {
-      const uint32_t v = T[s];
-      if (v == e) { h++; break; }
-      if (v == HASH_EMPTY) break;
-      s = (s + 1u) & mask;
+    b[q] = hash_bucket(e[q], lg);
+    v0[q] = T[2u * b[q]];
+    v1[q] = T[2u * b[q] + 1u];
+  }
+  uint32_t h = 0, pend = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const bool in = e[q] < bound, hit = (v0[q] == e[q]) | (v1[q] == e[q]);
+    h += (uint32_t)(in & hit);
+    pend |= (uint32_t)(in & !hit & (v1[q] != HASH_EMPTY)) << q;
+  }
+  while (pend) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      b[q] = (pend >> q) & 1u ? (b[q] + 1u) & bmask : b[q];
+      v0[q] = T[2u * b[q]];
+      v1[q] = T[2u * b[q] + 1u];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const bool p = (pend >> q) & 1u, hit = (v0[q] == e[q]) | (v1[q] == e[q]);
+      h += (uint32_t)(p & hit);
+      if (p & (hit | (v1[q] == HASH_EMPTY))) pend &= ~(1u << q);
     }
   }
   return h;
@@ -715,11 +740,24 @@ __device__ __forceinline__ PassAcc<W> pass_elem(const CountArgs &a, const AState
   }
 }
 
+// Rank of the pair that owns slot t = 64 kb + lane of a flattened remainder: the pairs
+// with remainder elements are numbered in lane order, `smask` holds per block of 64
+// slots a 64-bit mask of the slots where one of them starts (two words), and `sb` is the
+// lane's own pair's first slot (all ones if it has none).  The owner is the last pair
+// starting at or before t: those starting before the block (a ballot, so every lane of
+// the wave must be active) plus the block's starts at lanes 0..lane.  For slots past
+// the last element it is the last pair.
+__device__ __forceinline__ uint32_t slot_owner(const uint32_t *smask, uint32_t kb, uint32_t sb, uint32_t le_lo,
+                                               uint32_t le_hi) {
+  const uint32_t before = (uint32_t)__popcll(__ballot(sb < 64u * kb));
+  return before + __popc(smask[2u * kb] & le_lo) + __popc(smask[2u * kb + 1u] & le_hi) - 1u;
+}
+
 // One pass over up to 64 pairs (one per lane, c elements each).  Whole rounds of 64
 // elements run pair by pair with the pair's metadata wave-uniform (read from its
-// lane into scalar registers); the remainders (< 64 per pair) are flattened: an
-// exclusive scan gives every remainder element a slot t, lane t % 64 takes it and
-// finds its pair by a 6-step search over the scanned bases.
+// lane into scalar registers); the remainders (< 64 per pair, so at most 63 blocks of
+// 64 slots) are flattened: an exclusive scan gives every remainder element a slot t,
+// lane t % 64 takes it and finds its pair with slot_owner.
 template <int PASS, uint32_t KIND, bool W, int UNROLL>
 __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &S, uint32_t c, uint32_t p0,
                                               uint32_t p1, uint32_t p2, uint32_t lane, uint32_t *meta) {
@@ -731,9 +769,30 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
     const uint32_t q = (uint32_t)__builtin_ctzll(big);
     big &= big - 1;
     const uint32_t fq = rdl(full, q), q0 = rdl(p0, q), q1 = rdl(p1, q), q2 = rdl(p2, q);
-    // fq is a multiple of 64: unrolled, each lane keeps several loads in flight
+    if constexpr ((PASS == PASS_STREAM || PASS == PASS_STREAM16 || PASS == PASS_STREAMH) && UNROLL > 1) {
+      // fq is a multiple of 64: UNROLL vector loads per lane issued before any test
+      const uint4 *vb = ((PASS == PASS_STREAM16) ? a.lo16 : reinterpret_cast<const uint4 *>(a.unbr)) + q1;
+      for (uint32_t r0 = 0; r0 < fq; r0 += 64u * UNROLL) {
+        // a block past fq is skipped (a wave-uniform branch)
+        uint4 v[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++)
+          if (r0 + 64u * u < fq) v[u] = vb[r0 + 64u * u + lane];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+          if (r0 + 64u * u < fq) {
+            if constexpr (PASS == PASS_STREAM16) acc += (PassAcc<W>)bitmap16_hits(S.bm, v[u]);
+            else if constexpr (PASS == PASS_STREAMH) acc += (PassAcc<W>)hash_hits(S.ht, q0, q2, v[u]);
+            else acc += stream_hits<KIND, W, true>(S, a, v[u]);
+          }
+        }
+      }
+    } else {
+      // fq is a multiple of 64: unrolled, each lane keeps several loads in flight
 #pragma unroll UNROLL
-    for (uint32_t r = lane; r < fq; r += 64) acc += pass_elem<PASS, KIND, W, (UNROLL > 1)>(a, S, q0, q1, q2, r);
+      for (uint32_t r = lane; r < fq; r += 64) acc += pass_elem<PASS, KIND, W, (UNROLL > 1)>(a, S, q0, q1, q2, r);
+    }
   }
   // remainders start after the whole rounds: PROBE/AND/BLOCK index by v0, the streams by v1
   c -= full;
@@ -744,47 +803,62 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
   uint32_t tot;
   const uint32_t base = wave_excl_scan(c, lane, &tot);
   tot = rfl(tot);
-  uint32_t *mbase = meta, *m0 = meta + 65, *m1 = meta + 130, *m2 = meta + 195;
-  mbase[lane] = base;
-  m0[lane] = p0;
-  m1[lane] = p1;
-  m2[lane] = p2;
-  if (lane == 0) mbase[64] = tot;
+  uint32_t *mbase = meta, *m0 = meta + 64, *m1 = meta + 128, *m2 = meta + 192, *smask = meta + 256;
+  const bool live = c != 0u;
+  const uint32_t rank = mbcnt(__ballot(live));
+  smask[lane] = 0u;
+  smask[lane + 64u] = 0u;
   wave_sync();
+  if (live) {
+    mbase[rank] = base;
+    m0[rank] = p0;
+    m1[rank] = p1;
+    m2[rank] = p2;
+    atomicOr(&smask[base >> 5], 1u << (base & 31u));
+  }
+  wave_sync();
+  const uint32_t sb = live ? base : 0xFFFFFFFFu;
+  // slot-block lanes 0..lane (2u << 31 wraps to 0, so lane 31's low mask is all ones)
+  const uint32_t le_lo = lane < 32u ? (2u << lane) - 1u : 0xFFFFFFFFu;
+  const uint32_t le_hi = lane < 32u ? 0u : (2u << (lane - 32u)) - 1u;
+  // the block loops are wave-uniform (slot_owner's ballot needs every lane)
   if constexpr ((PASS == PASS_STREAM || PASS == PASS_STREAM16 || PASS == PASS_STREAMH) && UNROLL > 1) {
-    // UNROLL items per lane with their vector loads issued before any is tested; an item
-    // past the end re-reads the last one (always a valid address) and is not counted
+    // UNROLL blocks per step with their vector loads issued before any is tested; blocks
+    // past the end are skipped (a wave-uniform branch), and in the last block a slot past
+    // the end re-reads the last element (always a valid address) and counts zero
     const uint4 *vb = (PASS == PASS_STREAM16) ? a.lo16 : reinterpret_cast<const uint4 *>(a.unbr);
-    for (uint32_t t0 = lane; t0 < tot; t0 += 64u * UNROLL) {
+    for (uint32_t kb = 0; 64u * kb < tot; kb += UNROLL) {
       uint4 v[UNROLL];
       uint32_t hm[UNROLL], hb[UNROLL];
 #pragma unroll
       for (int u = 0; u < UNROLL; u++) {
-        const uint32_t t = min(t0 + 64u * u, tot - 1u);
-        uint32_t q = 0;
-#pragma unroll
-        for (uint32_t step = 32; step; step >>= 1)
-          if (mbase[q + step] <= t) q += step;
-        v[u] = vb[(uint64_t)m1[q] + (t - mbase[q])];
-        hm[u] = m0[q];
-        hb[u] = m2[q];
+        if (64u * (kb + u) < tot) {
+          const uint32_t t = min(64u * (kb + u) + lane, tot - 1u);
+          const uint32_t q = slot_owner(smask, kb + u, sb, le_lo, le_hi);
+          v[u] = vb[(uint64_t)m1[q] + (t - mbase[q])];
+          hm[u] = m0[q];
+          hb[u] = m2[q];
+        }
       }
+      // keep the scheduler from moving the tests between the loads
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int u = 0; u < UNROLL; u++) {
-        if (t0 + 64u * u < tot) {
-          if constexpr (PASS == PASS_STREAM16) acc += (PassAcc<W>)bitmap16_hits(S.bm, v[u]);
-          else if constexpr (PASS == PASS_STREAMH) acc += (PassAcc<W>)hash_hits(S.ht, hm[u], hb[u], v[u]);
-          else acc += stream_hits<KIND, W, true>(S, a, v[u]);
+        if (64u * (kb + u) < tot) {
+          const PassAcc<W> keep = (PassAcc<W>)0 - (PassAcc<W>)(64u * (kb + u) + lane < tot);
+          PassAcc<W> h;
+          if constexpr (PASS == PASS_STREAM16) h = (PassAcc<W>)bitmap16_hits(S.bm, v[u]);
+          else if constexpr (PASS == PASS_STREAMH) h = (PassAcc<W>)hash_hits(S.ht, hm[u], hb[u], v[u]);
+          else h = stream_hits<KIND, W, true>(S, a, v[u]);
+          acc += h & keep;
         }
       }
     }
   } else {
-    for (uint32_t t = lane; t < tot; t += 64) {
-      uint32_t q = 0;
-#pragma unroll
-      for (uint32_t step = 32; step; step >>= 1)
-        if (mbase[q + step] <= t) q += step;
-      acc += pass_elem<PASS, KIND, W>(a, S, m0[q], m1[q], m2[q], t - mbase[q]);
+    for (uint32_t kb = 0; 64u * kb < tot; kb++) {
+      const uint32_t t = 64u * kb + lane;
+      const uint32_t q = slot_owner(smask, kb, sb, le_lo, le_hi);
+      if (t < tot) acc += pass_elem<PASS, KIND, W>(a, S, m0[q], m1[q], m2[q], t - mbase[q]);
     }
   }
   wave_sync();
@@ -796,8 +870,8 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
 // 4-clique level sets, with more live state, run best at 1)
 // BLK: the matrix has bitset_new / new_type rows (their block passes are compiled in)
 template <uint32_t KIND, bool W, int UNROLL = 1, bool BLK = false>
-__device__ uint64_t sum_isect(const CountArgs &a, const AState &S, const uint32_t *K, uint32_t kb, uint32_t ke,
-                              uint32_t lane, uint32_t *meta) {
+__device__ __forceinline__ uint64_t sum_isect(const CountArgs &a, const AState &S, const uint32_t *K, uint32_t kb, uint32_t ke,
+                                              uint32_t lane, uint32_t *meta) {
   uint64_t acc = 0;
   for (uint32_t kc = kb; kc < ke; kc += 64) {
     const uint32_t nk = min(64u, ke - kc);
@@ -936,8 +1010,8 @@ __device__ uint64_t task_cliques4_tuned(const CountArgs &a, const AState &A, uin
 }
 
 template <int K, bool W, bool BLK>
-__device__ uint64_t run_task_tuned(const CountArgs &a, uint32_t kind, uint32_t i, uint32_t jb, uint32_t je,
-                                   uint32_t lane, uint32_t *L) {
+__device__ __forceinline__ uint64_t run_task_tuned(const CountArgs &a, uint32_t kind, uint32_t i, uint32_t jb,
+                                                   uint32_t je, uint32_t lane, uint32_t *L) {
   uint32_t *meta = L;
   uint32_t *Ls = L + META_WORDS;
   AState A;
@@ -961,8 +1035,8 @@ __device__ uint64_t run_task_tuned(const CountArgs &a, uint32_t kind, uint32_t i
 }
 
 template <int K, bool W, bool TUNED, bool BLK>
-__device__ uint64_t run_task(const CountArgs &a, uint32_t kind, uint32_t i, uint32_t jb, uint32_t je,
-                             uint32_t lane, uint32_t *L) {
+__device__ __forceinline__ uint64_t run_task(const CountArgs &a, uint32_t kind, uint32_t i, uint32_t jb,
+                                             uint32_t je, uint32_t lane, uint32_t *L) {
   if constexpr (TUNED) return run_task_tuned<K, W, BLK>(a, kind, i, jb, je, lane, L);
   AState A;
   const uint32_t used = stage_a(a, kind, i, lane, L, A);
@@ -1585,9 +1659,13 @@ __global__ void __launch_bounds__(256) k_bshg(CountArgs a, const uint4 *__restri
       const uint32_t x = a.unbr[(uint64_t)gerow[k] * 4 + (e - gebase[k])];
       const uint32_t m = gtab[k];
       uint32_t *T = tables + (m & 0xFFFFu);
-      const uint32_t tl = m >> 16, mask = (1u << tl) - 1u;
-      uint32_t hsh = (x * 2654435761u) >> (32u - tl);
-      while (atomicCAS(&T[hsh], HASH_EMPTY, x) != HASH_EMPTY) hsh = (hsh + 1u) & mask;
+      const uint32_t tl = m >> 16, bmask = (1u << (tl - 1u)) - 1u;
+      // first free slot from the id's bucket on, slot 0 of a bucket before slot 1 (at least
+      // half the slots stay free, so the walk ends)
+      uint32_t bk = hash_bucket(x, tl);
+      while (atomicCAS(&T[2u * bk], HASH_EMPTY, x) != HASH_EMPTY &&
+             atomicCAS(&T[2u * bk + 1u], HASH_EMPTY, x) != HASH_EMPTY)
+        bk = (bk + 1u) & bmask;
     }
     wave_sync();
     for (uint32_t c0 = pb; c0 < pe; c0 += 64) {
