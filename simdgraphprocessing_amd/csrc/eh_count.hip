@@ -1205,16 +1205,18 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
     if (lane == 0) mbase[64] = tot;
     wave_sync();
     {
-      // SMALL_NQ pairs per lane, t + 64 k, their searches interleaved
+      // SMALL_NQ pairs per lane, t + 64 k, their searches interleaved.  A lane's slots only
+      // grow, so its row index moves forward from the previous slot's (a row holds tens to
+      // thousands of wedges: usually no step, where a search over the 64 bases took six
+      // dependent LDS reads); mbase[64] = tot stops it
+      uint32_t qc = 0;
       for (uint32_t t = lane; t < tot; t += 64 * SMALL_NQ) {
         uint32_t q[SMALL_NQ], x[SMALL_NQ], j[SMALL_NQ];
 #pragma unroll
         for (int k = 0; k < SMALL_NQ; k++) {
           const uint32_t tk = t + 64u * k;
-          q[k] = 0;
-#pragma unroll
-          for (uint32_t step = 32; step; step >>= 1)
-            if (mbase[q[k] + step] <= tk) q[k] += step;
+          while (qc < 63u && mbase[qc + 1u] <= tk) qc++;
+          q[k] = qc;
           x[k] = 0;
           j[k] = 0;
           if (tk < tot) {
@@ -1310,16 +1312,16 @@ __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint3
       if (lane == 0) mbase[64] = tot;
       for (uint32_t w = lane; w < SMALL_ADJ_WORDS; w += 64) adj[w] = 0u;
       wave_sync();
-      // pass 1: local edges, SMALL_NQ membership tests per lane in flight
+      // pass 1: local edges, SMALL_NQ membership tests per lane in flight (a lane's row
+      // index moves forward with its slots, as in k_small_rows)
+      uint32_t qc = 0;
       for (uint32_t t = lane; t < tot; t += 64 * SMALL_NQ) {
         uint32_t q[SMALL_NQ], x[SMALL_NQ], j[SMALL_NQ], pp[SMALL_NQ], cc[SMALL_NQ];
 #pragma unroll
         for (int k = 0; k < SMALL_NQ; k++) {
           const uint32_t tk = t + 64u * k;
-          q[k] = 0;
-#pragma unroll
-          for (uint32_t step = 32; step; step >>= 1)
-            if (mbase[q[k] + step] <= tk) q[k] += step;
+          while (qc < 63u && mbase[qc + 1u] <= tk) qc++;
+          q[k] = qc;
           x[k] = 0;
           j[k] = 0;
           pp[k] = 0;
@@ -1348,11 +1350,10 @@ __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint3
       }
       wave_sync();
       // pass 2: every local edge (b, c) adds |adj(b) & adj(c)|
+      qc = 0;
       for (uint32_t t = lane; t < tot; t += 64) {
-        uint32_t q = 0;
-#pragma unroll
-        for (uint32_t step = 32; step; step >>= 1)
-          if (mbase[q + step] <= t) q += step;
+        while (qc < 63u && mbase[qc + 1u] <= t) qc++;
+        const uint32_t q = qc;
         uint32_t b, c;
         tri_decode(t - mbase[q], &b, &c);
         const uint32_t wq = (mdeg[q] + 31u) >> 5;
