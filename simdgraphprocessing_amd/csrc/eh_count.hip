@@ -1418,10 +1418,12 @@ __global__ void __launch_bounds__(256) k_bstat(CountArgs a, const uint4 *__restr
       if (t >= nbt) break;
       tend = min(t + a.grab, nbt);
     }
+    // {j | |row(j)| << 16, first in-pair, end, row(j)'s 16-byte offset}: the row is found
+    // without another dependent load, and the first 64 in-pairs load while the bitmap builds
     const uint4 task = btasks[t++];
-    const uint32_t j = rfl(task.x), b = rfl(task.y), e = rfl(task.z);
-    const uint32_t dj = rfl(a.card[j]);
-    const uint32_t *G = a.unbr + (uint64_t)a.aoff[j] * 4;
+    const uint32_t dj = rfl(task.x) >> 16, b = rfl(task.y), e = rfl(task.z);
+    const uint32_t *G = a.unbr + (uint64_t)rfl(task.w) * 4;
+    uint2 pr = (b + lane < e) ? in_pairs[b + lane] : make_uint2(0u, 0u);
     // the whole bitmap is cleared: a prefix vector's tail may test ids up to split - 1
     for (uint32_t w = lane; w < bm_words; w += 64) bm[w] = 0u;
     wave_sync();
@@ -1431,13 +1433,11 @@ __global__ void __launch_bounds__(256) k_bstat(CountArgs a, const uint4 *__restr
     }
     wave_sync();
     for (uint32_t c0 = b; c0 < e; c0 += 64) {
-      uint32_t nv = 0, off = 0;
-      if (c0 + lane < e) {
-        const uint2 pr = in_pairs[c0 + lane];
-        off = pr.x;
-        nv = (pr.y + 7u) >> 3;
-      }
+      const uint32_t nv = (c0 + lane < e) ? (pr.y + 7u) >> 3 : 0u, off = pr.x;
+      // the next 64 in-pairs load during this pass
+      const uint2 prn = (c0 + 64u + lane < e) ? in_pairs[c0 + 64u + lane] : make_uint2(0u, 0u);
       acc += flat_pass<PASS_STREAM16, KIND_BITMAP, false, 4>(a, S, nv, 0u, off, 0u, lane, meta);
+      pr = prn;
     }
     wave_sync();
   }
@@ -2078,15 +2078,18 @@ __global__ void k_bstat_ntask(const uint32_t *incount, uint32_t nj, uint32_t chu
     ntb[j] = (incount[j] + chunk - 1u) / chunk;
 }
 
+// task {j | |row(j)| << 16, b, e, row(j)'s offset in unbr}: j < 2^16 and row(j) holds ids
+// below j, so both fit in the first word
 __global__ void k_bstat_expand(const uint32_t *incount, const uint64_t *inoff, const uint64_t *boff, uint32_t nj,
-                               uint32_t chunk, uint4 *btasks) {
+                               uint32_t chunk, const uint32_t *card, const uint32_t *aoff, uint4 *btasks) {
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nj; j += gridDim.x * blockDim.x) {
     const uint32_t c = incount[j];
     const uint64_t b0 = inoff[j], t0 = boff[j];
+    const uint32_t head = j | (card[j] << 16), ao = aoff[j];
     for (uint32_t p = 0; p * chunk < c; p++) {
       const uint32_t b = (uint32_t)(b0 + (uint64_t)p * chunk);
       const uint32_t e = (uint32_t)(b0 + min<uint64_t>((uint64_t)(p + 1) * chunk, c));
-      btasks[t0 + p] = make_uint4(j, b, e, 0u);
+      btasks[t0 + p] = make_uint4(head, b, e, ao);
     }
   }
 }
@@ -2355,6 +2358,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     EH_CHECK(s.btasks.alloc(16 * nbt + 16));
     k_bstat_expand<<<grid_for(bstat_j), 256, 0, st>>>(incount.as<uint32_t>(), inoff.as<uint64_t>(),
                                                      boff.as<uint64_t>(), bstat_j, BSTAT_CHUNK,
+                                                     m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(),
                                                      s.btasks.as<uint4>());
     {
       // tasks in the order of their first pair's prefix offset: the tasks running at one time
