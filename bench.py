@@ -262,8 +262,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
-              "is described in DESIGN.md and not run here.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the counting path has "
+              "no multi-GPU mode.", file=sys.stderr)
         sys.exit(2)
 
     import simdgraphprocessing_amd as eh
