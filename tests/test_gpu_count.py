@@ -187,3 +187,22 @@ def test_deep_cliques_reject_large_k(ctx, facebook):
     M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "lower", "hybrid")
     with pytest.raises(eh.EHError):
         M.clique_count(17, "tuned")
+
+
+def test_complete_graph_full_remainders(ctx):
+    """K_600: every B row j of the oriented matrix has the in-pairs (i, j), i > j, each with a
+    prefix of j ids, so B rows 497-503 give 64 pairs of 63 remainder vectors per pass (the most
+    slots a flattened pass holds: 4,032 in 63 blocks of 64) and the hub-free dense rows exercise
+    the A-side and local-graph paths too; the counts are C(600, 3) and C(600, 4)."""
+    from math import comb
+    n = 600
+    src = [a for a in range(n) for b in range(a + 1, n)]
+    dst = [b for a in range(n) for b in range(a + 1, n)]
+    off, nbr = O.edges_to_csr(n, np.array(src), np.array(dst))
+    for pol in ("hybrid", "uint", "bs"):
+        M = ctx.matrix_from_csr(n, off, nbr, "lower", pol)
+        for bstat in (0, -1, 256):
+            assert M.triangle_count("tuned", bstat=bstat)[0] == comb(n, 3), (pol, bstat)
+        assert M.triangle_count("plain")[0] == comb(n, 3), pol
+        for local in (0, -1):
+            assert M.clique_count(4, "tuned", local_max=local)[0] == comb(n, 4), (pol, local)
