@@ -61,6 +61,15 @@ __global__ void k_zero64(unsigned long long *p, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
+// the error words of the triangle and 4-clique passes (queue-head blocks 0..7 and 8..15,
+// word 7 of each; a fill kernel, so no memset node is recorded into the captured graph)
+__global__ void k_zero_err(uint32_t *nx) {
+  if (threadIdx.x == 0) {
+    nx[7] = 0u;
+    nx[15] = 0u;
+  }
+}
+
 }  // namespace
 
 // what a captured graph or a per-graph plan bakes in: the schedule buffers it reads
@@ -290,6 +299,7 @@ extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tr
       if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
       EH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       k_zero64<<<grid_for(2ull * ng), 256, 0, st>>>(cnt, 2 * ng);
+      k_zero_err<<<1, 64, 0, st>>>(nx);
       eh_status s = EH_OK;
       for (int q = 0; q < 2 && s == EH_OK; q++)
         if (want[q])
@@ -314,6 +324,7 @@ extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tr
       if (want[q]) EH_CHECK(build_plan(b, q, key[q]));
     EH_HIP(hipEventRecord(ctx->ev0, st));
     k_zero64<<<grid_for(2ull * ng), 256, 0, st>>>(cnt, 2 * ng);
+    k_zero_err<<<1, 64, 0, st>>>(nx);
     for (uint32_t g = 0; g < ng; g++) {
       for (int q = 0; q < 2; q++) {
         if (!want[q]) continue;
@@ -331,6 +342,13 @@ extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tr
     EH_HIP(hipEventRecord(ctx->ev1, st));
     EH_HIP(hipEventSynchronize(ctx->ev1));
     EH_HIP(hipEventElapsedTime(&elapsed, ctx->ev0, ctx->ev1));
+  }
+  uint32_t nxh[16];
+  EH_HIP(hipMemcpy(nxh, nx, sizeof(nxh), hipMemcpyDeviceToHost));
+  if (nxh[7] | nxh[15]) {
+    eh_set_error("batch count: an LDS hash table had no free slot (device error words %#x %#x); the counts "
+                 "are not valid", nxh[7], nxh[15]);
+    return EH_ERR_INVALID;
   }
   std::vector<uint64_t> h(2ull * ng);
   EH_HIP(hipMemcpy(h.data(), b->counts.p, 16ull * ng, hipMemcpyDeviceToHost));

@@ -70,7 +70,12 @@ struct CountArgs {
                                      // position are counted by k_bstat (null: none)
   const uint4 *lo16;                 // B-stationary: those entries as 16-bit ids (they are all
                                      // below the split, at most 65536), padded to 8 per vector
+  uint32_t *err;                     // device error word: a hash probe or insert that walked
+                                     // its whole table ORs ERR_HASH_FULL (the host reports it)
 };
+
+// bits of the device error word (CountArgs::err)
+constexpr uint32_t ERR_HASH_FULL = 1u;
 
 __device__ __forceinline__ uint32_t rfl(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ uint32_t rdl(uint32_t x, uint32_t l) { return __builtin_amdgcn_readlane(x, l); }
@@ -128,6 +133,7 @@ struct AState {
   uint32_t P;
   uint32_t *ht;           // HASH: LDS table of 2^hlg slots
   uint32_t hlg;
+  uint32_t *err;          // HASH: the pass's device error word
 };
 
 template <uint32_t KIND>
@@ -147,14 +153,19 @@ __device__ __forceinline__ uint32_t memb(const AState &A, uint32_t x) {
     const uint32_t s = A.dstart[base], e = A.dstart[base + 1];
     return sorted_contains(A.list + s, e - s, x);
   } else if constexpr (KIND == KIND_HASH) {
+    // a table of 2^hlg slots holds at most half that many ids, so the walk meets a free
+    // slot; it is bounded by the table size all the same, and a walk that visits every
+    // slot flags the pass instead of spinning
     const uint32_t mask = (1u << A.hlg) - 1u;
-    uint32_t s = (x * 2654435761u) >> (32u - A.hlg);
-    for (;;) {
+    uint32_t s = ((x * 2654435761u) >> (32u - A.hlg)) & mask;
+    for (uint32_t it = 0; it <= mask; it++) {
       const uint32_t v = A.ht[s];
       if (v == x) return 1;
       if (v == HASH_EMPTY) return 0;
       s = (s + 1u) & mask;
     }
+    atomicOr(A.err, ERR_HASH_FULL);
+    return 0;
   } else {
     return sorted_contains(A.list, A.n, x);
   }
@@ -424,6 +435,7 @@ __device__ __forceinline__ uint32_t stage_a(const CountArgs &a, uint32_t kind, u
   A.P = 0;
   A.dpre = A.dstart = nullptr;
   A.ht = nullptr;
+  A.err = nullptr;
   A.hlg = 0;
   A.bm = L;
   if (kind == KIND_GLOBAL) {
@@ -491,6 +503,7 @@ __device__ __forceinline__ void make_cstate(const AState &A, uint32_t *clist, ui
   Cs.P = 0;
   Cs.dpre = Cs.dstart = nullptr;
   Cs.ht = nullptr;
+  Cs.err = nullptr;
   Cs.hlg = 0;
   Cs.aw0 = A.aw0;
   Cs.anw = A.anw;
@@ -667,16 +680,19 @@ __device__ __forceinline__ uint32_t bitmap16_hits(const uint32_t *bm, const uint
 // slots are used, so few lookups go past their first bucket: the four first buckets are
 // read together without branching, and a masked loop follows the rest.  Ids at or above
 // the row's id `bound` (the prefix tail, the padding) cannot be in it.
+// The bucket is the top lg - 1 bits of the 32-bit product (lg >= 4 for every table, so the
+// shift is at most 29); the mask restates that range at the use.
 __device__ __forceinline__ uint32_t hash_bucket(uint32_t e, uint32_t lg) { return (e * 2654435761u) >> (33u - lg); }
 
-__device__ __forceinline__ uint32_t hash_hits(const uint32_t *ht, uint32_t meta, uint32_t bound, const uint4 x) {
+__device__ __forceinline__ uint32_t hash_hits(const uint32_t *ht, uint32_t meta, uint32_t bound, const uint4 x,
+                                              uint32_t *err) {
   const uint32_t *T = ht + (meta & 0xFFFFu);
   const uint32_t lg = meta >> 16, bmask = (1u << (lg - 1u)) - 1u;
   const uint32_t e[4] = {x.x, x.y, x.z, x.w};
   uint32_t b[4], v0[4], v1[4];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    b[q] = hash_bucket(e[q], lg);
+    b[q] = hash_bucket(e[q], lg) & bmask;
     v0[q] = T[2u * b[q]];
     v1[q] = T[2u * b[q] + 1u];
   }
@@ -687,7 +703,9 @@ __device__ __forceinline__ uint32_t hash_hits(const uint32_t *ht, uint32_t meta,
     h += (uint32_t)(in & hit);
     pend |= (uint32_t)(in & !hit & (v1[q] != HASH_EMPTY)) << q;
   }
-  while (pend) {
+  // every further step visits one more bucket of each pending id; after bmask steps each
+  // has seen all 2^(lg-1) buckets, and one still pending means a table without a free slot
+  for (uint32_t it = 0; pend && it < bmask; it++) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       b[q] = (pend >> q) & 1u ? (b[q] + 1u) & bmask : b[q];
@@ -701,6 +719,7 @@ __device__ __forceinline__ uint32_t hash_hits(const uint32_t *ht, uint32_t meta,
       if (p & (hit | (v1[q] == HASH_EMPTY))) pend &= ~(1u << q);
     }
   }
+  if (pend) atomicOr(err, ERR_HASH_FULL);
   return h;
 }
 
@@ -733,7 +752,7 @@ __device__ __forceinline__ PassAcc<W> pass_elem(const CountArgs &a, const AState
   } else if constexpr (PASS == PASS_STREAM16) {
     return (PassAcc<W>)bitmap16_hits(S.bm, a.lo16[(uint64_t)v1 + r]);
   } else if constexpr (PASS == PASS_STREAMH) {
-    return (PassAcc<W>)hash_hits(S.ht, v0, v2, reinterpret_cast<const uint4 *>(a.unbr)[(uint64_t)v1 + r]);
+    return (PassAcc<W>)hash_hits(S.ht, v0, v2, reinterpret_cast<const uint4 *>(a.unbr)[(uint64_t)v1 + r], S.err);
   } else {
     const uint32_t *base = (PASS == PASS_NTSTREAM) ? a.ntu : a.unbr;
     return stream_hits<KIND, W, LOCKSTEP>(S, a, reinterpret_cast<const uint4 *>(base)[(uint64_t)v1 + r]);
@@ -783,7 +802,7 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
         for (int u = 0; u < UNROLL; u++) {
           if (r0 + 64u * u < fq) {
             if constexpr (PASS == PASS_STREAM16) acc += (PassAcc<W>)bitmap16_hits(S.bm, v[u]);
-            else if constexpr (PASS == PASS_STREAMH) acc += (PassAcc<W>)hash_hits(S.ht, q0, q2, v[u]);
+            else if constexpr (PASS == PASS_STREAMH) acc += (PassAcc<W>)hash_hits(S.ht, q0, q2, v[u], S.err);
             else acc += stream_hits<KIND, W, true>(S, a, v[u]);
           }
         }
@@ -848,7 +867,7 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
           const PassAcc<W> keep = (PassAcc<W>)0 - (PassAcc<W>)(64u * (kb + u) + lane < tot);
           PassAcc<W> h;
           if constexpr (PASS == PASS_STREAM16) h = (PassAcc<W>)bitmap16_hits(S.bm, v[u]);
-          else if constexpr (PASS == PASS_STREAMH) h = (PassAcc<W>)hash_hits(S.ht, hm[u], hb[u], v[u]);
+          else if constexpr (PASS == PASS_STREAMH) h = (PassAcc<W>)hash_hits(S.ht, hm[u], hb[u], v[u], S.err);
           else h = stream_hits<KIND, W, true>(S, a, v[u]);
           acc += h & keep;
         }
@@ -946,15 +965,22 @@ __device__ __forceinline__ uint32_t hash_lg(uint32_t n) {
 }
 
 // LDS hash table over a sorted list (linear probing, multiplicative hash)
+// (at most half the 2^lg slots are used; an insert that finds no free slot in a full walk
+// flags the pass through err instead of spinning)
 __device__ __forceinline__ void build_hash(const uint32_t *list, uint32_t n, uint32_t *ht, uint32_t lg,
-                                           uint32_t lane) {
+                                           uint32_t lane, uint32_t *err) {
   const uint32_t cap = 1u << lg, mask = cap - 1u;
   for (uint32_t w = lane; w < cap; w += 64) ht[w] = HASH_EMPTY;
   wave_sync();
   for (uint32_t k = lane; k < n; k += 64) {
     const uint32_t x = list[k];
-    uint32_t s = (x * 2654435761u) >> (32u - lg);
-    while (atomicCAS(&ht[s], HASH_EMPTY, x) != HASH_EMPTY) s = (s + 1u) & mask;
+    uint32_t s = ((x * 2654435761u) >> (32u - lg)) & mask;
+    uint32_t it = 0;
+    while (it <= mask && atomicCAS(&ht[s], HASH_EMPTY, x) != HASH_EMPTY) {
+      s = (s + 1u) & mask;
+      it++;
+    }
+    if (it > mask) atomicOr(err, ERR_HASH_FULL);
   }
   wave_sync();
 }
@@ -967,7 +993,8 @@ __device__ __forceinline__ uint32_t stage_tuned(const CountArgs &a, uint32_t kin
   const uint32_t lg = hash_lg(A.n);
   A.ht = L + used;
   A.hlg = lg;
-  build_hash(A.list, A.n, A.ht, lg, lane);
+  A.err = a.err;
+  build_hash(A.list, A.n, A.ht, lg, lane, a.err);
   return used + (1u << lg);
 }
 
@@ -997,7 +1024,8 @@ __device__ uint64_t task_cliques4_tuned(const CountArgs &a, const AState &A, uin
         make_cstate<KIND_LIST>(A, clist, nc, cmemb, lane, C);
         C.hlg = hash_lg(nc);
         C.ht = cmemb;
-        build_hash(clist, nc, cmemb, C.hlg, lane);
+        C.err = a.err;
+        build_hash(clist, nc, cmemb, C.hlg, lane, a.err);
         acc += sum_isect<KIND_HASH, false>(a, C, clist, 0, nc, lane, meta);
       } else {
         make_cstate<KIND_LIST>(A, clist, nc, cmemb, lane, C);
@@ -1409,6 +1437,7 @@ __global__ void __launch_bounds__(256) k_bstat(CountArgs a, const uint4 *__restr
   S.P = 0;
   S.dpre = S.dstart = nullptr;
   S.ht = nullptr;
+  S.err = nullptr;
   S.hlg = 0;
   for (;;) {
     if (t >= tend) {
@@ -1498,9 +1527,13 @@ __global__ void __launch_bounds__(256) k_local4(CountArgs a, const uint32_t *__r
     gsync();
     for (uint32_t k = gtid; k < d; k += gsize) {
       const uint32_t x = list[k];
-      uint32_t h = (x * 2654435761u) >> (32u - lg);
-      while (atomicCAS(&hkey[h], HASH_EMPTY, x) != HASH_EMPTY) h = (h + 1u) & mask;
-      hpos[h] = k;
+      uint32_t h = ((x * 2654435761u) >> (32u - lg)) & mask, it = 0;
+      while (it <= mask && atomicCAS(&hkey[h], HASH_EMPTY, x) != HASH_EMPTY) {
+        h = (h + 1u) & mask;
+        it++;
+      }
+      if (it > mask) atomicOr(a.err, ERR_HASH_FULL);
+      else hpos[h] = k;
     }
     gsync();
     // local edges: stream row(R[c]) for c in [1, d), wave w of the group taking every
@@ -1542,8 +1575,8 @@ __global__ void __launch_bounds__(256) k_local4(CountArgs a, const uint32_t *__r
           for (int e = 0; e < 4; e++) {
             const uint32_t x = xs[e];
             if (x == HASH_EMPTY) continue;  // the list padding
-            uint32_t h = (x * 2654435761u) >> (32u - lg);
-            for (;;) {
+            uint32_t h = ((x * 2654435761u) >> (32u - lg)) & mask, it = 0;
+            for (; it <= mask; it++) {
               const uint32_t key = hkey[h];
               if (key == x) {
                 const uint32_t pp = hpos[h];
@@ -1553,6 +1586,7 @@ __global__ void __launch_bounds__(256) k_local4(CountArgs a, const uint32_t *__r
               if (key == HASH_EMPTY) break;
               h = (h + 1u) & mask;
             }
+            if (it > mask) atomicOr(a.err, ERR_HASH_FULL);
           }
         }
       }
@@ -1604,6 +1638,7 @@ __global__ void __launch_bounds__(256) k_bshg(CountArgs a, const uint4 *__restri
   uint32_t *tables = gpbase + BSH_GROUP + 1;
   AState S;
   S.ht = tables;
+  S.err = a.err;
   S.bm = nullptr;
   S.list = nullptr;
   S.n = S.lo = S.span = S.aw0 = S.anw = S.base32 = S.P = S.hlg = 0;
@@ -1662,11 +1697,16 @@ __global__ void __launch_bounds__(256) k_bshg(CountArgs a, const uint4 *__restri
       uint32_t *T = tables + (m & 0xFFFFu);
       const uint32_t tl = m >> 16, bmask = (1u << (tl - 1u)) - 1u;
       // first free slot from the id's bucket on, slot 0 of a bucket before slot 1 (at least
-      // half the slots stay free, so the walk ends)
-      uint32_t bk = hash_bucket(x, tl);
-      while (atomicCAS(&T[2u * bk], HASH_EMPTY, x) != HASH_EMPTY &&
-             atomicCAS(&T[2u * bk + 1u], HASH_EMPTY, x) != HASH_EMPTY)
+      // half the slots stay free, so the walk ends; it is bounded by the bucket count all
+      // the same, and an insert that finds no free slot flags the pass)
+      uint32_t bk = hash_bucket(x, tl) & bmask, it = 0;
+      for (; it <= bmask; it++) {
+        if (atomicCAS(&T[2u * bk], HASH_EMPTY, x) == HASH_EMPTY ||
+            atomicCAS(&T[2u * bk + 1u], HASH_EMPTY, x) == HASH_EMPTY)
+          break;
         bk = (bk + 1u) & bmask;
+      }
+      if (it > bmask) atomicOr(a.err, ERR_HASH_FULL);
     }
     wave_sync();
     for (uint32_t c0 = pb; c0 < pe; c0 += 64) {
@@ -2604,6 +2644,9 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
                        uint64_t n_local0, const uint32_t *local_rows1, uint64_t n_local1, const Schedule *hs) {
   CountArgs a;
   a.grab = tp.grab ? tp.grab : 1u;
+  // d_next[7] is the pass's error word (the queue heads use d_next[0..4]); the caller
+  // clears it before the pass and reads it after
+  a.err = d_next + 7;
   a.jstart = jstart;
   a.lo16 = lo16;
   a.task_cycles = reinterpret_cast<unsigned long long *>(d_task_cycles);
@@ -2697,6 +2740,7 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
   float total_ms = 0;
   for (int r = 0; r < repeat; r++) {
     k_zero_u64<<<grid_for(ncounts ? ncounts : 1), 256, 0, st>>>(dc, ncounts ? ncounts : 1);
+    k_zero_u32<<<1, 64, 0, st>>>(next.as<uint32_t>() + 7, 1);  // the pass's error word
     EH_HIP(hipEventRecord(ctx->ev0, st));
     EH_CHECK(schedule_launch(m, k, sched, tp, s, reinterpret_cast<uint64_t *>(dc), next.as<uint32_t>(),
                              d_task_cycles));
@@ -2705,6 +2749,13 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
     float ms = 0;
     EH_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     total_ms += ms;
+  }
+  uint32_t dev_err = 0;
+  EH_HIP(hipMemcpy(&dev_err, next.as<uint32_t>() + 7, 4, hipMemcpyDeviceToHost));
+  if (dev_err) {
+    eh_set_error("counting pass: an LDS hash table had no free slot (device error word %#x); the count is "
+                 "not valid", dev_err);
+    return EH_ERR_INVALID;
   }
   if (count) {
     std::vector<uint64_t> h(ncounts ? ncounts : 1);
