@@ -253,9 +253,22 @@ eh_status eh_union(eh_ctx *ctx, const eh_row_view *a, const eh_row_view *b, void
  * (src/set/ops/difference.hpp:5-84): the elements of A not in B, as a sorted uint set. */
 eh_status eh_difference(eh_ctx *ctx, const eh_row_view *a, const eh_row_view *b, void *out_buf, size_t out_cap,
                         eh_row_view *out);
-/* Many pairs of rows of one matrix at once: counts[p] = |row(a[p]) & row(b[p])|. */
+/* Many pairs of rows of one matrix at once: counts[p] = |row(a[p]) & row(b[p])|
+ * (the per-pair loop of apps/undirected_triangle_counting.cpp:45-56 and
+ * apps/similar_nodes.cpp:45-60 as one device call; host arrays in and out). */
 eh_status eh_intersect_rows(eh_matrix *m, const uint32_t *a_rows, const uint32_t *b_rows,
                             uint64_t npairs, uint64_t *counts);
+/* No counterpart in the reference.  eh_intersect_rows for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): same arguments and results, but the pairs and counts are staged in a static
+ * device array of EH_STATIC_MAX_PAIRS pairs (16 bytes a pair: 16 MiB on every GPU the
+ * library runs on, held until the process ends) instead of buffers allocated per call.
+ * npairs above EH_STATIC_MAX_PAIRS is rejected with EH_ERR_CAPACITY.  All calls on one GPU
+ * share that array: calls that overlap on one GPU (different contexts or threads) race,
+ * and the caller must serialise them. */
+#define EH_STATIC_MAX_PAIRS 1048576ull
+eh_status eh_intersect_rows_static(eh_matrix *m, const uint32_t *a_rows, const uint32_t *b_rows,
+                                   uint64_t npairs, uint64_t *counts);
 
 /* ---------------------------------------------------------------- apps */
 /* undirected_triangle_counting<T,R>::run (apps/undirected_triangle_counting.cpp:27-65):

@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libeh.so")
 
 EH_BITSET, EH_PSHORT, EH_UINTEGER, EH_BITSET_NEW, EH_NEW_TYPE = 0, 1, 2, 7, 8
+STATIC_MAX_PAIRS = 1 << 20  # EH_STATIC_MAX_PAIRS: pairs eh_intersect_rows_static stages at most
 TYPE_NAMES = {0: "bitset", 1: "pshort", 2: "uint"}
 LAYOUTS = {"uint": 0, "pshort": 1, "bs": 2, "bitset": 2, "hybrid": 3, "hybrid3": 4, "learned": 5,
            "cost": 6, "bitset_new": 7, "new_type": 8, "hybrid_block": 9}
@@ -88,7 +89,7 @@ EXPORTS = [
     "eh_graph_relabel_by_degree", "eh_graph_reorder", "eh_graph_info_get", "eh_graph_download", "eh_graph_download_perm",
     "eh_graph_free", "eh_matrix_build", "eh_matrix_build_host", "eh_matrix_info_get", "eh_matrix_row",
     "eh_matrix_download", "eh_matrix_row_types", "eh_matrix_free", "eh_intersect", "eh_intersect_rows",
-    "eh_union", "eh_difference", "eh_triangle_count", "eh_clique_count", "eh_bfs_path", "eh_count_profile", "eh_batch_generate", "eh_batch_info",
+    "eh_intersect_rows_static", "eh_union", "eh_difference", "eh_triangle_count", "eh_clique_count", "eh_bfs_path", "eh_count_profile", "eh_batch_generate", "eh_batch_info",
     "eh_batch_graph_params", "eh_batch_graph_download", "eh_batch_count", "eh_batch_free",
     "eh_selector_set_params", "eh_selector_get_params", "eh_matrix_features",
     "eh_matrix_profile_layouts", "eh_mlp_forward", "eh_mlp_loss_grad", "eh_selector_train",
@@ -144,6 +145,7 @@ def lib():
     L.eh_intersect.argtypes = [_vp, C.POINTER(RowView), C.POINTER(RowView), C.c_int, _vp, C.c_size_t,
                                C.POINTER(RowView)]
     L.eh_intersect_rows.argtypes = [_vp, _vp, _vp, C.c_uint64, _vp]
+    L.eh_intersect_rows_static.argtypes = [_vp, _vp, _vp, C.c_uint64, _vp]
     for name in ("eh_union", "eh_difference"):
         getattr(L, name).argtypes = [_vp, C.POINTER(RowView), C.POINTER(RowView), _vp, C.c_size_t,
                                      C.POINTER(RowView)]
@@ -497,6 +499,15 @@ class SparseMatrix:
         b = np.ascontiguousarray(b_rows, np.uint32)
         out = np.zeros(len(a), np.uint64)
         _check(lib().eh_intersect_rows(self.h, _ptr(a), _ptr(b), len(a), _ptr(out)))
+        return out
+
+    def intersect_rows_static(self, a_rows, b_rows):
+        """intersect_rows without device allocation: staged in the library's static array
+        of STATIC_MAX_PAIRS pairs (more raises EHError with EH_ERR_CAPACITY)."""
+        a = np.ascontiguousarray(a_rows, np.uint32)
+        b = np.ascontiguousarray(b_rows, np.uint32)
+        out = np.zeros(len(a), np.uint64)
+        _check(lib().eh_intersect_rows_static(self.h, _ptr(a), _ptr(b), len(a), _ptr(out)))
         return out
 
 

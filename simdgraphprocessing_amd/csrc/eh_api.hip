@@ -66,7 +66,10 @@ extern "C" eh_status eh_ctx_create(int device, eh_ctx **out) {
     delete c;
     return EH_ERR_HIP;
   }
-  if (c->counter.alloc(256) != EH_OK) { delete c; return EH_ERR_OOM; }
+  if (c->counter.alloc(256) != EH_OK) { eh_ctx_destroy(c); return EH_ERR_OOM; }
+  // the device address of the static staging of eh_intersect_rows_static, once per device
+  const eh_status ss = static_scratch_init(device);
+  if (ss != EH_OK) { eh_ctx_destroy(c); return ss; }
   *out = c;
   return EH_OK;
 }

@@ -243,6 +243,9 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
                        const uint32_t *local_rows1 = nullptr, uint64_t n_local1 = 0,
                        const Schedule *hs = nullptr);
 
+// records this device's address of eh_intersect_rows_static's staging array (device current)
+eh_status static_scratch_init(int device);
+
 // timing helper: device ms between two events recorded on ctx->stream
 double ctx_elapsed_ms(eh_ctx *ctx);
 

@@ -20,6 +20,8 @@
 //
 // eh_intersect_rows counts |row(a[p]) & row(b[p])| for many pairs of one matrix with
 // one wave per pair over the rows' sorted u32 lists already resident in HBM.
+#include <mutex>
+
 #include "eh_internal.hpp"
 
 namespace {
@@ -826,29 +828,100 @@ extern "C" eh_status eh_difference(eh_ctx *ctx, const eh_row_view *a, const eh_r
   return EH_OK;
 }
 
-extern "C" eh_status eh_intersect_rows(eh_matrix *m, const uint32_t *a_rows, const uint32_t *b_rows,
-                                       uint64_t npairs, uint64_t *counts) {
+namespace {
+
+// Staging of eh_intersect_rows_static: the pairs' row ids and their counts live in this
+// file-scope array instead of buffers allocated per call -- EH_STATIC_MAX_PAIRS u32 a-rows,
+// then as many u32 b-rows, then as many u64 counts (16 MiB).  Each GPU the library runs on
+// holds its own copy until the process ends.  Host code never uses the array's name (it is
+// a host-side stand-in, not a device address): eh_ctx_create calls static_scratch_init with
+// the context's device current, which keeps that device's address in g_static_ptr.
+static __device__ __align__(256) uint8_t scratch[16ull * EH_STATIC_MAX_PAIRS];
+
+constexpr int STATIC_MAX_DEVICES = 64;
+std::mutex g_static_mu;
+void *g_static_ptr[STATIC_MAX_DEVICES] = {};
+
+// the pairs (device ra, rb) counted into c; a_rows / b_rows / counts are host arrays
+eh_status isect_rows_run(eh_matrix *m, const uint32_t *a_rows, const uint32_t *b_rows, uint64_t npairs,
+                         uint64_t *counts, uint32_t *ra, uint32_t *rb, unsigned long long *c) {
+  hipStream_t st = m->ctx->stream;
+  EH_HIP(hipMemcpyAsync(ra, a_rows, 4 * npairs, hipMemcpyHostToDevice, st));
+  EH_HIP(hipMemcpyAsync(rb, b_rows, 4 * npairs, hipMemcpyHostToDevice, st));
+  k_isect_rows<<<grid_for(npairs * 64), 256, 0, st>>>(m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(),
+                                                      m->dunbr.as<uint32_t>(), ra, rb, npairs, c);
+  EH_HIP(hipGetLastError());
+  EH_HIP(hipMemcpyAsync(counts, c, 8 * npairs, hipMemcpyDeviceToHost, st));
+  EH_HIP(hipStreamSynchronize(st));
+  return EH_OK;
+}
+
+eh_status isect_rows_check(const eh_matrix *m, const uint32_t *a_rows, const uint32_t *b_rows, uint64_t npairs,
+                           const uint64_t *counts) {
   if (!m || (npairs && (!a_rows || !b_rows || !counts))) { eh_set_error("null argument"); return EH_ERR_INVALID; }
-  if (npairs == 0) return EH_OK;
   for (uint64_t p = 0; p < npairs; p++)
     if (a_rows[p] >= m->n || b_rows[p] >= m->n) {
       eh_set_error("pair %llu names a row outside [0, %u)", (unsigned long long)p, m->n);
       return EH_ERR_INVALID;
     }
-  eh_ctx *ctx = m->ctx;
-  EH_HIP(hipSetDevice(ctx->device));
-  hipStream_t st = ctx->stream;
+  return EH_OK;
+}
+
+}  // namespace
+
+// Called once per context with `device` current: records that device's address of the
+// static staging array (hipGetSymbolAddress), once per device.
+eh_status static_scratch_init(int device) {
+  if (device < 0 || device >= STATIC_MAX_DEVICES) {
+    eh_set_error("device %d: the static staging table holds %d devices", device, STATIC_MAX_DEVICES);
+    return EH_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (g_static_ptr[device]) return EH_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    eh_set_error("hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    return EH_ERR_HIP;
+  }
+  g_static_ptr[device] = p;
+  return EH_OK;
+}
+
+extern "C" eh_status eh_intersect_rows(eh_matrix *m, const uint32_t *a_rows, const uint32_t *b_rows,
+                                       uint64_t npairs, uint64_t *counts) {
+  EH_CHECK(isect_rows_check(m, a_rows, b_rows, npairs, counts));
+  if (npairs == 0) return EH_OK;
+  EH_HIP(hipSetDevice(m->ctx->device));
   DevBuf ra, rb, c;
   EH_CHECK(ra.alloc(4 * npairs + 16));
   EH_CHECK(rb.alloc(4 * npairs + 16));
   EH_CHECK(c.alloc(8 * npairs + 16));
-  EH_HIP(hipMemcpyAsync(ra.p, a_rows, 4 * npairs, hipMemcpyHostToDevice, st));
-  EH_HIP(hipMemcpyAsync(rb.p, b_rows, 4 * npairs, hipMemcpyHostToDevice, st));
-  k_isect_rows<<<grid_for(npairs * 64), 256, 0, st>>>(m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(),
-                                                      m->dunbr.as<uint32_t>(), ra.as<uint32_t>(), rb.as<uint32_t>(),
-                                                      npairs, c.as<unsigned long long>());
-  EH_HIP(hipGetLastError());
-  EH_HIP(hipMemcpyAsync(counts, c.p, 8 * npairs, hipMemcpyDeviceToHost, st));
-  EH_HIP(hipStreamSynchronize(st));
-  return EH_OK;
+  return isect_rows_run(m, a_rows, b_rows, npairs, counts, ra.as<uint32_t>(), rb.as<uint32_t>(),
+                        c.as<unsigned long long>());
+}
+
+extern "C" eh_status eh_intersect_rows_static(eh_matrix *m, const uint32_t *a_rows, const uint32_t *b_rows,
+                                              uint64_t npairs, uint64_t *counts) {
+  EH_CHECK(isect_rows_check(m, a_rows, b_rows, npairs, counts));
+  if (npairs > EH_STATIC_MAX_PAIRS) {
+    eh_set_error("eh_intersect_rows_static: %llu pairs exceed the static staging of %llu pairs "
+                 "(split the call, or use eh_intersect_rows)",
+                 (unsigned long long)npairs, (unsigned long long)EH_STATIC_MAX_PAIRS);
+    return EH_ERR_CAPACITY;
+  }
+  if (npairs == 0) return EH_OK;
+  const int dev = m->ctx->device;
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    base = (dev >= 0 && dev < STATIC_MAX_DEVICES) ? g_static_ptr[dev] : nullptr;
+  }
+  if (!base) { eh_set_error("static staging of device %d not initialised", dev); return EH_ERR_HIP; }
+  EH_HIP(hipSetDevice(dev));
+  // every count is written by the kernel, so nothing in the array needs clearing
+  uint32_t *ra = static_cast<uint32_t *>(base);
+  uint32_t *rb = ra + EH_STATIC_MAX_PAIRS;
+  unsigned long long *c = reinterpret_cast<unsigned long long *>(rb + EH_STATIC_MAX_PAIRS);
+  return isect_rows_run(m, a_rows, b_rows, npairs, counts, ra, rb, c);
 }
