@@ -285,6 +285,64 @@ def test_communities_density():
     assert len(k[0]) == 100000 * 4 // 2 + got
 
 
+def test_chung_lu_cdf_closed_form():
+    """orc_chung_lu_cdf (the Chung-Lu weights w_i = (i+1)^-s, s = 1/(gamma-1), as a 2^32
+    fixed-point CDF): cdf[i] = floor(2^32 H(i+1, s) / H(n, s)) with H(k, s) = sum_{t<=k} t^-s
+    the generalized harmonic number, evaluated here in closed form through the Hurwitz zeta
+    function, H(k, s) = zeta(s) - zeta(s, k+1) (mpmath, 30 digits) -- no summation shared with
+    the oracle.  A wrong exponent, index offset or normalisation moves these by millions."""
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 30
+    two32 = mpmath.mpf(2) ** 32
+    for n, gamma in ((1000000, 2.1), (1000, 2.5), (50, 3.0)):
+        s = mpmath.mpf(1) / (mpmath.mpf(gamma) - 1)
+        cdf = O.chung_lu_cdf(n, gamma)
+        zs = mpmath.zeta(s)
+        Hn = zs - mpmath.zeta(s, n + 1)
+        for i in sorted({0, 1, 2, 9, n // 7, n // 2, n - 2}):
+            exact = int(mpmath.floor(two32 * (zs - mpmath.zeta(s, i + 2)) / Hn))
+            assert abs(int(cdf[i]) - exact) <= 2, (n, gamma, i, int(cdf[i]), exact)
+        assert int(cdf[-1]) == 1 << 32
+        assert np.all(np.diff(cdf.astype(np.int64)) > 0)
+
+
+@pytest.mark.parametrize("gamma", [2.1, 2.5, 3.0])
+def test_chung_lu_degree_tail(gamma):
+    """The generated degrees follow the power law: vertex of weight rank r has expected degree
+    proportional to r^(-1/(gamma-1)), so the rank-degree slope over ranks 100..10,000 gives
+    gamma back -- within 0.05 for the draws themselves (mean degree exactly 2m/n = 32), within
+    0.15 once duplicate draws are merged (the merge caps the hubs, as any simple graph must)."""
+    n = 100000
+    src, dst = O.gen_chung_lu(n, 32, gamma, 3)
+    r = np.arange(1, n + 1)
+    sel = (r >= 100) & (r <= 10000)
+    mdeg = np.bincount(src, minlength=n) + np.bincount(dst, minlength=n)
+    assert mdeg.mean() == 32.0
+    slope, _ = np.polyfit(np.log(r[sel]), np.log(np.sort(mdeg)[::-1][sel]), 1)
+    assert abs((1 - 1 / slope) - gamma) < 0.05
+    lo, hi = np.minimum(src, dst), np.maximum(src, dst)
+    key = np.unique(lo[lo != hi].astype(np.uint64) << np.uint64(32) | hi[lo != hi])
+    deg = np.bincount((key >> np.uint64(32)).astype(np.int64), minlength=n) + \
+        np.bincount((key & np.uint64(0xFFFFFFFF)).astype(np.int64), minlength=n)
+    slope, _ = np.polyfit(np.log(r[sel]), np.log(np.sort(deg)[::-1][sel]), 1)
+    assert abs((1 - 1 / slope) - gamma) < 0.15
+
+
+def test_communities_placement():
+    """Community c covers ids [c * (n / ncomm), + csize) and holds nothing else."""
+    n, ncomm, csize = 100000, 20, 256
+    L = O.lib()
+    k = L.orc_gen_communities(n, ncomm, csize, 3, None, None)
+    cs, cd = np.zeros(k, np.uint32), np.zeros(k, np.uint32)
+    L.orc_gen_communities(n, ncomm, csize, 3, cs.ctypes.data, cd.ctypes.data)
+    stride = n // ncomm
+    assert np.all(cs < cd)
+    assert np.array_equal(cs // stride, cd // stride)
+    assert np.all(cs % stride < csize) and np.all(cd % stride < csize)
+    per = np.bincount(cs // stride, minlength=ncomm)
+    assert per.min() > 0.45 * csize * (csize - 1) / 2 and per.max() < 0.55 * csize * (csize - 1) / 2
+
+
 def test_batch_params_ranges():
     ns, ms = [], []
     for g in range(4096):
