@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define EH_ABI_VERSION 1
+#define EH_ABI_VERSION 2
 
 typedef enum {
   EH_OK = 0,
@@ -134,19 +134,39 @@ typedef struct {
   int32_t local_max;      /* TUNED 4-cliques on oriented matrices: rows with more than small_degree
                              and at most this many entries count the triangles of their local
                              graph (bit-rows in LDS); 0 = default (512), <0 = off, at most 512 */
+  int32_t time_parts;     /* nonzero: record HIP events around each kernel class of the pass and
+                             report their device times in eh_run_stats.part_ms */
+  int32_t reserved;
 } eh_run_opts;
+
+/* Kernel classes of one counting pass (indices of eh_run_stats.part_ms). */
+enum {
+  EH_PART_SHORT_ROWS = 0, /* edge-centric short rows (k_small_rows, k_small_cliques4)        */
+  EH_PART_BSTAT = 1,      /* B-stationary pairs below the split (k_bstat)                      */
+  EH_PART_BSTAT_HASH = 2, /* hashed B-stationary pairs above the split (k_bshg)                */
+  EH_PART_LOCAL_MID = 3,  /* 4-cliques of rows up to 128 entries by their local graph (k_local4) */
+  EH_PART_LOCAL_BIG = 4,  /* the same for rows of 129..512 entries                             */
+  EH_PART_TASKS_BIG = 5,  /* A-side tasks of the large-LDS class (k_count)                    */
+  EH_PART_TASKS = 6,      /* A-side tasks (k_count); also the whole plain schedule             */
+  EH_PARTS = 8
+};
 
 typedef struct {
   double kernel_ms;       /* average device time of one counting pass (HIP events) */
   double sched_ms;        /* device time to build the task schedule (0 if cached) */
   uint64_t tasks;
-  uint64_t alg_bytes;     /* algorithmic bytes of one pass (canonical uint32 CSR) */
+  uint64_t alg_bytes;     /* algorithmic bytes of one pass on the canonical uint32 CSR: triangles
+                             B_TC = 4 (m + W) + 8 (n + 1), m kept entries, W = sum over kept (i,j)
+                             of card(j); 4-cliques B_4C = B_TC + 4 sum over triangles i>j>k of
+                             card(k); 0 for deeper cliques */
   uint64_t pairs[4];      /* (i,j) pairs by B-side type: U, PS, BS, empty */
   uint64_t work_units;    /* elements + words touched on the B side */
   int32_t schedule;
   int32_t lanes_per_row;
   uint32_t task_work;
   uint32_t lds_words;
+  double part_ms[EH_PARTS]; /* with eh_run_opts.time_parts: average device time of each kernel
+                               class (EH_PART_*) of a pass, 0 for classes the pass did not run */
 } eh_run_stats;
 
 /* Generator specification (this engine's seeded synthetic graphs). */
@@ -335,6 +355,9 @@ eh_status eh_batch_graph_download(eh_batch *b, uint32_t g, uint64_t *row_off, ui
  * the graph is recorded again whenever either pass changes. */
 eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tri_opts,
                          const eh_run_opts *k4_opts, uint64_t *tri, uint64_t *k4, double *ms);
+/* No reference counterpart: the algorithmic bytes of the batch's passes, summed over its
+ * graphs (B_TC of the triangle pass, B_4C of the 4-clique pass, as in eh_run_stats). */
+eh_status eh_batch_alg_bytes(eh_batch *b, uint64_t *tri_bytes, uint64_t *k4_bytes);
 void eh_batch_free(eh_batch *b);
 
 /* ---------------------------------------------------------------- learned selector */

@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libeh.so")
 
 EH_BITSET, EH_PSHORT, EH_UINTEGER, EH_BITSET_NEW, EH_NEW_TYPE = 0, 1, 2, 7, 8
+# kernel classes of a counting pass, in the order of eh_run_stats.part_ms (EH_PART_*)
+PARTS = ("k_small_rows", "k_bstat", "k_bshg", "k_local4_mid", "k_local4_big", "k_count_big", "k_count")
 STATIC_MAX_PAIRS = 1 << 20  # EH_STATIC_MAX_PAIRS: pairs eh_intersect_rows_static stages at most
 TYPE_NAMES = {0: "bitset", 1: "pshort", 2: "uint"}
 LAYOUTS = {"uint": 0, "pshort": 1, "bs": 2, "bitset": 2, "hybrid": 3, "hybrid3": 4, "learned": 5,
@@ -50,7 +52,7 @@ class RunOpts(C.Structure):
     _fields_ = [("schedule", C.c_int32), ("lanes_per_row", C.c_int32), ("task_work", C.c_uint32),
                 ("lds_words", C.c_uint32), ("waves_per_block", C.c_int32), ("repeat", C.c_int32),
                 ("small_degree", C.c_int32), ("stream_ratio", C.c_int32), ("bstat_split", C.c_int32),
-                ("local_max", C.c_int32)]
+                ("local_max", C.c_int32), ("time_parts", C.c_int32), ("reserved", C.c_int32)]
 
 
 class BfsInfo(C.Structure):
@@ -65,13 +67,14 @@ class RunStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("sched_ms", C.c_double), ("tasks", C.c_uint64),
                 ("alg_bytes", C.c_uint64), ("pairs", C.c_uint64 * 4), ("work_units", C.c_uint64),
                 ("schedule", C.c_int32), ("lanes_per_row", C.c_int32), ("task_work", C.c_uint32),
-                ("lds_words", C.c_uint32)]
+                ("lds_words", C.c_uint32), ("part_ms", C.c_double * 8)]
 
     def as_dict(self):
         return {"kernel_ms": self.kernel_ms, "sched_ms": self.sched_ms, "tasks": self.tasks,
                 "alg_bytes": self.alg_bytes, "pairs": list(self.pairs), "work_units": self.work_units,
                 "schedule": self.schedule, "lanes_per_row": self.lanes_per_row,
-                "task_work": self.task_work, "lds_words": self.lds_words}
+                "task_work": self.task_work, "lds_words": self.lds_words,
+                "part_ms": {name: self.part_ms[q] for q, name in enumerate(PARTS) if self.part_ms[q] > 0}}
 
 
 class GenSpec(C.Structure):
@@ -90,7 +93,7 @@ EXPORTS = [
     "eh_graph_free", "eh_matrix_build", "eh_matrix_build_host", "eh_matrix_info_get", "eh_matrix_row",
     "eh_matrix_download", "eh_matrix_row_types", "eh_matrix_free", "eh_intersect", "eh_intersect_rows",
     "eh_intersect_rows_static", "eh_union", "eh_difference", "eh_triangle_count", "eh_clique_count", "eh_bfs_path", "eh_count_profile", "eh_batch_generate", "eh_batch_info",
-    "eh_batch_graph_params", "eh_batch_graph_download", "eh_batch_count", "eh_batch_free",
+    "eh_batch_graph_params", "eh_batch_graph_download", "eh_batch_count", "eh_batch_alg_bytes", "eh_batch_free",
     "eh_selector_set_params", "eh_selector_get_params", "eh_matrix_features",
     "eh_matrix_profile_layouts", "eh_mlp_forward", "eh_mlp_loss_grad", "eh_selector_train",
 ]
@@ -162,6 +165,7 @@ def lib():
     L.eh_batch_graph_download.argtypes = [_vp, C.c_uint32, _vp, _vp, C.c_uint64]
     L.eh_batch_count.argtypes = [_vp, C.c_int, C.POINTER(RunOpts), C.POINTER(RunOpts), _vp, _vp,
                                  C.POINTER(C.c_double)]
+    L.eh_batch_alg_bytes.argtypes = [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.eh_batch_free.argtypes = [_vp]
     L.eh_batch_free.restype = None
     L.eh_selector_set_params.argtypes = [_vp, _vp]
@@ -187,8 +191,9 @@ def _ptr(a):
 
 
 def _opts(schedule="tuned", lanes=0, task_work=0, lds_words=0, waves=0, repeat=1, small_degree=0,
-          stream_ratio=0, bstat=0, local_max=0):
+          stream_ratio=0, bstat=0, local_max=0, time_parts=False):
     o = RunOpts()
+    o.time_parts = int(bool(time_parts))
     o.bstat_split = bstat
     o.local_max = local_max
     o.small_degree = small_degree
@@ -528,6 +533,12 @@ class Batch:
         e = C.c_uint64()
         _check(lib().eh_batch_info(self.h, C.byref(g), C.byref(v), C.byref(e)))
         return {"graphs": g.value, "vertices": v.value, "edges": e.value}
+
+    def alg_bytes(self):
+        """Algorithmic bytes of the triangle and 4-clique passes, summed over the graphs."""
+        t, k = C.c_uint64(), C.c_uint64()
+        _check(lib().eh_batch_alg_bytes(self.h, C.byref(t), C.byref(k)))
+        return int(t.value), int(k.value)
 
     def params(self):
         ng = self.info()["graphs"]

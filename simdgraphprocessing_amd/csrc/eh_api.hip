@@ -74,10 +74,22 @@ extern "C" eh_status eh_ctx_create(int device, eh_ctx **out) {
   return EH_OK;
 }
 
+eh_status ctx_part_events(eh_ctx *ctx) {
+  for (PartEv &p : ctx->parts) {
+    if (!p.b) EH_HIP(hipEventCreate(&p.b));
+    if (!p.e) EH_HIP(hipEventCreate(&p.e));
+  }
+  return EH_OK;
+}
+
 extern "C" void eh_ctx_destroy(eh_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  for (PartEv &p : ctx->parts) {
+    if (p.b) (void)hipEventDestroy(p.b);
+    if (p.e) (void)hipEventDestroy(p.e);
+  }
   ctx->scratch.release();
   ctx->counter.release();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

@@ -360,4 +360,17 @@ extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tr
   return EH_OK;
 }
 
+extern "C" eh_status eh_batch_alg_bytes(eh_batch *b, uint64_t *tri_bytes, uint64_t *k4_bytes) {
+  if (!b || !b->M || !tri_bytes || !k4_bytes) { eh_set_error("eh_batch_alg_bytes: null argument"); return EH_ERR_INVALID; }
+  EH_HIP(hipSetDevice(b->ctx->device));
+  const eh_matrix *M = b->M;
+  // per graph B_TC = 4 (m_g + W_g) + 8 (n_g + 1); the arena's m and W are the sums over graphs
+  const uint64_t tri = 4ull * (M->card + M->wedges) + 8ull * ((uint64_t)b->nv_total + b->ng);
+  uint64_t tw = 0;
+  EH_CHECK(matrix_triangle_weight(b->M, b->ng, &tw));
+  *tri_bytes = tri;
+  *k4_bytes = tri + 4ull * tw;
+  return EH_OK;
+}
+
 extern "C" void eh_batch_free(eh_batch *b) { delete b; }

@@ -2529,9 +2529,20 @@ int blocks_per_cu(const void *kern, int threads, size_t lds_bytes) {
   return per_cu;
 }
 
+// device-time marks around one kernel class of a pass (eh_run_opts.time_parts)
+inline void part_begin(eh_ctx *ctx, PartEv *pe) {
+  if (pe) (void)hipEventRecord(pe->b, ctx->stream);
+}
+inline void part_end(eh_ctx *ctx, PartEv *pe) {
+  if (pe) {
+    (void)hipEventRecord(pe->e, ctx->stream);
+    pe->used = true;
+  }
+}
+
 template <int K, bool W>
 eh_status launch_count(eh_ctx *ctx, const CountArgs &base, const uint4 *tasks, uint64_t ntasks,
-                       uint32_t lds_words, int waves, uint32_t *d_next, uint64_t task_base) {
+                       uint32_t lds_words, int waves, uint32_t *d_next, uint64_t task_base, PartEv *pe) {
   auto kern = !base.tuned ? k_count<K, W, false, false>
                           : base.blocks ? k_count<K, W, true, true> : k_count<K, W, true, false>;
   if (ntasks == 0) return EH_OK;
@@ -2548,13 +2559,16 @@ eh_status launch_count(eh_ctx *ctx, const CountArgs &base, const uint4 *tasks, u
   const uint64_t need = ceil_div(ntasks, (uint64_t)waves);
   if (blocks > need) blocks = need;
   k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
+  part_begin(ctx, pe);
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a);
   EH_HIP(hipGetLastError());
+  part_end(ctx, pe);
   return EH_OK;
 }
 
 template <int K, bool W>
-eh_status launch_small(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows, uint64_t nrows, uint32_t *d_next) {
+eh_status launch_small(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows, uint64_t nrows, uint32_t *d_next,
+                       PartEv *pe) {
   if (nrows == 0) return EH_OK;
   CountArgs a = base;
   a.next = d_next;
@@ -2567,13 +2581,15 @@ eh_status launch_small(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows,
   const uint64_t need = ceil_div(ceil_div(nrows, 64), (uint64_t)waves);
   if (blocks > need) blocks = need;
   k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
+  part_begin(ctx, pe);
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a, rows, (uint32_t)nrows);
   EH_HIP(hipGetLastError());
+  part_end(ctx, pe);
   return EH_OK;
 }
 
 eh_status launch_bstat(eh_ctx *ctx, const CountArgs &base, const uint4 *btasks, uint64_t nbt,
-                       const uint2 *in_pairs, uint32_t bstat_j, uint32_t *d_next) {
+                       const uint2 *in_pairs, uint32_t bstat_j, uint32_t *d_next, PartEv *pe) {
   if (nbt == 0) return EH_OK;
   CountArgs a = base;
   a.next = d_next;
@@ -2589,14 +2605,17 @@ eh_status launch_bstat(eh_ctx *ctx, const CountArgs &base, const uint4 *btasks, 
   const uint64_t need = ceil_div(ceil_div(nbt, (uint64_t)a.grab), (uint64_t)waves);
   if (blocks > need) blocks = need;
   k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
+  part_begin(ctx, pe);
   hipLaunchKernelGGL(k_bstat, dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a, btasks, nbt,
                      in_pairs, bm_words);
   EH_HIP(hipGetLastError());
+  part_end(ctx, pe);
   return EH_OK;
 }
 
 template <uint32_t LMAX, bool GROUP_BLOCK>
-eh_status launch_local(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows, uint64_t nrows, uint32_t *d_next) {
+eh_status launch_local(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows, uint64_t nrows, uint32_t *d_next,
+                       PartEv *pe) {
   if (nrows == 0) return EH_OK;
   CountArgs a = base;
   a.next = d_next;
@@ -2607,14 +2626,17 @@ eh_status launch_local(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows,
   const uint64_t need = GROUP_BLOCK ? nrows : ceil_div(nrows, (uint64_t)waves);
   if (blocks > need) blocks = need;
   k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
+  part_begin(ctx, pe);
   hipLaunchKernelGGL((k_local4<LMAX, GROUP_BLOCK>), dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a,
                      rows, (uint32_t)nrows);
   EH_HIP(hipGetLastError());
+  part_end(ctx, pe);
   return EH_OK;
 }
 
 eh_status launch_bshg(eh_ctx *ctx, const CountArgs &base, const uint4 *htasks, uint64_t nht, const uint2 *hpairs,
-                      const uint32_t *hj, const uint64_t *hoff, const uint32_t *bstart, uint32_t *d_next) {
+                      const uint32_t *hj, const uint64_t *hoff, const uint32_t *bstart, uint32_t *d_next,
+                      PartEv *pe) {
   if (nht == 0) return EH_OK;
   CountArgs a = base;
   a.next = d_next;
@@ -2628,9 +2650,11 @@ eh_status launch_bshg(eh_ctx *ctx, const CountArgs &base, const uint4 *htasks, u
   const uint64_t need = ceil_div(ceil_div(nht, (uint64_t)a.grab), (uint64_t)waves);
   if (blocks > need) blocks = need;
   k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
+  part_begin(ctx, pe);
   hipLaunchKernelGGL(k_bshg, dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a, htasks, nht, hpairs,
                      hj, hoff, bstart);
   EH_HIP(hipGetLastError());
+  part_end(ctx, pe);
   return EH_OK;
 }
 
@@ -2641,7 +2665,10 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
                        uint64_t n_small, uint32_t lds_small, uint32_t lds_big, uint64_t *d_counts, uint32_t *d_next,
                        uint64_t *d_task_cycles, const uint4 *btasks, uint64_t n_btasks, const uint2 *in_pairs,
                        uint32_t bstat_j, const uint32_t *jstart, const uint4 *lo16, const uint32_t *local_rows0,
-                       uint64_t n_local0, const uint32_t *local_rows1, uint64_t n_local1, const Schedule *hs) {
+                       uint64_t n_local0, const uint32_t *local_rows1, uint64_t n_local1, const Schedule *hs,
+                       PartEv *pev) {
+  // pev (may be null): one pair of events per kernel class, indexed by EH_PART_*
+  auto pe = [pev](int q) -> PartEv * { return pev ? pev + q : nullptr; };
   CountArgs a;
   a.grab = tp.grab ? tp.grab : 1u;
   // d_next[7] is the pass's error word (the queue heads use d_next[0..4]); the caller
@@ -2671,39 +2698,77 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
     return EH_ERR_INVALID;
   }
   if (k == 4) {
-    EH_CHECK((launch_small<4, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2)));
-    EH_CHECK((launch_local<LOCAL4_MID, false>(m->ctx, a, local_rows0, n_local0, d_next + 3)));
-    EH_CHECK((launch_local<LOCAL4_MAX, true>(m->ctx, a, local_rows1, n_local1, d_next + 4)));
-    EH_CHECK((launch_count<4, false>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0)));
+    EH_CHECK((launch_small<4, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2, pe(EH_PART_SHORT_ROWS))));
+    EH_CHECK((launch_local<LOCAL4_MID, false>(m->ctx, a, local_rows0, n_local0, d_next + 3, pe(EH_PART_LOCAL_MID))));
+    EH_CHECK((launch_local<LOCAL4_MAX, true>(m->ctx, a, local_rows1, n_local1, d_next + 4, pe(EH_PART_LOCAL_BIG))));
+    EH_CHECK((launch_count<4, false>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0,
+                                     pe(EH_PART_TASKS_BIG))));
     EH_CHECK((launch_count<4, false>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,
-                                     n_big)));
+                                     n_big, pe(EH_PART_TASKS))));
   } else if (weighted) {
-    EH_CHECK((launch_small<3, true>(m->ctx, a, small_rows, n_small_rows, d_next + 2)));
-    EH_CHECK((launch_count<3, true>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0)));
+    EH_CHECK((launch_small<3, true>(m->ctx, a, small_rows, n_small_rows, d_next + 2, pe(EH_PART_SHORT_ROWS))));
+    EH_CHECK((launch_count<3, true>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0,
+                                    pe(EH_PART_TASKS_BIG))));
     EH_CHECK((launch_count<3, true>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,
-                                    n_big)));
+                                    n_big, pe(EH_PART_TASKS))));
   } else {
-    EH_CHECK((launch_small<3, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2)));
-    EH_CHECK(launch_bstat(m->ctx, a, btasks, n_btasks, in_pairs, bstat_j, d_next + 3));
+    EH_CHECK((launch_small<3, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2, pe(EH_PART_SHORT_ROWS))));
+    EH_CHECK(launch_bstat(m->ctx, a, btasks, n_btasks, in_pairs, bstat_j, d_next + 3, pe(EH_PART_BSTAT)));
     if (hs && hs->n_htasks)
       EH_CHECK(launch_bshg(m->ctx, a, hs->htasks.as<uint4>(), hs->n_htasks, hs->hpairs.as<uint2>(),
-                           hs->hj.as<uint32_t>(), hs->hoff.as<uint64_t>(), hs->bstart.as<uint32_t>(), d_next + 4));
-    EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0)));
+                           hs->hj.as<uint32_t>(), hs->hoff.as<uint64_t>(), hs->bstart.as<uint32_t>(), d_next + 4,
+                           pe(EH_PART_BSTAT_HASH)));
+    EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0,
+                                     pe(EH_PART_TASKS_BIG))));
     EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,
-                                     n_big)));
+                                     n_big, pe(EH_PART_TASKS))));
   }
   return EH_OK;
 }
 
 eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, Schedule &s,
-                          uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles) {
+                          uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles, PartEv *pev) {
   const uint4 *t = s.tasks.as<uint4>();
   return launch_parts(m, k, sched, tp, s.small_rows.as<uint32_t>(), s.n_small_rows, t, s.n_big, t + s.n_big,
                       s.n_small, s.lds_small, s.lds_big, d_counts, d_next, d_task_cycles, s.btasks.as<uint4>(),
                       s.n_btasks, s.in_pairs.as<uint2>(), s.bstat_j,
                       s.bstat_j ? s.jstart.as<uint32_t>() : nullptr, s.bstat_j ? s.lo16.as<uint4>() : nullptr,
                       s.local_rows[0].as<uint32_t>(), s.n_local[0], s.local_rows[1].as<uint32_t>(), s.n_local[1],
-                      &s);
+                      &s, pev);
+}
+
+// Sum over the kept pairs (i, j) of sum over x in row(i) & row(j) of card(x): over the
+// triangles i > j > x of an oriented matrix, the card(x) term of the 4-clique byte model
+// (the weighted triangle pass, without the B-stationary kernels, which count only).
+// Computed once per matrix (summed over a batch arena's graphs, `ncounts` of them).
+eh_status matrix_triangle_weight(eh_matrix *m, uint32_t ncounts, uint64_t *out) {
+  if (m->tw_valid) { *out = m->tri_weight; return EH_OK; }
+  eh_ctx *ctx = m->ctx;
+  hipStream_t st = ctx->stream;
+  eh_run_opts o = {};
+  o.schedule = EH_SCHED_TUNED;
+  o.bstat_split = -1;
+  const TuneParams tp = tune_defaults(m, EH_SCHED_TUNED, 3, &o);
+  Schedule s;
+  EH_CHECK(schedule_build(m, EH_SCHED_TUNED, 3, tp, s));
+  if (ncounts == 0) ncounts = 1;
+  DevBuf cnt, next;
+  EH_CHECK(cnt.alloc(8ull * ncounts));
+  EH_CHECK(next.alloc(32));
+  k_zero_u64<<<grid_for(ncounts), 256, 0, st>>>(cnt.as<unsigned long long>(), ncounts);
+  k_zero_u32<<<1, 64, 0, st>>>(next.as<uint32_t>() + 7, 1);
+  EH_CHECK(schedule_launch(m, -3, EH_SCHED_TUNED, tp, s, cnt.as<uint64_t>(), next.as<uint32_t>(), nullptr, nullptr));
+  uint32_t dev_err = 0;
+  std::vector<uint64_t> h(ncounts);
+  EH_HIP(hipMemcpy(h.data(), cnt.p, 8ull * ncounts, hipMemcpyDeviceToHost));
+  EH_HIP(hipMemcpy(&dev_err, next.as<uint32_t>() + 7, 4, hipMemcpyDeviceToHost));
+  if (dev_err) { eh_set_error("weighted triangle pass: an LDS hash table had no free slot"); return EH_ERR_INVALID; }
+  uint64_t sum = 0;
+  for (uint64_t v : h) sum += v;
+  m->tri_weight = sum;
+  m->tw_valid = true;
+  *out = sum;
+  return EH_OK;
 }
 
 // k: 3 triangles, 4 four-cliques, -3 weighted triangle pass (sum of card(x) over
@@ -2737,18 +2802,33 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
   }
   EH_CHECK(next.alloc(32));
   const int repeat = (opts && opts->repeat > 1) ? opts->repeat : 1;
+  PartEv *pev = nullptr;
+  if (opts && opts->time_parts) {
+    EH_CHECK(ctx_part_events(ctx));
+    pev = ctx->parts;
+  }
+  double part_ms[EH_PARTS] = {};
   float total_ms = 0;
   for (int r = 0; r < repeat; r++) {
     k_zero_u64<<<grid_for(ncounts ? ncounts : 1), 256, 0, st>>>(dc, ncounts ? ncounts : 1);
     k_zero_u32<<<1, 64, 0, st>>>(next.as<uint32_t>() + 7, 1);  // the pass's error word
+    if (pev)
+      for (int q = 0; q < EH_PARTS; q++) pev[q].used = false;
     EH_HIP(hipEventRecord(ctx->ev0, st));
     EH_CHECK(schedule_launch(m, k, sched, tp, s, reinterpret_cast<uint64_t *>(dc), next.as<uint32_t>(),
-                             d_task_cycles));
+                             d_task_cycles, pev));
     EH_HIP(hipEventRecord(ctx->ev1, st));
     EH_HIP(hipEventSynchronize(ctx->ev1));
     float ms = 0;
     EH_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     total_ms += ms;
+    if (pev)
+      for (int q = 0; q < EH_PARTS; q++)
+        if (pev[q].used) {
+          float pm = 0;
+          EH_HIP(hipEventElapsedTime(&pm, pev[q].b, pev[q].e));
+          part_ms[q] += pm;
+        }
   }
   uint32_t dev_err = 0;
   EH_HIP(hipMemcpy(&dev_err, next.as<uint32_t>() + 7, 4, hipMemcpyDeviceToHost));
@@ -2767,6 +2847,12 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
     stats->sched_ms = sched_ms;
     stats->tasks = s.n_big + s.n_small;
     stats->alg_bytes = 4ull * (m->card + m->wedges) + 8ull * ((uint64_t)m->n + 1);
+    if (k == 4) {
+      uint64_t tw = 0;
+      EH_CHECK(matrix_triangle_weight(m, ncounts, &tw));
+      stats->alg_bytes += 4ull * tw;
+    }
+    for (int q = 0; q < EH_PARTS; q++) stats->part_ms[q] = part_ms[q] / repeat;
     for (int q = 0; q < 4; q++) stats->pairs[q] = s.stats[q];
     stats->work_units = s.stats[4];
     stats->schedule = sched;

@@ -131,8 +131,15 @@ struct TuneParams {
   uint32_t local_max = 0;     // 4-cliques: rows above small_deg and up to this run k_local4 (0: off)
 };
 
+// device-time marks around one kernel class of a counting pass
+struct PartEv {
+  hipEvent_t b = nullptr, e = nullptr;
+  bool used = false;
+};
+
 struct eh_ctx {
   int device = 0;
+  PartEv parts[EH_PARTS];  // created on the first pass that asks for per-class times
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int num_cus = 0;
@@ -196,6 +203,8 @@ struct eh_matrix {
   uint64_t ref_row_bytes = 0;
   uint64_t wedges = 0;
   double build_ms = 0;
+  bool tw_valid = false;    // tri_weight computed
+  uint64_t tri_weight = 0;  // sum over triangles i > j > x of card(x) (4-clique byte model)
   DevBuf dcard, daoff, dunbr, ddesc, dbsw, dpsd, dbid, dbwd, dntu;
   DevBuf dgraph_of;  // u32 per row: graph id (batch arenas); empty => all 0
   Schedule sched[2][2];  // [triangles, 4-cliques][plain, tuned]
@@ -232,7 +241,7 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
                     uint32_t ncounts, uint64_t *count, eh_run_stats *stats, uint64_t *d_task_cycles);
 eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, Schedule &s);
 eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, Schedule &s,
-                          uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles);
+                          uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles, PartEv *pev = nullptr);
 TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts);
 eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, const uint32_t *small_rows,
                        uint64_t n_small_rows, const uint4 *tasks_big, uint64_t n_big, const uint4 *tasks_small,
@@ -241,7 +250,9 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
                        const uint2 *in_pairs = nullptr, uint32_t bstat_j = 0, const uint32_t *jstart = nullptr,
                        const uint4 *lo16 = nullptr, const uint32_t *local_rows0 = nullptr, uint64_t n_local0 = 0,
                        const uint32_t *local_rows1 = nullptr, uint64_t n_local1 = 0,
-                       const Schedule *hs = nullptr);
+                       const Schedule *hs = nullptr, PartEv *pev = nullptr);
+eh_status matrix_triangle_weight(eh_matrix *m, uint32_t ncounts, uint64_t *out);
+eh_status ctx_part_events(eh_ctx *ctx);
 
 // records this device's address of eh_intersect_rows_static's staging array (device current)
 eh_status static_scratch_init(int device);

@@ -206,3 +206,54 @@ def test_complete_graph_full_remainders(ctx):
         assert M.triangle_count("plain")[0] == comb(n, 3), pol
         for local in (0, -1):
             assert M.clique_count(4, "tuned", local_max=local)[0] == comb(n, 4), (pol, local)
+
+
+def _oriented_bytes(n, off, nbr):
+    """B_TC and B_4C of an oriented CSR by sparse algebra: with A[i, j] = 1 for kept (i, j),
+    T = (A A) o A counts for every kept (i, x) the j with i > j > x closing a triangle, so the
+    4-clique term is 4 sum over (i, x) of T[i, x] card(x)."""
+    import scipy.sparse as sp
+    off = np.asarray(off, np.int64)
+    card = np.diff(off)
+    rows = np.repeat(np.arange(n), card)
+    A = sp.csr_matrix((np.ones(len(nbr), np.int64), (rows, np.asarray(nbr, np.int64))), shape=(n, n))
+    m, W = len(nbr), int(card[np.asarray(nbr, np.int64)].sum())
+    b_tc = 4 * (m + W) + 8 * (n + 1)
+    T = (A @ A).multiply(A).tocoo()
+    tw = int((T.data * card[T.col]).sum())
+    return b_tc, b_tc + 4 * tw, int(T.data.sum())
+
+
+def test_alg_bytes_and_kernel_class_times(ctx, facebook):
+    """eh_run_stats.alg_bytes is B_TC for triangles and B_4C for 4-cliques; with time_parts the
+    per-class device times cover the pass and the count is unchanged."""
+    import simdgraphprocessing_amd as eh
+    n = facebook["n"]
+    M = ctx.matrix_from_csr(n, facebook["off"], facebook["nbr"], "lower", "hybrid")
+    off, nbr = M.download()
+    b_tc, b_4c, tri = _oriented_bytes(n, off, nbr)
+    assert tri == FB_TRIANGLES
+    for sched in ("plain", "tuned"):
+        t, st = M.triangle_count(sched, time_parts=True, repeat=3)
+        assert t == FB_TRIANGLES and st["alg_bytes"] == b_tc
+        parts = st["part_ms"]
+        assert parts and set(parts) <= set(eh.PARTS), parts
+        assert sum(parts.values()) <= st["kernel_ms"] * 1.02 + 0.02, (parts, st["kernel_ms"])
+        k, st4 = M.clique_count(4, sched, time_parts=True)
+        assert k == FB_4CLIQUES and st4["alg_bytes"] == b_4c
+    assert set(M.triangle_count("tuned", time_parts=True)[1]["part_ms"]) >= {"k_bstat"}
+    assert M.triangle_count("tuned")[1]["part_ms"] == {}
+
+
+def test_batch_alg_bytes(ctx):
+    """The batch's byte counts are the per-graph B_TC and B_4C summed over its graphs."""
+    b = ctx.generate_batch(7, 6)
+    tri_b, k4_b = b.alg_bytes()
+    exp_t = exp_k = 0
+    for g in range(b.info()["graphs"]):
+        n, off, nbr = b.graph(g)  # graph g's rows, relabelled inside the graph, local ids
+        ooff, onbr = O.orient_lower(n, off, nbr)
+        t, k, _ = _oriented_bytes(n, ooff, onbr)
+        exp_t += t
+        exp_k += k
+    assert (tri_b, k4_b) == (exp_t, exp_k)
