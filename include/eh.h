@@ -170,13 +170,19 @@ typedef struct {
 } eh_run_stats;
 
 /* Generator specification (this engine's seeded synthetic graphs). */
-typedef enum { EH_GEN_ER = 0, EH_GEN_RMAT = 1, EH_GEN_CHUNG_LU = 2 } eh_gen_kind;
+typedef enum {
+  EH_GEN_ER = 0,       /* m pair draws (loops and repeats dropped by the canonicaliser) */
+  EH_GEN_RMAT = 1,
+  EH_GEN_CHUNG_LU = 2,
+  EH_GEN_ER_EXACT = 3  /* G(n, m): exactly m distinct non-loop pairs, the first m distinct
+                          pairs of EH_GEN_ER's draw stream */
+} eh_gen_kind;
 typedef struct {
   int32_t kind;          /* eh_gen_kind */
   uint32_t n;            /* vertices (ER, Chung-Lu) */
   uint32_t scale;        /* RMAT: n = 2^scale */
   uint32_t edge_factor;  /* RMAT: draws = edge_factor * n */
-  uint64_t m;            /* ER: pair draws; Chung-Lu: endpoint-pair draws */
+  uint64_t m;            /* ER: pair draws; ER_EXACT: distinct pairs; Chung-Lu: endpoint-pair draws */
   uint64_t seed;
   double a, b, c;        /* RMAT quadrant probabilities (d = 1-a-b-c) */
   double gamma;          /* Chung-Lu degree exponent */

@@ -411,6 +411,25 @@ def gen_er(n, m, seed, stream=0):
     return src, dst
 
 
+def gen_er_exact(n, m, seed):
+    """G(n, m) with exactly m distinct non-loop pairs: the first m distinct pairs of gen_er's
+    draw stream (first occurrences by np.unique, ordered by draw index).  Returns (src, dst)
+    with src < dst, in first-draw order."""
+    assert m <= n * (n - 1) // 2
+    K = m + m // 4 + 4096
+    while True:
+        src, dst = gen_er(n, K, seed)
+        lo, hi = np.minimum(src, dst).astype(np.uint64), np.maximum(src, dst).astype(np.uint64)
+        keys = (lo << np.uint64(32)) | hi
+        keys[src == dst] = np.uint64(0xFFFFFFFFFFFFFFFF)
+        uk, first = np.unique(keys, return_index=True)
+        first = np.sort(first[uk != np.uint64(0xFFFFFFFFFFFFFFFF)])
+        if len(first) >= m:
+            first = first[:m]
+            return lo[first].astype(np.uint32), hi[first].astype(np.uint32)
+        K *= 2
+
+
 def rmat_thresholds(a=0.57, b=0.19, c=0.19):
     thr = np.zeros(3, np.uint32)
     lib().orc_rmat_thresholds(a, b, c, thr)

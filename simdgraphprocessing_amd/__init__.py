@@ -247,7 +247,7 @@ class Context:
 
     def generate(self, kind, relabel=True, **kw):
         s = GenSpec()
-        s.kind = {"er": 0, "rmat": 1, "chung_lu": 2}[kind]
+        s.kind = {"er": 0, "rmat": 1, "chung_lu": 2, "er_exact": 3}[kind]
         s.relabel = int(relabel)
         s.a, s.b, s.c = kw.get("a", 0.57), kw.get("b", 0.19), kw.get("c", 0.19)
         s.gamma = kw.get("gamma", 2.1)
@@ -264,6 +264,11 @@ class Context:
 
     def generate_er(self, n, m, seed, relabel=True):
         return self.generate("er", relabel, n=n, m=m, seed=seed)
+
+    def generate_er_exact(self, n, m, seed, relabel=True):
+        """G(n, m): exactly m distinct non-loop pairs, the first m distinct pairs of generate_er's
+        draw stream."""
+        return self.generate("er_exact", relabel, n=n, m=m, seed=seed)
 
     def generate_rmat(self, scale, edge_factor, seed, a=0.57, b=0.19, c=0.19, relabel=True):
         return self.generate("rmat", relabel, scale=scale, edge_factor=edge_factor, seed=seed, a=a, b=b, c=c)

@@ -30,6 +30,38 @@ __global__ void k_gen_er(uint64_t *keys, uint64_t m, uint32_t n, uint64_t key) {
   }
 }
 
+// G(n, m) with exactly m distinct pairs: the ER draws with their draw index
+__global__ void k_gen_er_idx(uint64_t *keys, uint32_t *idx, uint64_t m, uint32_t n, uint64_t key) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t r = eh_draw(key, e);
+    keys[e] = undirected_key(eh_scale32((uint32_t)r, n), eh_scale32((uint32_t)(r >> 32), n));
+    idx[e] = (uint32_t)e;
+  }
+}
+
+// after a stable sort by key: the first entry of each pair's run holds its first draw
+__global__ void k_first_draw_flags(const uint64_t *keys, uint64_t m, uint32_t *flag) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    flag[e] = (keys[e] != EH_SENTINEL && (e == 0 || keys[e] != keys[e - 1])) ? 1u : 0u;
+}
+
+__global__ void k_first_draw_scatter(const uint32_t *flag, const uint64_t *pos, const uint32_t *idx, uint64_t m,
+                                     uint32_t *first) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    if (flag[e]) first[pos[e]] = idx[e];
+}
+
+__global__ void k_er_keys_of_draws(const uint32_t *draws, uint64_t m, uint32_t n, uint64_t key, uint64_t *keys) {
+  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < m;
+       q += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t r = eh_draw(key, draws[q]);
+    keys[q] = undirected_key(eh_scale32((uint32_t)r, n), eh_scale32((uint32_t)(r >> 32), n));
+  }
+}
+
 __global__ void k_gen_rmat(uint64_t *keys, uint64_t m, uint32_t scale, uint32_t t0, uint32_t t1,
                            uint32_t t2, uint64_t key) {
   const uint64_t per = (scale + 1) / 2;
@@ -364,6 +396,48 @@ eh_status graph_relabel(eh_graph *g) {
 
 // ---------------------------------------------------------------- C ABI
 
+// The first m distinct non-loop pairs of the ER draw stream of (n, seed) into keys (m of
+// them): K draws are sorted stably by pair with their draw index, the first draw of every
+// pair is kept, and the m smallest first draws name the pairs.  K grows until the draws
+// hold m distinct pairs; any K that does gives the same m pairs.
+static eh_status gen_er_exact(eh_ctx *ctx, uint32_t n, uint64_t m, uint64_t seed, DevBuf &keys) {
+  hipStream_t st = ctx->stream;
+  const uint64_t key = eh_stream_key(seed, 0);
+  for (uint64_t K = m + m / 4 + 4096;; K *= 2) {
+    if (K > 0xFFFFFFFFull) { eh_set_error("ER: %llu distinct pairs need more than 2^32 draws", (unsigned long long)m); return EH_ERR_UNSUPPORTED; }
+    DevBuf dk, dk2, di, di2, flag, pos;
+    EH_CHECK(dk.alloc(8 * K));
+    EH_CHECK(dk2.alloc(8 * K));
+    EH_CHECK(di.alloc(4 * K));
+    EH_CHECK(di2.alloc(4 * K));
+    EH_CHECK(flag.alloc(4 * K));
+    EH_CHECK(pos.alloc(8 * K));
+    k_gen_er_idx<<<grid_for(K), 256, 0, st>>>(dk.as<uint64_t>(), di.as<uint32_t>(), K, n, key);
+    EH_CHECK(prim_sort_pairs_u64_u32(ctx, dk.as<uint64_t>(), dk2.as<uint64_t>(), di.as<uint32_t>(),
+                                     di2.as<uint32_t>(), K));
+    k_first_draw_flags<<<grid_for(K), 256, 0, st>>>(dk.as<uint64_t>(), K, flag.as<uint32_t>());
+    EH_CHECK(prim_exscan_u32_to_u64(ctx, flag.as<uint32_t>(), pos.as<uint64_t>(), K));
+    uint64_t last_pos = 0;
+    uint32_t last_flag = 0;
+    EH_HIP(hipMemcpyAsync(&last_pos, pos.as<uint64_t>() + K - 1, 8, hipMemcpyDeviceToHost, st));
+    EH_HIP(hipMemcpyAsync(&last_flag, flag.as<uint32_t>() + K - 1, 4, hipMemcpyDeviceToHost, st));
+    EH_HIP(hipStreamSynchronize(st));
+    const uint64_t distinct = last_pos + last_flag;
+    if (distinct < m) continue;
+    DevBuf first, first2;
+    EH_CHECK(first.alloc(4 * distinct));
+    EH_CHECK(first2.alloc(4 * distinct));
+    k_first_draw_scatter<<<grid_for(K), 256, 0, st>>>(flag.as<uint32_t>(), pos.as<uint64_t>(), di.as<uint32_t>(), K,
+                                                       first.as<uint32_t>());
+    EH_CHECK(prim_sort_u32(ctx, first.as<uint32_t>(), first2.as<uint32_t>(), distinct));
+    EH_CHECK(keys.alloc(8 * m + 8));
+    k_er_keys_of_draws<<<grid_for(m), 256, 0, st>>>(first.as<uint32_t>(), m, n, key, keys.as<uint64_t>());
+    EH_HIP(hipGetLastError());
+    EH_HIP(hipStreamSynchronize(st));
+    return EH_OK;
+  }
+}
+
 extern "C" eh_status eh_graph_generate(eh_ctx *ctx, const eh_gen_spec *spec, eh_graph **out) {
   if (!ctx || !spec || !out) { eh_set_error("eh_graph_generate: null argument"); return EH_ERR_INVALID; }
   hipStream_t st = ctx->stream;
@@ -378,6 +452,16 @@ extern "C" eh_status eh_graph_generate(eh_ctx *ctx, const eh_gen_spec *spec, eh_
     EH_CHECK(keys.alloc(8 * nkeys + 8));
     k_gen_er<<<grid_for(nkeys), 256, 0, st>>>(keys.as<uint64_t>(), nkeys, n,
                                               eh_stream_key(spec->seed, 0));
+  } else if (spec->kind == EH_GEN_ER_EXACT) {
+    n = spec->n;
+    if (n < 2) { eh_set_error("ER needs n >= 2"); return EH_ERR_INVALID; }
+    if (spec->m > (uint64_t)n * (n - 1) / 2) {
+      eh_set_error("ER: %llu distinct pairs exceed the %llu a graph of %u vertices has", (unsigned long long)spec->m,
+                   (unsigned long long)n * (n - 1) / 2, n);
+      return EH_ERR_INVALID;
+    }
+    nkeys = spec->m;
+    EH_CHECK(gen_er_exact(ctx, n, nkeys, spec->seed, keys));
   } else if (spec->kind == EH_GEN_RMAT) {
     if (spec->scale < 1 || spec->scale > 31) { eh_set_error("RMAT scale out of range"); return EH_ERR_INVALID; }
     n = 1u << spec->scale;

@@ -221,6 +221,9 @@ eh_status clique_count_deep(eh_matrix *m, uint32_t k, uint64_t *count, double *m
 eh_status prim_sort_u64(eh_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int end_bit);
 eh_status prim_sort_pairs_u32(eh_ctx *ctx, uint32_t *keys, uint32_t *keys_tmp, uint32_t *vals,
                               uint32_t *vals_tmp, uint64_t n);
+eh_status prim_sort_pairs_u64_u32(eh_ctx *ctx, uint64_t *keys, uint64_t *keys_tmp, uint32_t *vals,
+                                  uint32_t *vals_tmp, uint64_t n);
+eh_status prim_sort_u32(eh_ctx *ctx, uint32_t *keys, uint32_t *tmp, uint64_t n);
 eh_status prim_unique_u64(eh_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n,
                           uint64_t *n_out);
 eh_status prim_exscan_u64(eh_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n);

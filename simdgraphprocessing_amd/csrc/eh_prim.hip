@@ -42,6 +42,31 @@ eh_status prim_sort_pairs_u32(eh_ctx *ctx, uint32_t *keys, uint32_t *keys_tmp, u
   return EH_OK;
 }
 
+// stable: values of equal keys keep their input order
+eh_status prim_sort_pairs_u64_u32(eh_ctx *ctx, uint64_t *keys, uint64_t *keys_tmp, uint32_t *vals,
+                                  uint32_t *vals_tmp, uint64_t n) {
+  if (n < 2) return EH_OK;
+  size_t bytes = 0;
+  EH_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys, keys_tmp, vals, vals_tmp, (size_t)n, 0, 64,
+                                   ctx->stream));
+  EH_CHECK(ensure_scratch(ctx, bytes));
+  EH_HIP(rocprim::radix_sort_pairs(ctx->scratch.p, bytes, keys, keys_tmp, vals, vals_tmp, (size_t)n,
+                                   0, 64, ctx->stream));
+  EH_HIP(hipMemcpyAsync(keys, keys_tmp, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  EH_HIP(hipMemcpyAsync(vals, vals_tmp, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  return EH_OK;
+}
+
+eh_status prim_sort_u32(eh_ctx *ctx, uint32_t *keys, uint32_t *tmp, uint64_t n) {
+  if (n < 2) return EH_OK;
+  size_t bytes = 0;
+  EH_HIP(rocprim::radix_sort_keys(nullptr, bytes, keys, tmp, (size_t)n, 0, 32, ctx->stream));
+  EH_CHECK(ensure_scratch(ctx, bytes));
+  EH_HIP(rocprim::radix_sort_keys(ctx->scratch.p, bytes, keys, tmp, (size_t)n, 0, 32, ctx->stream));
+  EH_HIP(hipMemcpyAsync(keys, tmp, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  return EH_OK;
+}
+
 eh_status prim_unique_u64(eh_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n,
                           uint64_t *n_out) {
   if (n == 0) { *n_out = 0; return EH_OK; }

@@ -39,11 +39,26 @@ def test_facebook_ref_row_bytes(ctx, facebook):
     assert M.info()["rows_by_type"]["bitset"] == om.type_counts()["bitset"]
 
 
+def test_er_exact_generator(ctx):
+    """G(n, m) with exactly m distinct pairs: the device picks the oracle's pairs; a complete
+    graph (m = n (n-1) / 2) gives C(n, 3) triangles and C(n, 4) 4-cliques."""
+    for n, m, seed in ((10, 45, 2), (300, 2000, 5), (5000, 60000, 9)):
+        g = ctx.generate_er_exact(n, m, seed)
+        off, nbr = g.download()
+        assert len(nbr) == 2 * m
+        ref = O.canonical_oriented(n, *O.gen_er_exact(n, m, seed))
+        assert np.array_equal(off, ref["relabel"][1]) and np.array_equal(nbr, ref["relabel"][2])
+    M = ctx.generate_er_exact(10, 45, 3).build_matrix("lower", "hybrid")
+    assert M.triangle_count("tuned")[0] == 120 and M.clique_count(4, "tuned")[0] == 210
+
+
 def test_er_c1_parity(ctx):
-    """ER G(10000, 100000 draws), seed 1: device generator + build + count vs oracle."""
-    g = ctx.generate_er(10000, 100000, 1)
+    """C1: ER G(10000, 100000 distinct non-loop pairs), seed 1: device generator + build + count
+    vs oracle."""
+    g = ctx.generate_er_exact(10000, 100000, 1)
     off, nbr = g.download()
-    src, dst = O.gen_er(10000, 100000, 1)
+    assert len(nbr) == 200000
+    src, dst = O.gen_er_exact(10000, 100000, 1)
     ref = O.canonical_oriented(10000, src, dst)
     roff, rnbr = ref["relabel"][1], ref["relabel"][2]
     assert np.array_equal(off, roff) and np.array_equal(nbr, rnbr)

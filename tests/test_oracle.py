@@ -328,6 +328,30 @@ def test_chung_lu_degree_tail(gamma):
     assert abs((1 - 1 / slope) - gamma) < 0.15
 
 
+def test_er_exact_first_distinct_pairs():
+    """gen_er_exact against a brute-force walk of the draw stream: keep each draw whose pair is
+    new and not a loop, stop at m pairs."""
+    for n, m, seed in ((50, 300, 1), (10, 45, 2), (1000, 5000, 7)):
+        src, dst = O.gen_er(n, 4 * m + 4096, seed)
+        seen, order = set(), []
+        for u, v in zip(src.tolist(), dst.tolist()):
+            if u == v:
+                continue
+            p = (min(u, v), max(u, v))
+            if p not in seen:
+                seen.add(p)
+                order.append(p)
+                if len(order) == m:
+                    break
+        assert len(order) == m
+        a, b = O.gen_er_exact(n, m, seed)
+        assert list(zip(a.tolist(), b.tolist())) == order
+    # C1: exactly 100,000 distinct pairs of 10,000 vertices
+    a, b = O.gen_er_exact(10000, 100000, 1)
+    assert len(a) == 100000 and np.all(a < b)
+    assert len(np.unique(a.astype(np.uint64) << np.uint64(32) | b)) == 100000
+
+
 def test_communities_placement():
     """Community c covers ids [c * (n / ncomm), + csize) and holds nothing else."""
     n, ncomm, csize = 100000, 20, 256
