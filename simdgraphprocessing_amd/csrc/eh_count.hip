@@ -70,8 +70,9 @@ struct CountArgs {
                                      // position are counted by k_bstat (null: none)
   const uint4 *lo16;                 // B-stationary: those entries as 16-bit ids (they are all
                                      // below the split, at most 65536), padded to 8 per vector
-  uint32_t *err;                     // device error word: a hash probe or insert that walked
-                                     // its whole table ORs ERR_HASH_FULL (the host reports it)
+  uint32_t *err;                     // device error word: a lane whose hash probe or insert
+                                     // walked a whole table ORs ERR_HASH_FULL into it once, when
+                                     // its task or kernel ends (the host reports it)
 };
 
 // bits of the device error word (CountArgs::err)
@@ -133,7 +134,8 @@ struct AState {
   uint32_t P;
   uint32_t *ht;           // HASH: LDS table of 2^hlg slots
   uint32_t hlg;
-  uint32_t *err;          // HASH: the pass's device error word
+  mutable uint32_t bad;   // HASH: this lane walked a whole table (kept in a register and
+                          // reported when the task ends, so no store sits in the probe loops)
 };
 
 template <uint32_t KIND>
@@ -153,19 +155,16 @@ __device__ __forceinline__ uint32_t memb(const AState &A, uint32_t x) {
     const uint32_t s = A.dstart[base], e = A.dstart[base + 1];
     return sorted_contains(A.list + s, e - s, x);
   } else if constexpr (KIND == KIND_HASH) {
-    // a table of 2^hlg slots holds at most half that many ids, so the walk meets a free
-    // slot; it is bounded by the table size all the same, and a walk that visits every
-    // slot flags the pass instead of spinning
+    // a table of 2^hlg slots holds at most half that many ids, and it is only probed when
+    // every insert found a slot (build_hash), so the walk meets a free slot
     const uint32_t mask = (1u << A.hlg) - 1u;
     uint32_t s = ((x * 2654435761u) >> (32u - A.hlg)) & mask;
-    for (uint32_t it = 0; it <= mask; it++) {
+    for (;;) {
       const uint32_t v = A.ht[s];
       if (v == x) return 1;
       if (v == HASH_EMPTY) return 0;
       s = (s + 1u) & mask;
     }
-    atomicOr(A.err, ERR_HASH_FULL);
-    return 0;
   } else {
     return sorted_contains(A.list, A.n, x);
   }
@@ -435,7 +434,7 @@ __device__ __forceinline__ uint32_t stage_a(const CountArgs &a, uint32_t kind, u
   A.P = 0;
   A.dpre = A.dstart = nullptr;
   A.ht = nullptr;
-  A.err = nullptr;
+  A.bad = 0u;
   A.hlg = 0;
   A.bm = L;
   if (kind == KIND_GLOBAL) {
@@ -503,7 +502,7 @@ __device__ __forceinline__ void make_cstate(const AState &A, uint32_t *clist, ui
   Cs.P = 0;
   Cs.dpre = Cs.dstart = nullptr;
   Cs.ht = nullptr;
-  Cs.err = nullptr;
+  Cs.bad = 0u;
   Cs.hlg = 0;
   Cs.aw0 = A.aw0;
   Cs.anw = A.anw;
@@ -684,8 +683,7 @@ __device__ __forceinline__ uint32_t bitmap16_hits(const uint32_t *bm, const uint
 // shift is at most 29); the mask restates that range at the use.
 __device__ __forceinline__ uint32_t hash_bucket(uint32_t e, uint32_t lg) { return (e * 2654435761u) >> (33u - lg); }
 
-__device__ __forceinline__ uint32_t hash_hits(const uint32_t *ht, uint32_t meta, uint32_t bound, const uint4 x,
-                                              uint32_t *err) {
+__device__ __forceinline__ uint32_t hash_hits(const uint32_t *ht, uint32_t meta, uint32_t bound, const uint4 x) {
   const uint32_t *T = ht + (meta & 0xFFFFu);
   const uint32_t lg = meta >> 16, bmask = (1u << (lg - 1u)) - 1u;
   const uint32_t e[4] = {x.x, x.y, x.z, x.w};
@@ -703,9 +701,9 @@ __device__ __forceinline__ uint32_t hash_hits(const uint32_t *ht, uint32_t meta,
     h += (uint32_t)(in & hit);
     pend |= (uint32_t)(in & !hit & (v1[q] != HASH_EMPTY)) << q;
   }
-  // every further step visits one more bucket of each pending id; after bmask steps each
-  // has seen all 2^(lg-1) buckets, and one still pending means a table without a free slot
-  for (uint32_t it = 0; pend && it < bmask; it++) {
+  // the table holds at most 2^(lg-1) ids in 2^(lg-1) two-slot buckets and is only probed
+  // when every insert found a slot, so some bucket has a free second slot and each walk ends
+  while (pend) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       b[q] = (pend >> q) & 1u ? (b[q] + 1u) & bmask : b[q];
@@ -719,7 +717,6 @@ __device__ __forceinline__ uint32_t hash_hits(const uint32_t *ht, uint32_t meta,
       if (p & (hit | (v1[q] == HASH_EMPTY))) pend &= ~(1u << q);
     }
   }
-  if (pend) atomicOr(err, ERR_HASH_FULL);
   return h;
 }
 
@@ -752,7 +749,7 @@ __device__ __forceinline__ PassAcc<W> pass_elem(const CountArgs &a, const AState
   } else if constexpr (PASS == PASS_STREAM16) {
     return (PassAcc<W>)bitmap16_hits(S.bm, a.lo16[(uint64_t)v1 + r]);
   } else if constexpr (PASS == PASS_STREAMH) {
-    return (PassAcc<W>)hash_hits(S.ht, v0, v2, reinterpret_cast<const uint4 *>(a.unbr)[(uint64_t)v1 + r], S.err);
+    return (PassAcc<W>)hash_hits(S.ht, v0, v2, reinterpret_cast<const uint4 *>(a.unbr)[(uint64_t)v1 + r]);
   } else {
     const uint32_t *base = (PASS == PASS_NTSTREAM) ? a.ntu : a.unbr;
     return stream_hits<KIND, W, LOCKSTEP>(S, a, reinterpret_cast<const uint4 *>(base)[(uint64_t)v1 + r]);
@@ -802,7 +799,7 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
         for (int u = 0; u < UNROLL; u++) {
           if (r0 + 64u * u < fq) {
             if constexpr (PASS == PASS_STREAM16) acc += (PassAcc<W>)bitmap16_hits(S.bm, v[u]);
-            else if constexpr (PASS == PASS_STREAMH) acc += (PassAcc<W>)hash_hits(S.ht, q0, q2, v[u], S.err);
+            else if constexpr (PASS == PASS_STREAMH) acc += (PassAcc<W>)hash_hits(S.ht, q0, q2, v[u]);
             else acc += stream_hits<KIND, W, true>(S, a, v[u]);
           }
         }
@@ -867,7 +864,7 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
           const PassAcc<W> keep = (PassAcc<W>)0 - (PassAcc<W>)(64u * (kb + u) + lane < tot);
           PassAcc<W> h;
           if constexpr (PASS == PASS_STREAM16) h = (PassAcc<W>)bitmap16_hits(S.bm, v[u]);
-          else if constexpr (PASS == PASS_STREAMH) h = (PassAcc<W>)hash_hits(S.ht, hm[u], hb[u], v[u], S.err);
+          else if constexpr (PASS == PASS_STREAMH) h = (PassAcc<W>)hash_hits(S.ht, hm[u], hb[u], v[u]);
           else h = stream_hits<KIND, W, true>(S, a, v[u]);
           acc += h & keep;
         }
@@ -966,9 +963,9 @@ __device__ __forceinline__ uint32_t hash_lg(uint32_t n) {
 
 // LDS hash table over a sorted list (linear probing, multiplicative hash)
 // (at most half the 2^lg slots are used; an insert that finds no free slot in a full walk
-// flags the pass through err instead of spinning)
+// sets `bad` instead of spinning)
 __device__ __forceinline__ void build_hash(const uint32_t *list, uint32_t n, uint32_t *ht, uint32_t lg,
-                                           uint32_t lane, uint32_t *err) {
+                                           uint32_t lane, uint32_t &bad) {
   const uint32_t cap = 1u << lg, mask = cap - 1u;
   for (uint32_t w = lane; w < cap; w += 64) ht[w] = HASH_EMPTY;
   wave_sync();
@@ -980,8 +977,10 @@ __device__ __forceinline__ void build_hash(const uint32_t *list, uint32_t n, uin
       s = (s + 1u) & mask;
       it++;
     }
-    if (it > mask) atomicOr(err, ERR_HASH_FULL);
+    bad |= (uint32_t)(it > mask);
   }
+  // wave-uniform: a table with a failed insert is never probed
+  bad = __ballot(bad != 0u) ? 1u : 0u;
   wave_sync();
 }
 
@@ -993,8 +992,8 @@ __device__ __forceinline__ uint32_t stage_tuned(const CountArgs &a, uint32_t kin
   const uint32_t lg = hash_lg(A.n);
   A.ht = L + used;
   A.hlg = lg;
-  A.err = a.err;
-  build_hash(A.list, A.n, A.ht, lg, lane, a.err);
+  A.bad = 0u;
+  build_hash(A.list, A.n, A.ht, lg, lane, A.bad);
   return used + (1u << lg);
 }
 
@@ -1024,9 +1023,10 @@ __device__ uint64_t task_cliques4_tuned(const CountArgs &a, const AState &A, uin
         make_cstate<KIND_LIST>(A, clist, nc, cmemb, lane, C);
         C.hlg = hash_lg(nc);
         C.ht = cmemb;
-        C.err = a.err;
-        build_hash(clist, nc, cmemb, C.hlg, lane, a.err);
-        acc += sum_isect<KIND_HASH, false>(a, C, clist, 0, nc, lane, meta);
+        C.bad = 0u;
+        build_hash(clist, nc, cmemb, C.hlg, lane, C.bad);
+        if (C.bad) A.bad = 1u;  // wave-uniform: C's table is not probed
+        else acc += sum_isect<KIND_HASH, false>(a, C, clist, 0, nc, lane, meta);
       } else {
         make_cstate<KIND_LIST>(A, clist, nc, cmemb, lane, C);
         acc += sum_isect<KIND_LIST, false>(a, C, clist, 0, nc, lane, meta);
@@ -1044,10 +1044,15 @@ __device__ __forceinline__ uint64_t run_task_tuned(const CountArgs &a, uint32_t 
   uint32_t *Ls = L + META_WORDS;
   AState A;
   const uint32_t used = stage_tuned(a, kind, i, lane, Ls, A);
+  if (A.bad) {  // row(i)'s hash table (wave-uniform flag): no free slot, nothing is probed
+    if (lane == 0) atomicOr(a.err, ERR_HASH_FULL);
+    return 0;
+  }
+  uint64_t r;
   if constexpr (K == 3) {
     switch (kind) {
       case KIND_BITMAP: return sum_isect<KIND_BITMAP, W, 4, BLK>(a, A, A.list, jb, je, lane, meta);
-      case KIND_HASH: return sum_isect<KIND_HASH, W, 4>(a, A, A.list, jb, je, lane, meta);
+      case KIND_HASH: r = sum_isect<KIND_HASH, W, 4>(a, A, A.list, jb, je, lane, meta); break;
       case KIND_LIST: return sum_isect<KIND_LIST, W, 4>(a, A, A.list, jb, je, lane, meta);
       // nothing staged: B's elements are searched in row(i) in HBM, the pairs flattened
       // like the staged kinds (the wave's slice holds the pass metadata only)
@@ -1055,11 +1060,14 @@ __device__ __forceinline__ uint64_t run_task_tuned(const CountArgs &a, uint32_t 
     }
   } else {
     switch (kind) {
-      case KIND_BITMAP: return task_cliques4_tuned<KIND_BITMAP, BLK>(a, A, jb, je, lane, Ls + used, meta);
-      case KIND_HASH: return task_cliques4_tuned<KIND_HASH, BLK>(a, A, jb, je, lane, Ls + used, meta);
-      default: return task_cliques4_tuned<KIND_LIST, BLK>(a, A, jb, je, lane, Ls + used, meta);
+      case KIND_BITMAP: r = task_cliques4_tuned<KIND_BITMAP, BLK>(a, A, jb, je, lane, Ls + used, meta); break;
+      case KIND_HASH: r = task_cliques4_tuned<KIND_HASH, BLK>(a, A, jb, je, lane, Ls + used, meta); break;
+      default: r = task_cliques4_tuned<KIND_LIST, BLK>(a, A, jb, je, lane, Ls + used, meta); break;
     }
   }
+  // a 4-clique task's C table that found no free slot
+  if (A.bad && lane == 0) atomicOr(a.err, ERR_HASH_FULL);
+  return r;
 }
 
 template <int K, bool W, bool TUNED, bool BLK>
@@ -1437,7 +1445,7 @@ __global__ void __launch_bounds__(256) k_bstat(CountArgs a, const uint4 *__restr
   S.P = 0;
   S.dpre = S.dstart = nullptr;
   S.ht = nullptr;
-  S.err = nullptr;
+  S.bad = 0u;
   S.hlg = 0;
   for (;;) {
     if (t >= tend) {
@@ -1509,6 +1517,7 @@ __global__ void __launch_bounds__(256) k_local4(CountArgs a, const uint32_t *__r
   };
   const bool per_graph = a.graph_of != nullptr;
   uint64_t acc = 0;
+  uint32_t bad = 0;  // a hash insert or lookup walked the whole table (reported at the end)
   for (;;) {
     if (gtid == 0) slot[0] = atomicAdd(a.next, 1u);
     gsync();
@@ -1524,6 +1533,7 @@ __global__ void __launch_bounds__(256) k_local4(CountArgs a, const uint32_t *__r
     for (uint32_t k = gtid; k < d; k += gsize) list[k] = R[k];
     for (uint32_t w = gtid; w < cap; w += gsize) hkey[w] = HASH_EMPTY;
     for (uint32_t w = gtid; w < d * wr; w += gsize) adj[w] = 0u;
+    if (gtid == 0) slot[1] = 0u;  // set when an insert finds no free slot
     gsync();
     for (uint32_t k = gtid; k < d; k += gsize) {
       const uint32_t x = list[k];
@@ -1532,10 +1542,16 @@ __global__ void __launch_bounds__(256) k_local4(CountArgs a, const uint32_t *__r
         h = (h + 1u) & mask;
         it++;
       }
-      if (it > mask) atomicOr(a.err, ERR_HASH_FULL);
+      if (it > mask) slot[1] = 1u;
       else hpos[h] = k;
     }
     gsync();
+    // a table with a failed insert is never probed (the lookups below rely on a free slot)
+    if (slot[1]) {
+      bad = 1u;
+      gsync();
+      continue;
+    }
     // local edges: stream row(R[c]) for c in [1, d), wave w of the group taking every
     // gwaves-th run of 64
     for (uint32_t c0 = 1 + 64u * gwave; c0 < d; c0 += 64u * gwaves) {
@@ -1575,8 +1591,9 @@ __global__ void __launch_bounds__(256) k_local4(CountArgs a, const uint32_t *__r
           for (int e = 0; e < 4; e++) {
             const uint32_t x = xs[e];
             if (x == HASH_EMPTY) continue;  // the list padding
-            uint32_t h = ((x * 2654435761u) >> (32u - lg)) & mask, it = 0;
-            for (; it <= mask; it++) {
+            // at most d ids in 2^lg >= 2d slots, every insert placed: the walk meets a free slot
+            uint32_t h = ((x * 2654435761u) >> (32u - lg)) & mask;
+            for (;;) {
               const uint32_t key = hkey[h];
               if (key == x) {
                 const uint32_t pp = hpos[h];
@@ -1586,7 +1603,6 @@ __global__ void __launch_bounds__(256) k_local4(CountArgs a, const uint32_t *__r
               if (key == HASH_EMPTY) break;
               h = (h + 1u) & mask;
             }
-            if (it > mask) atomicOr(a.err, ERR_HASH_FULL);
           }
         }
       }
@@ -1615,6 +1631,7 @@ __global__ void __launch_bounds__(256) k_local4(CountArgs a, const uint32_t *__r
     // every wave is done with the row's LDS (and has read the slot) before the next row
     gsync();
   }
+  if (bad) atomicOr(a.err, ERR_HASH_FULL);
   const uint64_t sum = wave_sum_u64(acc);
   if (lane == 0 && sum) atomicAdd(&a.counts[0], (unsigned long long)sum);
 }
@@ -1638,7 +1655,7 @@ __global__ void __launch_bounds__(256) k_bshg(CountArgs a, const uint4 *__restri
   uint32_t *tables = gpbase + BSH_GROUP + 1;
   AState S;
   S.ht = tables;
-  S.err = a.err;
+  S.bad = 0u;
   S.bm = nullptr;
   S.list = nullptr;
   S.n = S.lo = S.span = S.aw0 = S.anw = S.base32 = S.P = S.hlg = 0;
@@ -1687,6 +1704,7 @@ __global__ void __launch_bounds__(256) k_bshg(CountArgs a, const uint4 *__restri
     for (uint32_t w = lane; w < tsz; w += 64) tables[w] = HASH_EMPTY;
     wave_sync();
     // insert every element of the bin's rows into its row's table
+    uint32_t tbad = 0;
     for (uint32_t e = lane; e < tde; e += 64) {
       uint32_t k = 0;
 #pragma unroll
@@ -1706,9 +1724,14 @@ __global__ void __launch_bounds__(256) k_bshg(CountArgs a, const uint4 *__restri
           break;
         bk = (bk + 1u) & bmask;
       }
-      if (it > bmask) atomicOr(a.err, ERR_HASH_FULL);
+      tbad |= (uint32_t)(it > bmask);
     }
     wave_sync();
+    // a bin with a failed insert is not probed (the lookups rely on a free slot per table)
+    if (__ballot(tbad != 0u)) {
+      S.bad = 1u;
+      continue;
+    }
     for (uint32_t c0 = pb; c0 < pe; c0 += 64) {
       uint32_t nv = 0, off = 0, tm = 0, bound = 0;
       const uint32_t pi = c0 + lane;
@@ -1728,6 +1751,7 @@ __global__ void __launch_bounds__(256) k_bshg(CountArgs a, const uint4 *__restri
     }
     wave_sync();
   }
+  if (S.bad) atomicOr(a.err, ERR_HASH_FULL);
   const uint64_t sum = wave_sum_u64(acc);
   if (lane == 0 && sum) atomicAdd(&a.counts[0], (unsigned long long)sum);
 }
