@@ -332,7 +332,8 @@ typedef struct {
 } eh_bfs_info;
 /* mode: how wide rounds may be run; every mode gives the same frontiers, path length and
  * level sizes (a round's new frontier is the set of unvisited vertices adjacent to the
- * current one, whichever side enumerates the adjacency). */
+ * current one, whichever side enumerates the adjacency).  EH_BFS_SYMMETRIC and
+ * EH_BFS_IN_ROWS need a matrix built with EH_FILTER_ALL (EH_ERR_INVALID otherwise). */
 eh_status eh_bfs_path(eh_matrix *m, uint32_t n_vertices, uint32_t start, uint32_t max_depth, eh_bfs_mode mode,
                       uint64_t *level_sizes, uint32_t level_cap, eh_bfs_info *info);
 /* No reference counterpart (the reference times whole phases only,

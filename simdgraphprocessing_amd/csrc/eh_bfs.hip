@@ -227,6 +227,13 @@ extern "C" eh_status eh_bfs_path(eh_matrix *m, uint32_t n_vertices, uint32_t sta
     return EH_ERR_INVALID;
   }
   if (start >= n_vertices) { eh_set_error("eh_bfs_path: start %u outside [0, %u)", start, n_vertices); return EH_ERR_INVALID; }
+  // bottom-up rounds find a vertex's frontier neighbours in its own row (or in-row): only a
+  // matrix that kept every entry holds all of them
+  if (mode != EH_BFS_TOP_DOWN && m->filter != EH_FILTER_ALL) {
+    eh_set_error("eh_bfs_path: mode %d needs a matrix built with EH_FILTER_ALL (this one keeps a subset of "
+                 "each row); use EH_BFS_TOP_DOWN", (int)mode);
+    return EH_ERR_INVALID;
+  }
   eh_ctx *ctx = m->ctx;
   EH_HIP(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;

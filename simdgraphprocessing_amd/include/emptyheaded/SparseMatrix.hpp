@@ -84,6 +84,7 @@ class SparseMatrix {
       emptyheaded::check(eh_matrix_info_get(m, &info), "SparseMatrix::build");
     }
     auto *sm = new SparseMatrix(m, info, g);
+    sm->filter_ = filter;
     std::copy(lengths.begin(), lengths.begin() + n, sm->row_lengths);
     if (!g->symmetric) {
       std::cout << "Number of rows + columns: " << rows << std::endl;
@@ -138,9 +139,11 @@ class SparseMatrix {
     eh_bfs_info info;
     emptyheaded::Runtime &rt = emptyheaded::Runtime::get();
     std::lock_guard<std::mutex> lock(rt.mutex());
-    // a directed matrix holds its in-neighbourhoods as rows matrix_size .. 2 matrix_size - 1
-    emptyheaded::check(eh_bfs_path(handle, (uint32_t)matrix_size, start, max_depth,
-                                   symmetric ? EH_BFS_SYMMETRIC : EH_BFS_IN_ROWS, nullptr, 0, &info),
+    // a directed matrix holds its in-neighbourhoods as rows matrix_size .. 2 matrix_size - 1;
+    // rounds may run bottom-up only over rows that kept every entry
+    const eh_bfs_mode mode =
+        filter_ != EH_FILTER_ALL ? EH_BFS_TOP_DOWN : symmetric ? EH_BFS_SYMMETRIC : EH_BFS_IN_ROWS;
+    emptyheaded::check(eh_bfs_path(handle, (uint32_t)matrix_size, start, max_depth, mode, nullptr, 0, &info),
                        "SparseMatrix::bfs_path");
     return info;
   }
@@ -223,4 +226,5 @@ class SparseMatrix {
 
   size_t device_rows_;
   std::unique_ptr<RowCopy[]> rows_;
+  eh_edge_filter filter_ = EH_FILTER_ALL;  // the entries the build kept
 };

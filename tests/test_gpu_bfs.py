@@ -133,3 +133,16 @@ def test_union_and_difference(ctx):
     assert np.frombuffer(p.tobytes(), np.uint32).tolist() == [6, 65, 699]
     with pytest.raises(eh.EHError):
         ctx.union(O.set_payload([1, 2], O.UINTEGER), _bitset([1]))
+
+
+def test_bottom_up_modes_need_every_entry(ctx, facebook):
+    """A LOWER (oriented) matrix holds half of each neighbourhood: the bottom-up modes, which
+    search a vertex's own row for frontier members, reject it; top-down runs as the reference's
+    n_path does over whatever rows the matrix keeps."""
+    import simdgraphprocessing_amd as eh
+    M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "lower", "hybrid")
+    for mode in ("symmetric", "in_rows"):
+        with pytest.raises(eh.EHError) as e:
+            M.bfs_path(0, mode=mode)
+        assert e.value.status == 1 and "EH_FILTER_ALL" in str(e.value)
+    M.bfs_path(facebook["n"] - 1)  # top-down is accepted
