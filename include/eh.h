@@ -284,6 +284,17 @@ eh_status eh_difference(eh_ctx *ctx, const eh_row_view *a, const eh_row_view *b,
  * apps/similar_nodes.cpp:45-60 as one device call; host arrays in and out). */
 eh_status eh_intersect_rows(eh_matrix *m, const uint32_t *a_rows, const uint32_t *b_rows,
                             uint64_t npairs, uint64_t *counts);
+/* The nested ops::set_intersect calls of apps run under par_for_range -- row(i) & row(j)
+ * (apps/undirected_triangle_counting.cpp:45-52) and (row(i) & row(j)) & row(k) ...
+ * (apps/n_clique.cpp:30-50) -- answered many at once: request q intersects the rows
+ * row_ids[row_off[q] .. row_off[q+1]) of one matrix (1 to EH_ROW_SET_MAX rows; row_off has
+ * nreq + 1 entries, row_off[0] = 0) and counts[q] receives the result's size.  elem_off (may
+ * be NULL) receives nreq + 1 running sums of the counts; with elems non-NULL the sorted
+ * elements of request q are written to elems[elem_off[q] ..), and a total above elems_cap
+ * fails with EH_ERR_CAPACITY after counts and elem_off are filled. */
+#define EH_ROW_SET_MAX 16
+eh_status eh_intersect_row_sets(eh_matrix *m, const uint32_t *row_ids, const uint64_t *row_off, uint64_t nreq,
+                                uint64_t *counts, uint32_t *elems, uint64_t elems_cap, uint64_t *elem_off);
 /* No counterpart in the reference.  eh_intersect_rows for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc): same arguments and results, but the pairs and counts are staged in a static

@@ -92,7 +92,7 @@ EXPORTS = [
     "eh_graph_relabel_by_degree", "eh_graph_reorder", "eh_graph_info_get", "eh_graph_download", "eh_graph_download_perm",
     "eh_graph_free", "eh_matrix_build", "eh_matrix_build_host", "eh_matrix_info_get", "eh_matrix_row",
     "eh_matrix_download", "eh_matrix_row_types", "eh_matrix_free", "eh_intersect", "eh_intersect_rows",
-    "eh_intersect_rows_static", "eh_union", "eh_difference", "eh_triangle_count", "eh_clique_count", "eh_bfs_path", "eh_count_profile", "eh_batch_generate", "eh_batch_info",
+    "eh_intersect_rows_static", "eh_intersect_row_sets", "eh_union", "eh_difference", "eh_triangle_count", "eh_clique_count", "eh_bfs_path", "eh_count_profile", "eh_batch_generate", "eh_batch_info",
     "eh_batch_graph_params", "eh_batch_graph_download", "eh_batch_count", "eh_batch_alg_bytes", "eh_batch_free",
     "eh_selector_set_params", "eh_selector_get_params", "eh_matrix_features",
     "eh_matrix_profile_layouts", "eh_mlp_forward", "eh_mlp_loss_grad", "eh_selector_train",
@@ -149,6 +149,7 @@ def lib():
                                C.POINTER(RowView)]
     L.eh_intersect_rows.argtypes = [_vp, _vp, _vp, C.c_uint64, _vp]
     L.eh_intersect_rows_static.argtypes = [_vp, _vp, _vp, C.c_uint64, _vp]
+    L.eh_intersect_row_sets.argtypes = [_vp, _vp, _vp, C.c_uint64, _vp, _vp, C.c_uint64, _vp]
     for name in ("eh_union", "eh_difference"):
         getattr(L, name).argtypes = [_vp, C.POINTER(RowView), C.POINTER(RowView), _vp, C.c_size_t,
                                      C.POINTER(RowView)]
@@ -510,6 +511,25 @@ class SparseMatrix:
         out = np.zeros(len(a), np.uint64)
         _check(lib().eh_intersect_rows(self.h, _ptr(a), _ptr(b), len(a), _ptr(out)))
         return out
+
+    def intersect_row_sets(self, sets, materialize=False):
+        """sets: a list of row-id lists (1 to 16 rows each) -> (counts, [sorted elements per set]
+        or None)."""
+        row_off = np.zeros(len(sets) + 1, np.uint64)
+        row_off[1:] = np.cumsum([len(r) for r in sets])
+        ids = np.ascontiguousarray(np.concatenate([np.asarray(r, np.uint32) for r in sets]) if sets else
+                                   np.zeros(0, np.uint32), np.uint32)
+        counts = np.zeros(max(len(sets), 1), np.uint64)
+        eoff = np.zeros(len(sets) + 1, np.uint64)
+        L = lib()
+        _check(L.eh_intersect_row_sets(self.h, _ptr(ids), _ptr(row_off), len(sets), _ptr(counts), None, 0,
+                                       _ptr(eoff)))
+        if not materialize:
+            return counts[:len(sets)], None
+        elems = np.zeros(max(int(eoff[-1]), 1), np.uint32)
+        _check(L.eh_intersect_row_sets(self.h, _ptr(ids), _ptr(row_off), len(sets), _ptr(counts), _ptr(elems),
+                                       len(elems), _ptr(eoff)))
+        return counts[:len(sets)], [elems[int(eoff[q]):int(eoff[q + 1])] for q in range(len(sets))]
 
     def intersect_rows_static(self, a_rows, b_rows):
         """intersect_rows without device allocation: staged in the library's static array

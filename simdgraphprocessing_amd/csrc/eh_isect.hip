@@ -389,6 +389,51 @@ k_isect_rows(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr, c
   }
 }
 
+// one wave per request: the elements of the request's shortest row that every other row of
+// the request holds, counted (counts[p]) and, with elems, written in order from eoff[p]
+__global__ void __launch_bounds__(256)
+k_isect_row_sets(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr, const uint32_t *ids,
+                 const uint64_t *roff, uint64_t nreq, unsigned long long *counts, const uint64_t *eoff,
+                 uint32_t *elems) {
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+  for (uint64_t p = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); p < nreq; p += nwaves) {
+    const uint64_t r0 = roff[p];
+    const uint32_t nr = (uint32_t)(roff[p + 1] - r0);  // 1..EH_ROW_SET_MAX, checked by the host
+    uint32_t rid = 0, rc = 0xFFFFFFFFu;
+    if (lane < nr) {
+      rid = ids[r0 + lane];
+      rc = card[rid];
+    }
+    // the shortest row (ties: the lowest lane) as the base
+    uint64_t key = ((uint64_t)rc << 8) | lane;
+    for (int o = 32; o; o >>= 1) {
+      const uint64_t y = __shfl_xor(key, o, 64);
+      key = y < key ? y : key;
+    }
+    const uint32_t bl = (uint32_t)(key & 0xFFu);
+    const uint32_t nb = (uint32_t)(key >> 8);
+    const uint32_t *B = unbr + (uint64_t)aoff[__shfl(rid, (int)bl, 64)] * 4;
+    uint64_t wpos = elems ? eoff[p] : 0;
+    uint64_t hits = 0;
+    for (uint32_t k0 = 0; k0 < nb; k0 += 64) {
+      const uint32_t k = k0 + lane;
+      bool in = k < nb;
+      const uint32_t x = in ? B[k] : 0u;
+      for (uint32_t q = 0; q < nr; q++) {
+        const uint32_t rq = __shfl(rid, (int)q, 64), cq = __shfl(rc, (int)q, 64);
+        if (q != bl && in) in = list_contains(unbr + (uint64_t)aoff[rq] * 4, cq, x);
+      }
+      const uint64_t mask = __ballot(in);
+      if (elems && in)
+        elems[wpos + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u))] = x;
+      wpos += (uint64_t)__popcll(mask);
+      hits += (uint64_t)__popcll(mask);
+    }
+    if (lane == 0) counts[p] = hits;
+  }
+}
+
 inline unsigned grid_for(uint64_t work, unsigned block = 256) {
   uint64_t g = (work + block - 1) / block;
   if (g == 0) g = 1;
@@ -899,6 +944,66 @@ extern "C" eh_status eh_intersect_rows(eh_matrix *m, const uint32_t *a_rows, con
   EH_CHECK(c.alloc(8 * npairs + 16));
   return isect_rows_run(m, a_rows, b_rows, npairs, counts, ra.as<uint32_t>(), rb.as<uint32_t>(),
                         c.as<unsigned long long>());
+}
+
+extern "C" eh_status eh_intersect_row_sets(eh_matrix *m, const uint32_t *row_ids, const uint64_t *row_off,
+                                           uint64_t nreq, uint64_t *counts, uint32_t *elems, uint64_t elems_cap,
+                                           uint64_t *elem_off) {
+  if (!m || (nreq && (!row_ids || !row_off || !counts))) { eh_set_error("null argument"); return EH_ERR_INVALID; }
+  if (nreq == 0) {
+    if (elem_off) elem_off[0] = 0;
+    return EH_OK;
+  }
+  if (row_off[0] != 0) { eh_set_error("row_off[0] must be 0"); return EH_ERR_INVALID; }
+  for (uint64_t q = 0; q < nreq; q++) {
+    const uint64_t nr = row_off[q + 1] - row_off[q];
+    if (row_off[q + 1] < row_off[q] || nr < 1 || nr > EH_ROW_SET_MAX) {
+      eh_set_error("request %llu names %lld rows (1 to %d)", (unsigned long long)q,
+                   (long long)(row_off[q + 1] - row_off[q]), EH_ROW_SET_MAX);
+      return EH_ERR_INVALID;
+    }
+  }
+  const uint64_t nids = row_off[nreq];
+  for (uint64_t t = 0; t < nids; t++)
+    if (row_ids[t] >= m->n) { eh_set_error("row %u outside [0, %u)", row_ids[t], m->n); return EH_ERR_INVALID; }
+  eh_ctx *ctx = m->ctx;
+  EH_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  DevBuf ids, roff, cnt, eoff, out;
+  EH_CHECK(ids.alloc(4 * nids + 16));
+  EH_CHECK(roff.alloc(8 * (nreq + 1)));
+  EH_CHECK(cnt.alloc(8 * nreq));
+  EH_HIP(hipMemcpyAsync(ids.p, row_ids, 4 * nids, hipMemcpyHostToDevice, st));
+  EH_HIP(hipMemcpyAsync(roff.p, row_off, 8 * (nreq + 1), hipMemcpyHostToDevice, st));
+  k_isect_row_sets<<<grid_for(nreq * 64), 256, 0, st>>>(m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(),
+                                                          m->dunbr.as<uint32_t>(), ids.as<uint32_t>(),
+                                                          roff.as<uint64_t>(), nreq, cnt.as<unsigned long long>(),
+                                                          nullptr, nullptr);
+  EH_HIP(hipGetLastError());
+  EH_HIP(hipMemcpyAsync(counts, cnt.p, 8 * nreq, hipMemcpyDeviceToHost, st));
+  EH_HIP(hipStreamSynchronize(st));
+  std::vector<uint64_t> off(nreq + 1);
+  off[0] = 0;
+  for (uint64_t q = 0; q < nreq; q++) off[q + 1] = off[q] + counts[q];
+  if (elem_off) memcpy(elem_off, off.data(), 8 * (nreq + 1));
+  if (!elems) return EH_OK;
+  if (off[nreq] > elems_cap) {
+    eh_set_error("the results hold %llu elements (capacity %llu)", (unsigned long long)off[nreq],
+                 (unsigned long long)elems_cap);
+    return EH_ERR_CAPACITY;
+  }
+  if (off[nreq] == 0) return EH_OK;
+  EH_CHECK(eoff.alloc(8 * (nreq + 1)));
+  EH_CHECK(out.alloc(4 * off[nreq] + 16));
+  EH_HIP(hipMemcpyAsync(eoff.p, off.data(), 8 * (nreq + 1), hipMemcpyHostToDevice, st));
+  k_isect_row_sets<<<grid_for(nreq * 64), 256, 0, st>>>(m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(),
+                                                          m->dunbr.as<uint32_t>(), ids.as<uint32_t>(),
+                                                          roff.as<uint64_t>(), nreq, cnt.as<unsigned long long>(),
+                                                          eoff.as<uint64_t>(), out.as<uint32_t>());
+  EH_HIP(hipGetLastError());
+  EH_HIP(hipMemcpyAsync(elems, out.p, 4 * off[nreq], hipMemcpyDeviceToHost, st));
+  EH_HIP(hipStreamSynchronize(st));
+  return EH_OK;
 }
 
 extern "C" eh_status eh_intersect_rows_static(eh_matrix *m, const uint32_t *a_rows, const uint32_t *b_rows,

@@ -85,3 +85,36 @@ def test_intersect_rows_matches_oracle(ctx, facebook):
 
     exp = np.array([len(vals(x) & vals(y)) for x, y in zip(a, b)], np.uint64)
     assert np.array_equal(got, exp)
+
+
+def test_intersect_row_sets(ctx, facebook):
+    """eh_intersect_row_sets: the intersection of 1 to 16 rows per request, counted and
+    materialised, against numpy over the matrix's own rows; malformed requests are refused."""
+    M = ctx.matrix_from_csr(facebook["n"], facebook["off"], facebook["nbr"], "all", "hybrid")
+    off, nbr = M.download()
+    n = facebook["n"]
+    rng = np.random.default_rng(5)
+    sets = [[0], [3, 3], [int(n - 1), 0]]
+    # neighbourhood chains: rows that share neighbours, so deep intersections stay non-empty
+    for _ in range(400):
+        k = int(rng.integers(1, 17))
+        v = int(rng.integers(0, n))
+        row = nbr[off[v]:off[v + 1]]
+        pick = [v] + (rng.choice(row, min(k - 1, len(row)), replace=False).tolist() if len(row) else [])
+        sets.append([int(x) for x in pick])
+    counts, elems = M.intersect_row_sets(sets, materialize=True)
+    counts2, none = M.intersect_row_sets(sets)
+    assert none is None and np.array_equal(counts, counts2)
+    deep = 0
+    for s, c, e in zip(sets, counts, elems):
+        ref = nbr[off[s[0]]:off[s[0] + 1]]
+        for r in s[1:]:
+            ref = np.intersect1d(ref, nbr[off[r]:off[r + 1]], assume_unique=True)
+        assert c == len(ref) and np.array_equal(e, ref), s
+        deep += len(s) >= 3 and c > 0
+    assert deep > 20
+    import simdgraphprocessing_amd as eh
+    for bad in ([list(range(17))], [[n]], [[]]):
+        with pytest.raises(eh.EHError) as ex:
+            M.intersect_row_sets(bad)
+        assert ex.value.status == 1
