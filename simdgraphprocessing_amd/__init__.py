@@ -13,6 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libeh.so")
+REFERENCE_SRC = os.environ.get("EH_REFERENCE_SRC", "/root/reference")  # only read by build()
 
 EH_BITSET, EH_PSHORT, EH_UINTEGER, EH_BITSET_NEW, EH_NEW_TYPE = 0, 1, 2, 7, 8
 # kernel classes of a counting pass, in the order of eh_run_stats.part_ms (EH_PART_*)
@@ -107,8 +108,14 @@ def build(verbose=False):
     """Compile libeh.so for gfx950 (hipcc; cross-compiles without a GPU) and the C++ apps."""
     jobs = f"-j{min(os.cpu_count() or 4, 16)}"
     # the device library, then the host C++ apps and tests that link it
-    for sub in ("csrc", "apps"):
-        r = subprocess.run(["make", "-C", os.path.join(HERE, sub), jobs], capture_output=not verbose, text=True)
+    steps = [("csrc", []), ("apps", [])]
+    # where the reference's sources are, its apps are also compiled, unchanged, against the
+    # C++ surface (apps/Makefile target `ref`); nothing of the reference is copied
+    if os.path.isdir(os.path.join(REFERENCE_SRC, "apps")):
+        steps.append(("apps", ["ref", f"REF={REFERENCE_SRC}"]))
+    for sub, extra in steps:
+        r = subprocess.run(["make", "-C", os.path.join(HERE, sub), jobs] + extra, capture_output=not verbose,
+                           text=True)
         if r.returncode != 0:
             raise RuntimeError(f"{sub} build failed:\n" + (r.stdout or "")[-4000:] + (r.stderr or "")[-4000:])
 

@@ -783,7 +783,18 @@ extern "C" eh_status eh_union(eh_ctx *ctx, const eh_row_view *a, const eh_row_vi
     eh_set_error("eh_union: no set_union overload for B of type %d", b->type);
     return EH_ERR_INVALID;
   }
-  if (b->cardinality && !view_ok(b)) { eh_set_error("malformed set view (type or byte count)"); return EH_ERR_INVALID; }
+  // a bitset B is OR-ed word by word whatever its cardinality says (the reference's bitset x
+  // bitset union reads only the words, union.hpp:6-45, and the sets set_union fills keep the
+  // cardinality they had); a uint or pshort B is walked element by element
+  const bool b_words = b->type == EH_BS && b->number_of_bytes >= 16;
+  if (b_words && ((b->number_of_bytes - 8) % 8 || !b->data)) {
+    eh_set_error("malformed set view (bitset byte count)");
+    return EH_ERR_INVALID;
+  }
+  if (!b_words && b->cardinality && !view_ok(b)) {
+    eh_set_error("malformed set view (type or byte count)");
+    return EH_ERR_INVALID;
+  }
   const uint64_t ba = a->number_of_bytes;
   if (ba > out_cap) { eh_set_error("result needs %llu bytes", (unsigned long long)ba); return EH_ERR_CAPACITY; }
   EH_HIP(hipSetDevice(ctx->device));
@@ -795,7 +806,7 @@ extern "C" eh_status eh_union(eh_ctx *ctx, const eh_row_view *a, const eh_row_vi
   EH_CHECK(pa.alloc(ba + 16));
   EH_CHECK(res.alloc(64));
   EH_HIP(hipMemcpyAsync(pa.p, a->data, ba, hipMemcpyHostToDevice, st));
-  if (b->cardinality) {
+  if (b_words || (b->type != EH_BS && b->cardinality)) {
     const uint64_t bb = view_bytes(b);
     EH_CHECK(pb.alloc(bb + 16));
     EH_HIP(hipMemcpyAsync(pb.p, b->data, bb, hipMemcpyHostToDevice, st));

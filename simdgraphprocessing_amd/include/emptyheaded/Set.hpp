@@ -1,11 +1,19 @@
 // Set.hpp -- Set<T>, the non-owning view of one sorted set (src/set/Set.hpp:14-132):
 // {data, cardinality, number_of_bytes, density, type}.  The bytes are in the
 // reference's formats for `type`; foreach walks them on the host.
+//
+// Beyond the reference's fields a set records where it came from: the matrix and the rows
+// whose intersection it is (one row for SparseMatrix::get_row; the union of the operands'
+// rows for an ops::set_intersect result), so that par_for_range's rounds can answer the
+// intersection on the device in a batch (deferred.hpp).  Sets built from other data have no
+// origin.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 
+#include "deferred.hpp"
 #include "layouts.hpp"
 
 template <class T>
@@ -16,6 +24,7 @@ class Set {
   size_t number_of_bytes;
   double density;
   common::type type;
+  emptyheaded::RowSetKey origin;  // origin.origin == nullptr: not the rows of a matrix
 
   Set(uint8_t *data_in, size_t cardinality_in, size_t number_of_bytes_in, double density_in, common::type type_in)
       : data(data_in), cardinality(cardinality_in), number_of_bytes(number_of_bytes_in), density(density_in),
@@ -42,12 +51,20 @@ class Set {
   template <class U>
   Set(const Set<U> &in)
       : data(in.data), cardinality(in.cardinality), number_of_bytes(in.number_of_bytes), density(in.density),
-        type(in.type) {}
+        type(in.type), origin(in.origin) {}
 
   template <class U>
   Set(const Set<U> *in)
       : data(in->data), cardinality(in->cardinality), number_of_bytes(in->number_of_bytes), density(in->density),
-        type(in->type) {}
+        type(in->type), origin(in->origin) {}
+
+  // this set is row `row` of matrix `m` (SparseMatrix::get_row)
+  void set_origin_row(const void *m, uint32_t row) {
+    origin.origin = m;
+    origin.n = 1;
+    origin.rows[0] = row;
+  }
+  void clear_origin() { origin.origin = nullptr; origin.n = 0; }
 
   template <typename F>
   void foreach(F f) const {
@@ -101,5 +118,6 @@ class Set {
     number_of_bytes = src.number_of_bytes;
     density = src.density;
     type = src.type;
+    origin = src.origin;
   }
 };

@@ -105,13 +105,18 @@ class SparseMatrix {
 
   Set<T> get_row(uint32_t row) const {
     const RowCopy &rc = fetch(row);
-    return Set<T>(rc.bytes, rc.cardinality, rc.number_of_bytes, rc.type);
+    Set<T> s(rc.bytes, rc.cardinality, rc.number_of_bytes, rc.type);
+    s.set_origin_row(handle, row);
+    return s;
   }
 
   // in-neighbours of a directed graph; the row itself for an undirected one
   Set<T> get_column(uint32_t column) const {
-    const RowCopy &rc = fetch(symmetric ? column : (uint32_t)(matrix_size + column));
-    return Set<T>(rc.bytes, rc.cardinality, rc.number_of_bytes, rc.type);
+    const uint32_t row = symmetric ? column : (uint32_t)(matrix_size + column);
+    const RowCopy &rc = fetch(row);
+    Set<T> s(rc.bytes, rc.cardinality, rc.number_of_bytes, rc.type);
+    s.set_origin_row(handle, row);
+    return s;
   }
 
   // the first row of maximum length (src/SparseMatrix.hpp:134-145)
@@ -163,7 +168,9 @@ class SparseMatrix {
   Set<R> get_decoded_row(uint32_t row, uint32_t *decode_buffer) const {
     (void)decode_buffer;
     const RowCopy &rc = fetch(row);
-    return Set<R>(rc.bytes, rc.cardinality, rc.number_of_bytes, rc.type);
+    Set<R> s(rc.bytes, rc.cardinality, rc.number_of_bytes, rc.type);
+    s.set_origin_row(handle, row);
+    return s;
   }
 
   // undirected_triangle_counting's loop (apps/undirected_triangle_counting.cpp:39-61) on the device

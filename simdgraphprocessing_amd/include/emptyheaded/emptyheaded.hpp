@@ -1,6 +1,7 @@
 // emptyheaded.hpp -- the app framework (src/emptyheaded.hpp:1-85): application<T,R>,
 // init_app<T,R>, and main() with the --layout switch (compiled out when
-// GOOGLE_TEST is defined, as in the reference, so tests can include an app).
+// GOOGLE_TEST is defined, as in the reference, so tests can include an app), plus the
+// per-thread buffers and tables the reference's apps use (ParallelBuffer.hpp, Table.hpp).
 //
 // --layout=learned needs trained selector parameters: EH_SELECTOR_PARAMS names a
 // file of 5508 little-endian float32 values (a raw dump, or a .npy whose header
@@ -14,9 +15,11 @@
 #include <vector>
 
 #include "MutableGraph.hpp"
+#include "ParallelBuffer.hpp"
 #include "Parser.hpp"
 #include "Set.hpp"
 #include "SparseMatrix.hpp"
+#include "Table.hpp"
 #include "common.hpp"
 #include "ops.hpp"
 #include "runtime.hpp"

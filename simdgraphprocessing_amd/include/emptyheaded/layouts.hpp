@@ -38,10 +38,14 @@ inline void foreach_blocks(F f, const uint8_t *data, size_t number_of_bytes) {
 // (new_type.hpp:135-152).
 template <typename F>
 inline void foreach_bytes(F f, const uint8_t *data, size_t cardinality, size_t number_of_bytes, common::type t) {
-  if (cardinality == 0 || data == nullptr) return;
+  // a uint set is as long as its cardinality; the other layouts are walked by their bytes, as
+  // the reference's bitset and pshort loops are (bitset.hpp:148-173), so a bitset filled in
+  // place (bitset::set, set_union) iterates its bits whatever its cardinality field holds
+  if (data == nullptr || (t == common::UINTEGER && cardinality == 0)) return;
   if (t == common::BITSET_NEW) {
     foreach_blocks(f, data, number_of_bytes);
   } else if (t == common::NEW_TYPE) {
+    if (number_of_bytes < 8) return;
     uint64_t ub;
     std::memcpy(&ub, data, 8);
     for (size_t k = 0; k < ub / 4; k++) {
@@ -51,6 +55,7 @@ inline void foreach_bytes(F f, const uint8_t *data, size_t cardinality, size_t n
     }
     foreach_blocks(f, data + 8 + ub, number_of_bytes - 8 - ub);
   } else if (t == common::BITSET) {
+    if (number_of_bytes < 16) return;
     uint64_t w0;
     std::memcpy(&w0, data, 8);
     const size_t nw = (number_of_bytes - 8) / 8;

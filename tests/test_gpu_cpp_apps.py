@@ -221,3 +221,72 @@ def test_cli_edge_list_converter_rejects_half_pair(tmp_path):
     (tmp_path / "bad.txt").write_text("1 2\n3\n")
     rc, out, err = run(["undirectedEdgeListToBinary", str(tmp_path / "bad.txt"), str(tmp_path)])
     assert rc != 0 and "half a pair" in err
+
+
+# ---- the reference's own apps, unchanged, compiled against the C++ surface (apps/Makefile `ref`)
+
+REF_BIN = os.path.join(BIN, "ref")
+
+
+def run_ref(app, args, env=None, timeout=170):
+    exe = os.path.join(REF_BIN, app)
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built (the reference's sources were not present at build time)")
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=timeout,
+                       env={**os.environ, **(env or {})})
+    return r.returncode, r.stdout, r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("layout", ["hybrid", "uint", "bs", "pshort"])
+def test_reference_triangle_app(fb_path, layout):
+    """apps/undirected_triangle_counting.cpp as the reference wrote it (ParallelBuffer, one
+    ops::set_intersect per pair inside par_for_range): 1,612,010 on facebook.bin
+    (test/undirected_triangle_counting_test.cpp:11)."""
+    rc, out, err = run_ref("undirected_triangle_counting", [f"--graph={fb_path}", "--input_type=binary", "--t=4",
+                                                             f"--layout={layout}"])
+    assert rc == 0 and "Number of triangles: 1612010" in out, out[-1500:] + err[-1500:]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("layout", ["hybrid", "uint"])
+def test_reference_n_clique_app(fb_path, layout):
+    """apps/n_clique.cpp unchanged (ParallelTable, Table, the level-set recursion through
+    set_intersect): 30,004,668 4-cliques on facebook.bin (test/n_clique_test.cpp:11)."""
+    rc, out, err = run_ref("n_clique", [f"--graph={fb_path}", "--input_type=binary", "--t=4", f"--layout={layout}",
+                                        "--n=4"])
+    assert rc == 0 and "Number of cliques: 30004668" in out, out[-1500:] + err[-1500:]
+
+
+@pytest.mark.gpu
+def test_reference_other_apps(fb_path, dfb_path):
+    """similar_nodes -> 904, symbiosity -> 4039 undirected / 0 directed, lollipop -> 713,455,740,
+    n_path on dfacebook from 0 to depth 4 -> 4 (test/*_test.cpp), the reference's apps unchanged."""
+    base = ["--input_type=binary", "--t=4", "--layout=hybrid"]
+    rc, out, err = run_ref("similar_nodes", [f"--graph={fb_path}"] + base)
+    assert rc == 0 and "Count: 904" in out, out[-1500:] + err[-1500:]
+    rc, out, err = run_ref("symbiosity", [f"--graph={fb_path}"] + base)
+    assert rc == 0 and "Count: 4039" in out, out[-1500:] + err[-1500:]
+    rc, out, err = run_ref("symbiosity", [f"--graph={dfb_path}", "--asymmetric"] + base)
+    assert rc == 0 and "Count: 0" in out, out[-1500:] + err[-1500:]
+    rc, out, err = run_ref("undirected_lollipop_counting", [f"--graph={fb_path}"] + base)
+    assert rc == 0 and "Count: 713455740" in out, out[-1500:] + err[-1500:]
+    rc, out, err = run_ref("n_path", [f"--graph={dfb_path}", "--input_type=binary", "--t=4", "--layout=uint", "--n=4",
+                                      "--start_node=0", "--asymmetric"])
+    assert rc == 0 and "path length: 4" in out, out[-1500:] + err[-1500:]
+
+
+@pytest.mark.gpu
+def test_reference_triangle_app_one_call_per_pair(tmp_path):
+    """With the rounds off (EH_DEFERRED=0) every set_intersect is its own device call; the
+    count is the same (a small graph: one call per pair)."""
+    import oracle as O
+    src, dst = O.gen_er(300, 1500, 7)
+    g = O.canonical_oriented(300, src, dst)["sym"]
+    path = str(tmp_path / "er.bin")
+    O.write_binary(path, 300, g[0], g[1])
+    exp = O.Matrix(300, g[0], g[1], "hybrid").triangles(2)[0]
+    for env in ({}, {"EH_DEFERRED": "0"}):
+        rc, out, err = run_ref("undirected_triangle_counting", [f"--graph={path}", "--input_type=binary", "--t=2",
+                                                                 "--layout=hybrid"], env=env)
+        assert rc == 0 and f"Number of triangles: {exp}" in out, (env, out[-1500:] + err[-1500:])

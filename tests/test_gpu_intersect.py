@@ -87,6 +87,7 @@ def test_intersect_rows_matches_oracle(ctx, facebook):
     assert np.array_equal(got, exp)
 
 
+@pytest.mark.gpu
 def test_intersect_row_sets(ctx, facebook):
     """eh_intersect_row_sets: the intersection of 1 to 16 rows per request, counted and
     materialised, against numpy over the matrix's own rows; malformed requests are refused."""
