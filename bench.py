@@ -373,7 +373,7 @@ def profile_run(args):
 
 
 # kernels of a counting pass (the batch adds its counter and error-word fills)
-PASS_KERNELS = ("k_small_rows", "k_small_cliques4", "k_bstat", "k_bshg", "k_local4", "k_count",
+PASS_KERNELS = ("k_small_rows", "k_small_cliques4", "k_bstat", "k_bshl", "k_local4", "k_count",
                 "k_zero64", "k_zero_err")
 
 

@@ -143,7 +143,7 @@ typedef struct {
 enum {
   EH_PART_SHORT_ROWS = 0, /* edge-centric short rows (k_small_rows, k_small_cliques4)        */
   EH_PART_BSTAT = 1,      /* B-stationary pairs below the split (k_bstat)                      */
-  EH_PART_BSTAT_HASH = 2, /* hashed B-stationary pairs above the split (k_bshg)                */
+  EH_PART_BSTAT_ABOVE = 2,/* B-stationary pairs whose B row is at or above the split (k_bshl)   */
   EH_PART_LOCAL_MID = 3,  /* 4-cliques of rows up to 128 entries by their local graph (k_local4) */
   EH_PART_LOCAL_BIG = 4,  /* the same for rows of 129..512 entries                             */
   EH_PART_TASKS_BIG = 5,  /* A-side tasks of the large-LDS class (k_count)                    */

@@ -17,7 +17,7 @@ REFERENCE_SRC = os.environ.get("EH_REFERENCE_SRC", "/root/reference")  # only re
 
 EH_BITSET, EH_PSHORT, EH_UINTEGER, EH_BITSET_NEW, EH_NEW_TYPE = 0, 1, 2, 7, 8
 # kernel classes of a counting pass, in the order of eh_run_stats.part_ms (EH_PART_*)
-PARTS = ("k_small_rows", "k_bstat", "k_bshg", "k_local4_mid", "k_local4_big", "k_count_big", "k_count")
+PARTS = ("k_small_rows", "k_bstat", "k_bshl", "k_local4_mid", "k_local4_big", "k_count_big", "k_count")
 STATIC_MAX_PAIRS = 1 << 20  # EH_STATIC_MAX_PAIRS: pairs eh_intersect_rows_static stages at most
 TYPE_NAMES = {0: "bitset", 1: "pshort", 2: "uint"}
 LAYOUTS = {"uint": 0, "pshort": 1, "bs": 2, "bitset": 2, "hybrid": 3, "hybrid3": 4, "learned": 5,

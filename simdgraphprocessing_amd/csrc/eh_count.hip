@@ -602,10 +602,11 @@ __device__ uint64_t task_cliques4(const CountArgs &a, const AState &A, uint32_t 
 // pshort B rows are walked partition by partition, one pair after another.
 
 constexpr uint32_t PASS_PROBE = 0, PASS_AND = 1, PASS_STREAM = 2, PASS_BLOCK = 3, PASS_NTSTREAM = 4,
-                   PASS_STREAM16 = 5, PASS_STREAMH = 6;
-// hashed B-stationary: longest B row staged, rows per bin, table words per bin (plus one table)
-constexpr uint32_t BSH_MAX_D = 256, BSH_GROUP = 16, BSH_TABLE_WORDS = 1024;
-constexpr uint32_t BSH_MAX_TABLE = 512;  // 2^hash_lg(BSH_MAX_D)
+                   PASS_STREAM16 = 5;
+// B-stationary pairs above the split (k_bshl): the longest B row, and the bins of consecutive
+// B rows that form its tasks (a row weighs 64 plus the power of two at or above twice its ids
+// at or above the split; a bin holds BSH_BIN_WEIGHT of weight, at most BSH_GROUP rows)
+constexpr uint32_t BSH_MAX_D = 256, BSH_GROUP = 16, BSH_BIN_WEIGHT = 1024;
 // flat_pass metadata: four rank-ordered arrays of 64 words and 64 slot-block start masks
 constexpr uint32_t META_WORDS = 384;
 constexpr uint32_t SMALL_META_WORDS = 260;
@@ -672,54 +673,6 @@ __device__ __forceinline__ uint32_t bitmap16_hits(const uint32_t *bm, const uint
   return h;
 }
 
-// four ids of a streamed prefix vector against one B row's LDS hash table (`meta`: table
-// offset | log2 size << 16).  The table is 2^lg words in buckets of two: an id hashes to a
-// bucket, an insert takes the first free slot from there on (slot 0 before slot 1), so a
-// lookup stops at the first bucket that holds the id or a free slot.  At most half the
-// slots are used, so few lookups go past their first bucket: the four first buckets are
-// read together without branching, and a masked loop follows the rest.  Ids at or above
-// the row's id `bound` (the prefix tail, the padding) cannot be in it.
-// The bucket is the top lg - 1 bits of the 32-bit product (lg >= 4 for every table, so the
-// shift is at most 29); the mask restates that range at the use.
-__device__ __forceinline__ uint32_t hash_bucket(uint32_t e, uint32_t lg) { return (e * 2654435761u) >> (33u - lg); }
-
-__device__ __forceinline__ uint32_t hash_hits(const uint32_t *ht, uint32_t meta, uint32_t bound, const uint4 x) {
-  const uint32_t *T = ht + (meta & 0xFFFFu);
-  const uint32_t lg = meta >> 16, bmask = (1u << (lg - 1u)) - 1u;
-  const uint32_t e[4] = {x.x, x.y, x.z, x.w};
-  uint32_t b[4], v0[4], v1[4];
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    b[q] = hash_bucket(e[q], lg) & bmask;
-    v0[q] = T[2u * b[q]];
-    v1[q] = T[2u * b[q] + 1u];
-  }
-  uint32_t h = 0, pend = 0;
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const bool in = e[q] < bound, hit = (v0[q] == e[q]) | (v1[q] == e[q]);
-    h += (uint32_t)(in & hit);
-    pend |= (uint32_t)(in & !hit & (v1[q] != HASH_EMPTY)) << q;
-  }
-  // the table holds at most 2^(lg-1) ids in 2^(lg-1) two-slot buckets and is only probed
-  // when every insert found a slot, so some bucket has a free second slot and each walk ends
-  while (pend) {
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      b[q] = (pend >> q) & 1u ? (b[q] + 1u) & bmask : b[q];
-      v0[q] = T[2u * b[q]];
-      v1[q] = T[2u * b[q] + 1u];
-    }
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const bool p = (pend >> q) & 1u, hit = (v0[q] == e[q]) | (v1[q] == e[q]);
-      h += (uint32_t)(p & hit);
-      if (p & (hit | (v1[q] == HASH_EMPTY))) pend &= ~(1u << q);
-    }
-  }
-  return h;
-}
-
 template <int PASS, uint32_t KIND, bool W, bool LOCKSTEP = false>
 __device__ __forceinline__ PassAcc<W> pass_elem(const CountArgs &a, const AState &S, uint32_t v0, uint32_t v1,
                                                 uint32_t v2, uint32_t r) {
@@ -748,8 +701,6 @@ __device__ __forceinline__ PassAcc<W> pass_elem(const CountArgs &a, const AState
     return (PassAcc<W>)block_and<W>(S, a, v0 + r);
   } else if constexpr (PASS == PASS_STREAM16) {
     return (PassAcc<W>)bitmap16_hits(S.bm, a.lo16[(uint64_t)v1 + r]);
-  } else if constexpr (PASS == PASS_STREAMH) {
-    return (PassAcc<W>)hash_hits(S.ht, v0, v2, reinterpret_cast<const uint4 *>(a.unbr)[(uint64_t)v1 + r]);
   } else {
     const uint32_t *base = (PASS == PASS_NTSTREAM) ? a.ntu : a.unbr;
     return stream_hits<KIND, W, LOCKSTEP>(S, a, reinterpret_cast<const uint4 *>(base)[(uint64_t)v1 + r]);
@@ -785,7 +736,7 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
     const uint32_t q = (uint32_t)__builtin_ctzll(big);
     big &= big - 1;
     const uint32_t fq = rdl(full, q), q0 = rdl(p0, q), q1 = rdl(p1, q), q2 = rdl(p2, q);
-    if constexpr ((PASS == PASS_STREAM || PASS == PASS_STREAM16 || PASS == PASS_STREAMH) && UNROLL > 1) {
+    if constexpr ((PASS == PASS_STREAM || PASS == PASS_STREAM16) && UNROLL > 1) {
       // fq is a multiple of 64: UNROLL vector loads per lane issued before any test
       const uint4 *vb = ((PASS == PASS_STREAM16) ? a.lo16 : reinterpret_cast<const uint4 *>(a.unbr)) + q1;
       for (uint32_t r0 = 0; r0 < fq; r0 += 64u * UNROLL) {
@@ -799,7 +750,6 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
         for (int u = 0; u < UNROLL; u++) {
           if (r0 + 64u * u < fq) {
             if constexpr (PASS == PASS_STREAM16) acc += (PassAcc<W>)bitmap16_hits(S.bm, v[u]);
-            else if constexpr (PASS == PASS_STREAMH) acc += (PassAcc<W>)hash_hits(S.ht, q0, q2, v[u]);
             else acc += stream_hits<KIND, W, true>(S, a, v[u]);
           }
         }
@@ -812,7 +762,7 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
   }
   // remainders start after the whole rounds: PROBE/AND/BLOCK index by v0, the streams by v1
   c -= full;
-  if constexpr (PASS == PASS_STREAM || PASS == PASS_NTSTREAM || PASS == PASS_STREAM16 || PASS == PASS_STREAMH)
+  if constexpr (PASS == PASS_STREAM || PASS == PASS_NTSTREAM || PASS == PASS_STREAM16)
     p1 += full;
   else p0 += full;
   if (__ballot(c != 0u) == 0ull) return acc;
@@ -838,22 +788,19 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
   const uint32_t le_lo = lane < 32u ? (2u << lane) - 1u : 0xFFFFFFFFu;
   const uint32_t le_hi = lane < 32u ? 0u : (2u << (lane - 32u)) - 1u;
   // the block loops are wave-uniform (slot_owner's ballot needs every lane)
-  if constexpr ((PASS == PASS_STREAM || PASS == PASS_STREAM16 || PASS == PASS_STREAMH) && UNROLL > 1) {
+  if constexpr ((PASS == PASS_STREAM || PASS == PASS_STREAM16) && UNROLL > 1) {
     // UNROLL blocks per step with their vector loads issued before any is tested; blocks
     // past the end are skipped (a wave-uniform branch), and in the last block a slot past
     // the end re-reads the last element (always a valid address) and counts zero
     const uint4 *vb = (PASS == PASS_STREAM16) ? a.lo16 : reinterpret_cast<const uint4 *>(a.unbr);
     for (uint32_t kb = 0; 64u * kb < tot; kb += UNROLL) {
       uint4 v[UNROLL];
-      uint32_t hm[UNROLL], hb[UNROLL];
 #pragma unroll
       for (int u = 0; u < UNROLL; u++) {
         if (64u * (kb + u) < tot) {
           const uint32_t t = min(64u * (kb + u) + lane, tot - 1u);
           const uint32_t q = slot_owner(smask, kb + u, sb, le_lo, le_hi);
           v[u] = vb[(uint64_t)m1[q] + (t - mbase[q])];
-          hm[u] = m0[q];
-          hb[u] = m2[q];
         }
       }
       // keep the scheduler from moving the tests between the loads
@@ -864,7 +811,6 @@ __device__ __forceinline__ uint64_t flat_pass(const CountArgs &a, const AState &
           const PassAcc<W> keep = (PassAcc<W>)0 - (PassAcc<W>)(64u * (kb + u) + lane < tot);
           PassAcc<W> h;
           if constexpr (PASS == PASS_STREAM16) h = (PassAcc<W>)bitmap16_hits(S.bm, v[u]);
-          else if constexpr (PASS == PASS_STREAMH) h = (PassAcc<W>)hash_hits(S.ht, hm[u], hb[u], v[u]);
           else h = stream_hits<KIND, W, true>(S, a, v[u]);
           acc += h & keep;
         }
@@ -1636,30 +1582,46 @@ __global__ void __launch_bounds__(256) k_local4(CountArgs a, const uint32_t *__r
   if (lane == 0 && sum) atomicAdd(&a.counts[0], (unsigned long long)sum);
 }
 
-// Hashed B-stationary pairs: the pairs (i, j) with j at or above the split, for rows i whose
-// B rows there all have at most BSH_MAX_D entries.  B rows are grouped: a bin holds up to
-// BSH_GROUP consecutive such rows whose LDS hash tables (2^hash_lg(|row(j)|) slots each) fit
-// in BSH_TABLE_WORDS plus one table.  A task {bin, b, e} stages every table of its bin and
-// streams the prefixes row(i)[0, p) of the in-pairs [b, e) of the bin's rows (listed in row
-// order, so a pair's B row follows from its index), 16-byte vectors of the canonical list
-// tested in the pair's table (ids at or above j cannot be in row(j)).  Grouping the B rows
-// amortises the chain of dependent loads that staging one short row per task would cost.
-__global__ void __launch_bounds__(256) k_bshg(CountArgs a, const uint4 *__restrict__ htasks, uint64_t nht,
-                                              const uint2 *__restrict__ hpairs, const uint32_t *__restrict__ hj,
-                                              const uint64_t *__restrict__ hoff, const uint32_t *__restrict__ bstart) {
+// B-stationary pairs above the split: the pairs (i, j) with j at or above the split, for rows
+// i whose B rows there all have at most BSH_MAX_D entries.  Every id of row(i) below the
+// split is below j too, so the pair's count is |LO(i) & LO(j)| + |HI(i, j) & HI(j)|, with LO a
+// row's ids below the split, HI(j) row(j)'s ids at or above it and HI(i, j) those of row(i)
+// before j.  A task {bin, b, e} takes the bin's B rows one at a time: LO(j)'s bits go into
+// the wave's bitmap over [0, split) and HI(j) into an LDS list; the 16-bit copies of LO(i)
+// of the row's in-pairs in [b, e) stream through the bitmap (as in k_bstat) and the vectors
+// of row(i) from its first id at or above the split to j's through the list (binary search,
+// four ids in lockstep); then LO(j)'s words are cleared again, so a B row costs its own
+// length instead of the bitmap's.  The rows are pipelined: a lane keeps ids k, k + 64,
+// k + 128, k + 192 of row(j) in registers for setting and clearing them, and the next row
+// and its first 64 in-pairs load during the current row's passes.  The 16-bit copies are
+// padded with split - 1, a bit never set here: a pair whose LO(i) and LO(j) both end with
+// split - 1 counts that id apart.
+__global__ void __launch_bounds__(256) k_bshl(CountArgs a, const uint4 *__restrict__ htasks, uint64_t nht,
+                                              const uint4 *__restrict__ hpairs, const uint32_t *__restrict__ hj,
+                                              const uint32_t *__restrict__ hjlo, const uint64_t *__restrict__ hoff,
+                                              const uint32_t *__restrict__ bstart, uint32_t bm_words, uint32_t pad) {
   extern __shared__ uint32_t lds_all[];
   const uint32_t lane = lane_id();
   uint32_t *meta = lds_all + (threadIdx.x >> 6) * a.lds_words;
-  uint32_t *gj = meta + META_WORDS, *gtab = gj + BSH_GROUP, *gerow = gtab + BSH_GROUP,
-           *gebase = gerow + BSH_GROUP, *gpbase = gebase + BSH_GROUP + 1;
-  uint32_t *tables = gpbase + BSH_GROUP + 1;
-  AState S;
-  S.ht = tables;
-  S.bad = 0u;
-  S.bm = nullptr;
+  uint32_t *bm = meta + META_WORDS, *hl = bm + bm_words;
+  const uint16_t *lo16s = reinterpret_cast<const uint16_t *>(a.lo16);
+  AState S;  // the bitmap (LO(j)) for the 16-bit streams
+  S.bm = bm;
+  S.base32 = S.aw0 = S.lo = 0;
+  S.anw = bm_words >> 1;
+  S.span = 0xFFFFFFFFu;
   S.list = nullptr;
-  S.n = S.lo = S.span = S.aw0 = S.anw = S.base32 = S.P = S.hlg = 0;
+  S.n = 0;
+  S.P = 0;
   S.dpre = S.dstart = nullptr;
+  S.ht = nullptr;
+  S.bad = 0u;
+  S.hlg = 0;
+  AState H = S;  // the list (HI(j)) for the full-width streams
+  H.bm = nullptr;
+  H.list = hl;
+  for (uint32_t w = lane; w < bm_words; w += 64) bm[w] = 0u;
+  wave_sync();
   uint64_t acc = 0;
   uint64_t t = 0, tend = 0;
   for (;;) {
@@ -1672,86 +1634,69 @@ __global__ void __launch_bounds__(256) k_bshg(CountArgs a, const uint4 *__restri
     }
     const uint4 task = htasks[t++];
     const uint32_t bin = rfl(task.x), pb = rfl(task.y), pe = rfl(task.z);
-    const uint32_t h0 = rfl(bstart[bin]), nj = rfl(bstart[bin + 1]) - h0;
-    const uint64_t p0 = hoff[h0];
-    // the bin's rows: tables and element ranges
-    uint32_t j = 0, d = 0, sz = 0, lg = 0, eoff = 0, poff = 0;
-    if (lane < nj) {
-      j = hj[h0 + lane];
-      d = a.card[j];
-      eoff = a.aoff[j];
-      lg = 4;
-      while ((1u << lg) < 2u * d) lg++;
-      sz = 1u << lg;
-      poff = (uint32_t)(hoff[h0 + lane] - p0);
-    }
-    uint32_t tsz, tde;
-    const uint32_t tb = wave_excl_scan(sz, lane, &tsz);
-    const uint32_t eb = wave_excl_scan(d, lane, &tde);
-    tsz = rfl(tsz);
-    tde = rfl(tde);
-    if (lane < nj) {
-      gj[lane] = j;
-      gtab[lane] = tb | (lg << 16);
-      gerow[lane] = eoff;
-      gebase[lane] = eb;
-      gpbase[lane] = poff;
-    }
-    if (lane == 0) {
-      gebase[nj] = tde;
-      gpbase[nj] = (uint32_t)(hoff[h0 + nj] - p0);
-    }
-    for (uint32_t w = lane; w < tsz; w += 64) tables[w] = HASH_EMPTY;
-    wave_sync();
-    // insert every element of the bin's rows into its row's table
-    uint32_t tbad = 0;
-    for (uint32_t e = lane; e < tde; e += 64) {
-      uint32_t k = 0;
+    const uint32_t h0 = rfl(bstart[bin]), h1 = rfl(bstart[bin + 1]);
+    uint32_t xn[4], xc[4];
+    uint4 prn = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t dn = 0, ln = 0, q0n = 0, q1n = 0;
+    auto fetch = [&](uint32_t h) {
+      q0n = max((uint32_t)hoff[h], pb);
+      q1n = min((uint32_t)hoff[h + 1], pe);
+      const uint32_t j = rfl(hj[h]);
+      dn = (q0n < q1n) ? rfl(a.card[j]) : 0u;  // row(j) has at most BSH_MAX_D = 256 ids
+      ln = rfl(hjlo[h]);
+      const uint32_t *G = a.unbr + (uint64_t)a.aoff[j] * 4;
 #pragma unroll
-      for (uint32_t step = BSH_GROUP / 2; step; step >>= 1)
-        if (k + step < nj && gebase[k + step] <= e) k += step;
-      const uint32_t x = a.unbr[(uint64_t)gerow[k] * 4 + (e - gebase[k])];
-      const uint32_t m = gtab[k];
-      uint32_t *T = tables + (m & 0xFFFFu);
-      const uint32_t tl = m >> 16, bmask = (1u << (tl - 1u)) - 1u;
-      // first free slot from the id's bucket on, slot 0 of a bucket before slot 1 (at least
-      // half the slots stay free, so the walk ends; it is bounded by the bucket count all
-      // the same, and an insert that finds no free slot flags the pass)
-      uint32_t bk = hash_bucket(x, tl) & bmask, it = 0;
-      for (; it <= bmask; it++) {
-        if (atomicCAS(&T[2u * bk], HASH_EMPTY, x) == HASH_EMPTY ||
-            atomicCAS(&T[2u * bk + 1u], HASH_EMPTY, x) == HASH_EMPTY)
-          break;
-        bk = (bk + 1u) & bmask;
-      }
-      tbad |= (uint32_t)(it > bmask);
-    }
-    wave_sync();
-    // a bin with a failed insert is not probed (the lookups rely on a free slot per table)
-    if (__ballot(tbad != 0u)) {
-      S.bad = 1u;
-      continue;
-    }
-    for (uint32_t c0 = pb; c0 < pe; c0 += 64) {
-      uint32_t nv = 0, off = 0, tm = 0, bound = 0;
-      const uint32_t pi = c0 + lane;
-      if (pi < pe) {
-        const uint32_t rel = (uint32_t)(pi - p0);
-        uint32_t k = 0;
+      for (int u = 0; u < 4; u++) xn[u] = (lane + 64u * u < dn) ? G[lane + 64u * u] : pad;
+      prn = (dn && q0n + lane < q1n) ? hpairs[q0n + lane] : make_uint4(0u, 0u, 0u, 0u);
+    };
+    fetch(h0);
+    for (uint32_t h = h0; h < h1; h++) {
+      const uint32_t dj = dn, dlo = ln, q0 = q0n, q1 = q1n;
+      uint4 pr = prn;
 #pragma unroll
-        for (uint32_t step = BSH_GROUP / 2; step; step >>= 1)
-          if (k + step < nj && gpbase[k + step] <= rel) k += step;
-        const uint2 pr = hpairs[pi];
-        off = pr.x;
-        nv = (pr.y + 3u) >> 2;
-        tm = gtab[k];
-        bound = gj[k];
+      for (int u = 0; u < 4; u++) xc[u] = xn[u];
+      if (h + 1 < h1) fetch(h + 1);
+      if (dj == 0) continue;  // no pairs of this row in the task
+      const uint32_t dhi = dj - dlo;
+      bool has_pad = false;
+      if (dlo) {
+        // LO(j) is sorted: it ends with the pad id exactly when its last id is split - 1
+        const uint32_t qd = (dlo - 1u) >> 6;  // wave-uniform: which register holds it
+        const uint32_t xl = qd == 0u ? xc[0] : qd == 1u ? xc[1] : qd == 2u ? xc[2] : xc[3];
+        has_pad = __shfl(xl, (int)((dlo - 1u) & 63u), 64) == pad;
       }
-      acc += flat_pass<PASS_STREAMH, KIND_HASH, false, 4>(a, S, nv, tm, off, bound, lane, meta);
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const uint32_t k = lane + 64u * u;
+        if (k < dlo && xc[u] != pad) atomicOr(&bm[xc[u] >> 5], 1u << (xc[u] & 31u));
+        else if (k >= dlo && k < dj) hl[k - dlo] = xc[u];
+      }
+      wave_sync();
+      if (dhi) {
+        H.n = dhi;
+        H.lo = hl[0];
+        H.span = hl[dhi - 1u] - H.lo;
+      }
+      for (uint32_t c0 = q0; c0 < q1; c0 += 64) {
+        const uint4 pnext = (c0 + 64u + lane < q1) ? hpairs[c0 + 64u + lane] : make_uint4(0u, 0u, 0u, 0u);
+        if (dlo)
+          acc += flat_pass<PASS_STREAM16, KIND_BITMAP, false, 4>(a, S, (pr.y + 7u) >> 3, 0u, pr.x, 0u, lane, meta);
+        if (has_pad && pr.y && lo16s[(uint64_t)pr.x * 8 + pr.y - 1u] == pad) acc++;
+        if (dhi) {
+          // row(i) from the vector of its first id at or above the split to j's vector: the
+          // ids below the split and from j on are in no HI(j)
+          const uint32_t v0 = pr.y >> 2, nv = (pr.z | pr.w) ? ((pr.w + 3u) >> 2) - v0 : 0u;
+          acc += flat_pass<PASS_STREAM, KIND_LIST, false, 4>(a, H, nv, 0u, pr.z + v0, 0u, lane, meta);
+        }
+        pr = pnext;
+      }
+      wave_sync();
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (lane + 64u * u < dlo && xc[u] != pad) bm[xc[u] >> 5] = 0u;
+      wave_sync();
     }
-    wave_sync();
   }
-  if (S.bad) atomicOr(a.err, ERR_HASH_FULL);
   const uint64_t sum = wave_sum_u64(acc);
   if (lane == 0 && sum) atomicAdd(&a.counts[0], (unsigned long long)sum);
 }
@@ -1819,7 +1764,7 @@ struct SchedArgs {
   uint32_t local_max;     // 4-cliques: rows above small_deg and up to this go to k_local4
   uint32_t bstat_j;       // task rows hand their pairs with j < bstat_j to the B-stationary pass
   uint32_t bsh_max;       // rows whose B rows at or above bstat_j all have at most this many
-                          // entries hand those pairs to the hashed B-stationary pass (0: off)
+                          // entries hand those pairs to the B-stationary pass above the split (0: off)
   uint32_t *incount;      // out (bstat): task rows containing j, per j < bstat_j
   uint32_t *hcount;       // out (bsh): such rows containing j, per j >= bstat_j (index j - bstat_j)
   uint32_t *jstart;       // out: first entry of row i its A-side tasks cover
@@ -1871,7 +1816,7 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
       work = wave_sum_u64(work);
       below = (uint32_t)wave_sum_u64(below);
       loc[4] += (lane == 0) ? work - 16ull * (dA - below) : 0ull;
-      // every pair at or above the split goes hashed B-stationary when no B row there is longer
+      // every pair at or above the split goes B-stationary (k_bshl) when no B row there is longer
       // than bsh_max: the row then has no A-side or edge-centric work left
       full = s.bstat_j && s.bsh_max && __ballot(bad != 0u) == 0ull;
       if (full && below < dA) {
@@ -2021,7 +1966,7 @@ __global__ void k_bstat_task_gather(const uint4 *btasks, uint64_t nbt, const uin
     out[t] = btasks[tkeys[t] & 0xFFFFFFFFu];
 }
 
-// ---- hashed B-stationary schedule: B rows with in-pairs, their bins, in-pairs, tasks
+// ---- the B-stationary schedule above the split: B rows with in-pairs, their bins, in-pairs, tasks
 __global__ void k_nonzero_flag(const uint32_t *c, uint32_t n, uint32_t *flag) {
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
     flag[q] = c[q] ? 1u : 0u;
@@ -2029,23 +1974,24 @@ __global__ void k_nonzero_flag(const uint32_t *c, uint32_t n, uint32_t *flag) {
 
 // compacted B rows (in row order), their pair counts and table-word weights
 __global__ void k_bsh_rows(const uint32_t *hcount, const uint64_t *hpos, uint32_t nh, uint32_t split,
-                           const uint32_t *card, uint32_t *hj, uint32_t *hc, uint32_t *hw) {
+                           const uint32_t *card, const uint32_t *jbelow, uint32_t *hj, uint32_t *hjlo, uint32_t *hc,
+                           uint32_t *hw) {
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += gridDim.x * blockDim.x) {
     if (!hcount[q]) continue;
     const uint64_t h = hpos[q];
-    const uint32_t j = split + q, d = card[j];
+    const uint32_t j = split + q, d = card[j] - jbelow[j];  // the row's ids at or above the split
     uint32_t lg = 4;
     while ((1u << lg) < 2u * d) lg++;
     hj[h] = j;
+    hjlo[h] = jbelow[j];
     hc[h] = hcount[q];
-    // a bin spans BSH_TABLE_WORDS of these weights: at most BSH_GROUP - 3 rows each
-    hw[h] = (1u << lg) + BSH_TABLE_WORDS / BSH_GROUP;
+    hw[h] = (1u << lg) + BSH_BIN_WEIGHT / BSH_GROUP;
   }
 }
 
 __global__ void k_bsh_bin_flag(const uint64_t *pw, uint32_t nhr, uint32_t *flag) {
   for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nhr; h += gridDim.x * blockDim.x)
-    flag[h] = (h == 0 || pw[h] / BSH_TABLE_WORDS != pw[h - 1] / BSH_TABLE_WORDS) ? 1u : 0u;
+    flag[h] = (h == 0 || pw[h] / BSH_BIN_WEIGHT != pw[h - 1] / BSH_BIN_WEIGHT) ? 1u : 0u;
 }
 
 __global__ void k_bsh_bin_start(const uint32_t *flag, const uint64_t *bpos, uint32_t nhr, uint32_t *bstart) {
@@ -2053,13 +1999,13 @@ __global__ void k_bsh_bin_start(const uint32_t *flag, const uint64_t *bpos, uint
     if (flag[h]) bstart[bpos[h]] = h;
 }
 
-// every row whose pairs at or above the split all go hashed B-stationary (jstart == card)
+// every row whose pairs at or above the split all go B-stationary (jstart == card)
 // writes (its offset / 16 bytes, position) into the list of each non-empty B row there
 __global__ void __launch_bounds__(256) k_bsh_fill(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr,
                                                   const RowDesc *desc, const uint32_t *jstart,
                                                   const uint32_t *jbelow, uint32_t n, uint32_t split,
                                                   const uint64_t *hpos, const uint64_t *hoff, uint32_t *hfill,
-                                                  uint2 *hpairs) {
+                                                  const uint64_t *lo16off, uint4 *hpairs) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
@@ -2072,7 +2018,7 @@ __global__ void __launch_bounds__(256) k_bsh_fill(const uint32_t *card, const ui
       if (desc[j].card == 0) continue;
       const uint64_t h = hpos[j - split];
       const uint32_t slot = atomicAdd(&hfill[h], 1u);
-      hpairs[hoff[h] + slot] = make_uint2(aoff[i], k);
+      hpairs[hoff[h] + slot] = make_uint4((uint32_t)lo16off[i], jb, aoff[i], k);
     }
   }
 }
@@ -2351,6 +2297,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   EH_HIP(hipGetLastError());
   s.bstat_j = 0;
   s.n_btasks = s.n_in_pairs = 0;
+  DevBuf lo16off;  // per row, its 16-bit copy's first vector (the B-stationary and hashed pairs)
   if (bstat_j) {
     // in-lists of the B rows below bstat_j, then tasks of at most BSTAT_CHUNK pairs each
     // (each task clears and fills an 8 KB bitmap: on RMAT-22 256 pairs per task 3.74 ms,
@@ -2376,7 +2323,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     EH_CHECK(pkeys.alloc(8 * npairs + 16));
     EH_CHECK(pkeys2.alloc(8 * npairs + 16));
     // 16-bit prefixes
-    DevBuf nvec, lo16off;
+    DevBuf nvec;
     EH_CHECK(nvec.alloc(4ull * n + 8));
     EH_CHECK(lo16off.alloc(8ull * n + 16));
     {
@@ -2461,6 +2408,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     EH_HIP(hipStreamSynchronize(st));
     if (nhr) {
       EH_CHECK(s.hj.alloc(4 * nhr + 16));
+      EH_CHECK(s.hjlo.alloc(4 * nhr + 16));
       EH_CHECK(hc.alloc(4 * nhr + 16));
       EH_CHECK(hw.alloc(4 * nhr + 16));
       EH_CHECK(s.hoff.alloc(8 * nhr + 16));
@@ -2469,8 +2417,8 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
       EH_CHECK(bpos.alloc(8 * nhr + 16));
       EH_CHECK(hfill.alloc(4 * nhr + 16));
       k_bsh_rows<<<grid_for(nh), 256, 0, st>>>(hcount.as<uint32_t>(), hpos.as<uint64_t>(), nh, bstat_j,
-                                               m->dcard.as<uint32_t>(), s.hj.as<uint32_t>(), hc.as<uint32_t>(),
-                                               hw.as<uint32_t>());
+                                               m->dcard.as<uint32_t>(), jbelow.as<uint32_t>(), s.hj.as<uint32_t>(),
+                                               s.hjlo.as<uint32_t>(), hc.as<uint32_t>(), hw.as<uint32_t>());
       k_zero_u32<<<1, 64, 0, st>>>(hc.as<uint32_t>() + nhr, 1);
       EH_CHECK(prim_exscan_u32_to_u64(ctx, hc.as<uint32_t>(), s.hoff.as<uint64_t>(), nhr + 1));
       EH_CHECK(prim_exscan_u32_to_u64(ctx, hw.as<uint32_t>(), pw.as<uint64_t>(), nhr));
@@ -2482,7 +2430,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
       EH_HIP(hipMemcpyAsync(&nhp, s.hoff.as<uint64_t>() + nhr, 8, hipMemcpyDeviceToHost, st));
       EH_HIP(hipStreamSynchronize(st));
       if (nhp >= (1ull << 32)) {
-        eh_set_error("%llu hashed B-stationary pairs exceed the 32-bit pair index", (unsigned long long)nhp);
+        eh_set_error("%llu B-stationary pairs above the split exceed the 32-bit pair index", (unsigned long long)nhp);
         return EH_ERR_UNSUPPORTED;
       }
       EH_CHECK(s.bstart.alloc(4 * nb + 16));
@@ -2491,12 +2439,12 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
       const uint32_t last = (uint32_t)nhr;
       EH_HIP(hipStreamSynchronize(st));
       EH_HIP(hipMemcpy(s.bstart.as<uint32_t>() + nb, &last, 4, hipMemcpyHostToDevice));
-      EH_CHECK(s.hpairs.alloc(8 * nhp + 16));
+      EH_CHECK(s.hpairs.alloc(16 * nhp + 16));
       k_zero_u32<<<grid_for(nhr + 1), 256, 0, st>>>(hfill.as<uint32_t>(), (uint32_t)(nhr + 1));
       k_bsh_fill<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(
           m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(), m->dunbr.as<uint32_t>(), m->ddesc.as<RowDesc>(),
           jstart.as<uint32_t>(), jbelow.as<uint32_t>(), n, bstat_j, hpos.as<uint64_t>(), s.hoff.as<uint64_t>(),
-          hfill.as<uint32_t>(), s.hpairs.as<uint2>());
+          hfill.as<uint32_t>(), lo16off.as<uint64_t>(), s.hpairs.as<uint4>());
       EH_CHECK(ntb.alloc(4 * nb + 16));
       EH_CHECK(boff.alloc(8 * nb + 16));
       k_bsh_bin_ntask<<<grid_for(nb), 256, 0, st>>>(s.bstart.as<uint32_t>(), s.hoff.as<uint64_t>(), (uint32_t)nb,
@@ -2658,25 +2606,30 @@ eh_status launch_local(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows,
   return EH_OK;
 }
 
-eh_status launch_bshg(eh_ctx *ctx, const CountArgs &base, const uint4 *htasks, uint64_t nht, const uint2 *hpairs,
-                      const uint32_t *hj, const uint64_t *hoff, const uint32_t *bstart, uint32_t *d_next,
-                      PartEv *pe) {
+eh_status launch_bshl(eh_ctx *ctx, const CountArgs &base, const uint4 *htasks, uint64_t nht, const uint4 *hpairs,
+                      const uint32_t *hj, const uint32_t *hjlo, const uint64_t *hoff, const uint32_t *bstart,
+                      uint32_t bstat_j, uint32_t *d_next, PartEv *pe) {
   if (nht == 0) return EH_OK;
   CountArgs a = base;
   a.next = d_next;
-  // a wave's slice: the pass metadata, the bin's rows (j, table, element offset, element and
-  // pair bases) and its tables (at most BSH_TABLE_WORDS + one largest table)
-  a.lds_words = META_WORDS + 5 * BSH_GROUP + 2 + BSH_TABLE_WORDS + BSH_MAX_TABLE + BSH_TABLE_WORDS / BSH_GROUP;
-  const int waves = 4, threads = 256;
+  // a wave's slice: the pass metadata, a bitmap over the ids [0, bstat_j) and a B row's ids
+  // at or above the split
+  const uint32_t bm_words = 2u * (((bstat_j - 1u) >> 6) + 1u);
+  a.lds_words = META_WORDS + bm_words + BSH_MAX_D;
+  // two waves per block: the 10.75 KB slices then fill 14 waves per CU (four per block fit 12;
+  // RMAT-22 0.93 against 1.01 ms)
+  int waves = 2;
+  while (waves > 1 && (size_t)a.lds_words * 4 * waves > 64 * 1024) waves--;
   const size_t lds_bytes = (size_t)a.lds_words * 4 * waves;
-  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(k_bshg), threads, lds_bytes);
+  const int threads = 64 * waves;
+  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(k_bshl), threads, lds_bytes);
   uint64_t blocks = (uint64_t)per_cu * (ctx->num_cus ? ctx->num_cus : 256);
   const uint64_t need = ceil_div(ceil_div(nht, (uint64_t)a.grab), (uint64_t)waves);
   if (blocks > need) blocks = need;
   k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
   part_begin(ctx, pe);
-  hipLaunchKernelGGL(k_bshg, dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a, htasks, nht, hpairs,
-                     hj, hoff, bstart);
+  hipLaunchKernelGGL(k_bshl, dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a, htasks, nht, hpairs,
+                     hj, hjlo, hoff, bstart, bm_words, bstat_j - 1u);
   EH_HIP(hipGetLastError());
   part_end(ctx, pe);
   return EH_OK;
@@ -2738,10 +2691,11 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
   } else {
     EH_CHECK((launch_small<3, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2, pe(EH_PART_SHORT_ROWS))));
     EH_CHECK(launch_bstat(m->ctx, a, btasks, n_btasks, in_pairs, bstat_j, d_next + 3, pe(EH_PART_BSTAT)));
-    if (hs && hs->n_htasks)
-      EH_CHECK(launch_bshg(m->ctx, a, hs->htasks.as<uint4>(), hs->n_htasks, hs->hpairs.as<uint2>(),
-                           hs->hj.as<uint32_t>(), hs->hoff.as<uint64_t>(), hs->bstart.as<uint32_t>(), d_next + 4,
-                           pe(EH_PART_BSTAT_HASH)));
+    if (hs && hs->n_htasks) {
+      EH_CHECK(launch_bshl(m->ctx, a, hs->htasks.as<uint4>(), hs->n_htasks, hs->hpairs.as<uint4>(),
+                           hs->hj.as<uint32_t>(), hs->hjlo.as<uint32_t>(), hs->hoff.as<uint64_t>(),
+                           hs->bstart.as<uint32_t>(), bstat_j, d_next + 5, pe(EH_PART_BSTAT_ABOVE)));
+    }
     EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0,
                                      pe(EH_PART_TASKS_BIG))));
     EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,

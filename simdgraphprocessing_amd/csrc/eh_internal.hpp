@@ -126,7 +126,7 @@ struct TuneParams {
   uint32_t stream_ratio = 0;  // tuned: stream bitset B rows when 16 |B| < ratio * probes (0: never)
   uint32_t bstat_j = 0;       // tuned triangles: pairs with B row id below this run B-stationary (0: off)
   uint32_t bsh_max = 0;       // and rows whose B rows above it have at most this many entries run
-                              // those pairs hashed B-stationary (0: off)
+                              // those pairs B-stationary, k_bshl (0: off)
   uint32_t grab = 2;          // tasks a wave takes from the queue per atomic
   uint32_t local_max = 0;     // 4-cliques: rows above small_deg and up to this run k_local4 (0: off)
 };
@@ -178,9 +178,9 @@ struct Schedule {
   DevBuf in_pairs, btasks, jstart;  // jstart: per row, entries counted B-stationary
   DevBuf lo16;                      // those entries as 16-bit ids, 8 per 16-byte vector
   uint32_t local_req = 0;           // 4-cliques: the local_max the schedule was built for
-  // hashed B-stationary part: B rows at or above the split with in-pairs (row order), the
+  // B-stationary part above the split (k_bshl): B rows at or above the split with in-pairs (row order), the
   // offset of each one's in-pairs, the first row of each bin, the pairs and the tasks
-  DevBuf hj, hoff, bstart, hpairs, htasks;
+  DevBuf hj, hjlo, hoff, bstart, hpairs, htasks;  // hjlo: each such B row's ids below the split
   uint64_t n_htasks = 0;
   DevBuf local_rows[2];             // rows of the two k_local4 classes (<= 128, <= 512 entries)
   uint64_t n_local[2] = {0, 0};
