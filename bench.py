@@ -305,41 +305,22 @@ def config_c4(ctx, with_cpu):
 
 
 def config_c5(ctx, with_cpu):
-    """Learned selector (trained on engine timings of RMAT 18-21, seeds 10-19) evaluated on RMAT-24,
-    edge factor 16, seed 24: triangles with density-rule vs learned layouts, each at its
-    committed launch shape."""
-    out = {"workload": f"triangles, RMAT scale {C5[0]} ef {C5[1]} seed {C5[2]}"}
+    """Learned selector (trained on engine-timed pass costs of RMAT 18-21, seeds 10-19) evaluated
+    on RMAT-24, edge factor 16, seed 24: the tuned triangle pass under the learned layouts and
+    under every rule the network is measured against (the reference's density rule, bitsets at
+    density 1/512 and 1/4096, all uint), each with its build time; plus the plain schedule."""
+    from simdgraphprocessing_amd import selector
     g = ctx.generate_rmat(*C5)
-    res = {}
-    count = None
-    for name, policy, sched in (("plain_hybrid", "hybrid", "plain"), ("tuned_hybrid", "hybrid", "tuned"),
-                                ("tuned_learned", "learned", "tuned")):
-        M = g.build_matrix("lower", policy)
-        params = committed_shape(f"c5:{policy}") if sched == "tuned" else {}
-        c, st = M.triangle_count(sched, **params)
-        ms = []
-        for _ in range(5):
-            c2, st = M.triangle_count(sched, **params)
-            assert c2 == c
-            ms.append(st["kernel_ms"])
-        assert count is None or c == count
-        count = c
-        kms = sorted(ms)[len(ms) // 2]
-        res[name] = {"count": c, "kernel_ms": kms, "build_ms": M.info()["build_ms"], "sched_ms": st["sched_ms"],
-                     "rows_by_type": M.info()["rows_by_type"], "params": params}
-        if sched == "tuned":
-            res[name].update(byte_figures(st["alg_bytes"], kms, traffic_record("c5", policy, params)))
-        del M
-    out.update(res)
-    out["learned_vs_heuristic"] = res["tuned_hybrid"]["kernel_ms"] / res["tuned_learned"]["kernel_ms"]
-    out["learned_vs_heuristic_with_build"] = (
-        (res["tuned_hybrid"]["kernel_ms"] + res["tuned_hybrid"]["build_ms"]) /
-        (res["tuned_learned"]["kernel_ms"] + res["tuned_learned"]["build_ms"]))
-    out["learned_vs_plain"] = res["plain_hybrid"]["kernel_ms"] / res["tuned_learned"]["kernel_ms"]
+    out = selector.evaluate_graph(ctx, g, steps=5, log=log)
+    out["workload"] = f"triangles, RMAT scale {C5[0]} ef {C5[1]} seed {C5[2]}"
+    # counter records exist for the profiled layout policies (density rule = hybrid, learned)
+    for name, r in out["policies"].items():
+        pol = {"density_1_256": "hybrid", "learned": "learned"}.get(name)
+        r.update(byte_figures(out["alg_bytes"], r["kernel_ms"], traffic_record("c5", pol, {}) if pol else None))
     if with_cpu:
         off, nbr = g.download()
         del g
-        out["cpu"] = cpu_triangles(1 << C5[0], off, nbr, count, runs=3, budget_s=40.0)
+        out["cpu"] = cpu_triangles(1 << C5[0], off, nbr, out["policies"]["learned"]["count"], runs=3, budget_s=40.0)
     return out
 
 
@@ -552,7 +533,13 @@ def main():
         policies["hybrid"]["kernel_ms"] = rule_ms
     policies[policy]["kernel_ms"] = kernel_ms
     legs.clear()
-    del M, g
+    del M
+    # the learned layouts against every rule on this graph, pass and build (default launch shape)
+    sel_c2 = None
+    if has_sel:
+        from simdgraphprocessing_amd import selector
+        sel_c2 = selector.compare(ctx, g, steps=5, log=log)
+    del g
 
     extra = {}
     wanted = [c for c in args.configs.split(",") if c]
@@ -596,6 +583,7 @@ def main():
                   "edges_per_s": m / (plain_ms * 1e-3)},
         "speedup_vs_plain": plain_ms / kernel_ms,
         "learned_vs_rule": (rule_ms / kernel_ms) if rule_ms else None,
+        "selector": sel_c2,
         "policies": policies,
         "rows_by_type": info["rows_by_type"],
         "configs": extra,

@@ -136,7 +136,9 @@ typedef struct {
                              graph (bit-rows in LDS); 0 = default (512), <0 = off, at most 512 */
   int32_t time_parts;     /* nonzero: record HIP events around each kernel class of the pass and
                              report their device times in eh_run_stats.part_ms */
-  int32_t reserved;
+  int32_t bsh_max;        /* TUNED triangles with the split on: a row whose B rows at or above the
+                             split all have at most this many entries counts those pairs
+                             B-stationary too (k_bshl); 0 = default (256), <0 = off, else 1..256 */
 } eh_run_opts;
 
 /* Kernel classes of one counting pass (indices of eh_run_stats.part_ms). */
@@ -387,15 +389,24 @@ void eh_batch_free(eh_batch *b);
 #define EH_MLP_PARAMS 5508
 eh_status eh_selector_set_params(eh_ctx *ctx, const float *params);
 eh_status eh_selector_get_params(eh_ctx *ctx, float *params);
-/* Per-row selector arms (eh_matrix_row_types values): 0 bitset probe, 1 pshort
- * partition walk, 2 uint binary search, 3 uint galloping search. */
+/* Per-row selector arms (eh_matrix_row_types values) and what the tuned triangle pass runs
+ * for each when the row is the B side: 0 bitset (queries probe a bit; pairs probe B's words
+ * with A's ids or AND them with A's bitmap), 1 pshort (queries walk the partitions; pairs
+ * stream B's list), 2 uint (queries binary-search; pairs stream B's list), 3 uint searched
+ * (queries binary-search; pairs search each of A's ids within B's first and last id in B's
+ * list: the per-pair algorithm for long B rows few of A's ids fall in). */
+/* EH_LAYOUT_LEARNED's rule: a bitset only where its words are at most guard_words per id
+ * (default 8: density 1/512, 16x the uint row's bytes); fixed_arm -1 takes the network's arm (default), 0..3 gives
+ * that arm to every row whose arm can matter (the ablations the network is measured against).
+ * Rows of fewer than two ids, or held by no row, are uint under every rule.  No counterpart
+ * in the reference. */
+eh_status eh_selector_set_rule(eh_ctx *ctx, uint32_t guard_words, int32_t fixed_arm);
 /* per-row features (n x 16 floats) of an oriented matrix's rows */
 eh_status eh_matrix_features(eh_matrix *m, float *features);
-/* Engine timings: for every stride-th row that answers membership queries in the
- * triangle pass, time each arm on the row's own queries (the elements below the row
- * id of every source row that contains it) with the shader clock; fills targets
- * (n x 4, log2 cycles per query) and mask (n x 4, 1 where measured). */
-eh_status eh_matrix_profile_layouts(eh_matrix *m, uint32_t stride, float *targets, float *mask);
+/* A matrix whose rows take the caller's arms (row_types: n values 0..3; a bitset only within
+ * the guard of eh_selector_set_rule, rows of fewer than two ids uint): the selector's labels
+ * time the tuned pass over such matrices.  No counterpart in the reference. */
+eh_status eh_matrix_build_typed(eh_graph *g, eh_edge_filter filter, const uint8_t *row_types, eh_matrix **out);
 /* device forward over a host batch (x: batch x 16, y: batch x 4) */
 eh_status eh_mlp_forward(eh_ctx *ctx, const float *params, const float *x, int batch, float *y);
 /* device loss + gradient (grad: 5508 floats) */

@@ -53,7 +53,7 @@ class RunOpts(C.Structure):
     _fields_ = [("schedule", C.c_int32), ("lanes_per_row", C.c_int32), ("task_work", C.c_uint32),
                 ("lds_words", C.c_uint32), ("waves_per_block", C.c_int32), ("repeat", C.c_int32),
                 ("small_degree", C.c_int32), ("stream_ratio", C.c_int32), ("bstat_split", C.c_int32),
-                ("local_max", C.c_int32), ("time_parts", C.c_int32), ("reserved", C.c_int32)]
+                ("local_max", C.c_int32), ("time_parts", C.c_int32), ("bsh_max", C.c_int32)]
 
 
 class BfsInfo(C.Structure):
@@ -95,8 +95,8 @@ EXPORTS = [
     "eh_matrix_download", "eh_matrix_row_types", "eh_matrix_free", "eh_intersect", "eh_intersect_rows",
     "eh_intersect_rows_static", "eh_intersect_row_sets", "eh_union", "eh_difference", "eh_triangle_count", "eh_clique_count", "eh_bfs_path", "eh_count_profile", "eh_batch_generate", "eh_batch_info",
     "eh_batch_graph_params", "eh_batch_graph_download", "eh_batch_count", "eh_batch_alg_bytes", "eh_batch_free",
-    "eh_selector_set_params", "eh_selector_get_params", "eh_matrix_features",
-    "eh_matrix_profile_layouts", "eh_mlp_forward", "eh_mlp_loss_grad", "eh_selector_train",
+    "eh_selector_set_params", "eh_selector_get_params", "eh_selector_set_rule", "eh_matrix_features",
+    "eh_matrix_build_typed", "eh_mlp_forward", "eh_mlp_loss_grad", "eh_selector_train",
 ]
 
 _lib = None
@@ -178,8 +178,9 @@ def lib():
     L.eh_batch_free.restype = None
     L.eh_selector_set_params.argtypes = [_vp, _vp]
     L.eh_selector_get_params.argtypes = [_vp, _vp]
+    L.eh_selector_set_rule.argtypes = [_vp, C.c_uint32, C.c_int32]
     L.eh_matrix_features.argtypes = [_vp, _vp]
-    L.eh_matrix_profile_layouts.argtypes = [_vp, C.c_uint32, _vp, _vp]
+    L.eh_matrix_build_typed.argtypes = [_vp, C.c_int, _vp, _pp]
     L.eh_mlp_forward.argtypes = [_vp, _vp, _vp, C.c_int, _vp]
     L.eh_mlp_loss_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_float), _vp]
     L.eh_selector_train.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64,
@@ -199,8 +200,9 @@ def _ptr(a):
 
 
 def _opts(schedule="tuned", lanes=0, task_work=0, lds_words=0, waves=0, repeat=1, small_degree=0,
-          stream_ratio=0, bstat=0, local_max=0, time_parts=False):
+          stream_ratio=0, bstat=0, local_max=0, time_parts=False, bsh=0):
     o = RunOpts()
+    o.bsh_max = bsh
     o.time_parts = int(bool(time_parts))
     o.bstat_split = bstat
     o.local_max = local_max
@@ -342,6 +344,11 @@ class Context:
         assert p.size == MLP_PARAMS
         _check(lib().eh_selector_set_params(self.h, _ptr(p)))
 
+    def set_selector_rule(self, guard_words=8, fixed_arm=-1):
+        """The LEARNED policy's bitset guard (words per id) and, for the ablations, a fixed arm
+        (0 bitset, 1 pshort, 2 uint, 3 uint searched) in place of the network's (-1)."""
+        _check(lib().eh_selector_set_rule(self.h, int(guard_words), int(fixed_arm)))
+
     def get_selector(self):
         p = np.zeros(MLP_PARAMS, np.float32)
         _check(lib().eh_selector_get_params(self.h, _ptr(p)))
@@ -415,6 +422,15 @@ class Graph:
         """One of ORDERINGS (MutableGraph::reorder_*) on the device."""
         _check(lib().eh_graph_reorder(self.h, ORDERINGS[ordering], seed))
 
+    def build_matrix_typed(self, types, filter="lower"):
+        """A matrix whose rows take the given arms (0 bitset, 1 pshort, 2 uint, 3 uint searched;
+        a bitset only within the context's guard)."""
+        t = np.ascontiguousarray(types, np.uint8)
+        assert t.shape == (self.info()["n"],)
+        h = C.c_void_p()
+        _check(lib().eh_matrix_build_typed(self.h, FILTERS[filter], _ptr(t), C.byref(h)))
+        return SparseMatrix(self.ctx, h)
+
     def build_matrix(self, filter="lower", policy="hybrid", mask=None):
         mk = None if mask is None else np.ascontiguousarray(mask, np.uint8)
         h = C.c_void_p()
@@ -464,14 +480,6 @@ class SparseMatrix:
         f = np.zeros((self.info()["n"], 16), np.float32)
         _check(lib().eh_matrix_features(self.h, _ptr(f)))
         return f
-
-    def profile_layouts(self, stride=1):
-        """Engine-timed log2 cycles per membership query for each selector arm."""
-        n = self.info()["n"]
-        t = np.zeros((n, 4), np.float32)
-        m = np.zeros((n, 4), np.float32)
-        _check(lib().eh_matrix_profile_layouts(self.h, stride, _ptr(t), _ptr(m)))
-        return t, m
 
     def triangle_count(self, schedule="tuned", **kw):
         c = C.c_uint64()

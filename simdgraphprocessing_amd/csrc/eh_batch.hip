@@ -167,7 +167,8 @@ extern "C" eh_status eh_batch_generate(eh_ctx *ctx, uint64_t seed, uint32_t ngra
   if ((s = graph_canonicalize(ctx, (uint32_t)vt, keys, moff[ngraphs], 1, &b->g, b->vgraph.as<uint32_t>())) != EH_OK)
     return fail(s);
   if ((s = matrix_build_dev(ctx, b->g->n, b->g->off.as<uint64_t>(), b->g->nbr.as<uint32_t>(), b->g->entries,
-                            EH_FILTER_LOWER, nullptr, policy, b->vgraph.as<uint32_t>(), &b->M)) != EH_OK)
+                            EH_FILTER_LOWER, nullptr, policy, b->vgraph.as<uint32_t>(), b->g->symmetric,
+                            &b->M)) != EH_OK)
     return fail(s);
   if ((s = b->counts.alloc(16ull * ngraphs + 64)) != EH_OK || (s = b->next.alloc(64)) != EH_OK) return fail(s);
   *out = b;

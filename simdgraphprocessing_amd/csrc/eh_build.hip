@@ -58,32 +58,88 @@ __global__ void k_aoff(const uint64_t *poff, uint32_t n, uint32_t *aoff) {
     aoff[i] = (uint32_t)(poff[i] >> 2);
 }
 
-// one wave per row: copy the kept entries, pad with the sentinel
+// copy the kept entries of every row, pad with the sentinel.  A wave takes 64 consecutive
+// rows: a lane copies a row of at most PACK_LANE_MAX entries itself (most rows), then the wave
+// copies the longer ones together; a masked row is always copied by the wave (its kept entries
+// are found with ballots).
+constexpr uint32_t PACK_LANE_MAX = 32;
+
 __global__ void __launch_bounds__(256) k_pack(const uint64_t *off, const uint32_t *nbr, uint32_t n, int filter,
                                               const uint8_t *mask, const uint32_t *card, const uint64_t *poff,
                                               uint32_t *unbr) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t i = wid; i < n; i += nw) {
-    const uint64_t b = off[i], e = off[i + 1];
-    const uint32_t c = card[i];
-    uint32_t *dst = unbr + poff[i];
-    if (filter != EH_FILTER_MASK) {
-      for (uint32_t k = lane; k < c; k += 64) dst[k] = nbr[b + k];
-    } else {
-      uint32_t w = 0;
-      for (uint64_t k0 = b; k0 < e; k0 += 64) {
-        const uint64_t k = k0 + lane;
-        const uint32_t keep = (k < e) && mask[k];
-        const uint64_t m = __ballot(keep);
-        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        if (keep) dst[w + below] = nbr[k];
-        w += (uint32_t)__popcll(m);
-      }
+  const uint64_t nwv = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t base = wid * 64; base < n; base += nwv * 64) {
+    const uint64_t i = base + lane;
+    const bool valid = i < n;
+    const uint32_t c = valid ? card[i] : 0u;
+    const bool own = valid && filter != EH_FILTER_MASK && c <= PACK_LANE_MAX;
+    if (own) {
+      const uint32_t *src = nbr + off[i];
+      uint32_t *dst = unbr + poff[i];
+      for (uint32_t k = 0; k < c; k++) dst[k] = src[k];
+      for (uint32_t k = c; k < ((c + 3u) & ~3u); k++) dst[k] = EH_PAD_VALUE;
     }
-    const uint32_t padded = (c + 3u) & ~3u;
-    for (uint32_t k = c + lane; k < padded; k += 64) dst[k] = EH_PAD_VALUE;
+    uint64_t lm = __ballot(valid && !own);
+    while (lm) {
+      const uint32_t q = (uint32_t)__builtin_ctzll(lm);
+      lm &= lm - 1;
+      const uint64_t r = base + q;
+      const uint64_t b = off[r], e = off[r + 1];
+      const uint32_t cr = __builtin_amdgcn_readlane(c, q);
+      uint32_t *dst = unbr + poff[r];
+      if (filter != EH_FILTER_MASK) {
+        for (uint32_t k = lane; k < cr; k += 64) dst[k] = nbr[b + k];
+      } else {
+        uint32_t w = 0;
+        for (uint64_t k0 = b; k0 < e; k0 += 64) {
+          const uint64_t k = k0 + lane;
+          const uint32_t keep = (k < e) && mask[k];
+          const uint64_t m = __ballot(keep);
+          const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+          if (keep) dst[w + below] = nbr[k];
+          w += (uint32_t)__popcll(m);
+        }
+      }
+      const uint32_t padded = (cr + 3u) & ~3u;
+      for (uint32_t k = cr + lane; k < padded; k += 64) dst[k] = EH_PAD_VALUE;
+    }
+  }
+}
+
+// 16-bit partitions of every row (the pshort layouts' headers): a wave takes 64 consecutive
+// rows, a lane counts a row of at most PACK_LANE_MAX ids itself, the wave the longer ones
+__global__ void __launch_bounds__(256) k_parts(const uint32_t *card, const uint64_t *poff, const uint32_t *unbr,
+                                               uint32_t n, const uint8_t *only_ps, uint32_t *parts) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwv = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t base = wid * 64; base < n; base += nwv * 64) {
+    const uint64_t i = base + lane;
+    // only_ps (the learned layouts): rows given another layout need no count
+    const bool skip = i < n && only_ps && only_ps[i] != EH_PS;
+    if (skip) parts[i] = 0u;
+    const bool valid = i < n && !skip;
+    const uint32_t c = valid ? card[i] : 0u;
+    if (valid && c <= PACK_LANE_MAX) {
+      const uint32_t *r = unbr + poff[i];
+      uint32_t p = c ? 1u : 0u;
+      for (uint32_t k = 1; k < c; k++) p += (r[k] >> 16) != (r[k - 1] >> 16);
+      parts[i] = p;
+    }
+    uint64_t lm = __ballot(valid && c > PACK_LANE_MAX);
+    while (lm) {
+      const uint32_t q = (uint32_t)__builtin_ctzll(lm);
+      lm &= lm - 1;
+      const uint64_t row = base + q;
+      const uint32_t cr = __builtin_amdgcn_readlane(c, q);
+      const uint32_t *r = unbr + poff[row];
+      uint32_t p = 0;
+      for (uint32_t k = 1 + lane; k < cr; k += 64) p += (r[k] >> 16) != (r[k - 1] >> 16);
+      for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
+      if (lane == 0) parts[row] = 1u + p;
+    }
   }
 }
 
@@ -92,6 +148,7 @@ struct TypeArgs {
   const uint64_t *poff;
   const uint32_t *unbr;
   const uint8_t *forced;  // learned per-row types or null
+  const uint32_t *parts_in;  // 16-bit partitions per row (k_parts) or null where no row can be pshort
   uint32_t n;
   int policy;
   uint8_t *type;
@@ -105,8 +162,11 @@ struct TypeArgs {
 };
 
 __global__ void k_select(TypeArgs a) {
-  // wave-uniform grid stride, so the per-type row counts are added once per wave
+  // wave-uniform grid stride over a bounded grid: each lane counts its rows per type and the
+  // wave adds its counts once at the end (one add per wave per row group serialised on the three
+  // counters)
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint32_t cnt[3] = {0u, 0u, 0u};
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < a.n; base += stride) {
     const uint64_t i = base + (threadIdx.x & 63u);
     const bool valid = i < a.n;
@@ -116,13 +176,8 @@ __global__ void k_select(TypeArgs a) {
       const uint32_t *r = a.unbr + a.poff[i];
       uint32_t lo = c ? r[0] : 0, hi = c ? r[c - 1] : 0;
       const uint32_t nw = c ? (hi >> 6) - (lo >> 6) + 1u : 0u;
-      uint32_t parts = 0;
-      const bool need_parts = a.policy == EH_LAYOUT_PSHORT || a.policy == EH_LAYOUT_HYBRID3 ||
-                              a.policy == EH_LAYOUT_COST || a.policy == EH_LAYOUT_LEARNED;
-      if (c && need_parts) {
-        parts = 1;
-        for (uint32_t k = 1; k < c; k++) parts += (r[k] >> 16) != (r[k - 1] >> 16);
-      }
+      // partitions (k_parts) decide the pshort policies' choice and size every pshort row
+      const uint32_t parts = a.parts_in ? a.parts_in[i] : 0u;
       // 128-value blocks: all of them (bitset_new), and new_type's split into blocks with
       // more than 8 elements (density c/128 > 1/16, new_type.hpp:61-63) and a uint part
       uint32_t nblk = 0, ndense = 0, nsparse = 0;
@@ -200,61 +255,117 @@ __global__ void k_select(TypeArgs a) {
       // block rows are counted with the bitsets
       if (c) counted = t == EH_UG ? EH_U : is_block_type(t) ? EH_BS : t;
     }
-    for (uint32_t q = 0; q < 3; q++) {
-      const uint64_t b = __ballot(counted == q);
-      if ((threadIdx.x & 63u) == 0 && b) atomicAdd(&a.type_rows[q], (unsigned long long)__popcll(b));
-    }
+    for (uint32_t q = 0; q < 3; q++) cnt[q] += counted == q ? 1u : 0u;
+  }
+  for (uint32_t q = 0; q < 3; q++) {
+    uint32_t v = cnt[q];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63u) == 0 && v) atomicAdd(&a.type_rows[q], (unsigned long long)v);
   }
 }
 
-// one wave per bitset row: OR the bits into zeroed words
+// one wave per bitset row, over words cleared beforehand, 64 ids a step: the ids of one word
+// are OR-ed together across lanes (a segmented reduction toward the word's first lane, the ids
+// being sorted) and that lane ORs the word in, one atomic per word and step instead of one per id
 __global__ void __launch_bounds__(256) k_fill_bitset(const uint32_t *card, const uint64_t *poff, const uint32_t *unbr,
                                                      const uint8_t *type, const uint64_t *bsoff, uint32_t n,
                                                      unsigned long long *bsw) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t i = wid; i < n; i += nw) {
+  const uint64_t nwv = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t i = wid; i < n; i += nwv) {
     if (type[i] != EH_BS) continue;
     const uint32_t c = card[i];
     if (!c) continue;
     const uint32_t *r = unbr + poff[i];
     const uint32_t w0 = r[0] >> 6;
     unsigned long long *W = bsw + bsoff[i];
-    for (uint32_t k = lane; k < c; k += 64) {
-      const uint32_t x = r[k];
-      atomicOr(&W[(x >> 6) - w0], 1ull << (x & 63u));
+    for (uint32_t k0 = 0; k0 < c; k0 += 64) {
+      const uint32_t k = k0 + lane;
+      const bool in = k < c;
+      const uint32_t x = in ? r[k] : 0u;
+      const uint32_t wi = in ? (x >> 6) - w0 : 0xFFFFFFFFu;
+      unsigned long long bits = in ? 1ull << (x & 63u) : 0ull;
+      for (uint32_t o = 1; o < 64; o <<= 1) {
+        const unsigned long long ob = __shfl_down(bits, o, 64);
+        const uint32_t ow = __shfl_down(wi, o, 64);
+        if (lane + o < 64u && ow == wi) bits |= ob;
+      }
+      const uint32_t pw = __shfl_up(wi, 1, 64);
+      if (in && (lane == 0u || pw != wi)) atomicOr(&W[wi], bits);
     }
   }
 }
 
-// one thread per pshort row: [prefix, count-1, low16...] per 16-bit prefix (pshort.hpp:46-78)
-__global__ void k_fill_pshort(const uint32_t *card, const uint64_t *poff, const uint32_t *unbr,
-                              const uint8_t *type, const uint64_t *psoff, const uint32_t *ps_n, uint32_t n,
-                              uint16_t *psd) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    if (type[i] != EH_PS) continue;
-    const uint32_t c = card[i];
-    if (!c) continue;
-    const uint32_t *r = unbr + poff[i];
-    uint16_t *S = psd + psoff[i];
-    uint32_t w = 0, hdr = 0, run = 0, cur = 0xFFFFFFFFu;
-    for (uint32_t k = 0; k < c; k++) {
-      const uint32_t hi = r[k] >> 16;
-      if (hi != cur) {
-        if (cur != 0xFFFFFFFFu) S[hdr + 1] = (uint16_t)(run - 1u);
-        hdr = w;
-        S[w++] = (uint16_t)hi;
-        S[w++] = 0;
-        cur = hi;
-        run = 0;
+// pshort rows: [prefix, count-1, low16...] per 16-bit prefix (pshort.hpp:46-78), then zeros to
+// the row's padded length.  A wave takes 64 consecutive rows: a lane encodes a row of at most
+// lane_max ids itself, then the wave encodes the longer ones together, 64 ids a step (stores
+// of neighbouring shorts): an id at position k in partition p goes to k + 2 (p + 1); a
+// partition's first id writes its prefix, its last id the count.  lane_max 0: every row by
+// the wave.
+
+__global__ void __launch_bounds__(256) k_fill_pshort(const uint32_t *card, const uint64_t *poff, const uint32_t *unbr,
+                                                     const uint8_t *type, const uint64_t *psoff, const uint32_t *ps_n,
+                                                     uint32_t n, uint32_t lane_max, uint16_t *psd) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwv = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t base = wid * 64; base < n; base += nwv * 64) {
+    const uint64_t i = base + lane;
+    const bool ps = i < n && type[i] == EH_PS;
+    const uint32_t c = ps ? card[i] : 0u;
+    if (ps && c && c <= lane_max) {
+      const uint32_t *r = unbr + poff[i];
+      uint16_t *S = psd + psoff[i];
+      uint32_t w = 0, hdr = 0, cnt = 0, cur = 0xFFFFFFFFu;
+      for (uint32_t k = 0; k < c; k++) {
+        const uint32_t hi = r[k] >> 16;
+        if (hi != cur) {
+          if (cur != 0xFFFFFFFFu) S[hdr + 1] = (uint16_t)(cnt - 1u);
+          hdr = w;
+          S[w++] = (uint16_t)hi;
+          S[w++] = 0;
+          cur = hi;
+          cnt = 0;
+        }
+        S[w++] = (uint16_t)(r[k] & 0xFFFFu);
+        cnt++;
       }
-      S[w++] = (uint16_t)(r[k] & 0xFFFFu);
-      run++;
+      S[hdr + 1] = (uint16_t)(cnt - 1u);
+      for (uint32_t k = w; k < ps_n[i]; k++) S[k] = 0;
     }
-    S[hdr + 1] = (uint16_t)(run - 1u);
-    for (uint32_t k = w; k < ps_n[i]; k++) S[k] = 0;
+    uint64_t lm = __ballot(ps && c > lane_max);
+    while (lm) {
+      const uint32_t q = (uint32_t)__builtin_ctzll(lm);
+      lm &= lm - 1;
+      const uint64_t row = base + q;
+      const uint32_t cr = __builtin_amdgcn_readlane(c, q);
+      const uint32_t *r = unbr + poff[row];
+      uint16_t *S = psd + psoff[row];
+      uint32_t parts = 0, start = 0;  // partitions before this step; the open partition's first position
+      for (uint32_t k0 = 0; k0 < cr; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        const bool in = k < cr;
+        const uint32_t x = in ? r[k] : 0u;
+        const uint32_t hi = x >> 16;
+        const bool first = in && (k == 0 || (r[k - 1] >> 16) != hi);
+        const bool last = in && (k + 1 == cr || (r[k + 1] >> 16) != hi);
+        const uint64_t fm = __ballot(first);
+        const uint64_t upto = fm & (lane == 63u ? ~0ull : ((2ull << lane) - 1ull));
+        const uint32_t p = parts + (uint32_t)__popcll(upto) - 1u;  // this id's partition
+        const uint32_t s = upto ? k0 + 63u - (uint32_t)__builtin_clzll(upto) : start;
+        if (in) {
+          S[k + 2u * (p + 1u)] = (uint16_t)(x & 0xFFFFu);
+          if (first) S[k + 2u * p] = (uint16_t)hi;
+          if (last) S[s + 2u * p + 1u] = (uint16_t)(k - s);
+        }
+        // carry the partition still open at the step's end
+        parts += (uint32_t)__popcll(fm);
+        if (fm) start = k0 + 63u - (uint32_t)__builtin_clzll(fm);
+      }
+      const uint32_t w = cr + 2u * parts;
+      for (uint32_t k = w + lane; k < ps_n[row]; k += 64) S[k] = 0;
+    }
   }
 }
 
@@ -331,19 +442,36 @@ __global__ void k_desc(const uint32_t *card, const uint64_t *poff, const uint32_
   }
 }
 
-// one wave per row: W_i = sum over kept j of card(j)
-__global__ void __launch_bounds__(256) k_wedges(const uint32_t *card, const uint64_t *poff, const uint32_t *unbr,
-                                                uint32_t n, uint64_t *w) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t i = wid; i < n; i += nw) {
+// sum over every kept entry j of card(j) (the wedges of the triangle pass), flat over the
+// padded rows (the pad sentinel is no id): per-lane sums, one add per wave
+__global__ void __launch_bounds__(256) k_wedge_sum(const uint32_t *card, const uint32_t *unbr, uint64_t words,
+                                                   unsigned long long *out) {
+  const uint4 *U = reinterpret_cast<const uint4 *>(unbr);
+  const uint64_t nv = words >> 2;
+  unsigned long long s = 0;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nv; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint4 x = U[v];
+    if (x.x != EH_PAD_VALUE) s += card[x.x];
+    if (x.y != EH_PAD_VALUE) s += card[x.y];
+    if (x.z != EH_PAD_VALUE) s += card[x.z];
+    if (x.w != EH_PAD_VALUE) s += card[x.w];
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63u) == 0 && s) atomicAdd(out, s);
+}
+
+// caller-given arms, a bitset only within `guard` words per id (else uint), unknown values uint
+__global__ void k_guard_types(const uint32_t *card, const uint64_t *poff, const uint32_t *unbr, const uint8_t *in,
+                              uint32_t n, uint32_t guard, uint8_t *out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t c = card[i];
-    const uint32_t *r = unbr + poff[i];
-    uint64_t s = 0;
-    for (uint32_t k = lane; k < c; k += 64) s += card[r[k]];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0) w[i] = s;
+    uint8_t t = in[i];
+    if (t > EH_UG || c < 2) t = EH_U;
+    if (t == EH_BS) {
+      const uint32_t *r = unbr + poff[i];
+      if ((uint64_t)(r[c - 1] >> 6) - (r[0] >> 6) + 1ull > (uint64_t)guard * c) t = EH_U;
+    }
+    out[i] = t;
   }
 }
 
@@ -357,7 +485,8 @@ __global__ void k_u32_to_u64_plus(const uint32_t *in, uint32_t n, uint64_t *out)
 
 eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const uint32_t *d_nbr, uint64_t entries,
                            eh_edge_filter filter, const uint8_t *d_mask, eh_layout_policy policy,
-                           const uint32_t *d_graph_of, eh_matrix **out) {
+                           const uint32_t *d_graph_of, bool symmetric, eh_matrix **out,
+                           const uint8_t *d_types_in) {
   (void)entries;
   if (policy < EH_LAYOUT_UINT || policy > EH_LAYOUT_HYBRID_BLOCK) { eh_set_error("unknown layout policy %d", (int)policy); return EH_ERR_INVALID; }
   if (filter < EH_FILTER_ALL || filter > EH_FILTER_MASK) { eh_set_error("unknown edge filter %d", (int)filter); return EH_ERR_INVALID; }
@@ -370,7 +499,7 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   m->filter = filter;
   eh_status s = EH_OK;
   auto fail = [&](eh_status e) { delete m; return e; };
-  DevBuf pc, poff, type, parts, bs_n, ps_n, blk_n, ntu_n, bsoff, psoff, blkoff, ntuoff, refb, stats, wed;
+  DevBuf pc, poff, type, parts, bs_n, ps_n, blk_n, ntu_n, bsoff, psoff, blkoff, ntuoff, refb, stats;
   if ((s = m->dcard.alloc(4ull * n + 4)) != EH_OK) return fail(s);
   if ((s = m->daoff.alloc(4ull * n + 4)) != EH_OK) return fail(s);
   if ((s = pc.alloc(4ull * n + 8)) != EH_OK) return fail(s);
@@ -385,7 +514,7 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   m->unbr_words = total;
   if ((s = m->dunbr.alloc(4 * total + 64)) != EH_OK) return fail(s);
   k_aoff<<<grid_for(n), 256, 0, st>>>(poff.as<uint64_t>(), n, m->daoff.as<uint32_t>());
-  k_pack<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(d_off, d_nbr, n, filter, d_mask, m->dcard.as<uint32_t>(),
+  k_pack<<<grid_for(n), 256, 0, st>>>(d_off, d_nbr, n, filter, d_mask, m->dcard.as<uint32_t>(),
                                                      poff.as<uint64_t>(), m->dunbr.as<uint32_t>());
   if ((s = prim_reduce_max_u32(ctx, m->dcard.as<uint32_t>(), n, &m->max_row)) != EH_OK) return fail(s);
   // learned types are computed from the packed rows before selection
@@ -393,7 +522,12 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   if (policy == EH_LAYOUT_LEARNED) {
     if ((s = forced.alloc((size_t)n + 16)) != EH_OK) return fail(s);
     m->card = 0;
-    if ((s = matrix_select_learned(m, forced.as<uint8_t>())) != EH_OK) return fail(s);
+    if (d_types_in) {
+      k_guard_types<<<grid_for(n), 256, 0, st>>>(m->dcard.as<uint32_t>(), poff.as<uint64_t>(), m->dunbr.as<uint32_t>(),
+                                                 d_types_in, n, ctx->sel_guard, forced.as<uint8_t>());
+    } else if ((s = matrix_select_learned(m, forced.as<uint8_t>(), symmetric ? d_off : nullptr)) != EH_OK) {
+      return fail(s);
+    }
   }
   if ((s = type.alloc((size_t)n + 16)) != EH_OK) return fail(s);
   if ((s = parts.alloc(4ull * n + 4)) != EH_OK) return fail(s);
@@ -413,6 +547,17 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   ta.poff = poff.as<uint64_t>();
   ta.unbr = m->dunbr.as<uint32_t>();
   ta.forced = forced.p ? forced.as<uint8_t>() : nullptr;
+  // partitions, where a row can be pshort
+  const bool any_ps = policy == EH_LAYOUT_PSHORT || policy == EH_LAYOUT_HYBRID3 || policy == EH_LAYOUT_COST ||
+                      policy == EH_LAYOUT_LEARNED;
+  DevBuf parts_in;
+  if (any_ps) {
+    if ((s = parts_in.alloc(4ull * n + 4)) != EH_OK) return fail(s);
+    k_parts<<<grid_for(n), 256, 0, st>>>(m->dcard.as<uint32_t>(), poff.as<uint64_t>(), m->dunbr.as<uint32_t>(), n,
+                                         policy == EH_LAYOUT_LEARNED ? forced.as<uint8_t>() : nullptr,
+                                         parts_in.as<uint32_t>());
+  }
+  ta.parts_in = any_ps ? parts_in.as<uint32_t>() : nullptr;
   ta.n = n;
   ta.policy = policy;
   ta.type = type.as<uint8_t>();
@@ -423,7 +568,8 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   ta.ntu_n = ntu_n.as<uint32_t>();
   ta.refb = refb.as<uint64_t>();
   ta.type_rows = stats.as<unsigned long long>();
-  k_select<<<grid_for(n), 256, 0, st>>>(ta);
+  const unsigned bounded = std::min(grid_for(n), (unsigned)(ctx->num_cus ? ctx->num_cus : 256) * 8u);
+  k_select<<<bounded, 256, 0, st>>>(ta);
   if ((s = bsoff.alloc(8ull * n + 16)) != EH_OK) return fail(s);
   if ((s = psoff.alloc(8ull * n + 16)) != EH_OK) return fail(s);
   if ((s = prim_exscan_u32_to_u64(ctx, bs_n.as<uint32_t>(), bsoff.as<uint64_t>(), (uint64_t)n + 1)) != EH_OK) return fail(s);
@@ -459,7 +605,7 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
                                                               m->dbsw.as<unsigned long long>());
   if (hb[1])
     k_fill_pshort<<<grid_for(n), 256, 0, st>>>(m->dcard.as<uint32_t>(), poff.as<uint64_t>(), m->dunbr.as<uint32_t>(),
-                                               type.as<uint8_t>(), psoff.as<uint64_t>(), ps_n.as<uint32_t>(), n,
+                                               type.as<uint8_t>(), psoff.as<uint64_t>(), ps_n.as<uint32_t>(), n, 0u,
                                                m->dpsd.as<uint16_t>());
   if ((s = m->dbid.alloc(4 * hb[2] + 64)) != EH_OK) return fail(s);
   if ((s = m->dbwd.alloc(16 * hb[2] + 64)) != EH_OK) return fail(s);
@@ -475,11 +621,11 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
                                       bsoff.as<uint64_t>(), psoff.as<uint64_t>(), blkoff.as<uint64_t>(), n,
                                       m->ddesc.as<RowDesc>());
   if (hipEventRecord(ctx->ev1, st) != hipSuccess) return fail(EH_ERR_HIP);
-  if ((s = wed.alloc(8ull * n + 8)) != EH_OK) return fail(s);
-  k_wedges<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(m->dcard.as<uint32_t>(), poff.as<uint64_t>(),
-                                                       m->dunbr.as<uint32_t>(), n, wed.as<uint64_t>());
+  k_wedge_sum<<<bounded, 256, 0, st>>>(m->dcard.as<uint32_t>(), m->dunbr.as<uint32_t>(), m->unbr_words,
+                                       stats.as<unsigned long long>() + 4);
   if (hipGetLastError() != hipSuccess) { eh_set_error("build kernel launch failed"); return fail(EH_ERR_HIP); }
-  if ((s = prim_reduce_sum_u64(ctx, wed.as<uint64_t>(), n, &m->wedges)) != EH_OK) return fail(s);
+  if (hipMemcpy(&m->wedges, stats.as<unsigned long long>() + 4, 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(EH_ERR_HIP);
   if ((s = prim_reduce_sum_u64(ctx, refb.as<uint64_t>(), n, &m->ref_row_bytes)) != EH_OK) return fail(s);
   // kept entries = sum of card
   {
@@ -513,7 +659,19 @@ extern "C" eh_status eh_matrix_build(eh_graph *g, eh_edge_filter filter, const u
     if (g->entries) EH_HIP(hipMemcpy(dmask.p, edge_mask, g->entries, hipMemcpyHostToDevice));
   }
   return matrix_build_dev(g->ctx, g->n, g->off.as<uint64_t>(), g->nbr.as<uint32_t>(), g->entries, filter,
-                          dmask.as<uint8_t>(), policy, nullptr, out);
+                          dmask.as<uint8_t>(), policy, nullptr, g->symmetric, out);
+}
+
+extern "C" eh_status eh_matrix_build_typed(eh_graph *g, eh_edge_filter filter, const uint8_t *row_types,
+                                           eh_matrix **out) {
+  if (!g || !out || !row_types) { eh_set_error("eh_matrix_build_typed: null argument"); return EH_ERR_INVALID; }
+  if (filter == EH_FILTER_MASK) { eh_set_error("eh_matrix_build_typed: EH_FILTER_MASK is not supported"); return EH_ERR_INVALID; }
+  EH_HIP(hipSetDevice(g->ctx->device));
+  DevBuf dt;
+  EH_CHECK(dt.alloc((size_t)g->n + 16));
+  EH_HIP(hipMemcpy(dt.p, row_types, g->n, hipMemcpyHostToDevice));
+  return matrix_build_dev(g->ctx, g->n, g->off.as<uint64_t>(), g->nbr.as<uint32_t>(), g->entries, filter, nullptr,
+                          EH_LAYOUT_LEARNED, nullptr, g->symmetric, out, dt.as<uint8_t>());
 }
 
 extern "C" eh_status eh_matrix_build_host(eh_ctx *ctx, uint32_t n, const uint64_t *row_off, const uint32_t *nbrs,

@@ -602,7 +602,7 @@ __device__ uint64_t task_cliques4(const CountArgs &a, const AState &A, uint32_t 
 // pshort B rows are walked partition by partition, one pair after another.
 
 constexpr uint32_t PASS_PROBE = 0, PASS_AND = 1, PASS_STREAM = 2, PASS_BLOCK = 3, PASS_NTSTREAM = 4,
-                   PASS_STREAM16 = 5;
+                   PASS_STREAM16 = 5, PASS_SEARCH = 6;
 // B-stationary pairs above the split (k_bshl): the longest B row, and the bins of consecutive
 // B rows that form its tasks (a row weighs 64 plus the power of two at or above twice its ids
 // at or above the split; a bin holds BSH_BIN_WEIGHT of weight, at most BSH_GROUP rows)
@@ -627,6 +627,7 @@ __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t *L, uint32_t 
 //   STREAM  B's elements unbr[4 (v1 + r) .. +4] tested against A's LDS structure
 //   BLOCK   B's block slot v0 + r AND-ed with A's LDS bitmap
 //   NTSTREAM  B's new_type uint part ntu[4 (v1 + r) .. +4] tested against A
+//   SEARCH  A's element S.list[v0 + r] searched in B's sorted uint list (unbr[4 v1 ..], v2 ids)
 // a lane's plain count in one pass adds at most 64 per element it visits, far below 2^32;
 // weighted sums (card of every hit) need 64 bits
 template <bool W>
@@ -697,6 +698,11 @@ __device__ __forceinline__ PassAcc<W> pass_elem(const CountArgs &a, const AState
     } else {
       return (uint32_t)__popcll(v);
     }
+  } else if constexpr (PASS == PASS_SEARCH) {
+    const uint32_t x = S.list[v0 + r];
+    const uint32_t h = sorted_contains(a.unbr + 4ull * v1, v2, x);
+    if constexpr (W) return h ? (uint64_t)a.card[x] : 0ull;
+    else return h;
   } else if constexpr (PASS == PASS_BLOCK) {
     return (PassAcc<W>)block_and<W>(S, a, v0 + r);
   } else if constexpr (PASS == PASS_STREAM16) {
@@ -847,8 +853,12 @@ __device__ __forceinline__ uint64_t sum_isect(const CountArgs &a, const AState &
     // per pair, the cheapest of: probing A's elements in B's bitset words (PROBE),
     // AND-ing A's LDS bitmap with B's words (AND), streaming B's uint list against
     // A's LDS structure (STREAM).  Every row keeps its uint list, so pshort rows and
-    // bitset rows whose probes would outnumber their elements are streamed.
+    // bitset rows whose probes would outnumber their elements are streamed.  A uint row
+    // tagged for search (UG) is never streamed: A's elements between B's first and last
+    // id are each searched in B's list (SEARCH), the per-pair algorithm for long B rows
+    // that few of A's elements fall in.
     uint32_t pc = 0, ps = 0, ac = 0, as = 0, sc = 0, soff = d.off;
+    uint32_t uc = 0, us = 0;                    // SEARCH items
     uint32_t bc = 0, bs = 0, nc = 0, noff = 0;  // BLOCK and NTSTREAM items
     if (d.card) {
       if (BLK && KIND == KIND_BITMAP && is_block_type(t)) {
@@ -884,11 +894,19 @@ __device__ __forceinline__ uint64_t sum_isect(const CountArgs &a, const AState &
           const uint32_t ov = oe > os ? oe - os : 0u;
           if (16ull * ov < best) { ac = ov; as = os; pc = 0; sc = 0; }
         }
+      } else if (t == EH_UG) {
+        const uint32_t *Bl = a.unbr + (uint64_t)d.off * 4;
+        const uint32_t blo = Bl[0], bhi = Bl[d.card - 1];
+        const uint32_t s = lds_lower_bound(S.list, S.n, blo);
+        const uint32_t e = (bhi == 0xFFFFFFFFu) ? S.n : lds_lower_bound(S.list, S.n, bhi + 1u);
+        uc = e > s ? e - s : 0u;
+        us = s;
       } else {
         sc = (d.card + 3u) >> 2;
         if (!is_uint_type(t)) soff = a.aoff[key];
       }
     }
+    acc += flat_pass<PASS_SEARCH, KIND, W, UNROLL>(a, S, uc, us, d.off, d.card, lane, meta);
     acc += flat_pass<PASS_PROBE, KIND, W, UNROLL>(a, S, pc, ps, d.off, d.w0, lane, meta);
     if constexpr (KIND == KIND_BITMAP)
       acc += flat_pass<PASS_AND, KIND, W, UNROLL>(a, S, ac, as, d.off, d.w0, lane, meta);
@@ -2161,6 +2179,8 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
     if (opts->stream_ratio < 0) tp.stream_ratio = 0;
     else if (opts->stream_ratio > 0) tp.stream_ratio = (uint32_t)opts->stream_ratio;
     if (opts->bstat_split < 0) tp.bstat_j = tp.bsh_max = 0;
+    if (opts->bsh_max < 0) tp.bsh_max = 0;
+    else if (opts->bsh_max > 0 && tp.bsh_max) tp.bsh_max = (uint32_t)std::min<int32_t>(opts->bsh_max, (int32_t)BSH_MAX_D);
     if (opts->local_max < 0) tp.local_max = 0;
     else if (opts->local_max > 0 && tp.local_max)
       tp.local_max = (uint32_t)std::min<int32_t>(opts->local_max, (int32_t)LOCAL4_MAX);
@@ -2763,12 +2783,13 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
   const int skey = kk * 8 + sched;
   double sched_ms = 0;
   if (!s.valid || s.task_work != tp.task_work || s.lds_small != tp.lds_small || s.lds_big != tp.lds_big ||
-      (int)s.sched != skey || s.small_deg != tp.small_deg || s.bstat_req != tp.bstat_j ||
+      (int)s.sched != skey || s.small_deg != tp.small_deg || s.bstat_req != tp.bstat_j || s.bsh_req != tp.bsh_max ||
       s.local_req != tp.local_max) {
     s = Schedule();
     EH_CHECK(schedule_build(m, sched, kk, tp, s));
     s.sched = skey;
     s.bstat_req = tp.bstat_j;
+    s.bsh_req = tp.bsh_max;
     s.local_req = tp.local_max;
     sched_ms = s.ms;
   }

@@ -341,6 +341,7 @@ eh_status graph_canonicalize(eh_ctx *ctx, uint32_t n, DevBuf &keys, uint64_t nke
   g->ctx = ctx;
   eh_status s = finish_graph(ctx, n, tmp.as<uint64_t>(), mu, relabel, g, vgraph);
   if (s != EH_OK) { delete g; return s; }
+  g->symmetric = true;  // every undirected key is stored as both of its entries
   *out = g;
   return EH_OK;
 }

@@ -145,6 +145,9 @@ struct eh_ctx {
   int num_cus = 0;
   std::string name;
   std::vector<float> mlp;  // selector parameters (EH_MLP_PARAMS) or empty
+  DevBuf mlp_dev;          // their device copy for the selection kernel
+  uint32_t sel_guard = 8;  // LEARNED: a bitset only where its words are at most this many per id
+  int sel_fixed = -1;      // LEARNED: -1 the network's arm, 0..3 that arm for every row (ablations)
   DevBuf scratch;          // reusable temp storage for primitives
   DevBuf counter;          // small device counters
 };
@@ -157,6 +160,7 @@ struct eh_graph {
   DevBuf off;      // u64[n+1]
   DevBuf nbr;      // u32[entries]
   DevBuf new2old;  // u32[n] (empty => identity)
+  bool symmetric = false;  // built by canonicalisation: every edge stored both ways
 };
 
 struct Schedule {
@@ -175,6 +179,7 @@ struct Schedule {
   // (row offset / 4, position of j) of every task row containing j; btasks are
   // {j, first pair, end pair, 0}
   uint32_t bstat_j = 0, bstat_req = 0;  // in effect / as asked (0 where it does not apply)
+  uint32_t bsh_req = 0;                 // the k_bshl row bound the schedule was built for
   DevBuf in_pairs, btasks, jstart;  // jstart: per row, entries counted B-stationary
   DevBuf lo16;                      // those entries as 16-bit ids, 8 per 16-byte vector
   uint32_t local_req = 0;           // 4-cliques: the local_max the schedule was built for
@@ -238,8 +243,11 @@ eh_status graph_canonicalize(eh_ctx *ctx, uint32_t n, DevBuf &keys, uint64_t nke
 eh_status graph_relabel(eh_graph *g);
 eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const uint32_t *d_nbr,
                            uint64_t entries, eh_edge_filter filter, const uint8_t *d_mask,
-                           eh_layout_policy policy, const uint32_t *d_graph_of, eh_matrix **out);
-eh_status matrix_select_learned(eh_matrix *m, uint8_t *d_types);
+                           eh_layout_policy policy, const uint32_t *d_graph_of, bool symmetric,
+                           eh_matrix **out, const uint8_t *d_types_in = nullptr);
+// the learned (or fixed-arm) layout of every row; sym_off: the row offsets of the symmetric
+// graph whose lower triangle m holds (its degrees give the uses), or null
+eh_status matrix_select_learned(eh_matrix *m, uint8_t *d_types, const uint64_t *sym_off);
 eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_counts,
                     uint32_t ncounts, uint64_t *count, eh_run_stats *stats, uint64_t *d_task_cycles);
 eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, Schedule &s);

@@ -1,20 +1,26 @@
 // eh_mlp.hip -- the learned per-row selector.
 //
-// Every row that answers membership queries in the triangle pass can do so with
-// one of four arms (RowDesc type values):
-//   0 BS  probe one bit of the row's bitset          (intersection.hpp:1023-1058)
-//   1 PS  walk the pshort partitions, search the run (intersection.hpp:1062-1173)
-//   2 U   binary search in the sorted uint row       (the merge kernels' role)
-//   3 UG  galloping search from the front            (set_intersect_galloping, :393-584)
-// An FP32 MLP 16 -> 64 -> 64 -> 4 (ReLU) maps 16 row features to the predicted
-// log2 shader cycles per query of each arm.  It is trained on timings the engine
-// takes itself (k_profile_arms times every arm on the row's real queries), and the
-// LEARNED layout policy gives each row the arm with the smallest prediction.  The
-// arms only change how a count is computed, never which elements are common, so
+// Every row that is the B side of a pair can be stored and searched in one of four arms
+// (RowDesc type values), and the tuned triangle pass runs a different algorithm for each:
+//   0 BS  bitset: a short source row's wedge queries probe one bit (intersection.hpp:1023-1058);
+//         a task pair probes B's words with A's ids or ANDs them with A's LDS bitmap
+//   1 PS  pshort: queries walk the partitions (intersection.hpp:1062-1173); pairs stream
+//         B's canonical list against A's LDS structure
+//   2 U   uint: queries binary-search the list; pairs stream it
+//   3 UG  uint searched: queries binary-search the list; pairs search each of A's ids
+//         within B's first and last id in B's list instead of streaming B (the per-pair
+//         algorithm for long rows few of A's ids fall in; set_intersect_galloping's role,
+//         :393-584)
+// An FP32 MLP 16 -> 64 -> 64 -> 4 (ReLU) maps 16 row features to the predicted log2 cost of
+// each arm in the tuned pass: shader cycles of the row's own wedge queries and pairs, timed
+// with the pass's own code (k_selector_labels in eh_count.hip), plus the cycles of building
+// the arm's payload.  The LEARNED layout policy gives each row the arm with the smallest
+// prediction.  Arms change only how a count is computed, never which elements are common, so
 // counts are identical under every selection.
 //
-// Kernels here: feature extraction, arm profiling, MLP forward, MLP loss + per-sample
-// deltas, deterministic per-parameter gradient sums, Adam.
+// Kernels here: uses and features, the selection (MLP forward with the parameters staged in
+// LDS, over the rows whose arm can matter), MLP loss + per-sample deltas, deterministic
+// per-parameter gradient sums, Adam.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -34,8 +40,9 @@ inline unsigned grid_for(uint64_t work, unsigned block = 256) {
   return (unsigned)(g > 65536 ? 65536 : g);
 }
 
-// ---------------------------------------------------------------- features
+// ---------------------------------------------------------------- uses and features
 
+// uses[j] = rows holding j, one atomic per entry (any matrix)
 __global__ void k_uses(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr, uint32_t n, uint32_t *uses) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
@@ -47,185 +54,315 @@ __global__ void k_uses(const uint32_t *card, const uint32_t *aoff, const uint32_
   }
 }
 
+// uses of the lower-triangle rows of a symmetric graph: row(j) holds j's neighbours below j, so
+// the rows holding j are its neighbours above j, degree(j) - card(j) of them
+__global__ void k_uses_sym(const uint64_t *goff, const uint32_t *card, uint32_t n, uint32_t *uses) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    uses[i] = (uint32_t)(goff[i + 1] - goff[i]) - card[i];
+}
+
 __device__ __forceinline__ float lg1(float x) { return log2f(1.0f + x); }
 
-// 16 features of row i (all scaled to roughly [0, 1])
-__global__ void k_features(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr, const uint32_t *uses,
-                           uint32_t n, float *F) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t c = card[i];
-    const uint32_t *r = unbr + (uint64_t)aoff[i] * 4;
-    float f[NI];
-    for (int q = 0; q < NI; q++) f[q] = 0.0f;
-    const float u = (float)uses[i];
-    if (c) {
-      const uint32_t lo = r[0], hi = r[c - 1];
-      const float span = (float)(hi - lo);
-      uint32_t parts = 1, low64k = 0, quarter = 0, maxgap = 0;
-      const uint32_t qcut = lo + (hi - lo) / 4u;
-      for (uint32_t k = 0; k < c; k++) {
-        const uint32_t x = r[k];
-        if (k) {
-          parts += (x >> 16) != (r[k - 1] >> 16);
-          maxgap = max(maxgap, x - r[k - 1]);
-        }
-        low64k += x < 65536u;
-        quarter += x <= qcut;
-      }
-      f[0] = lg1((float)c) / 16.0f;
-      f[1] = lg1(span) / 32.0f;
-      f[2] = (float)c / (span + 1.0f);
-      f[3] = lg1((float)parts) / 16.0f;
-      f[6] = lg1((float)((hi >> 6) - (lo >> 6) + 1u)) / 24.0f;
-      f[7] = (float)low64k / (float)c;
-      f[8] = lg1(span / (float)c) / 32.0f;
-      f[9] = lg1((float)maxgap) / 32.0f;
-      f[11] = lg1((float)lo) / 32.0f;
-      f[12] = lg1((float)hi) / 32.0f;
-      f[13] = lg1((float)r[c / 2]) / 32.0f;
-      f[14] = (float)quarter / (float)c;
-    }
-    f[4] = lg1(u) / 24.0f;
-    f[5] = lg1(u * (float)c) / 40.0f;
-    f[10] = (float)i / (float)n;
-    f[15] = 1.0f;
-    for (int q = 0; q < NI; q++) F[i * NI + q] = f[q];
-  }
-}
-
-// ---------------------------------------------------------------- arm profiling
-
-struct ProfArgs {
-  const uint32_t *card, *aoff, *unbr;         // the oriented matrix (uint rows)
-  const uint64_t *toff;                       // in-neighbour CSR of the oriented matrix
-  const uint32_t *tsrc;
-  const RowDesc *desc_bs;                     // same rows encoded as bitsets
-  const uint64_t *bsw;
-  const RowDesc *desc_ps;                     // ... and as pshort streams
-  const uint16_t *psd;
-  const uint32_t *rows;                       // sampled rows
-  uint32_t nrows;
-  uint32_t qcap;                              // queries timed per row and arm
-  unsigned long long *cycles;                 // nrows x 4
-  unsigned long long *queries;                // nrows
-  unsigned long long *hits;                   // nrows x 4
+// the statistics behind a row's features
+struct RowStats {
+  uint32_t c, lo, hi, mid, parts, low64k, quarter, maxgap;
 };
 
-__device__ __forceinline__ uint32_t arm_member(const ProfArgs &p, uint32_t arm, uint32_t j, uint32_t x) {
-  if (arm == EH_BS) {
-    const RowDesc d = p.desc_bs[j];
-    const uint32_t w = (x >> 6) - d.w0;
-    return (w < desc_aux(d)) ? (uint32_t)((p.bsw[d.off + w] >> (x & 63u)) & 1ull) : 0u;
+// 16 features of row i, all in [0, 1] and free of the graph's size where they can be (ids and
+// spans relative to n, gaps relative to the span, counts relative to the row), so a network
+// trained on small graphs reads a large graph's rows on the same scale:
+//   0 log length            1 log span / n         2 density (ids per id of span)
+//   3 16-bit partitions per id   4 log uses        5 log (uses x length)
+//   6 log bitset words per id    7 share of ids below 65536 (the B-stationary split)
+//   8 log span per id       9 largest gap / span   10 row id / n
+//   11 first id / n         12 last id / n         13 middle id / n
+//   14 share of ids in the first quarter of the span   15 constant 1
+__device__ __forceinline__ void write_features(const RowStats &r, uint32_t u_raw, uint32_t i, uint32_t n, float *out) {
+  float f[NI];
+  for (int q = 0; q < NI; q++) f[q] = 0.0f;
+  const float u = (float)u_raw, fn = (float)n;
+  if (r.c) {
+    const float span = (float)(r.hi - r.lo), c = (float)r.c;
+    const float words = (float)((r.hi >> 6) - (r.lo >> 6) + 1u);
+    f[0] = lg1(c) / 16.0f;
+    f[1] = lg1(1048576.0f * span / fn) / 20.0f;
+    f[2] = c / (span + 1.0f);
+    f[3] = (float)r.parts / c;
+    f[6] = lg1(words / c) / 16.0f;
+    f[7] = (float)r.low64k / c;
+    f[8] = lg1(span / c) / 32.0f;
+    f[9] = span > 0.0f ? (float)r.maxgap / span : 0.0f;
+    f[11] = (float)r.lo / fn;
+    f[12] = (float)r.hi / fn;
+    f[13] = (float)r.mid / fn;
+    f[14] = (float)r.quarter / c;
   }
-  if (arm == EH_PS) {
-    const RowDesc d = p.desc_ps[j];
-    const uint16_t *S = p.psd + (uint64_t)d.off * 4;
-    const uint32_t len = desc_aux(d), hx = x >> 16;
-    uint32_t pos = 0;
-    while (pos < len) {
-      const uint32_t pre = S[pos], cnt = (uint32_t)S[pos + 1] + 1u;
-      if (pre == hx) {
-        uint32_t lo = 0, hi = cnt;
-        const uint32_t v = x & 0xFFFFu;
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (S[pos + 2 + mid] < v) lo = mid + 1; else hi = mid;
-        }
-        return lo < cnt && S[pos + 2 + lo] == v;
-      }
-      if (pre > hx) return 0;
-      pos += cnt + 2u;
-    }
-    return 0;
-  }
-  const uint32_t *R = p.unbr + (uint64_t)p.aoff[j] * 4;
-  const uint32_t n = p.card[j];
-  if (arm == EH_U) {
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (R[mid] < x) lo = mid + 1; else hi = mid;
-    }
-    return lo < n && R[lo] == x;
-  }
-  if (R[0] >= x) return R[0] == x;
-  uint32_t b = 1;
-  while (b < n && R[b] < x) b <<= 1;
-  uint32_t lo = (b >> 1) + 1, hi = min(b, n - 1) + 1;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (R[mid] < x) lo = mid + 1; else hi = mid;
-  }
-  return lo < n && R[lo] == x;
+  f[4] = lg1(u) / 24.0f;
+  f[5] = lg1(u * (float)r.c) / 40.0f;
+  f[10] = (float)i / fn;
+  f[15] = 1.0f;
+  float4 *o = reinterpret_cast<float4 *>(out);
+#pragma unroll
+  for (int q = 0; q < NI / 4; q++) o[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
 }
 
-// one wave per sampled row j: its queries are the elements below j of every source
-// row that contains j; each arm runs twice over the first qcap of them and the
-// second run is timed
-__global__ void __launch_bounds__(64) k_profile_arms(ProfArgs p) {
+// the same 16 features into an LDS queue entry (word stores: entries are an odd stride apart)
+__device__ __forceinline__ void write_features_lds(const RowStats &r, uint32_t u, uint32_t i, uint32_t n, float *out) {
+  float f[NI];
+  write_features(r, u, i, n, f);
+#pragma unroll
+  for (int q = 0; q < NI; q++) out[q] = f[q];
+}
+
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr uint32_t FEAT_LANE_MAX = 32;  // rows up to this long are read by one lane
+
+// the statistics of a row of at most FEAT_LANE_MAX ids, read by one lane
+__device__ __forceinline__ RowStats lane_stats(const uint32_t *r, uint32_t c) {
+  RowStats st = {c, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  if (!c) return st;
+  st.lo = r[0];
+  st.hi = r[c - 1];
+  st.mid = r[c / 2];
+  st.parts = 1;
+  const uint32_t qcut = st.lo + (st.hi - st.lo) / 4u;
+  uint32_t prev = st.lo;
+  for (uint32_t k = 0; k < c; k++) {
+    const uint32_t x = r[k];
+    if (k) {
+      st.parts += (x >> 16) != (prev >> 16);
+      st.maxgap = max(st.maxgap, x - prev);
+    }
+    st.low64k += x < 65536u;
+    st.quarter += x <= qcut;
+    prev = x;
+  }
+  return st;
+}
+
+// the statistics of a longer row, read by the whole wave (every lane returns them)
+__device__ __forceinline__ RowStats wave_stats(const uint32_t *r, uint32_t c, uint32_t lane) {
+  RowStats st = {c, r[0], r[c - 1], r[c / 2], 0u, 0u, 0u, 0u};
+  const uint32_t qcut = st.lo + (st.hi - st.lo) / 4u;
+  uint32_t parts = 0, low = 0, quart = 0, gap = 0;
+  for (uint32_t k = lane; k < c; k += 64) {
+    const uint32_t x = r[k];
+    if (k) {
+      const uint32_t pv = r[k - 1];
+      parts += (x >> 16) != (pv >> 16);
+      gap = max(gap, x - pv);
+    }
+    low += x < 65536u;
+    quart += x <= qcut;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    parts += __shfl_xor(parts, o, 64);
+    low += __shfl_xor(low, o, 64);
+    quart += __shfl_xor(quart, o, 64);
+    gap = max(gap, (uint32_t)__shfl_xor(gap, o, 64));
+  }
+  st.parts = 1u + parts;
+  st.low64k = low;
+  st.quarter = quart;
+  st.maxgap = gap;
+  return st;
+}
+
+// Features of every row, at its own index.  A wave takes 64 consecutive rows: each lane reads
+// a row of at most FEAT_LANE_MAX ids itself, then the wave reads the longer ones together.
+__global__ void __launch_bounds__(256) k_features(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr,
+                                                  const uint32_t *uses, uint32_t n, float *F) {
   const uint32_t lane = threadIdx.x & 63u;
-  for (uint32_t s = blockIdx.x; s < p.nrows; s += gridDim.x) {
-    const uint32_t j = p.rows[s];
-    const uint64_t t0 = p.toff[j], t1 = p.toff[j + 1];
-    uint64_t q_done = 0;
-    for (uint32_t arm = 0; arm < 4; arm++) {
-      uint64_t cyc = 0, hit = 0;
-      for (int rep = 0; rep < 2; rep++) {
-        __builtin_amdgcn_s_waitcnt(0);
-        const uint64_t c0 = clock64();
-        uint32_t h = 0, q = 0;
-        for (uint64_t e = t0; e < t1 && q < p.qcap; e++) {
-          const uint32_t i = p.tsrc[e];
-          const uint32_t d = p.card[i];
-          const uint32_t *A = p.unbr + (uint64_t)p.aoff[i] * 4;
-          for (uint32_t k = lane; k < d; k += 64) {
-            const uint32_t x = A[k];
-            if (x < j) h += arm_member(p, arm, j, x);
-          }
-          uint32_t below = 0;
-          for (uint32_t k = lane; k < d; k += 64) below += A[k] < j;
-          for (int o = 32; o > 0; o >>= 1) below += __shfl_xor(below, o, 64);
-          q += __builtin_amdgcn_readfirstlane(below);
-        }
-        for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o, 64);
-        const uint64_t c1 = clock64();
-        cyc = c1 - c0;
-        hit = h;
-        q_done = q;
-      }
-      if (lane == 0) {
-        p.cycles[4ull * s + arm] = cyc;
-        p.hits[4ull * s + arm] = hit;
-      }
+  const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwv = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t base = wid * 64; base < n; base += nwv * 64) {
+    const uint64_t i = base + lane;
+    const bool valid = i < n;
+    const uint32_t c = valid ? card[i] : 0u;
+    const uint32_t u = valid ? uses[i] : 0u;
+    if (valid && c <= FEAT_LANE_MAX)
+      write_features(lane_stats(unbr + (uint64_t)aoff[i] * 4, c), u, (uint32_t)i, n, F + i * NI);
+    uint64_t lm = __ballot(valid && c > FEAT_LANE_MAX);
+    while (lm) {
+      const uint32_t q = (uint32_t)__builtin_ctzll(lm);
+      lm &= lm - 1;
+      const uint32_t ii = (uint32_t)base + q;
+      const uint32_t cc = __builtin_amdgcn_readlane(c, q), uu = __builtin_amdgcn_readlane(u, q);
+      const RowStats st = wave_stats(unbr + (uint64_t)aoff[ii] * 4, cc, lane);
+      if (lane == 0) write_features(st, uu, ii, n, F + (uint64_t)ii * NI);
     }
-    if (lane == 0) p.queries[s] = q_done;
   }
 }
 
-// ---------------------------------------------------------------- MLP kernels
+// ---------------------------------------------------------------- MLP forward and selection
 
 __device__ __forceinline__ float relu(float v) { return v > 0.0f ? v : 0.0f; }
 
-// y[b] = MLP(x[b]); one thread per sample
-__global__ void k_mlp_forward(const float *__restrict__ P, const float *__restrict__ X, int batch, float *Y) {
+// y = MLP(x) with the parameters W (LDS: every lane of a wave reads the same word)
+__device__ __forceinline__ void mlp_eval(const float *W, const float *__restrict__ X, float (&y)[NO]) {
+  float x[NI], h1[NH];
+  const float4 *X4 = reinterpret_cast<const float4 *>(X);
+#pragma unroll
+  for (int q = 0; q < NI / 4; q++) {
+    const float4 v = X4[q];
+    x[4 * q] = v.x;
+    x[4 * q + 1] = v.y;
+    x[4 * q + 2] = v.z;
+    x[4 * q + 3] = v.w;
+  }
+#pragma unroll
+  for (int o = 0; o < NH; o++) {
+    float a = W[OB1 + o];
+#pragma unroll
+    for (int i = 0; i < NI; i++) a += W[OW1 + o * NI + i] * x[i];
+    h1[o] = relu(a);
+  }
+#pragma unroll
+  for (int k = 0; k < NO; k++) y[k] = W[OB3 + k];
+  for (int o = 0; o < NH; o++) {
+    float a = W[OB2 + o];
+#pragma unroll
+    for (int i = 0; i < NH; i++) a += W[OW2 + o * NH + i] * h1[i];
+    const float h2 = relu(a);
+#pragma unroll
+    for (int k = 0; k < NO; k++) y[k] += W[OW3 + k * NH + o] * h2;
+  }
+}
+
+// the block stages the parameters in LDS once (22 KB), then one thread per sample
+__global__ void __launch_bounds__(256) k_mlp_forward(const float *__restrict__ P, const float *__restrict__ X,
+                                                     int batch, float *Y) {
+  __shared__ float W[NPARAM];
+  for (int q = threadIdx.x; q < NPARAM; q += blockDim.x) W[q] = P[q];
+  __syncthreads();
   for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < batch; b += gridDim.x * blockDim.x) {
-    float x[NI], h1[NH], y[NO];
-    for (int i = 0; i < NI; i++) x[i] = X[(size_t)b * NI + i];
-    for (int o = 0; o < NH; o++) {
-      float a = P[OB1 + o];
-      for (int i = 0; i < NI; i++) a += P[OW1 + o * NI + i] * x[i];
-      h1[o] = relu(a);
-    }
-    for (int k = 0; k < NO; k++) y[k] = P[OB3 + k];
-    for (int o = 0; o < NH; o++) {
-      float a = P[OB2 + o];
-      for (int i = 0; i < NH; i++) a += P[OW2 + o * NH + i] * h1[i];
-      const float h2 = relu(a);
-      for (int k = 0; k < NO; k++) y[k] += P[OW3 + k * NH + o] * h2;
-    }
+    float y[NO];
+    mlp_eval(W, X + (size_t)b * NI, y);
     for (int k = 0; k < NO; k++) Y[(size_t)b * NO + k] = y[k];
   }
+}
+
+// The layout of every row under the LEARNED policy, in one pass.  Rows of fewer than two ids,
+// or held by no row, are uint (their arm cannot matter).  The others ("candidates") take the
+// arm with the smallest predicted cost (fixed < 0) or arm `fixed` (the ablations); a bitset
+// only where its words stay within `guard` per id.
+// Each wave walks 64 consecutive rows at a time and appends its candidates' features to its own
+// LDS queue (short rows read by one lane, long rows by the wave, as in k_features); whenever 64
+// are queued, every lane evaluates the network for one of them with the parameters the block
+// staged in LDS (every lane reads the same word), then the queue shifts down.  No features
+// leave the CU and no candidate list is built in HBM.
+constexpr uint32_t SEL_Q = 128;        // queue entries per wave (below 64 + one step's 64)
+constexpr uint32_t SEL_STRIDE = NI + 1;  // LDS words per entry (odd: lanes read distinct banks)
+
+__device__ __forceinline__ uint8_t pick_arm(const float *W, const float *x, uint32_t c, uint32_t nw, uint32_t guard) {
+  float xi[NI], h1[NH], y[NO];
+#pragma unroll
+  for (int q = 0; q < NI; q++) xi[q] = x[q];
+#pragma unroll
+  for (int o = 0; o < NH; o++) {
+    float a = W[OB1 + o];
+#pragma unroll
+    for (int i = 0; i < NI; i++) a += W[OW1 + o * NI + i] * xi[i];
+    h1[o] = relu(a);
+  }
+#pragma unroll
+  for (int k = 0; k < NO; k++) y[k] = W[OB3 + k];
+  for (int o = 0; o < NH; o++) {
+    float a = W[OB2 + o];
+#pragma unroll
+    for (int i = 0; i < NH; i++) a += W[OW2 + o * NH + i] * h1[i];
+    const float h2 = relu(a);
+#pragma unroll
+    for (int k = 0; k < NO; k++) y[k] += W[OW3 + k * NH + o] * h2;
+  }
+  const bool bs_ok = (uint64_t)nw <= (uint64_t)guard * c;
+  uint8_t pick = EH_U;
+  float best = 3.0e38f;
+  for (uint32_t k = 0; k < NO; k++) {
+    if (k == EH_BS && !bs_ok) continue;
+    if (y[k] < best) { best = y[k]; pick = (uint8_t)k; }
+  }
+  return pick;
+}
+
+__global__ void __launch_bounds__(256) k_select_learned(const uint32_t *card, const uint32_t *aoff,
+                                                        const uint32_t *unbr, const uint32_t *uses, uint32_t n,
+                                                        const float *__restrict__ P, uint32_t guard, int fixed,
+                                                        uint8_t *types) {
+  __shared__ float Wl[NPARAM];
+  __shared__ float Qf[4][SEL_Q * SEL_STRIDE];
+  __shared__ uint32_t Qr[4][SEL_Q], Qc[4][SEL_Q], Qw[4][SEL_Q];
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  if (fixed < 0) {
+    for (uint32_t q = threadIdx.x; q < (uint32_t)NPARAM; q += blockDim.x) Wl[q] = P[q];
+  }
+  __syncthreads();
+  float *QF = Qf[wv];
+  uint32_t *QR = Qr[wv], *QC = Qc[wv], *QW = Qw[wv];
+  uint32_t qn = 0;
+  auto drain = [&](uint32_t cnt) {
+    // one queued row per lane
+    if (lane < cnt) types[QR[lane]] = pick_arm(Wl, QF + lane * SEL_STRIDE, QC[lane], QW[lane], guard);
+    wave_sync();
+    const uint32_t rest = qn - cnt;
+    for (uint32_t e = lane; e < rest; e += 64) {
+      for (uint32_t q = 0; q < NI; q++) QF[e * SEL_STRIDE + q] = QF[(e + cnt) * SEL_STRIDE + q];
+      QR[e] = QR[e + cnt];
+      QC[e] = QC[e + cnt];
+      QW[e] = QW[e + cnt];
+    }
+    wave_sync();
+    qn = rest;
+  };
+  const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwv = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t base = wid * 64; base < n; base += nwv * 64) {
+    const uint64_t i = base + lane;
+    const bool valid = i < n;
+    const uint32_t c = valid ? card[i] : 0u;
+    const uint32_t u = valid ? uses[i] : 0u;
+    const bool want = valid && c >= 2u && u >= 1u;
+    const uint32_t *r = unbr + (uint64_t)(valid ? aoff[i] : 0u) * 4;
+    if (valid && !want) types[i] = EH_U;
+    if (fixed >= 0) {
+      if (want) {
+        const uint32_t nw = (r[c - 1] >> 6) - (r[0] >> 6) + 1u;
+        types[i] = (fixed == EH_BS && (uint64_t)nw > (uint64_t)guard * c) ? (uint8_t)EH_U : (uint8_t)fixed;
+      }
+      continue;
+    }
+    const uint64_t m = __ballot(want);
+    const uint32_t slot = qn + mbcnt64(m);
+    if (want) {
+      QR[slot] = (uint32_t)i;
+      QC[slot] = c;
+      QW[slot] = (r[c - 1] >> 6) - (r[0] >> 6) + 1u;
+      if (c <= FEAT_LANE_MAX) write_features_lds(lane_stats(r, c), u, (uint32_t)i, n, QF + slot * SEL_STRIDE);
+    }
+    uint64_t lm = __ballot(want && c > FEAT_LANE_MAX);
+    while (lm) {
+      const uint32_t q = (uint32_t)__builtin_ctzll(lm);
+      lm &= lm - 1;
+      const uint32_t cc = __builtin_amdgcn_readlane(c, q), uu = __builtin_amdgcn_readlane(u, q);
+      const uint32_t sq = __builtin_amdgcn_readlane(slot, q);
+      const uint32_t ii = (uint32_t)base + q;
+      const RowStats st = wave_stats(unbr + (uint64_t)aoff[ii] * 4, cc, lane);
+      if (lane == 0) write_features_lds(st, uu, ii, n, QF + sq * SEL_STRIDE);
+    }
+    qn += (uint32_t)__popcll(m);
+    wave_sync();
+    if (qn >= 64u) drain(64u);
+  }
+  if (fixed < 0 && qn) drain(qn);
 }
 
 // per-sample activations and deltas of the masked mean squared error
@@ -324,123 +461,68 @@ __global__ void k_adam(float *P, const float *G, float *M1, float *M2, float lr,
   P[p] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
 }
 
-// the arm with the smallest predicted cost; bitsets only where the words stay
-// within 8 per element (64 bytes, 16x the uint row), rows of one element stay uint
-__global__ void k_pick_arm(const float *Y, const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr, uint32_t n,
-                           uint8_t *types) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t c = card[i];
-    uint8_t t = EH_U;
-    if (c > 1) {
-      const uint32_t *r = unbr + (uint64_t)aoff[i] * 4;
-      const uint64_t nw = (uint64_t)(r[c - 1] >> 6) - (r[0] >> 6) + 1ull;
-      float best = 3.0e38f;
-      for (uint32_t k = 0; k < NO; k++) {
-        if (k == EH_BS && nw > 8ull * c) continue;
-        const float v = Y[i * NO + k];
-        if (v < best) { best = v; t = (uint8_t)k; }
-      }
-    }
-    types[i] = t;
-  }
-}
-
-__global__ void k_sample_rows(const uint32_t *card, const uint32_t *uses, uint32_t n, uint32_t stride, uint32_t *flag) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    flag[i] = (i < n && card[i] >= 2u && uses[i] >= 1u && (i % stride) == 0) ? 1u : 0u;
-}
-
-__global__ void k_scatter_flag(const uint32_t *flag, const uint64_t *pos, uint32_t n, uint32_t *out) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    if (flag[i]) out[pos[i]] = (uint32_t)i;
-}
-
-// keep only the sampled rows (compact CSR of the oriented matrix)
-__global__ void k_card_if(const uint32_t *card, const uint32_t *flag, uint32_t n, uint32_t *out) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    out[i] = (i < n && flag[i]) ? card[i] : 0u;
-}
-
-__global__ void k_copy_rows(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr, const uint32_t *flag,
-                            const uint64_t *off, uint32_t n, uint32_t *nbr) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t i = wid; i < n; i += nw) {
-    if (flag && !flag[i]) continue;
-    const uint32_t c = card[i];
-    const uint32_t *r = unbr + (uint64_t)aoff[i] * 4;
-    for (uint32_t k = lane; k < c; k += 64) nbr[off[i] + k] = r[k];
-  }
-}
-
-// transposed oriented CSR: keys (target << 32 | source)
-__global__ void k_transpose_keys(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr, const uint64_t *off,
-                                 uint32_t n, uint64_t *keys) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t i = wid; i < n; i += nw) {
-    const uint32_t c = card[i];
-    const uint32_t *r = unbr + (uint64_t)aoff[i] * 4;
-    for (uint32_t k = lane; k < c; k += 64) keys[off[i] + k] = ((uint64_t)r[k] << 32) | (uint32_t)i;
-  }
-}
-
-__global__ void k_hi_counts(const uint64_t *keys, uint64_t m, uint32_t *cnt) {
-  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m; e += (uint64_t)gridDim.x * blockDim.x)
-    atomicAdd(&cnt[(uint32_t)(keys[e] >> 32)], 1u);
-}
-
-__global__ void k_lo_words(const uint64_t *keys, uint64_t m, uint32_t *out) {
-  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m; e += (uint64_t)gridDim.x * blockDim.x)
-    out[e] = (uint32_t)keys[e];
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------- host helpers
 
-static eh_status features_dev(eh_matrix *m, DevBuf &F, DevBuf &uses) {
+// uses of every row: from the source graph's degrees when the matrix holds the lower
+// triangle of a symmetric graph (sym_off: its row offsets), else one atomic per entry
+static eh_status uses_dev(eh_matrix *m, const uint64_t *sym_off, DevBuf &uses) {
   hipStream_t st = m->ctx->stream;
   const uint32_t n = m->n;
   EH_CHECK(uses.alloc(4ull * n + 4));
-  EH_HIP(hipMemsetAsync(uses.p, 0, 4ull * n + 4, st));
-  k_uses<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(),
-                                                     m->dunbr.as<uint32_t>(), n, uses.as<uint32_t>());
+  if (sym_off && m->filter == EH_FILTER_LOWER) {
+    k_uses_sym<<<grid_for(n), 256, 0, st>>>(sym_off, m->dcard.as<uint32_t>(), n, uses.as<uint32_t>());
+  } else {
+    EH_HIP(hipMemsetAsync(uses.p, 0, 4ull * n + 4, st));
+    k_uses<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(),
+                                                       m->dunbr.as<uint32_t>(), n, uses.as<uint32_t>());
+  }
+  EH_HIP(hipGetLastError());
+  return EH_OK;
+}
+
+// features of every row, at its own index
+static eh_status features_dev(eh_matrix *m, DevBuf &F, DevBuf &uses) {
+  hipStream_t st = m->ctx->stream;
+  const uint32_t n = m->n;
+  EH_CHECK(uses_dev(m, nullptr, uses));
   EH_CHECK(F.alloc(4ull * NI * n + 64));
-  k_features<<<grid_for(n), 256, 0, st>>>(m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(), m->dunbr.as<uint32_t>(),
-                                          uses.as<uint32_t>(), n, F.as<float>());
+  k_features<<<grid_for((uint64_t)n, 256), 256, 0, st>>>(m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(),
+                                                         m->dunbr.as<uint32_t>(), uses.as<uint32_t>(), n,
+                                                         F.as<float>());
   EH_HIP(hipGetLastError());
   return EH_OK;
 }
 
 static eh_status mlp_forward_dev(eh_ctx *ctx, const float *d_params, const float *d_x, int batch, float *d_y) {
-  k_mlp_forward<<<grid_for((uint64_t)batch, 128), 128, 0, ctx->stream>>>(d_params, d_x, batch, d_y);
+  k_mlp_forward<<<grid_for((uint64_t)batch, 256), 256, 0, ctx->stream>>>(d_params, d_x, batch, d_y);
   EH_HIP(hipGetLastError());
   return EH_OK;
 }
 
-eh_status matrix_select_learned(eh_matrix *m, uint8_t *d_types) {
+eh_status matrix_select_learned(eh_matrix *m, uint8_t *d_types, const uint64_t *sym_off) {
   eh_ctx *ctx = m->ctx;
-  if (ctx->mlp.size() != (size_t)NPARAM) {
+  const bool fixed = ctx->sel_fixed >= 0;
+  if (!fixed && ctx->mlp.size() != (size_t)NPARAM) {
     eh_set_error("EH_LAYOUT_LEARNED needs trained selector parameters (eh_selector_set_params / eh_selector_train)");
     return EH_ERR_INVALID;
   }
-  DevBuf F, uses, P, Y;
-  EH_CHECK(features_dev(m, F, uses));
-  EH_CHECK(P.alloc(4ull * NPARAM));
-  EH_HIP(hipMemcpy(P.p, ctx->mlp.data(), 4ull * NPARAM, hipMemcpyHostToDevice));
-  EH_CHECK(Y.alloc(4ull * NO * m->n + 64));
-  EH_CHECK(mlp_forward_dev(ctx, P.as<float>(), F.as<float>(), (int)m->n, Y.as<float>()));
-  k_pick_arm<<<grid_for(m->n), 256, 0, ctx->stream>>>(Y.as<float>(), m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(),
-                                                      m->dunbr.as<uint32_t>(), m->n, d_types);
+  hipStream_t st = ctx->stream;
+  const uint32_t n = m->n;
+  DevBuf uses;
+  EH_CHECK(uses_dev(m, sym_off, uses));
+  if (!fixed) {
+    if (ctx->mlp_dev.bytes < 4ull * NPARAM) EH_CHECK(ctx->mlp_dev.alloc(4ull * NPARAM));
+    EH_HIP(hipMemcpyAsync(ctx->mlp_dev.p, ctx->mlp.data(), 4ull * NPARAM, hipMemcpyHostToDevice, st));
+  }
+  // every block stages the parameters (22 KB, from L2) once: a few thousand blocks at most
+  const unsigned blocks = std::min(grid_for(n), 8192u);
+  k_select_learned<<<blocks, 256, 0, st>>>(m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(), m->dunbr.as<uint32_t>(),
+                                           uses.as<uint32_t>(), n, fixed ? nullptr : ctx->mlp_dev.as<float>(),
+                                           ctx->sel_guard, ctx->sel_fixed, d_types);
   EH_HIP(hipGetLastError());
-  EH_HIP(hipStreamSynchronize(ctx->stream));
+  EH_HIP(hipStreamSynchronize(st));
   return EH_OK;
 }
 
@@ -459,6 +541,15 @@ extern "C" eh_status eh_selector_get_params(eh_ctx *ctx, float *params) {
   return EH_OK;
 }
 
+extern "C" eh_status eh_selector_set_rule(eh_ctx *ctx, uint32_t guard_words, int32_t fixed_arm) {
+  if (!ctx) { eh_set_error("null argument"); return EH_ERR_INVALID; }
+  if (guard_words == 0 || guard_words > (1u << 20)) { eh_set_error("bitset guard must be 1 .. 2^20 words per id"); return EH_ERR_INVALID; }
+  if (fixed_arm < -1 || fixed_arm > 3) { eh_set_error("fixed arm must be -1 (learned) or 0 .. 3"); return EH_ERR_INVALID; }
+  ctx->sel_guard = guard_words;
+  ctx->sel_fixed = fixed_arm;
+  return EH_OK;
+}
+
 extern "C" eh_status eh_matrix_features(eh_matrix *m, float *features) {
   if (!m || !features) { eh_set_error("null argument"); return EH_ERR_INVALID; }
   EH_HIP(hipSetDevice(m->ctx->device));
@@ -466,119 +557,6 @@ extern "C" eh_status eh_matrix_features(eh_matrix *m, float *features) {
   EH_CHECK(features_dev(m, F, uses));
   EH_HIP(hipStreamSynchronize(m->ctx->stream));
   EH_HIP(hipMemcpy(features, F.p, 4ull * NI * m->n, hipMemcpyDeviceToHost));
-  return EH_OK;
-}
-
-extern "C" eh_status eh_matrix_profile_layouts(eh_matrix *m, uint32_t stride, float *targets, float *mask) {
-  if (!m || !targets || !mask || stride == 0) { eh_set_error("bad argument"); return EH_ERR_INVALID; }
-  if (m->filter != EH_FILTER_LOWER) { eh_set_error("arm profiling needs an oriented (EH_FILTER_LOWER) matrix"); return EH_ERR_INVALID; }
-  eh_ctx *ctx = m->ctx;
-  EH_HIP(hipSetDevice(ctx->device));
-  hipStream_t st = ctx->stream;
-  const uint32_t n = m->n;
-  DevBuf F, uses, flag, fpos, rows;
-  EH_CHECK(features_dev(m, F, uses));
-  EH_CHECK(flag.alloc(4ull * n + 8));
-  EH_CHECK(fpos.alloc(8ull * n + 16));
-  k_sample_rows<<<grid_for((uint64_t)n + 1), 256, 0, st>>>(m->dcard.as<uint32_t>(), uses.as<uint32_t>(), n, stride,
-                                                          flag.as<uint32_t>());
-  EH_CHECK(prim_exscan_u32_to_u64(ctx, flag.as<uint32_t>(), fpos.as<uint64_t>(), (uint64_t)n + 1));
-  uint64_t ns = 0;
-  EH_HIP(hipMemcpy(&ns, fpos.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost));
-  for (uint64_t q = 0; q < 4ull * n; q++) { targets[q] = 0.0f; mask[q] = 0.0f; }
-  if (ns == 0) return EH_OK;
-  EH_CHECK(rows.alloc(4 * ns + 16));
-  k_scatter_flag<<<grid_for(n), 256, 0, st>>>(flag.as<uint32_t>(), fpos.as<uint64_t>(), n, rows.as<uint32_t>());
-  // compact CSR of the sampled rows, encoded once as bitsets and once as pshort
-  DevBuf cs, offs, nbrs;
-  EH_CHECK(cs.alloc(4ull * n + 8));
-  EH_CHECK(offs.alloc(8ull * n + 16));
-  k_card_if<<<grid_for((uint64_t)n + 1), 256, 0, st>>>(m->dcard.as<uint32_t>(), flag.as<uint32_t>(), n, cs.as<uint32_t>());
-  EH_CHECK(prim_exscan_u32_to_u64(ctx, cs.as<uint32_t>(), offs.as<uint64_t>(), (uint64_t)n + 1));
-  uint64_t es = 0;
-  EH_HIP(hipMemcpy(&es, offs.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost));
-  EH_CHECK(nbrs.alloc(4 * es + 16));
-  k_copy_rows<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(),
-                                                          m->dunbr.as<uint32_t>(), flag.as<uint32_t>(),
-                                                          offs.as<uint64_t>(), n, nbrs.as<uint32_t>());
-  eh_matrix *Mb = nullptr, *Mp = nullptr;
-  EH_CHECK(matrix_build_dev(ctx, n, offs.as<uint64_t>(), nbrs.as<uint32_t>(), es, EH_FILTER_ALL, nullptr,
-                            EH_LAYOUT_BITSET, nullptr, &Mb));
-  eh_status s = matrix_build_dev(ctx, n, offs.as<uint64_t>(), nbrs.as<uint32_t>(), es, EH_FILTER_ALL, nullptr,
-                                 EH_LAYOUT_PSHORT, nullptr, &Mp);
-  if (s != EH_OK) { eh_matrix_free(Mb); return s; }
-  // in-neighbours of every row in the oriented matrix
-  DevBuf coff, keys, keys2, tcnt, toff, tsrc;
-  uint64_t mm = m->card;
-  s = coff.alloc(8ull * n + 16);
-  if (s == EH_OK) s = keys.alloc(8 * mm + 16);
-  if (s == EH_OK) s = keys2.alloc(8 * mm + 16);
-  if (s == EH_OK) s = tcnt.alloc(4ull * n + 8);
-  if (s == EH_OK) s = toff.alloc(8ull * n + 16);
-  if (s == EH_OK) s = tsrc.alloc(4 * mm + 16);
-  if (s == EH_OK) s = prim_exscan_u32_to_u64(ctx, m->dcard.as<uint32_t>(), coff.as<uint64_t>(), (uint64_t)n);
-  if (s == EH_OK) {
-    k_transpose_keys<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(),
-                                                                 m->dunbr.as<uint32_t>(), coff.as<uint64_t>(), n,
-                                                                 keys.as<uint64_t>());
-    s = prim_sort_u64(ctx, keys.as<uint64_t>(), keys2.as<uint64_t>(), mm, 64);
-  }
-  if (s == EH_OK) {
-    (void)hipMemsetAsync(tcnt.p, 0, 4ull * n + 8, st);
-    k_hi_counts<<<grid_for(mm), 256, 0, st>>>(keys.as<uint64_t>(), mm, tcnt.as<uint32_t>());
-    s = prim_exscan_u32_to_u64(ctx, tcnt.as<uint32_t>(), toff.as<uint64_t>(), (uint64_t)n + 1);
-  }
-  DevBuf cyc, qs, hits;
-  if (s == EH_OK) {
-    k_lo_words<<<grid_for(mm), 256, 0, st>>>(keys.as<uint64_t>(), mm, tsrc.as<uint32_t>());
-    s = cyc.alloc(32 * ns + 64);
-    if (s == EH_OK) s = qs.alloc(8 * ns + 64);
-    if (s == EH_OK) s = hits.alloc(32 * ns + 64);
-  }
-  if (s == EH_OK) {
-    ProfArgs p;
-    p.card = m->dcard.as<uint32_t>();
-    p.aoff = m->daoff.as<uint32_t>();
-    p.unbr = m->dunbr.as<uint32_t>();
-    p.toff = toff.as<uint64_t>();
-    p.tsrc = tsrc.as<uint32_t>();
-    p.desc_bs = Mb->ddesc.as<RowDesc>();
-    p.bsw = Mb->dbsw.as<uint64_t>();
-    p.desc_ps = Mp->ddesc.as<RowDesc>();
-    p.psd = Mp->dpsd.as<uint16_t>();
-    p.rows = rows.as<uint32_t>();
-    p.nrows = (uint32_t)ns;
-    p.qcap = 2048;
-    p.cycles = cyc.as<unsigned long long>();
-    p.queries = qs.as<unsigned long long>();
-    p.hits = hits.as<unsigned long long>();
-    const unsigned blocks = (unsigned)std::min<uint64_t>(ns, (uint64_t)(ctx->num_cus ? ctx->num_cus : 256) * 4);
-    k_profile_arms<<<blocks, 64, 0, st>>>(p);
-    if (hipGetLastError() != hipSuccess) s = EH_ERR_HIP;
-  }
-  std::vector<uint32_t> hr(ns);
-  std::vector<uint64_t> hc(4 * ns), hq(ns), hh(4 * ns);
-  if (s == EH_OK && (hipMemcpy(hr.data(), rows.p, 4 * ns, hipMemcpyDeviceToHost) != hipSuccess ||
-                     hipMemcpy(hc.data(), cyc.p, 32 * ns, hipMemcpyDeviceToHost) != hipSuccess ||
-                     hipMemcpy(hq.data(), qs.p, 8 * ns, hipMemcpyDeviceToHost) != hipSuccess ||
-                     hipMemcpy(hh.data(), hits.p, 32 * ns, hipMemcpyDeviceToHost) != hipSuccess))
-    s = EH_ERR_HIP;
-  eh_matrix_free(Mb);
-  eh_matrix_free(Mp);
-  if (s != EH_OK) return s;
-  for (uint64_t q = 0; q < ns; q++) {
-    // every arm must find the same common elements
-    if (hh[4 * q] != hh[4 * q + 1] || hh[4 * q] != hh[4 * q + 2] || hh[4 * q] != hh[4 * q + 3]) {
-      eh_set_error("arm profiling: arms disagree on row %u", hr[q]);
-      return EH_ERR_HIP;
-    }
-    if (hq[q] < 8) continue;
-    const uint32_t r = hr[q];
-    for (int k = 0; k < 4; k++) {
-      targets[4ull * r + k] = (float)std::log2((double)hc[4 * q + k] / (double)hq[q]);
-      mask[4ull * r + k] = 1.0f;
-    }
-  }
   return EH_OK;
 }
 
