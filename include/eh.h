@@ -138,7 +138,7 @@ typedef struct {
                              report their device times in eh_run_stats.part_ms */
   int32_t bsh_max;        /* TUNED triangles with the split on: a row whose B rows at or above the
                              split all have at most this many entries counts those pairs
-                             B-stationary too (k_bshl); 0 = default (256), <0 = off, else 1..256 */
+                             B-stationary too (k_bshl); 0 = default (512), <0 = off, else 1..512 */
 } eh_run_opts;
 
 /* Kernel classes of one counting pass (indices of eh_run_stats.part_ms). */

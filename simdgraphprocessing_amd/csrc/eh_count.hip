@@ -606,7 +606,7 @@ constexpr uint32_t PASS_PROBE = 0, PASS_AND = 1, PASS_STREAM = 2, PASS_BLOCK = 3
 // B-stationary pairs above the split (k_bshl): the longest B row, and the bins of consecutive
 // B rows that form its tasks (a row weighs 64 plus the power of two at or above twice its ids
 // at or above the split; a bin holds BSH_BIN_WEIGHT of weight, at most BSH_GROUP rows)
-constexpr uint32_t BSH_MAX_D = 256, BSH_GROUP = 16, BSH_BIN_WEIGHT = 1024;
+constexpr uint32_t BSH_MAX_D = 512, BSH_GROUP = 16, BSH_BIN_WEIGHT = 1024;
 // flat_pass metadata: four rank-ordered arrays of 64 words and 64 slot-block start masks
 constexpr uint32_t META_WORDS = 384;
 constexpr uint32_t SMALL_META_WORDS = 260;
@@ -1609,15 +1609,18 @@ __global__ void __launch_bounds__(256) k_local4(CountArgs a, const uint32_t *__r
 // of the row's in-pairs in [b, e) stream through the bitmap (as in k_bstat) and the vectors
 // of row(i) from its first id at or above the split to j's through the list (binary search,
 // four ids in lockstep); then LO(j)'s words are cleared again, so a B row costs its own
-// length instead of the bitmap's.  The rows are pipelined: a lane keeps ids k, k + 64,
-// k + 128, k + 192 of row(j) in registers for setting and clearing them, and the next row
-// and its first 64 in-pairs load during the current row's passes.  The 16-bit copies are
-// padded with split - 1, a bit never set here: a pair whose LO(i) and LO(j) both end with
-// split - 1 counts that id apart.
+// length instead of the bitmap's.  The rows are pipelined: a lane keeps ids k, k + 64, ...,
+// k + 64 (R - 1) of row(j) in registers for setting and clearing them, and the next row and
+// its first 64 in-pairs load during the current row's passes.  The 16-bit copies are padded
+// with split - 1, a bit never set here: a pair whose LO(i) and LO(j) both end with split - 1
+// counts that id apart.  R is 4 when no B row of the schedule holds more than 256 ids, else 8
+// (BSH_MAX_D = 512); the LDS list holds 64 R ids.
+template <int R>
 __global__ void __launch_bounds__(256) k_bshl(CountArgs a, const uint4 *__restrict__ htasks, uint64_t nht,
                                               const uint4 *__restrict__ hpairs, const uint32_t *__restrict__ hj,
                                               const uint32_t *__restrict__ hjlo, const uint64_t *__restrict__ hoff,
                                               const uint32_t *__restrict__ bstart, uint32_t bm_words, uint32_t pad) {
+  static_assert(R >= 1 && 64 * R <= (int)BSH_MAX_D, "row(j) register rounds");
   extern __shared__ uint32_t lds_all[];
   const uint32_t lane = lane_id();
   uint32_t *meta = lds_all + (threadIdx.x >> 6) * a.lds_words;
@@ -1653,18 +1656,18 @@ __global__ void __launch_bounds__(256) k_bshl(CountArgs a, const uint4 *__restri
     const uint4 task = htasks[t++];
     const uint32_t bin = rfl(task.x), pb = rfl(task.y), pe = rfl(task.z);
     const uint32_t h0 = rfl(bstart[bin]), h1 = rfl(bstart[bin + 1]);
-    uint32_t xn[4], xc[4];
+    uint32_t xn[R], xc[R];
     uint4 prn = make_uint4(0u, 0u, 0u, 0u);
     uint32_t dn = 0, ln = 0, q0n = 0, q1n = 0;
     auto fetch = [&](uint32_t h) {
       q0n = max((uint32_t)hoff[h], pb);
       q1n = min((uint32_t)hoff[h + 1], pe);
       const uint32_t j = rfl(hj[h]);
-      dn = (q0n < q1n) ? rfl(a.card[j]) : 0u;  // row(j) has at most BSH_MAX_D = 256 ids
+      dn = (q0n < q1n) ? rfl(a.card[j]) : 0u;  // row(j) has at most 64 R ids
       ln = rfl(hjlo[h]);
       const uint32_t *G = a.unbr + (uint64_t)a.aoff[j] * 4;
 #pragma unroll
-      for (int u = 0; u < 4; u++) xn[u] = (lane + 64u * u < dn) ? G[lane + 64u * u] : pad;
+      for (int u = 0; u < R; u++) xn[u] = (lane + 64u * u < dn) ? G[lane + 64u * u] : pad;
       prn = (dn && q0n + lane < q1n) ? hpairs[q0n + lane] : make_uint4(0u, 0u, 0u, 0u);
     };
     fetch(h0);
@@ -1672,7 +1675,7 @@ __global__ void __launch_bounds__(256) k_bshl(CountArgs a, const uint4 *__restri
       const uint32_t dj = dn, dlo = ln, q0 = q0n, q1 = q1n;
       uint4 pr = prn;
 #pragma unroll
-      for (int u = 0; u < 4; u++) xc[u] = xn[u];
+      for (int u = 0; u < R; u++) xc[u] = xn[u];
       if (h + 1 < h1) fetch(h + 1);
       if (dj == 0) continue;  // no pairs of this row in the task
       const uint32_t dhi = dj - dlo;
@@ -1680,11 +1683,14 @@ __global__ void __launch_bounds__(256) k_bshl(CountArgs a, const uint4 *__restri
       if (dlo) {
         // LO(j) is sorted: it ends with the pad id exactly when its last id is split - 1
         const uint32_t qd = (dlo - 1u) >> 6;  // wave-uniform: which register holds it
-        const uint32_t xl = qd == 0u ? xc[0] : qd == 1u ? xc[1] : qd == 2u ? xc[2] : xc[3];
+        uint32_t xl = xc[0];
+#pragma unroll
+        for (int u = 1; u < R; u++)
+          if (qd == (uint32_t)u) xl = xc[u];
         has_pad = __shfl(xl, (int)((dlo - 1u) & 63u), 64) == pad;
       }
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < R; u++) {
         const uint32_t k = lane + 64u * u;
         if (k < dlo && xc[u] != pad) atomicOr(&bm[xc[u] >> 5], 1u << (xc[u] & 31u));
         else if (k >= dlo && k < dj) hl[k - dlo] = xc[u];
@@ -1710,7 +1716,7 @@ __global__ void __launch_bounds__(256) k_bshl(CountArgs a, const uint4 *__restri
       }
       wave_sync();
 #pragma unroll
-      for (int u = 0; u < 4; u++)
+      for (int u = 0; u < R; u++)
         if (lane + 64u * u < dlo && xc[u] != pad) bm[xc[u] >> 5] = 0u;
       wave_sync();
     }
@@ -1993,11 +1999,12 @@ __global__ void k_nonzero_flag(const uint32_t *c, uint32_t n, uint32_t *flag) {
 // compacted B rows (in row order), their pair counts and table-word weights
 __global__ void k_bsh_rows(const uint32_t *hcount, const uint64_t *hpos, uint32_t nh, uint32_t split,
                            const uint32_t *card, const uint32_t *jbelow, uint32_t *hj, uint32_t *hjlo, uint32_t *hc,
-                           uint32_t *hw) {
+                           uint32_t *hw, uint32_t *longest) {
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += gridDim.x * blockDim.x) {
     if (!hcount[q]) continue;
     const uint64_t h = hpos[q];
     const uint32_t j = split + q, d = card[j] - jbelow[j];  // the row's ids at or above the split
+    atomicMax(longest, card[j]);
     uint32_t lg = 4;
     while ((1u << lg) < 2u * d) lg++;
     hj[h] = j;
@@ -2413,6 +2420,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     s.n_in_pairs = npairs;
   }
   s.n_htasks = 0;
+  s.bsh_longest = 0;
   if (nh) {
     // hashed part: the B rows at or above the split with in-pairs (compacted, row order),
     // bins of them by table words, the in-pairs of each, tasks of at most BSH_CHUNK pairs
@@ -2436,9 +2444,11 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
       EH_CHECK(bflag.alloc(4 * nhr + 16));
       EH_CHECK(bpos.alloc(8 * nhr + 16));
       EH_CHECK(hfill.alloc(4 * nhr + 16));
+      k_zero_u32<<<1, 64, 0, st>>>(cnt2.as<uint32_t>(), 1);
       k_bsh_rows<<<grid_for(nh), 256, 0, st>>>(hcount.as<uint32_t>(), hpos.as<uint64_t>(), nh, bstat_j,
                                                m->dcard.as<uint32_t>(), jbelow.as<uint32_t>(), s.hj.as<uint32_t>(),
-                                               s.hjlo.as<uint32_t>(), hc.as<uint32_t>(), hw.as<uint32_t>());
+                                               s.hjlo.as<uint32_t>(), hc.as<uint32_t>(), hw.as<uint32_t>(),
+                                               cnt2.as<uint32_t>());
       k_zero_u32<<<1, 64, 0, st>>>(hc.as<uint32_t>() + nhr, 1);
       EH_CHECK(prim_exscan_u32_to_u64(ctx, hc.as<uint32_t>(), s.hoff.as<uint64_t>(), nhr + 1));
       EH_CHECK(prim_exscan_u32_to_u64(ctx, hw.as<uint32_t>(), pw.as<uint64_t>(), nhr));
@@ -2446,9 +2456,17 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
       k_zero_u32<<<1, 64, 0, st>>>(bflag.as<uint32_t>() + nhr, 1);
       EH_CHECK(prim_exscan_u32_to_u64(ctx, bflag.as<uint32_t>(), bpos.as<uint64_t>(), nhr + 1));
       uint64_t nb = 0, nhp = 0;
+      uint32_t longest = 0;
       EH_HIP(hipMemcpyAsync(&nb, bpos.as<uint64_t>() + nhr, 8, hipMemcpyDeviceToHost, st));
       EH_HIP(hipMemcpyAsync(&nhp, s.hoff.as<uint64_t>() + nhr, 8, hipMemcpyDeviceToHost, st));
+      EH_HIP(hipMemcpyAsync(&longest, cnt2.p, 4, hipMemcpyDeviceToHost, st));
       EH_HIP(hipStreamSynchronize(st));
+      if (longest > BSH_MAX_D) {
+        eh_set_error("a B row of %u ids above the split exceeds the %u the B-stationary pass holds", longest,
+                     BSH_MAX_D);
+        return EH_ERR_INVALID;
+      }
+      s.bsh_longest = longest;
       if (nhp >= (1ull << 32)) {
         eh_set_error("%llu B-stationary pairs above the split exceed the 32-bit pair index", (unsigned long long)nhp);
         return EH_ERR_UNSUPPORTED;
@@ -2628,27 +2646,30 @@ eh_status launch_local(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows,
 
 eh_status launch_bshl(eh_ctx *ctx, const CountArgs &base, const uint4 *htasks, uint64_t nht, const uint4 *hpairs,
                       const uint32_t *hj, const uint32_t *hjlo, const uint64_t *hoff, const uint32_t *bstart,
-                      uint32_t bstat_j, uint32_t *d_next, PartEv *pe) {
+                      uint32_t bstat_j, uint32_t longest, uint32_t *d_next, PartEv *pe) {
   if (nht == 0) return EH_OK;
   CountArgs a = base;
   a.next = d_next;
+  // row(j) in 4 register rounds (256 ids) unless a B row is longer, then 8 (512)
+  const bool wide = longest > 256u;
+  auto kern = wide ? k_bshl<8> : k_bshl<4>;
   // a wave's slice: the pass metadata, a bitmap over the ids [0, bstat_j) and a B row's ids
   // at or above the split
   const uint32_t bm_words = 2u * (((bstat_j - 1u) >> 6) + 1u);
-  a.lds_words = META_WORDS + bm_words + BSH_MAX_D;
+  a.lds_words = META_WORDS + bm_words + (wide ? 512u : 256u);
   // two waves per block: the 10.75 KB slices then fill 14 waves per CU (four per block fit 12;
   // RMAT-22 0.93 against 1.01 ms)
   int waves = 2;
   while (waves > 1 && (size_t)a.lds_words * 4 * waves > 64 * 1024) waves--;
   const size_t lds_bytes = (size_t)a.lds_words * 4 * waves;
   const int threads = 64 * waves;
-  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(k_bshl), threads, lds_bytes);
+  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(kern), threads, lds_bytes);
   uint64_t blocks = (uint64_t)per_cu * (ctx->num_cus ? ctx->num_cus : 256);
   const uint64_t need = ceil_div(ceil_div(nht, (uint64_t)a.grab), (uint64_t)waves);
   if (blocks > need) blocks = need;
   k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
   part_begin(ctx, pe);
-  hipLaunchKernelGGL(k_bshl, dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a, htasks, nht, hpairs,
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a, htasks, nht, hpairs,
                      hj, hjlo, hoff, bstart, bm_words, bstat_j - 1u);
   EH_HIP(hipGetLastError());
   part_end(ctx, pe);
@@ -2714,7 +2735,8 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
     if (hs && hs->n_htasks) {
       EH_CHECK(launch_bshl(m->ctx, a, hs->htasks.as<uint4>(), hs->n_htasks, hs->hpairs.as<uint4>(),
                            hs->hj.as<uint32_t>(), hs->hjlo.as<uint32_t>(), hs->hoff.as<uint64_t>(),
-                           hs->bstart.as<uint32_t>(), bstat_j, d_next + 5, pe(EH_PART_BSTAT_ABOVE)));
+                           hs->bstart.as<uint32_t>(), bstat_j, hs->bsh_longest, d_next + 5,
+                           pe(EH_PART_BSTAT_ABOVE)));
     }
     EH_CHECK((launch_count<3, false>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0,
                                      pe(EH_PART_TASKS_BIG))));

@@ -187,6 +187,7 @@ struct Schedule {
   // offset of each one's in-pairs, the first row of each bin, the pairs and the tasks
   DevBuf hj, hjlo, hoff, bstart, hpairs, htasks;  // hjlo: each such B row's ids below the split
   uint64_t n_htasks = 0;
+  uint32_t bsh_longest = 0;         // the longest of those B rows (ids)
   DevBuf local_rows[2];             // rows of the two k_local4 classes (<= 128, <= 512 entries)
   uint64_t n_local[2] = {0, 0};
   uint64_t n_btasks = 0, n_in_pairs = 0;

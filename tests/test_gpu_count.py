@@ -204,6 +204,27 @@ def test_deep_cliques_reject_large_k(ctx, facebook):
         M.clique_count(17, "tuned")
 
 
+@pytest.mark.parametrize("split", [64, 1024])
+def test_b_stationary_long_b_rows(ctx, split):
+    """A dense ER graph (n 2,500, 500,000 edges): oriented row j holds about j / 6 ids, so the B
+    rows above the split hold up to ~400, and the k_bshl bound (B rows up to 256 ids in four
+    register rounds, up to 512 in eight) decides which rows go wholly B-stationary; every bound
+    gives the oracle's count."""
+    n = 2500
+    g = ctx.generate_er_exact(n, 500000, 7)
+    off, nbr = g.download()
+    exp = O.Matrix(n, off, nbr, "hybrid").triangles(8)[0]
+    ooff, _ = O.orient_lower(n, off, nbr)
+    card = np.diff(ooff.astype(np.int64))
+    assert card[split:].max() > 256 and card.max() <= 512
+    for pol in ("hybrid", "uint"):
+        M = g.build_matrix("lower", pol)
+        for bsh in (0, 128, 256, 300, 512, -1):
+            for small in (-1, 32):
+                got = M.triangle_count("tuned", bstat=split, bsh=bsh, small_degree=small)[0]
+                assert got == exp, (pol, bsh, small)
+
+
 def test_complete_graph_full_remainders(ctx):
     """K_600: every B row j of the oriented matrix has the in-pairs (i, j), i > j, each with a
     prefix of j ids, so B rows 497-503 give 64 pairs of 63 remainder vectors per pass (the most
@@ -218,6 +239,11 @@ def test_complete_graph_full_remainders(ctx):
         M = ctx.matrix_from_csr(n, off, nbr, "lower", pol)
         for bstat in (0, -1, 256):
             assert M.triangle_count("tuned", bstat=bstat)[0] == comb(n, 3), (pol, bstat)
+        # split 256: the rows up to 512 hand their pairs above the split to k_bshl, whose B rows
+        # then hold 256-511 ids (row(j) in eight register rounds); a bound of 256 or 300 leaves
+        # more of them A-side
+        for bsh in (256, 300, 511, 512, -1):
+            assert M.triangle_count("tuned", bstat=256, bsh=bsh)[0] == comb(n, 3), (pol, bsh)
         assert M.triangle_count("plain")[0] == comb(n, 3), pol
         for local in (0, -1):
             assert M.clique_count(4, "tuned", local_max=local)[0] == comb(n, 4), (pol, local)
