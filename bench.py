@@ -308,19 +308,23 @@ def config_c5(ctx, with_cpu):
     """Learned selector (trained on engine-timed pass costs of RMAT 18-21, seeds 10-19) evaluated
     on RMAT-24, edge factor 16, seed 24: the tuned triangle pass under the learned layouts and
     under every rule the network is measured against (the reference's density rule, bitsets at
-    density 1/512 and 1/4096, all uint), each with its build time; plus the plain schedule."""
+    density 1/512 and 1/4096, all uint), each with its build time, in the default pass ("tuned":
+    every pair B-stationary, no layout read) and with the B-stationary kernels off ("aside":
+    every pair A-side, where the layouts decide); plus the plain schedule."""
     from simdgraphprocessing_amd import selector
     g = ctx.generate_rmat(*C5)
     out = selector.evaluate_graph(ctx, g, steps=5, log=log)
     out["workload"] = f"triangles, RMAT scale {C5[0]} ef {C5[1]} seed {C5[2]}"
-    # counter records exist for the profiled layout policies (density rule = hybrid, learned)
-    for name, r in out["policies"].items():
-        pol = {"density_1_256": "hybrid", "learned": "learned"}.get(name)
-        r.update(byte_figures(out["alg_bytes"], r["kernel_ms"], traffic_record("c5", pol, {}) if pol else None))
+    # counter records exist for the profiled layout policies of the default pass
+    for regime in ("tuned", "aside"):
+        for name, r in out[regime]["policies"].items():
+            pol = {"density_1_256": "hybrid", "learned": "learned"}.get(name) if regime == "tuned" else None
+            r.update(byte_figures(out["alg_bytes"], r["kernel_ms"], traffic_record("c5", pol, {}) if pol else None))
     if with_cpu:
         off, nbr = g.download()
         del g
-        out["cpu"] = cpu_triangles(1 << C5[0], off, nbr, out["policies"]["learned"]["count"], runs=3, budget_s=40.0)
+        out["cpu"] = cpu_triangles(1 << C5[0], off, nbr, out["tuned"]["policies"]["learned"]["count"], runs=3,
+                                   budget_s=40.0)
     return out
 
 
@@ -538,7 +542,7 @@ def main():
     sel_c2 = None
     if has_sel:
         from simdgraphprocessing_amd import selector
-        sel_c2 = selector.compare(ctx, g, steps=5, log=log)
+        sel_c2 = {r: selector.compare(ctx, g, steps=5, log=log, regime=r) for r in ("tuned", "aside")}
     del g
 
     extra = {}

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define EH_ABI_VERSION 2
+#define EH_ABI_VERSION 3
 
 typedef enum {
   EH_OK = 0,
@@ -401,6 +401,17 @@ eh_status eh_selector_get_params(eh_ctx *ctx, float *params);
  * Rows of fewer than two ids, or held by no row, are uint under every rule.  No counterpart
  * in the reference. */
 eh_status eh_selector_set_rule(eh_ctx *ctx, uint32_t guard_words, int32_t fixed_arm);
+/* The tuned triangle schedule EH_LAYOUT_LEARNED selects for (eh_run_opts.bstat_split and
+ * bsh_max of the pass that will read the matrix; 0 = their defaults, 65536 and 512).  A row's
+ * arm only changes the pairs and short-row queries whose B row it is and that the pass runs
+ * A-side or edge-centric: with the split on, the pairs at or above the split of the rows that
+ * hold a B row there of more than bsh_max ids (the others are B-stationary and read canonical
+ * lists).  Only rows some such pair reads are candidates, and a row's "uses" feature counts
+ * those rows; the others are uint.  bstat_split < 0: every pair is A-side (every row held by
+ * some row is a candidate), as with bstat_split < 0 in the pass; bsh_max < 0: no row's pairs
+ * above the split are B-stationary.  Unoriented matrices and batch arenas always count every
+ * use.  No counterpart in the reference. */
+eh_status eh_selector_set_schedule(eh_ctx *ctx, int32_t bstat_split, int32_t bsh_max);
 /* per-row features (n x 16 floats) of an oriented matrix's rows */
 eh_status eh_matrix_features(eh_matrix *m, float *features);
 /* A matrix whose rows take the caller's arms (row_types: n values 0..3; a bitset only within

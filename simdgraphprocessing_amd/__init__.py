@@ -95,7 +95,8 @@ EXPORTS = [
     "eh_matrix_download", "eh_matrix_row_types", "eh_matrix_free", "eh_intersect", "eh_intersect_rows",
     "eh_intersect_rows_static", "eh_intersect_row_sets", "eh_union", "eh_difference", "eh_triangle_count", "eh_clique_count", "eh_bfs_path", "eh_count_profile", "eh_batch_generate", "eh_batch_info",
     "eh_batch_graph_params", "eh_batch_graph_download", "eh_batch_count", "eh_batch_alg_bytes", "eh_batch_free",
-    "eh_selector_set_params", "eh_selector_get_params", "eh_selector_set_rule", "eh_matrix_features",
+    "eh_selector_set_params", "eh_selector_get_params", "eh_selector_set_rule", "eh_selector_set_schedule",
+    "eh_matrix_features",
     "eh_matrix_build_typed", "eh_mlp_forward", "eh_mlp_loss_grad", "eh_selector_train",
 ]
 
@@ -179,6 +180,7 @@ def lib():
     L.eh_selector_set_params.argtypes = [_vp, _vp]
     L.eh_selector_get_params.argtypes = [_vp, _vp]
     L.eh_selector_set_rule.argtypes = [_vp, C.c_uint32, C.c_int32]
+    L.eh_selector_set_schedule.argtypes = [_vp, C.c_int32, C.c_int32]
     L.eh_matrix_features.argtypes = [_vp, _vp]
     L.eh_matrix_build_typed.argtypes = [_vp, C.c_int, _vp, _pp]
     L.eh_mlp_forward.argtypes = [_vp, _vp, _vp, C.c_int, _vp]
@@ -348,6 +350,12 @@ class Context:
         """The LEARNED policy's bitset guard (words per id) and, for the ablations, a fixed arm
         (0 bitset, 1 pshort, 2 uint, 3 uint searched) in place of the network's (-1)."""
         _check(lib().eh_selector_set_rule(self.h, int(guard_words), int(fixed_arm)))
+
+    def set_selector_schedule(self, bstat=0, bsh=0):
+        """The tuned triangle schedule the LEARNED policy selects for (the pass's bstat_split and
+        bsh_max; 0: their defaults, bstat < 0: every pair A-side): only rows that some A-side or
+        edge-centric pair of that schedule reads are candidates."""
+        _check(lib().eh_selector_set_schedule(self.h, int(bstat), int(bsh)))
 
     def get_selector(self):
         p = np.zeros(MLP_PARAMS, np.float32)

@@ -148,6 +148,10 @@ struct eh_ctx {
   DevBuf mlp_dev;          // their device copy for the selection kernel
   uint32_t sel_guard = 8;  // LEARNED: a bitset only where its words are at most this many per id
   int sel_fixed = -1;      // LEARNED: -1 the network's arm, 0..3 that arm for every row (ablations)
+  // LEARNED: the tuned triangle schedule the selection serves (the B-stationary split and the
+  // k_bshl row bound; split 0: every pair A-side): only rows some A-side or edge-centric pair of
+  // that schedule reads are candidates
+  uint32_t sel_split = 65536, sel_bsh = 512;
   DevBuf scratch;          // reusable temp storage for primitives
   DevBuf counter;          // small device counters
 };

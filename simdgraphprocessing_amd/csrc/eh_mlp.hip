@@ -11,12 +11,17 @@
 //         within B's first and last id in B's list instead of streaming B (the per-pair
 //         algorithm for long rows few of A's ids fall in; set_intersect_galloping's role,
 //         :393-584)
-// An FP32 MLP 16 -> 64 -> 64 -> 4 (ReLU) maps 16 row features to the predicted log2 cost of
-// each arm in the tuned pass: shader cycles of the row's own wedge queries and pairs, timed
-// with the pass's own code (k_selector_labels in eh_count.hip), plus the cycles of building
-// the arm's payload.  The LEARNED layout policy gives each row the arm with the smallest
-// prediction.  Arms change only how a count is computed, never which elements are common, so
-// counts are identical under every selection.
+// An FP32 MLP 16 -> 64 -> 64 -> 4 (ReLU) maps 16 row features to the predicted cost of each
+// arm against uint (signed log2 of ns): the row's share of the change in the tuned pass's time
+// (every pair A-side) and in the build time when the row's group of similar rows takes the arm,
+// both measured on the engine (simdgraphprocessing_amd/selector.py, label_graph).  The LEARNED
+// layout policy gives each candidate row (eh_selector_set_schedule) the arm with the smallest
+// prediction among bitset, uint and uint searched: in the tuned pass a pshort B row's pairs stream
+// its canonical list, as a uint row's do, and only its short-row queries differ -- too little to
+// learn from timings (its labels are within the noise of uint's), and a network output that left
+// its training range on graphs larger than the training graphs (RMAT-22 and up), so the pshort arm
+// stays for the fixed-arm ablations and the typed builds only.  Arms change only how a count is
+// computed, never which elements are common, so counts are identical under every selection.
 //
 // Kernels here: uses and features, the selection (MLP forward with the parameters staged in
 // LDS, over the rows whose arm can matter), MLP loss + per-sample deltas, deterministic
@@ -61,35 +66,66 @@ __global__ void k_uses_sym(const uint64_t *goff, const uint32_t *card, uint32_t 
     uses[i] = (uint32_t)(goff[i + 1] - goff[i]) - card[i];
 }
 
+// uses[j] under a tuned triangle schedule (split, bsh): the rows holding j whose pair (i, j) the
+// pass runs A-side or edge-centric -- j at or above the split in a row that holds some B row
+// there of more than bsh ids (k_row_work's test; the other rows' pairs are B-stationary and
+// read canonical lists).  One wave per row; uses must start at zero.
+__global__ void __launch_bounds__(256) k_uses_tuned(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr,
+                                                    uint32_t n, uint32_t split, uint32_t bsh, uint32_t *uses) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t i = wid; i < n; i += nw) {
+    const uint32_t c = card[i];
+    if (c == 0) continue;
+    const uint32_t *r = unbr + (uint64_t)aoff[i] * 4;
+    if (r[c - 1] < split) continue;  // every pair below the split
+    uint32_t bad = 0;
+    for (uint32_t k = lane; k < c; k += 64) {
+      const uint32_t x = r[k];
+      if (x >= split && card[x] > bsh) bad = 1u;
+    }
+    if (__ballot(bad != 0u) == 0ull) continue;
+    for (uint32_t k = lane; k < c; k += 64) {
+      const uint32_t x = r[k];
+      if (x >= split && card[x]) atomicAdd(&uses[x], 1u);
+    }
+  }
+}
+
 __device__ __forceinline__ float lg1(float x) { return log2f(1.0f + x); }
 
 // the statistics behind a row's features
 struct RowStats {
-  uint32_t c, lo, hi, mid, parts, low64k, quarter, maxgap;
+  uint32_t c, lo, hi, mid, parts, low, quarter, maxgap;  // low: ids below n / 64
 };
 
 // 16 features of row i, all in [0, 1] and free of the graph's size where they can be (ids and
 // spans relative to n, gaps relative to the span, counts relative to the row), so a network
-// trained on small graphs reads a large graph's rows on the same scale:
+// trained on small graphs reads a large graph's rows on the same scale.  The absolute sizes
+// saturate where the training graphs (RMAT 18-21) end -- a length of FEAT_CARD_CAP, FEAT_USES_CAP
+// uses -- so a larger graph's heaviest rows read as the heaviest rows the network has seen
+// instead of outside its range:
 //   0 log length            1 log span / n         2 density (ids per id of span)
 //   3 16-bit partitions per id   4 log uses        5 log (uses x length)
-//   6 log bitset words per id    7 share of ids below 65536 (the B-stationary split)
+//   6 log bitset words per id    7 share of ids below n / 64
 //   8 log span per id       9 largest gap / span   10 row id / n
 //   11 first id / n         12 last id / n         13 middle id / n
 //   14 share of ids in the first quarter of the span   15 constant 1
+constexpr float FEAT_CARD_CAP = 1023.0f, FEAT_USES_CAP = 16383.0f;
 __device__ __forceinline__ void write_features(const RowStats &r, uint32_t u_raw, uint32_t i, uint32_t n, float *out) {
   float f[NI];
   for (int q = 0; q < NI; q++) f[q] = 0.0f;
-  const float u = (float)u_raw, fn = (float)n;
+  const float u = fminf((float)u_raw, FEAT_USES_CAP), fn = (float)n;
   if (r.c) {
     const float span = (float)(r.hi - r.lo), c = (float)r.c;
     const float words = (float)((r.hi >> 6) - (r.lo >> 6) + 1u);
-    f[0] = lg1(c) / 16.0f;
+    f[0] = lg1(fminf(c, FEAT_CARD_CAP)) / 16.0f;
     f[1] = lg1(1048576.0f * span / fn) / 20.0f;
     f[2] = c / (span + 1.0f);
     f[3] = (float)r.parts / c;
     f[6] = lg1(words / c) / 16.0f;
-    f[7] = (float)r.low64k / c;
+    f[7] = (float)r.low / c;
     f[8] = lg1(span / c) / 32.0f;
     f[9] = span > 0.0f ? (float)r.maxgap / span : 0.0f;
     f[11] = (float)r.lo / fn;
@@ -98,7 +134,7 @@ __device__ __forceinline__ void write_features(const RowStats &r, uint32_t u_raw
     f[14] = (float)r.quarter / c;
   }
   f[4] = lg1(u) / 24.0f;
-  f[5] = lg1(u * (float)r.c) / 40.0f;
+  f[5] = lg1(u * fminf((float)r.c, FEAT_CARD_CAP)) / 40.0f;
   f[10] = (float)i / fn;
   f[15] = 1.0f;
   float4 *o = reinterpret_cast<float4 *>(out);
@@ -127,7 +163,7 @@ __device__ __forceinline__ void wave_sync() {
 constexpr uint32_t FEAT_LANE_MAX = 32;  // rows up to this long are read by one lane
 
 // the statistics of a row of at most FEAT_LANE_MAX ids, read by one lane
-__device__ __forceinline__ RowStats lane_stats(const uint32_t *r, uint32_t c) {
+__device__ __forceinline__ RowStats lane_stats(const uint32_t *r, uint32_t c, uint32_t lowcut) {
   RowStats st = {c, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
   if (!c) return st;
   st.lo = r[0];
@@ -142,7 +178,7 @@ __device__ __forceinline__ RowStats lane_stats(const uint32_t *r, uint32_t c) {
       st.parts += (x >> 16) != (prev >> 16);
       st.maxgap = max(st.maxgap, x - prev);
     }
-    st.low64k += x < 65536u;
+    st.low += x < lowcut;
     st.quarter += x <= qcut;
     prev = x;
   }
@@ -150,7 +186,7 @@ __device__ __forceinline__ RowStats lane_stats(const uint32_t *r, uint32_t c) {
 }
 
 // the statistics of a longer row, read by the whole wave (every lane returns them)
-__device__ __forceinline__ RowStats wave_stats(const uint32_t *r, uint32_t c, uint32_t lane) {
+__device__ __forceinline__ RowStats wave_stats(const uint32_t *r, uint32_t c, uint32_t lane, uint32_t lowcut) {
   RowStats st = {c, r[0], r[c - 1], r[c / 2], 0u, 0u, 0u, 0u};
   const uint32_t qcut = st.lo + (st.hi - st.lo) / 4u;
   uint32_t parts = 0, low = 0, quart = 0, gap = 0;
@@ -161,7 +197,7 @@ __device__ __forceinline__ RowStats wave_stats(const uint32_t *r, uint32_t c, ui
       parts += (x >> 16) != (pv >> 16);
       gap = max(gap, x - pv);
     }
-    low += x < 65536u;
+    low += x < lowcut;
     quart += x <= qcut;
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -171,7 +207,7 @@ __device__ __forceinline__ RowStats wave_stats(const uint32_t *r, uint32_t c, ui
     gap = max(gap, (uint32_t)__shfl_xor(gap, o, 64));
   }
   st.parts = 1u + parts;
-  st.low64k = low;
+  st.low = low;
   st.quarter = quart;
   st.maxgap = gap;
   return st;
@@ -190,14 +226,14 @@ __global__ void __launch_bounds__(256) k_features(const uint32_t *card, const ui
     const uint32_t c = valid ? card[i] : 0u;
     const uint32_t u = valid ? uses[i] : 0u;
     if (valid && c <= FEAT_LANE_MAX)
-      write_features(lane_stats(unbr + (uint64_t)aoff[i] * 4, c), u, (uint32_t)i, n, F + i * NI);
+      write_features(lane_stats(unbr + (uint64_t)aoff[i] * 4, c, n >> 6), u, (uint32_t)i, n, F + i * NI);
     uint64_t lm = __ballot(valid && c > FEAT_LANE_MAX);
     while (lm) {
       const uint32_t q = (uint32_t)__builtin_ctzll(lm);
       lm &= lm - 1;
       const uint32_t ii = (uint32_t)base + q;
       const uint32_t cc = __builtin_amdgcn_readlane(c, q), uu = __builtin_amdgcn_readlane(u, q);
-      const RowStats st = wave_stats(unbr + (uint64_t)aoff[ii] * 4, cc, lane);
+      const RowStats st = wave_stats(unbr + (uint64_t)aoff[ii] * 4, cc, lane, n >> 6);
       if (lane == 0) write_features(st, uu, ii, n, F + (uint64_t)ii * NI);
     }
   }
@@ -288,7 +324,7 @@ __device__ __forceinline__ uint8_t pick_arm(const float *W, const float *x, uint
   uint8_t pick = EH_U;
   float best = 3.0e38f;
   for (uint32_t k = 0; k < NO; k++) {
-    if (k == EH_BS && !bs_ok) continue;
+    if (k == EH_PS || (k == EH_BS && !bs_ok)) continue;
     if (y[k] < best) { best = y[k]; pick = (uint8_t)k; }
   }
   return pick;
@@ -346,7 +382,7 @@ __global__ void __launch_bounds__(256) k_select_learned(const uint32_t *card, co
       QR[slot] = (uint32_t)i;
       QC[slot] = c;
       QW[slot] = (r[c - 1] >> 6) - (r[0] >> 6) + 1u;
-      if (c <= FEAT_LANE_MAX) write_features_lds(lane_stats(r, c), u, (uint32_t)i, n, QF + slot * SEL_STRIDE);
+      if (c <= FEAT_LANE_MAX) write_features_lds(lane_stats(r, c, n >> 6), u, (uint32_t)i, n, QF + slot * SEL_STRIDE);
     }
     uint64_t lm = __ballot(want && c > FEAT_LANE_MAX);
     while (lm) {
@@ -355,7 +391,7 @@ __global__ void __launch_bounds__(256) k_select_learned(const uint32_t *card, co
       const uint32_t cc = __builtin_amdgcn_readlane(c, q), uu = __builtin_amdgcn_readlane(u, q);
       const uint32_t sq = __builtin_amdgcn_readlane(slot, q);
       const uint32_t ii = (uint32_t)base + q;
-      const RowStats st = wave_stats(unbr + (uint64_t)aoff[ii] * 4, cc, lane);
+      const RowStats st = wave_stats(unbr + (uint64_t)aoff[ii] * 4, cc, lane, n >> 6);
       if (lane == 0) write_features_lds(st, uu, ii, n, QF + sq * SEL_STRIDE);
     }
     qn += (uint32_t)__popcll(m);
@@ -467,11 +503,29 @@ __global__ void k_adam(float *P, const float *G, float *M1, float *M2, float lr,
 
 // uses of every row: from the source graph's degrees when the matrix holds the lower
 // triangle of a symmetric graph (sym_off: its row offsets), else one atomic per entry
-static eh_status uses_dev(eh_matrix *m, const uint64_t *sym_off, DevBuf &uses) {
+// uses of every row: under the context's selector schedule for an oriented single-graph
+// matrix with the split on (k_uses_tuned), else every row that holds it.  *none: set when no
+// row has a use (no row at or above the split is longer than the k_bshl bound, so every pair
+// is B-stationary: then the rows are not scanned)
+static eh_status uses_dev(eh_matrix *m, const uint64_t *sym_off, DevBuf &uses, bool *none = nullptr) {
   hipStream_t st = m->ctx->stream;
   const uint32_t n = m->n;
   EH_CHECK(uses.alloc(4ull * n + 4));
-  if (sym_off && m->filter == EH_FILTER_LOWER) {
+  const eh_ctx *ctx = m->ctx;
+  if (none) *none = false;
+  if (ctx->sel_split && m->filter == EH_FILTER_LOWER && !m->dgraph_of.p) {
+    EH_HIP(hipMemsetAsync(uses.p, 0, 4ull * n + 4, st));
+    uint32_t longest = 0;
+    if (ctx->sel_split < n)
+      EH_CHECK(prim_reduce_max_u32(m->ctx, m->dcard.as<uint32_t>() + ctx->sel_split, n - ctx->sel_split, &longest));
+    if (longest <= ctx->sel_bsh) {
+      if (none) *none = true;
+      return EH_OK;
+    }
+    k_uses_tuned<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(),
+                                                             m->dunbr.as<uint32_t>(), n, ctx->sel_split,
+                                                             ctx->sel_bsh, uses.as<uint32_t>());
+  } else if (sym_off && m->filter == EH_FILTER_LOWER) {
     k_uses_sym<<<grid_for(n), 256, 0, st>>>(sym_off, m->dcard.as<uint32_t>(), n, uses.as<uint32_t>());
   } else {
     EH_HIP(hipMemsetAsync(uses.p, 0, 4ull * n + 4, st));
@@ -511,7 +565,13 @@ eh_status matrix_select_learned(eh_matrix *m, uint8_t *d_types, const uint64_t *
   hipStream_t st = ctx->stream;
   const uint32_t n = m->n;
   DevBuf uses;
-  EH_CHECK(uses_dev(m, sym_off, uses));
+  bool none = false;
+  EH_CHECK(uses_dev(m, sym_off, uses, &none));
+  if (none) {  // no candidate: every row uint
+    EH_HIP(hipMemsetAsync(d_types, EH_U, n, st));
+    EH_HIP(hipStreamSynchronize(st));
+    return EH_OK;
+  }
   if (!fixed) {
     if (ctx->mlp_dev.bytes < 4ull * NPARAM) EH_CHECK(ctx->mlp_dev.alloc(4ull * NPARAM));
     EH_HIP(hipMemcpyAsync(ctx->mlp_dev.p, ctx->mlp.data(), 4ull * NPARAM, hipMemcpyHostToDevice, st));
@@ -547,6 +607,18 @@ extern "C" eh_status eh_selector_set_rule(eh_ctx *ctx, uint32_t guard_words, int
   if (fixed_arm < -1 || fixed_arm > 3) { eh_set_error("fixed arm must be -1 (learned) or 0 .. 3"); return EH_ERR_INVALID; }
   ctx->sel_guard = guard_words;
   ctx->sel_fixed = fixed_arm;
+  return EH_OK;
+}
+
+extern "C" eh_status eh_selector_set_schedule(eh_ctx *ctx, int32_t bstat_split, int32_t bsh_max) {
+  if (!ctx) { eh_set_error("null argument"); return EH_ERR_INVALID; }
+  if (bstat_split > 65536 || (bstat_split > 0 && bstat_split < 64)) {
+    eh_set_error("bstat_split must be 0 (default), < 0 (off) or 64 .. 65536");
+    return EH_ERR_INVALID;
+  }
+  if (bsh_max > 512) { eh_set_error("bsh_max must be at most 512"); return EH_ERR_INVALID; }
+  ctx->sel_split = bstat_split < 0 ? 0u : bstat_split == 0 ? 65536u : (uint32_t)bstat_split;
+  ctx->sel_bsh = bsh_max < 0 ? 0u : bsh_max == 0 ? 512u : (uint32_t)bsh_max;
   return EH_OK;
 }
 

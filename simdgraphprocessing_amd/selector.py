@@ -2,21 +2,25 @@
 
 Each row that is the B side of a pair has four arms (bitset, pshort, uint, uint searched:
 ``ARMS``); the tuned triangle pass runs a different algorithm for each (probe/AND, stream,
-stream, per-pair search).  The labels are what each arm costs the tuned pass itself: rows are
-grouped by their statistics (length, bitset words per id, how many rows hold them), and for
-every group and arm the engine builds a matrix where only that group takes the arm (the rest
-uint) and times its build and its tuned pass against the all-uint matrix (``label_graph``).
-The difference, per row of the group, is the label of each row in it.  The FP32 MLP
-16 -> 64 -> 64 -> 4 learns those costs from the rows' own features with the engine's own HIP
-forward / backward / Adam kernels (``Context.train_selector``), and
-``build_matrix(policy="learned")`` gives each row its cheapest predicted arm.  The reference's
-density rule (``policy="hybrid"``) stays the fallback and the yardstick, with more rules as
-ablations (``RULES``).
+stream, per-pair search) wherever it reads the row's layout: in the A-side tasks and the
+edge-centric short rows.  The B-stationary kernels read canonical lists whatever the layout, and
+with the default split and row bound (65,536 and 512) every pair of RMAT-22 and RMAT-24 is
+B-stationary, so there no layout changes the pass: the LEARNED policy is schedule-aware
+(``Context.set_selector_schedule``), takes as candidates only the rows that some A-side or
+edge-centric pair of the schedule reads, and leaves the rest uint (the cheapest build).
 
-Layouts act only on A-side work (the B-stationary kernels read canonical lists whatever the
-layout), and on the small training graphs nearly every pair runs B-stationary, so the labels
-time the tuned pass with the B-stationary kernels off: every pair is A-side work, short source
-rows through k_small_rows and the others through the task kernel.
+The network is trained where the arms matter: the tuned pass with the B-stationary kernels off
+(``REGIMES["aside"]``), every pair A-side.  Labels are what each arm costs that pass itself: rows
+are grouped by their statistics (length, bitset words per id, uses), and for every group and
+arm the engine builds a matrix where only that group takes the arm (the rest uint) and times
+its build and its pass against the all-uint matrix, the two passes alternated
+(``label_graph``).  The group's pass difference is shared among its rows in proportion to
+their stake in the pass (uses x length, the ids streamed when the row is read as uint), the
+build difference evenly.  The FP32 MLP 16 -> 64 -> 64 -> 4 learns those costs from the rows'
+own features with the engine's own HIP forward / backward / Adam kernels
+(``Context.train_selector``), and ``build_matrix(policy="learned")`` gives each candidate row
+its cheapest predicted arm.  The reference's density rule (``policy="hybrid"``) stays the
+fallback and the yardstick, with more rules as ablations (``RULES``).
 """
 import os
 import time
@@ -33,6 +37,22 @@ RULES = (("density_1_256", "hybrid", None, None),
          ("bitset_1_4096", "learned", 64, 0),
          ("uint", "uint", None, None))
 U = 2
+# the arms the network chooses among (the uint baseline aside): bitset and uint searched.  In the
+# tuned pass a pshort B row's pairs stream its canonical list as a uint row's do and only its
+# short-row queries differ; its labels sit within the timing noise of uint's, and the network's
+# pshort output left its training range on larger graphs, so pshort stays a fixed arm only
+LEARNED_ARMS = (0, 3)
+# where the arms are timed and compared: the selector schedule the LEARNED build serves
+# (Context.set_selector_schedule) and the options of the tuned pass that reads the matrix.
+# "tuned": the default pass (B-stationary split 65,536, k_bshl rows up to 512 ids);
+# "aside": the B-stationary kernels off, every pair A-side (where the network is trained)
+REGIMES = {"tuned": (0, {}), "aside": (-1, {"bstat": -1})}
+
+
+def set_regime(ctx, regime):
+    split, opts = REGIMES[regime]
+    ctx.set_selector_schedule(split)
+    return opts
 
 
 def slog(x):
@@ -47,7 +67,8 @@ def unslog(y):
 
 
 def row_stats(M):
-    """(card, uses, bitset words) of every row of an oriented matrix."""
+    """(card, uses, bitset words) of every row of an oriented matrix (uses: the rows that hold
+    it, every one of them A-side)."""
     off, nbr = M.download()
     off = off.astype(np.int64)
     card = np.diff(off)
@@ -60,14 +81,29 @@ def row_stats(M):
     return card, uses, words
 
 
+def tuned_uses(off, nbr, split=65536, bsh=512):
+    """The uses the schedule-aware selection counts (k_uses_tuned): per row j, the rows holding
+    j at or above the split that also hold some B row there of more than bsh ids."""
+    off = np.asarray(off, np.int64)
+    card = np.diff(off)
+    n = len(card)
+    row = np.repeat(np.arange(n, dtype=np.int64), card)
+    nbr = np.asarray(nbr, np.int64)
+    hi = nbr >= split
+    bad = np.zeros(n, bool)
+    bad[row[hi & (card[nbr] > bsh)]] = True
+    sel = hi & bad[row] & (card[nbr] > 0)
+    return np.bincount(nbr[sel], minlength=n)
+
+
 def groups_of(card, uses, words, min_rows):
     """Row groups (lists of row ids) by length, bitset words per id and uses; only rows of at
     least two ids that some row holds, and groups of at least min_rows."""
     cand = (card >= 2) & (uses >= 1)
     cb = np.digitize(card, [4, 8, 16, 32, 64, 256])
     db = np.digitize(words / np.maximum(card, 1), [1.5, 8.5, 64.5])
-    ub = np.digitize(uses, [2, 8])
-    key = (cb * 4 + db) * 3 + ub
+    ub = np.digitize(uses, [2, 8, 64, 512])
+    key = (cb * 4 + db) * 5 + ub
     out = []
     for k in np.unique(key[cand]):
         rows = np.nonzero(cand & (key == k))[0]
@@ -76,66 +112,115 @@ def groups_of(card, uses, words, min_rows):
     return out
 
 
-def _pass(g, types, split, bsh, reps):
-    M = g.build_matrix_typed(types)
-    b = M.info()["build_ms"]
-    c, _ = M.triangle_count("tuned", bstat=split, bsh=bsh)
-    ms = []
-    for _ in range(reps):
-        c2, st = M.triangle_count("tuned", bstat=split, bsh=bsh)
-        assert c2 == c
-        ms.append(st["kernel_ms"])
-    del M
-    return c, b, float(np.median(ms))
+def _pass_ms(M, opts):
+    c, st = M.triangle_count("tuned", **opts)
+    return c, st["kernel_ms"]
 
 
-def label_graph(ctx, g, reps=3, min_frac=0.001, max_rows=40000, seed=0, log=None):
-    """Labels of one graph: (features, targets n x 4, mask n x 4) of its labelled rows (a seeded
-    sample of at most max_rows of them: a group's rows share their labels)."""
+def row_parts(M):
+    """16-bit partitions (distinct id >> 16) of every row of a matrix: a pshort row's headers."""
+    off, nbr = M.download()
+    card = np.diff(off.astype(np.int64))
+    row = np.repeat(np.arange(len(card), dtype=np.int64), card)
+    hi = np.asarray(nbr, np.int64) >> 16
+    new = np.ones(len(hi), np.float64)
+    if len(hi) > 1:
+        new[1:] = (hi[1:] != hi[:-1]) | (row[1:] != row[:-1])
+    return np.bincount(row, weights=new, minlength=len(card)).astype(np.int64)
+
+
+def significant(dk, base_ms):
+    """The median of alternated pass-time differences (ms), or 0 where it is within the timing
+    noise: 3 standard errors (from the median absolute deviation) plus 0.1% of the pass."""
+    dk = np.asarray(dk, np.float64)
+    med = float(np.median(dk))
+    sd = 1.4826 * float(np.median(np.abs(dk - med)))
+    return med if abs(med) > 3.0 * sd / np.sqrt(len(dk)) + 1e-3 * base_ms else 0.0
+
+
+def label_graph(ctx, g, reps=4, min_rows=64, per_group=500, seed=0, log=None):
+    """Labels of one graph in the A-side regime: (features, targets n x 4, weights n x 4) of its
+    labelled rows (a seeded sample of at most per_group rows of every group, so the few heavy
+    rows -- the hubs every pass is bound by -- are not swamped by the many light ones).
+
+    Pass: per group and arm, the pass of the matrix where the group takes the arm and the
+    all-uint pass are timed alternately, reps times; the median difference, where it stands out
+    of the timing noise (``significant``; else 0), is shared among the group's rows by their
+    stake, uses x length (the ids streamed when the row is read as uint).
+    Build: per arm, the build with every candidate in the arm and the all-uint build are timed
+    alternately four times; the median difference is shared among the rows by the payload the
+    arm writes for them (bitset: 8 bytes a word; uint searched writes none).  The arms labelled
+    are LEARNED_ARMS, with uint the baseline.  A row's label is the signed log of the sum (ns), its weight in the training
+    loss 1 + log2(1 + stake) / 4."""
     ctx.set_selector_rule(GUARD, -1)
+    opts = set_regime(ctx, "aside")
     n = g.info()["n"]
-    split, bsh = -1, 0  # the B-stationary kernels off: every pair is A-side work
-    M0 = g.build_matrix("lower", "uint")
-    F = M0.features()
-    card, uses, words = row_stats(M0)
-    del M0
-    cand = (card >= 2) & (uses >= 1)
-    groups = groups_of(card, uses, words, max(64, int(min_frac * cand.sum())))
     base = np.full(n, U, np.uint8)
-    c0, b0s, k0s = [], [], []
-    for _ in range(3):
-        c, b, k = _pass(g, base, split, bsh, reps)
-        c0.append(c)
-        b0s.append(b)
-        k0s.append(k)
-    b0, k0 = float(np.median(b0s)), float(np.median(k0s))
+    Mb = g.build_matrix_typed(base)
+    F = Mb.features()
+    card, uses, words = row_stats(Mb)
+    parts = row_parts(Mb)
+    cand = (card >= 2) & (uses >= 1)
+    stake = (uses * card).astype(np.float64)
+    groups = groups_of(card, uses, words, min_rows)
+    c0, k0 = _pass_ms(Mb, opts)  # the schedule, then a warm pass
+    bs_ok = words <= GUARD * card
+    # build cost of each arm, shared by payload
+    bcost = {3: np.zeros(n)}
+    payload = {0: 8.0 * words + 8.0, 1: 2.0 * card + 4.0 * parts}
+    for arm in (a for a in LEARNED_ARMS if a in payload):
+        sel = cand & bs_ok if arm == 0 else cand
+        t = base.copy()
+        t[sel] = arm
+        d = []
+        for _ in range(4):
+            Mv = g.build_matrix_typed(t)
+            bv = Mv.info()["build_ms"]
+            del Mv
+            M2 = g.build_matrix_typed(base)
+            d.append(bv - M2.info()["build_ms"])
+            del M2
+        tot = max(float(np.median(d)), 0.0)
+        bc = np.zeros(n)
+        if sel.any():
+            bc[sel] = tot * payload[arm][sel] / payload[arm][sel].sum()
+        bcost[arm] = bc
     T = np.zeros((n, 4), np.float32)
     Mk = np.zeros((n, 4), np.float32)
-    bs_ok = words <= GUARD * card
+    wt = (1.0 + np.log2(1.0 + stake) / 4.0).astype(np.float32)
     for rows in groups:
-        Mk[rows, U] = 1.0
-        for arm in (0, 1, 3):
+        Mk[rows, U] = wt[rows]
+        for arm in LEARNED_ARMS:
             sel = rows[bs_ok[rows]] if arm == 0 else rows
             if len(sel) == 0:
                 continue
             t = base.copy()
             t[sel] = arm
-            c, b, k = _pass(g, t, split, bsh, reps)
-            assert c == c0[0]
-            T[sel, arm] = slog(1e6 * ((k - k0) + (b - b0)) / len(sel))
-            Mk[sel, arm] = 1.0
-    keep = Mk.max(axis=1) > 0
-    idx = np.nonzero(keep)[0]
-    if max_rows and len(idx) > max_rows:
-        idx = np.sort(np.random.default_rng(seed).choice(idx, max_rows, replace=False))
+            Mv = g.build_matrix_typed(t)
+            c, _ = _pass_ms(Mv, opts)
+            assert c == c0
+            dk = []
+            for _ in range(reps):
+                kb = _pass_ms(Mb, opts)[1]
+                kv = _pass_ms(Mv, opts)[1]
+                dk.append(kv - kb)
+            del Mv
+            w = stake[sel] / max(stake[sel].sum(), 1.0)
+            T[sel, arm] = slog(1e6 * (significant(dk, k0) * w + bcost[arm][sel]))
+            Mk[sel, arm] = wt[sel]
+    rng = np.random.default_rng(seed)
     keep = np.zeros(n, bool)
-    keep[idx] = True
+    for rows in groups:
+        keep[rows if len(rows) <= per_group else rng.choice(rows, per_group, replace=False)] = True
     if log:
-        log(f"    {len(groups)} groups, {int(keep.sum())} rows labelled; base build {b0:.2f} ms, pass {k0:.2f} ms")
+        log(f"    {len(groups)} groups, {int(keep.sum())} rows labelled; all-uint pass {k0:.2f} ms; "
+            f"build +{bcost[0].sum():.2f} ms all bitset")
+    del Mb
+    set_regime(ctx, "tuned")
     return F[keep], T[keep], Mk[keep]
 
 
-def collect(ctx, scales=(18, 19, 20, 21), seeds=range(10, 20), edge_factor=16, reps=3, log=None):
+def collect(ctx, scales=(18, 19, 20, 21), seeds=range(10, 20), edge_factor=16, reps=4, log=None):
     """Engine-timed training rows from RMAT graphs: (features, targets, mask, seed of each row)."""
     xs, ts, ms, gs = [], [], [], []
     for scale in scales:
@@ -197,23 +282,24 @@ def bitset_choice(F, guard):
     return np.where((card > 1.5) & (words <= guard * card + 0.5), 0, 2).astype(np.int64)
 
 
-def _timed(M, steps):
-    c, _ = M.triangle_count("tuned")
+def _timed(M, steps, opts):
+    c, _ = M.triangle_count("tuned", **opts)
     ms = []
     for _ in range(steps):
-        c2, st = M.triangle_count("tuned")
+        c2, st = M.triangle_count("tuned", **opts)
         assert c2 == c
         ms.append(st["kernel_ms"])
     return c, float(np.median(ms))
 
 
-def compare(ctx, g, steps=5, params=None, log=None):
-    """The tuned triangle pass of graph g under the learned layouts and under every rule of
-    RULES, each with its build time; counts must agree.  Returns per-policy results and the
-    learned policy's gain over the best rule, with and without the build."""
+def compare(ctx, g, steps=5, params=None, log=None, regime="tuned"):
+    """The tuned triangle pass of graph g in `regime` under the learned layouts and under every
+    rule of RULES, each with its build time; counts must agree.  Returns per-policy results and
+    the learned policy's gain over the best rule, with and without the build."""
     res = {}
     if params is not None:
         ctx.set_selector(params)
+    opts = set_regime(ctx, regime)
     for name, policy, guard, fixed in RULES + (("learned", "learned", GUARD, -1),):
         if policy == "learned":
             ctx.set_selector_rule(guard, fixed)
@@ -223,20 +309,21 @@ def compare(ctx, g, steps=5, params=None, log=None):
             builds.append(M.info()["build_ms"])
         info = M.info()
         info["build_ms"] = float(np.median(builds))
-        c, kms = _timed(M, steps)
+        c, kms = _timed(M, steps, opts)
         res[name] = {"count": c, "kernel_ms": kms, "build_ms": info["build_ms"],
                      "rows_by_type": info["rows_by_type"]}
         if log:
-            log(f"  {name}: {c} in {kms:.3f} ms, build {info['build_ms']:.2f} ms, {info['rows_by_type']}")
+            log(f"  {regime} {name}: {c} in {kms:.3f} ms, build {info['build_ms']:.2f} ms, {info['rows_by_type']}")
         del M
     ctx.set_selector_rule(GUARD, -1)
+    set_regime(ctx, "tuned")
     counts = {v["count"] for v in res.values()}
     assert len(counts) == 1, res
     rules = [r[0] for r in RULES]
     best_pass = min(rules, key=lambda k: res[k]["kernel_ms"])
     best_total = min(rules, key=lambda k: res[k]["kernel_ms"] + res[k]["build_ms"])
     L = res["learned"]
-    return {"policies": res,
+    return {"regime": regime, "pass_options": opts, "policies": res,
             "best_rule_pass": best_pass,
             "learned_vs_best_rule": res[best_pass]["kernel_ms"] / L["kernel_ms"],
             "best_rule_with_build": best_total,
@@ -245,22 +332,22 @@ def compare(ctx, g, steps=5, params=None, log=None):
 
 
 def evaluate_graph(ctx, g, steps=5, log=None):
-    """compare() on graph g, plus the plain schedule under the density rule and the pass's
-    algorithmic bytes; counts must agree."""
-    out = compare(ctx, g, steps=steps, log=log)
+    """compare() on graph g in both regimes, plus the plain schedule under the density rule and
+    the pass's algorithmic bytes; counts must agree."""
+    out = {r: compare(ctx, g, steps=steps, log=log, regime=r) for r in ("tuned", "aside")}
     M = g.build_matrix("lower", "hybrid")
     c, st = M.triangle_count("plain")
     ms = [M.triangle_count("plain")[1]["kernel_ms"] for _ in range(steps)]
-    assert c == out["policies"]["learned"]["count"]
+    assert c == out["tuned"]["policies"]["learned"]["count"]
     out["alg_bytes"] = M.triangle_count("tuned")[1]["alg_bytes"]
     out["plain_density_1_256_ms"] = float(np.median(ms))
-    out["learned_vs_plain"] = out["plain_density_1_256_ms"] / out["policies"]["learned"]["kernel_ms"]
+    out["learned_vs_plain"] = out["plain_density_1_256_ms"] / out["tuned"]["policies"]["learned"]["kernel_ms"]
     return out
 
 
 def evaluate(ctx, scale=24, seed=24, edge_factor=16, steps=5, log=None):
-    """Triangle count on RMAT-scale: the learned layouts against the rules, all on the tuned
-    schedule, plus the plain schedule under the density rule; counts must agree."""
+    """Triangle count on RMAT-scale: the learned layouts against the rules in both regimes, plus
+    the plain schedule under the density rule; counts must agree."""
     g = ctx.generate_rmat(scale, edge_factor, seed)
     out = {"workload": f"triangles, RMAT scale {scale} ef {edge_factor} seed {seed}"}
     out.update(evaluate_graph(ctx, g, steps=steps, log=log))

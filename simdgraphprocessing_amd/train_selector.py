@@ -36,13 +36,22 @@ def main():
     ap.add_argument("--eval-scale", type=int, default=24)
     ap.add_argument("--eval-seed", type=int, default=24)
     ap.add_argument("--epochs", type=int, default=80)
+    ap.add_argument("--check-scales", default="21,22", help="held-out RMAT scales timed in the A-side regime")
+    ap.add_argument("--save-data", default=None, help="write the collected labels (.npz) here")
+    ap.add_argument("--data", default=None, help="train on labels written by --save-data instead of collecting")
     a = ap.parse_args()
     s0, s1 = (int(x) for x in a.seeds.split("-"))
     scales = tuple(int(x) for x in a.scales.split(","))
     ctx = eh.Context(0)
     log(f"device {ctx.device_name}")
     t0 = time.time()
-    X, T, Mk, seed_of = selector.collect(ctx, scales=scales, seeds=range(s0, s1 + 1), log=log)
+    if a.data:
+        with np.load(a.data, allow_pickle=False) as z:
+            X, T, Mk, seed_of = z["X"], z["T"], z["Mk"], z["seed_of"]
+    else:
+        X, T, Mk, seed_of = selector.collect(ctx, scales=scales, seeds=range(s0, s1 + 1), log=log)
+        if a.save_data:
+            np.savez_compressed(a.save_data, X=X, T=T, Mk=Mk, seed_of=seed_of)
     t_collect = time.time() - t0
     # hold out the graphs of the last seed (their rows share no group with the training rows)
     test = seed_of == s1
@@ -71,6 +80,12 @@ def main():
         f" (learned saving vs best rule {best_rule}: {gain:.3f}x)")
     selector.save(params, a.out)
     ctx.set_selector(params)
+    # the pass each regime runs on a held-out graph at a training scale, then at the evaluation scale
+    for sc in a.check_scales.split(","):
+        if sc:
+            gc = ctx.generate_rmat(int(sc), 16, s1 + 30)
+            selector.compare(ctx, gc, steps=3, log=log, regime="aside")
+            del gc
     ev = selector.evaluate(ctx, scale=a.eval_scale, seed=a.eval_seed, log=log)
     log(json.dumps(ev))
     report = {"train_rows": int(len(X)), "scales": scales, "seeds": [s0, s1], "epochs": a.epochs,

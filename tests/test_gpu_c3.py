@@ -42,6 +42,7 @@ def test_c3_4cliques(c3, policy, schedule):
     c = eh.Context(0)
     if policy == "learned":
         c.set_selector(selector.load())
+        c.set_selector_schedule(-1)  # the 4-clique passes read every row's layout
     _, roff, rnbr = c3["ref"]
     M = c.matrix_from_csr(N, roff, rnbr, "lower", policy)
     k4, st = M.clique_count(4, schedule)

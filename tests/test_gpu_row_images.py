@@ -47,6 +47,9 @@ def test_selected_row_images_are_exact_encodings(ctx, facebook, policy):
     c = eh.Context(0)
     if policy == "learned":
         c.set_selector(selector.load())
+        # every pair A-side: every row held by another is a candidate (under the default schedule
+        # these graphs, all below the B-stationary split, would leave every row uint)
+        c.set_selector_schedule(-1)
     seen = set()
     for name, n, off, nbr in _graphs(facebook):
         M = c.matrix_from_csr(n, off, nbr, "lower", policy)
