@@ -1789,7 +1789,8 @@ struct SchedArgs {
   uint32_t bstat_j;       // task rows hand their pairs with j < bstat_j to the B-stationary pass
   uint32_t bsh_max;       // rows whose B rows at or above bstat_j all have at most this many
                           // entries hand those pairs to the B-stationary pass above the split (0: off)
-  uint32_t *incount;      // out (bstat): task rows containing j, per j < bstat_j
+  uint32_t *bpairs;       // out (bstat): row i's B-stationary in-pairs (entries k in [1, below)
+                          // with a non-empty row(j)), one word per row
   uint32_t *hcount;       // out (bsh): such rows containing j, per j >= bstat_j (index j - bstat_j)
   uint32_t *jstart;       // out: first entry of row i its A-side tasks cover
   uint32_t *jbelow;       // out: entries of row i below bstat_j
@@ -1809,7 +1810,7 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
   for (uint32_t i = wid; i < s.n; i += nw) {
     const uint32_t dA = s.card[i];
     uint64_t work = 0;
-    uint32_t lo = 0, hi = 0, below = 0, bad = 0;
+    uint32_t lo = 0, hi = 0, below = 0, bad = 0, bp = 0;
     bool full = false;
     // with the B-stationary pass on, every row's pairs with j < bstat_j (a prefix of the
     // sorted row) leave its A-side tasks or its edge-centric wedges
@@ -1831,7 +1832,7 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
         if (jv < s.bstat_j) {
           below++;
           // the row's first entry has an empty prefix: nothing to count
-          if (d.card && k > 0) atomicAdd(&s.incount[jv], 1u);
+          bp += (d.card && k > 0) ? 1u : 0u;
           continue;
         }
         if (d.card > s.bsh_max) bad = 1u;
@@ -1839,6 +1840,7 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
       }
       work = wave_sum_u64(work);
       below = (uint32_t)wave_sum_u64(below);
+      bp = (uint32_t)wave_sum_u64(bp);
       loc[4] += (lane == 0) ? work - 16ull * (dA - below) : 0ull;
       // every pair at or above the split goes B-stationary (k_bshl) when no B row there is longer
       // than bsh_max: the row then has no A-side or edge-centric work left
@@ -1851,6 +1853,7 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
       }
     }
     if (lane != 0) continue;
+    if (s.bpairs) s.bpairs[i] = bp;
     const uint32_t js = full ? dA : below;
     s.jstart[i] = js;
     s.jbelow[i] = below;
@@ -1912,9 +1915,19 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
     const uint64_t order = s.tuned ? (0x3FFFFFFFull - min<uint64_t>(work, 0x3FFFFFFFull)) : 0ull;
     s.sortkey[i] = ((uint64_t)cls << 62) | (order << 32) | i;
   }
+  // the statistics: summed over the block's waves in LDS, then one atomic per block and value
+  // (one per wave had every wave of the grid queue on the same six words)
+  __shared__ unsigned long long part[4][6];
+  const uint32_t wv = threadIdx.x >> 6;
   for (int q = 0; q < 6; q++) {
     const unsigned long long v = wave_sum_u64(loc[q]);
-    if (lane == 0 && v) atomicAdd(&s.stats[q], v);
+    if (lane == 0) part[wv][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    unsigned long long v = 0;
+    for (uint32_t w = 0; w < (blockDim.x >> 6); w++) v += part[w][threadIdx.x];
+    if (v) atomicAdd(&s.stats[threadIdx.x], v);
   }
 }
 
@@ -1949,27 +1962,52 @@ __global__ void k_expand_tasks(const uint64_t *sorted, const uint32_t *nt_sorted
 
 // B-stationary in-pairs: every task row writes (its offset / 4, position) into the list
 // of each non-empty B row j < bstat_j it holds (one wave per row, positions [0, jstart))
-__global__ void __launch_bounds__(256) k_bstat_fill(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr,
-                                                    const RowDesc *desc, const uint32_t *jstart, uint32_t n,
-                                                    const uint64_t *inoff, uint32_t *infill, const uint64_t *lo16off,
-                                                    uint64_t *keys) {
+// every row writes the keys of its B-stationary in-pairs at its own offset (rowpos, the scan of
+// the per-row counts): no atomics, so the hub rows' in-lists do not serialise on counters; the
+// radix sort that follows groups the keys by j
+__global__ void __launch_bounds__(256) k_bstat_fill(const uint32_t *aoff, const uint32_t *unbr, const RowDesc *desc,
+                                                    const uint32_t *jbelow, uint32_t n, const uint64_t *rowpos,
+                                                    const uint64_t *lo16off, uint64_t *keys) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
   for (uint32_t i = wid; i < n; i += nw) {
-    const uint32_t js = jstart[i];
-    if (js == 0) continue;
+    const uint32_t jb = jbelow[i];
+    if (jb <= 1) continue;
     const uint32_t *G = unbr + (uint64_t)aoff[i] * 4;
-    for (uint32_t k = lane; k < js; k += 64) {
-      const uint32_t j = G[k];
-      if (desc[j].card == 0 || k == 0) continue;
-      const uint32_t slot = atomicAdd(&infill[j], 1u);
+    const uint64_t pre = (uint64_t)(uint32_t)lo16off[i] << 16;
+    uint64_t pos = rowpos[i];
+    for (uint32_t k0 = 0; k0 < jb; k0 += 64) {
+      const uint32_t k = k0 + lane;
+      uint32_t j = 0;
+      bool q = false;
+      if (k < jb && k > 0) {
+        j = G[k];
+        q = desc[j].card != 0;
+      }
+      const uint64_t m = __ballot(q);
       // j (16 bits) | row i's prefix vector offset (32 bits) | p (16 bits, p < j < 2^16):
       // sorting the keys orders every in-list by prefix offset (row order within each region
       // of the prefix copy)
-      keys[inoff[j] + slot] = ((uint64_t)j << 48) | ((uint64_t)(uint32_t)lo16off[i] << 16) | k;
+      if (q) keys[pos + mbcnt(m)] = ((uint64_t)j << 48) | pre | k;
+      pos += (uint64_t)__popcll(m);
     }
   }
+}
+
+// in-list bounds from the sorted keys: inoff[j] = the first key of j, incount[j] = how many
+// (incount starts at zero; j without keys keep count 0)
+__global__ void k_bstat_bounds(const uint64_t *keys, uint64_t np, uint64_t *inoff, uint32_t *inend) {
+  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < np; q += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t j = (uint32_t)(keys[q] >> 48);
+    if (q == 0 || (uint32_t)(keys[q - 1] >> 48) != j) inoff[j] = q;
+    if (q + 1 == np || (uint32_t)(keys[q + 1] >> 48) != j) inend[j] = (uint32_t)(q + 1);
+  }
+}
+
+__global__ void k_bstat_counts(const uint64_t *inoff, const uint32_t *inend, uint32_t nj, uint32_t *incount) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nj; j += gridDim.x * blockDim.x)
+    incount[j] = inend[j] ? (uint32_t)(inend[j] - inoff[j]) : 0u;
 }
 
 __global__ void k_bstat_unpack(const uint64_t *keys, uint64_t np, uint2 *in_pairs) {
@@ -2209,7 +2247,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   eh_ctx *ctx = m->ctx;
   hipStream_t st = ctx->stream;
   const uint32_t n = m->n;
-  DevBuf key, key2, ntask, rowinfo, stats, ntsorted, toff, cnt2, is_small, spos, incount;
+  DevBuf key, key2, ntask, rowinfo, stats, ntsorted, toff, cnt2, is_small, spos, bpairs;
   DevBuf &jstart = s.jstart;
   // B-stationary pairs: tuned triangles on an oriented single-graph matrix only (the
   // prefix identity needs every row(j) below j)
@@ -2221,10 +2259,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   // rows whose B rows at or above the split are all short go wholly B-stationary (hashed)
   const uint32_t bsh_max = (bstat_j && bstat_j < n) ? tp.bsh_max : 0u;
   const uint32_t nh = bsh_max ? n - bstat_j : 0u;
-  if (bstat_j) {
-    EH_CHECK(incount.alloc(4ull * bstat_j + 8));
-    k_zero_u32<<<grid_for(bstat_j), 256, 0, st>>>(incount.as<uint32_t>(), bstat_j);
-  }
+  if (bstat_j) EH_CHECK(bpairs.alloc(4ull * n + 8));
   if (nh) {
     EH_CHECK(hcount.alloc(4ull * nh + 8));
     k_zero_u32<<<grid_for((uint64_t)nh + 1), 256, 0, st>>>(hcount.as<uint32_t>(), nh + 1);
@@ -2258,7 +2293,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   sa.local_max = local_max;
   sa.bstat_j = bstat_j;
   sa.bsh_max = bsh_max;
-  sa.incount = bstat_j ? incount.as<uint32_t>() : nullptr;
+  sa.bpairs = bstat_j ? bpairs.as<uint32_t>() : nullptr;
   sa.hcount = nh ? hcount.as<uint32_t>() : nullptr;
   sa.jstart = jstart.as<uint32_t>();
   sa.jbelow = jbelow.as<uint32_t>();
@@ -2267,7 +2302,8 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   sa.ntask = ntask.as<uint32_t>();
   sa.rowinfo = rowinfo.as<uint32_t>();
   sa.stats = stats.as<unsigned long long>();
-  k_row_work<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(sa);
+  // a persistent grid: every block adds its statistics once
+  k_row_work<<<std::min(grid_for((uint64_t)n * 64), (unsigned)(ctx->num_cus ? ctx->num_cus : 256) * 8u), 256, 0, st>>>(sa);
   EH_HIP(hipGetLastError());
   // rows of each k_local4 class, in row order
   for (int c = 0; c < 2; c++) {
@@ -2330,16 +2366,12 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     // (each task clears and fills an 8 KB bitmap: on RMAT-22 256 pairs per task 3.74 ms,
     // 512 3.54, 1024 3.47, 2048 3.55)
     constexpr uint32_t BSTAT_CHUNK = 1024;
-    DevBuf inoff, infill, ntb, boff;
-    EH_CHECK(inoff.alloc(8ull * bstat_j + 16));
-    EH_CHECK(infill.alloc(4ull * bstat_j + 8));
-    EH_CHECK(ntb.alloc(4ull * bstat_j + 8));
-    EH_CHECK(boff.alloc(8ull * bstat_j + 16));
-    k_zero_u32<<<grid_for(bstat_j + 1), 256, 0, st>>>(infill.as<uint32_t>(), bstat_j + 1);
-    k_zero_u32<<<1, 64, 0, st>>>(incount.as<uint32_t>() + bstat_j, 1);
-    EH_CHECK(prim_exscan_u32_to_u64(ctx, incount.as<uint32_t>(), inoff.as<uint64_t>(), (uint64_t)bstat_j + 1));
+    DevBuf inoff, inend, incount, rowpos, ntb, boff;
+    EH_CHECK(rowpos.alloc(8ull * n + 16));
+    k_zero_u32<<<1, 64, 0, st>>>(bpairs.as<uint32_t>() + n, 1);
+    EH_CHECK(prim_exscan_u32_to_u64(ctx, bpairs.as<uint32_t>(), rowpos.as<uint64_t>(), (uint64_t)n + 1));
     uint64_t npairs = 0;
-    EH_HIP(hipMemcpyAsync(&npairs, inoff.as<uint64_t>() + bstat_j, 8, hipMemcpyDeviceToHost, st));
+    EH_HIP(hipMemcpyAsync(&npairs, rowpos.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, st));
     EH_HIP(hipStreamSynchronize(st));
     if (npairs >= (1ull << 32)) {
       eh_set_error("%llu B-stationary pairs exceed the 32-bit pair index", (unsigned long long)npairs);
@@ -2349,6 +2381,12 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     DevBuf pkeys, pkeys2;
     EH_CHECK(pkeys.alloc(8 * npairs + 16));
     EH_CHECK(pkeys2.alloc(8 * npairs + 16));
+    EH_CHECK(inoff.alloc(8ull * bstat_j + 16));
+    EH_CHECK(inend.alloc(4ull * bstat_j + 8));
+    EH_CHECK(incount.alloc(4ull * bstat_j + 8));
+    EH_CHECK(ntb.alloc(4ull * bstat_j + 8));
+    EH_CHECK(boff.alloc(8ull * bstat_j + 16));
+    k_zero_u32<<<grid_for(bstat_j + 1), 256, 0, st>>>(inend.as<uint32_t>(), bstat_j + 1);
     // 16-bit prefixes
     DevBuf nvec;
     EH_CHECK(nvec.alloc(4ull * n + 8));
@@ -2378,12 +2416,17 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     k_lo16_fill<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(m->daoff.as<uint32_t>(), m->dunbr.as<uint32_t>(),
                                                           jbelow.as<uint32_t>(), n, lo16off.as<uint64_t>(),
                                                           bstat_j - 1u, s.lo16.as<uint16_t>());
-    k_bstat_fill<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(
-        m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(), m->dunbr.as<uint32_t>(), m->ddesc.as<RowDesc>(),
-        jbelow.as<uint32_t>(), n, inoff.as<uint64_t>(), infill.as<uint32_t>(), lo16off.as<uint64_t>(),
-        pkeys.as<uint64_t>());
+    k_bstat_fill<<<grid_for((uint64_t)n * 64), 256, 0, st>>>(m->daoff.as<uint32_t>(), m->dunbr.as<uint32_t>(),
+                                                           m->ddesc.as<RowDesc>(), jbelow.as<uint32_t>(), n,
+                                                           rowpos.as<uint64_t>(), lo16off.as<uint64_t>(),
+                                                           pkeys.as<uint64_t>());
     EH_CHECK(prim_sort_u64(ctx, pkeys.as<uint64_t>(), pkeys2.as<uint64_t>(), npairs, 64));
     k_bstat_unpack<<<grid_for(npairs), 256, 0, st>>>(pkeys.as<uint64_t>(), npairs, s.in_pairs.as<uint2>());
+    if (npairs)
+      k_bstat_bounds<<<grid_for(npairs), 256, 0, st>>>(pkeys.as<uint64_t>(), npairs, inoff.as<uint64_t>(),
+                                                       inend.as<uint32_t>());
+    k_bstat_counts<<<grid_for(bstat_j + 1), 256, 0, st>>>(inoff.as<uint64_t>(), inend.as<uint32_t>(), bstat_j + 1,
+                                                          incount.as<uint32_t>());
     EH_HIP(hipStreamSynchronize(st));
     pkeys.release();
     pkeys2.release();
@@ -2393,6 +2436,13 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     uint64_t nbt = 0;
     EH_HIP(hipMemcpyAsync(&nbt, boff.as<uint64_t>() + bstat_j, 8, hipMemcpyDeviceToHost, st));
     EH_HIP(hipStreamSynchronize(st));
+    // the in-list bounds come from sorted keys: every list non-empty has at least one pair per
+    // task, so more tasks than pairs would mean unsorted keys -- refuse rather than launch them
+    if (nbt > npairs) {
+      eh_set_error("B-stationary in-lists inconsistent (%llu tasks for %llu pairs)", (unsigned long long)nbt,
+                   (unsigned long long)npairs);
+      return EH_ERR_INVALID;
+    }
     EH_CHECK(s.btasks.alloc(16 * nbt + 16));
     k_bstat_expand<<<grid_for(bstat_j), 256, 0, st>>>(incount.as<uint32_t>(), inoff.as<uint64_t>(),
                                                      boff.as<uint64_t>(), bstat_j, BSTAT_CHUNK,

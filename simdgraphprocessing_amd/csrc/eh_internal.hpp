@@ -228,6 +228,9 @@ eh_status graph_apply_order(eh_graph *g, const uint32_t *d_new2old);
 // k-cliques for k >= 3 by the general level-set recursion (eh_clique.hip)
 eh_status clique_count_deep(eh_matrix *m, uint32_t k, uint64_t *count, double *ms);
 
+// keys sorted on bits [0, end_bit) (always from bit 0: rocPRIM's merge-sort path, taken for
+// inputs up to its merge-sort limit, builds its digit mask as (1 << (begin_bit + bits)) - 1, a
+// shift by 64 for a range ending at bit 64 with begin_bit > 0, and then orders by the wrong bits)
 eh_status prim_sort_u64(eh_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int end_bit);
 eh_status prim_sort_pairs_u32(eh_ctx *ctx, uint32_t *keys, uint32_t *keys_tmp, uint32_t *vals,
                               uint32_t *vals_tmp, uint64_t n);
