@@ -392,18 +392,27 @@ def traffic_summary(args):
     cf, cw = copy_kib(fr), copy_kib(wr)
     ff, wf = COPY_BYTES / (cf * 1024.0), COPY_BYTES / (cw * 1024.0)
     passes = args.passes + 1
-    kernels = {}
+    # per kernel class: the mean bytes of one dispatch times the dispatches every pass makes;
+    # classes dispatched fewer times than there were passes ran outside the pass (the one-time
+    # weighted pass behind the 4-clique byte model) and are left out
+    acc = {}
     for rows, key, fac in ((fr, "fetch_bytes", ff), (wr, "write_bytes", wf)):
         for name, kib in rows:
             base = _kernel_base(name)
             if base:
-                k = kernels.setdefault(base, {"fetch_bytes": 0.0, "write_bytes": 0.0, "dispatches": 0})
-                k[key] += kib * 1024.0 * fac / passes
-                if key == "fetch_bytes":
-                    k["dispatches"] += 1
-    for k in kernels.values():
-        k["dispatches"] /= passes
+                a = acc.setdefault(base, {"fetch_bytes": [], "write_bytes": []})
+                a[key].append(kib * 1024.0 * fac)
+    kernels = {}
+    for base, a in acc.items():
+        nd = len(a["fetch_bytes"])
+        per_pass = nd // passes
+        if per_pass == 0 or not a["write_bytes"]:
+            continue
+        k = {"dispatches": per_pass,
+             "fetch_bytes": per_pass * sum(a["fetch_bytes"]) / nd,
+             "write_bytes": per_pass * sum(a["write_bytes"]) / len(a["write_bytes"])}
         k["bytes"] = k["fetch_bytes"] + k["write_bytes"]
+        kernels[base] = k
     policy = args.policy if args.policy != "auto" else "hybrid"
     params = committed_shape(shape_tag(args.profile_run, policy))
     entry = {"bytes_per_pass": sum(k["bytes"] for k in kernels.values()), "kernels": kernels,

@@ -276,8 +276,9 @@ extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tr
     if (!want[q]) continue;
     const int sched = opts[q] ? opts[q]->schedule : EH_SCHED_TUNED;
     if (sched != EH_SCHED_PLAIN && sched != EH_SCHED_TUNED) { eh_set_error("bad schedule %d", sched); return EH_ERR_INVALID; }
-    // count_run builds (or reuses) the pass's schedule; its result is discarded
-    EH_CHECK(count_run(M, kk[q], opts[q], b->counts.as<uint64_t>() + (uint64_t)q * ng, ng, nullptr, nullptr, nullptr));
+    // the pass's schedule, built or reused (no pass runs here: the counts come from the graph
+    // or the per-graph launches below, whose error words are checked after them)
+    EH_CHECK(schedule_ensure(M, kk[q], opts[q], nullptr));
     const Schedule &S = M->sched[q][sched];
     key[q].sched = sched;
     key[q].tp = tune_defaults(M, sched, kk[q], opts[q]);

@@ -2843,17 +2843,14 @@ eh_status matrix_triangle_weight(eh_matrix *m, uint32_t ncounts, uint64_t *out) 
 
 // k: 3 triangles, 4 four-cliques, -3 weighted triangle pass (sum of card(x) over
 // common x; the per-triangle byte term of the 4-clique model)
-eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_counts, uint32_t ncounts,
-                    uint64_t *count, eh_run_stats *stats, uint64_t *d_task_cycles) {
-  eh_ctx *ctx = m->ctx;
-  hipStream_t st = ctx->stream;
+eh_status schedule_ensure(eh_matrix *m, int k, const eh_run_opts *opts, double *sched_ms) {
   const int sched = opts ? opts->schedule : EH_SCHED_TUNED;
   if (sched != EH_SCHED_PLAIN && sched != EH_SCHED_TUNED) { eh_set_error("bad schedule %d", sched); return EH_ERR_INVALID; }
   const int kk = (k == 4) ? 4 : 3;
-  TuneParams tp = tune_defaults(m, sched, kk, opts);
+  const TuneParams tp = tune_defaults(m, sched, kk, opts);
   Schedule &s = m->sched[kk == 4 ? 1 : 0][sched];
   const int skey = kk * 8 + sched;
-  double sched_ms = 0;
+  if (sched_ms) *sched_ms = 0;
   if (!s.valid || s.task_work != tp.task_work || s.lds_small != tp.lds_small || s.lds_big != tp.lds_big ||
       (int)s.sched != skey || s.small_deg != tp.small_deg || s.bstat_req != tp.bstat_j || s.bsh_req != tp.bsh_max ||
       s.local_req != tp.local_max) {
@@ -2863,8 +2860,21 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
     s.bstat_req = tp.bstat_j;
     s.bsh_req = tp.bsh_max;
     s.local_req = tp.local_max;
-    sched_ms = s.ms;
+    if (sched_ms) *sched_ms = s.ms;
   }
+  return EH_OK;
+}
+
+eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_counts, uint32_t ncounts,
+                    uint64_t *count, eh_run_stats *stats, uint64_t *d_task_cycles) {
+  eh_ctx *ctx = m->ctx;
+  hipStream_t st = ctx->stream;
+  const int sched = opts ? opts->schedule : EH_SCHED_TUNED;
+  const int kk = (k == 4) ? 4 : 3;
+  double sched_ms = 0;
+  EH_CHECK(schedule_ensure(m, k, opts, &sched_ms));
+  const TuneParams tp = tune_defaults(m, sched, kk, opts);
+  Schedule &s = m->sched[kk == 4 ? 1 : 0][sched];
   DevBuf own_counts, next;
   unsigned long long *dc = reinterpret_cast<unsigned long long *>(d_counts);
   if (!dc) {

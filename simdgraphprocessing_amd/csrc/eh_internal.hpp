@@ -259,6 +259,9 @@ eh_status matrix_select_learned(eh_matrix *m, uint8_t *d_types, const uint64_t *
 eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_counts,
                     uint32_t ncounts, uint64_t *count, eh_run_stats *stats, uint64_t *d_task_cycles);
 eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, Schedule &s);
+// the schedule of a k-pass under opts: built, or reused while every parameter it was built for
+// holds; *sched_ms = its build time when built here, else 0
+eh_status schedule_ensure(eh_matrix *m, int k, const eh_run_opts *opts, double *sched_ms);
 eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, Schedule &s,
                           uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles, PartEv *pev = nullptr);
 TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts);
