@@ -48,8 +48,11 @@ application<T, R> *init_app(Parser input_data);
 
 namespace emptyheaded {
 
-// load the learned selector's parameters into the shared context
-inline void load_selector_params() {
+// load the learned selector's parameters into the shared context.  The selection serves the
+// tuned triangle pass (only the rows it reads as an A-side or edge-centric B row are candidates);
+// every other app, and the plain schedule, read every row's layout, so there every row held by
+// another is a candidate.
+inline void load_selector_params(const std::string &app, int schedule) {
   const char *path = std::getenv("EH_SELECTOR_PARAMS");
   if (!path) {
     std::cerr << "ERROR: --layout=learned needs EH_SELECTOR_PARAMS=<parameter file>" << std::endl;
@@ -66,6 +69,8 @@ inline void load_selector_params() {
   Runtime &rt = Runtime::get();
   std::lock_guard<std::mutex> lock(rt.mutex());
   check(eh_selector_set_params(rt.ctx(), p), "selector parameters");
+  const bool tuned_triangles = app == "undirected_triangle_counting" && schedule == EH_SCHED_TUNED;
+  check(eh_selector_set_schedule(rt.ctx(), tuned_triangles ? 0 : -1, 0), "selector schedule");
 }
 
 template <class T>
@@ -97,7 +102,7 @@ int main(int argc, char *argv[]) {
   if (l == "bitset_new") return emptyheaded::run_layout<bitset_new>(input_data);
   if (l == "hybrid_block") return emptyheaded::run_layout<hybrid_block>(input_data);
   if (l == "learned") {
-    emptyheaded::load_selector_params();
+    emptyheaded::load_selector_params(app, input_data.schedule);
     return emptyheaded::run_layout<learned_hybrid>(input_data);
   }
   std::cout << "No valid layout entered." << std::endl;
