@@ -1656,69 +1656,85 @@ __global__ void __launch_bounds__(256) k_bshl(CountArgs a, const uint4 *__restri
     const uint4 task = htasks[t++];
     const uint32_t bin = rfl(task.x), pb = rfl(task.y), pe = rfl(task.z);
     const uint32_t h0 = rfl(bstart[bin]), h1 = rfl(bstart[bin + 1]);
-    uint32_t xn[R], xc[R];
-    uint4 prn = make_uint4(0u, 0u, 0u, 0u);
-    uint32_t dn = 0, ln = 0, q0n = 0, q1n = 0;
-    auto fetch = [&](uint32_t h) {
-      q0n = max((uint32_t)hoff[h], pb);
-      q1n = min((uint32_t)hoff[h + 1], pe);
-      const uint32_t j = rfl(hj[h]);
-      dn = (q0n < q1n) ? rfl(a.card[j]) : 0u;  // row(j) has at most 64 R ids
-      ln = rfl(hjlo[h]);
-      const uint32_t *G = a.unbr + (uint64_t)a.aoff[j] * 4;
-#pragma unroll
-      for (int u = 0; u < R; u++) xn[u] = (lane + 64u * u < dn) ? G[lane + 64u * u] : pad;
-      prn = (dn && q0n + lane < q1n) ? hpairs[q0n + lane] : make_uint4(0u, 0u, 0u, 0u);
-    };
-    fetch(h0);
-    for (uint32_t h = h0; h < h1; h++) {
-      const uint32_t dj = dn, dlo = ln, q0 = q0n, q1 = q1n;
-      uint4 pr = prn;
-#pragma unroll
-      for (int u = 0; u < R; u++) xc[u] = xn[u];
-      if (h + 1 < h1) fetch(h + 1);
-      if (dj == 0) continue;  // no pairs of this row in the task
-      const uint32_t dhi = dj - dlo;
-      bool has_pad = false;
-      if (dlo) {
-        // LO(j) is sorted: it ends with the pad id exactly when its last id is split - 1
-        const uint32_t qd = (dlo - 1u) >> 6;  // wave-uniform: which register holds it
-        uint32_t xl = xc[0];
-#pragma unroll
-        for (int u = 1; u < R; u++)
-          if (qd == (uint32_t)u) xl = xc[u];
-        has_pad = __shfl(xl, (int)((dlo - 1u) & 63u), 64) == pad;
+    for (uint32_t hc = h0; hc < h1; hc += 64) {
+      const uint32_t nr = min(64u, h1 - hc);
+      // the metadata of the bin's B rows (at most BSH_GROUP), one row per lane, loaded together:
+      // a row's fetch below is then one dependent load (its ids) instead of a chain of three
+      // (row id -> length and offset -> ids), which the one-row-ahead prefetch could not hide
+      // behind the few in-pairs most rows have
+      uint32_t mq0 = 0, mq1 = 0, md = 0, mlo = 0, mao = 0;
+      if (lane < nr) {
+        const uint32_t h = hc + lane;
+        mq0 = max((uint32_t)hoff[h], pb);
+        mq1 = min((uint32_t)hoff[h + 1], pe);
+        const uint32_t j = hj[h];
+        md = (mq0 < mq1) ? a.card[j] : 0u;  // row(j) has at most 64 R ids
+        mlo = hjlo[h];
+        mao = a.aoff[j];
       }
+      uint32_t xn[R], xc[R];
+      uint4 prn = make_uint4(0u, 0u, 0u, 0u);
+      uint32_t dn = 0, ln = 0, q0n = 0, q1n = 0;
+      auto fetch = [&](uint32_t r) {
+        q0n = rdl(mq0, r);
+        q1n = rdl(mq1, r);
+        dn = rdl(md, r);
+        ln = rdl(mlo, r);
+        const uint32_t *G = a.unbr + (uint64_t)rdl(mao, r) * 4;
 #pragma unroll
-      for (int u = 0; u < R; u++) {
-        const uint32_t k = lane + 64u * u;
-        if (k < dlo && xc[u] != pad) atomicOr(&bm[xc[u] >> 5], 1u << (xc[u] & 31u));
-        else if (k >= dlo && k < dj) hl[k - dlo] = xc[u];
-      }
-      wave_sync();
-      if (dhi) {
-        H.n = dhi;
-        H.lo = hl[0];
-        H.span = hl[dhi - 1u] - H.lo;
-      }
-      for (uint32_t c0 = q0; c0 < q1; c0 += 64) {
-        const uint4 pnext = (c0 + 64u + lane < q1) ? hpairs[c0 + 64u + lane] : make_uint4(0u, 0u, 0u, 0u);
-        if (dlo)
-          acc += flat_pass<PASS_STREAM16, KIND_BITMAP, false, 4>(a, S, (pr.y + 7u) >> 3, 0u, pr.x, 0u, lane, meta);
-        if (has_pad && pr.y && lo16s[(uint64_t)pr.x * 8 + pr.y - 1u] == pad) acc++;
-        if (dhi) {
-          // row(i) from the vector of its first id at or above the split to j's vector: the
-          // ids below the split and from j on are in no HI(j)
-          const uint32_t v0 = pr.y >> 2, nv = (pr.z | pr.w) ? ((pr.w + 3u) >> 2) - v0 : 0u;
-          acc += flat_pass<PASS_STREAM, KIND_LIST, false, 4>(a, H, nv, 0u, pr.z + v0, 0u, lane, meta);
+        for (int u = 0; u < R; u++) xn[u] = (lane + 64u * u < dn) ? G[lane + 64u * u] : pad;
+        prn = (dn && q0n + lane < q1n) ? hpairs[q0n + lane] : make_uint4(0u, 0u, 0u, 0u);
+      };
+      fetch(0);
+      for (uint32_t r = 0; r < nr; r++) {
+        const uint32_t dj = dn, dlo = ln, q0 = q0n, q1 = q1n;
+        uint4 pr = prn;
+#pragma unroll
+        for (int u = 0; u < R; u++) xc[u] = xn[u];
+        if (r + 1 < nr) fetch(r + 1);
+        if (dj == 0) continue;  // no pairs of this row in the task
+        const uint32_t dhi = dj - dlo;
+        bool has_pad = false;
+        if (dlo) {
+          // LO(j) is sorted: it ends with the pad id exactly when its last id is split - 1
+          const uint32_t qd = (dlo - 1u) >> 6;  // wave-uniform: which register holds it
+          uint32_t xl = xc[0];
+#pragma unroll
+          for (int u = 1; u < R; u++)
+            if (qd == (uint32_t)u) xl = xc[u];
+          has_pad = __shfl(xl, (int)((dlo - 1u) & 63u), 64) == pad;
         }
-        pr = pnext;
-      }
-      wave_sync();
 #pragma unroll
-      for (int u = 0; u < R; u++)
-        if (lane + 64u * u < dlo && xc[u] != pad) bm[xc[u] >> 5] = 0u;
-      wave_sync();
+        for (int u = 0; u < R; u++) {
+          const uint32_t k = lane + 64u * u;
+          if (k < dlo && xc[u] != pad) atomicOr(&bm[xc[u] >> 5], 1u << (xc[u] & 31u));
+          else if (k >= dlo && k < dj) hl[k - dlo] = xc[u];
+        }
+        wave_sync();
+        if (dhi) {
+          H.n = dhi;
+          H.lo = hl[0];
+          H.span = hl[dhi - 1u] - H.lo;
+        }
+        for (uint32_t c0 = q0; c0 < q1; c0 += 64) {
+          const uint4 pnext = (c0 + 64u + lane < q1) ? hpairs[c0 + 64u + lane] : make_uint4(0u, 0u, 0u, 0u);
+          if (dlo)
+            acc += flat_pass<PASS_STREAM16, KIND_BITMAP, false, 4>(a, S, (pr.y + 7u) >> 3, 0u, pr.x, 0u, lane, meta);
+          if (has_pad && pr.y && lo16s[(uint64_t)pr.x * 8 + pr.y - 1u] == pad) acc++;
+          if (dhi) {
+            // row(i) from the vector of its first id at or above the split to j's vector: the
+            // ids below the split and from j on are in no HI(j)
+            const uint32_t v0 = pr.y >> 2, nv = (pr.z | pr.w) ? ((pr.w + 3u) >> 2) - v0 : 0u;
+            acc += flat_pass<PASS_STREAM, KIND_LIST, false, 4>(a, H, nv, 0u, pr.z + v0, 0u, lane, meta);
+          }
+          pr = pnext;
+        }
+        wave_sync();
+#pragma unroll
+        for (int u = 0; u < R; u++)
+          if (lane + 64u * u < dlo && xc[u] != pad) bm[xc[u] >> 5] = 0u;
+        wave_sync();
+      }
     }
   }
   const uint64_t sum = wave_sum_u64(acc);
