@@ -517,6 +517,12 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   k_pack<<<grid_for(n), 256, 0, st>>>(d_off, d_nbr, n, filter, d_mask, m->dcard.as<uint32_t>(),
                                                      poff.as<uint64_t>(), m->dunbr.as<uint32_t>());
   if ((s = prim_reduce_max_u32(ctx, m->dcard.as<uint32_t>(), n, &m->max_row)) != EH_OK) return fail(s);
+  // the graph of every row (batch arenas), before the learned selection, which treats an arena
+  // as read wholly by A-side and edge-centric passes (every row held by another is a candidate)
+  if (d_graph_of) {
+    if ((s = m->dgraph_of.alloc(4ull * n + 4)) != EH_OK) return fail(s);
+    if (hipMemcpyAsync(m->dgraph_of.p, d_graph_of, 4ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) return fail(EH_ERR_HIP);
+  }
   // learned types are computed from the packed rows before selection
   DevBuf forced;
   if (policy == EH_LAYOUT_LEARNED) {
@@ -637,10 +643,6 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   float ms = 0;
   if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) ms = 0;
   m->build_ms = ms;
-  if (d_graph_of) {
-    if ((s = m->dgraph_of.alloc(4ull * n + 4)) != EH_OK) return fail(s);
-    if (hipMemcpyAsync(m->dgraph_of.p, d_graph_of, 4ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) return fail(EH_ERR_HIP);
-  }
   if (hipStreamSynchronize(st) != hipSuccess) return fail(EH_ERR_HIP);
   *out = m;
   return EH_OK;
