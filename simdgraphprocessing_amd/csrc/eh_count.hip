@@ -2891,13 +2891,14 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
   EH_CHECK(schedule_ensure(m, k, opts, &sched_ms));
   const TuneParams tp = tune_defaults(m, sched, kk, opts);
   Schedule &s = m->sched[kk == 4 ? 1 : 0][sched];
-  DevBuf own_counts, next;
-  unsigned long long *dc = reinterpret_cast<unsigned long long *>(d_counts);
-  if (!dc) {
-    EH_CHECK(own_counts.alloc(8ull * (ncounts ? ncounts : 1)));
-    dc = own_counts.as<unsigned long long>();
-  }
-  EH_CHECK(next.alloc(32));
+  // the pass's queue heads and error word (32 words) and, when the caller passes no device
+  // counters, its counts: the context's buffer, grown when needed
+  const uint64_t own = d_counts ? 0ull : (ncounts ? ncounts : 1);
+  DevBuf &pw = ctx->pass_words;
+  if (pw.bytes < 128 + 8 * own) EH_CHECK(pw.alloc(128 + 8 * own + 1024));
+  uint32_t *const nx = pw.as<uint32_t>();
+  unsigned long long *dc = d_counts ? reinterpret_cast<unsigned long long *>(d_counts)
+                                    : reinterpret_cast<unsigned long long *>(pw.as<uint8_t>() + 128);
   const int repeat = (opts && opts->repeat > 1) ? opts->repeat : 1;
   PartEv *pev = nullptr;
   if (opts && opts->time_parts) {
@@ -2908,11 +2909,11 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
   float total_ms = 0;
   for (int r = 0; r < repeat; r++) {
     k_zero_u64<<<grid_for(ncounts ? ncounts : 1), 256, 0, st>>>(dc, ncounts ? ncounts : 1);
-    k_zero_u32<<<1, 64, 0, st>>>(next.as<uint32_t>() + 7, 1);  // the pass's error word
+    k_zero_u32<<<1, 64, 0, st>>>(nx + 7, 1);  // the pass's error word
     if (pev)
       for (int q = 0; q < EH_PARTS; q++) pev[q].used = false;
     EH_HIP(hipEventRecord(ctx->ev0, st));
-    EH_CHECK(schedule_launch(m, k, sched, tp, s, reinterpret_cast<uint64_t *>(dc), next.as<uint32_t>(),
+    EH_CHECK(schedule_launch(m, k, sched, tp, s, reinterpret_cast<uint64_t *>(dc), nx,
                              d_task_cycles, pev));
     EH_HIP(hipEventRecord(ctx->ev1, st));
     EH_HIP(hipEventSynchronize(ctx->ev1));
@@ -2928,7 +2929,7 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
         }
   }
   uint32_t dev_err = 0;
-  EH_HIP(hipMemcpy(&dev_err, next.as<uint32_t>() + 7, 4, hipMemcpyDeviceToHost));
+  EH_HIP(hipMemcpy(&dev_err, nx + 7, 4, hipMemcpyDeviceToHost));
   if (dev_err) {
     eh_set_error("counting pass: an LDS hash table had no free slot (device error word %#x); the count is "
                  "not valid", dev_err);

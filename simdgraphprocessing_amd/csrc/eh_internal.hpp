@@ -154,6 +154,8 @@ struct eh_ctx {
   uint32_t sel_split = 65536, sel_bsh = 512;
   DevBuf scratch;          // reusable temp storage for primitives
   DevBuf counter;          // small device counters
+  DevBuf pass_words;       // a counting pass's queue heads, error word and count (kept between
+                           // passes: a per-pass hipMalloc / hipFree pair synchronised the device)
 };
 
 struct eh_graph {
