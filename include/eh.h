@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define EH_ABI_VERSION 3
+#define EH_ABI_VERSION 4
 
 typedef enum {
   EH_OK = 0,
@@ -139,6 +139,10 @@ typedef struct {
   int32_t bsh_max;        /* TUNED triangles with the split on: a row whose B rows at or above the
                              split all have at most this many entries counts those pairs
                              B-stationary too (k_bshl); 0 = default (512), <0 = off, else 1..512 */
+  int32_t short_queues;   /* TUNED on batch arenas: the edge-centric short rows are cut into 8
+                             ranges of equal wedge work, one queue per XCD, taken first by the
+                             blocks of that XCD (each L2 then mostly holds its own graphs);
+                             0 = default (on), <0 = one queue */
 } eh_run_opts;
 
 /* Kernel classes of one counting pass (indices of eh_run_stats.part_ms). */

@@ -53,7 +53,8 @@ class RunOpts(C.Structure):
     _fields_ = [("schedule", C.c_int32), ("lanes_per_row", C.c_int32), ("task_work", C.c_uint32),
                 ("lds_words", C.c_uint32), ("waves_per_block", C.c_int32), ("repeat", C.c_int32),
                 ("small_degree", C.c_int32), ("stream_ratio", C.c_int32), ("bstat_split", C.c_int32),
-                ("local_max", C.c_int32), ("time_parts", C.c_int32), ("bsh_max", C.c_int32)]
+                ("local_max", C.c_int32), ("time_parts", C.c_int32), ("bsh_max", C.c_int32),
+                ("short_queues", C.c_int32)]
 
 
 class BfsInfo(C.Structure):
@@ -202,9 +203,10 @@ def _ptr(a):
 
 
 def _opts(schedule="tuned", lanes=0, task_work=0, lds_words=0, waves=0, repeat=1, small_degree=0,
-          stream_ratio=0, bstat=0, local_max=0, time_parts=False, bsh=0):
+          stream_ratio=0, bstat=0, local_max=0, time_parts=False, bsh=0, short_queues=0):
     o = RunOpts()
     o.bsh_max = bsh
+    o.short_queues = short_queues
     o.time_parts = int(bool(time_parts))
     o.bstat_split = bstat
     o.local_max = local_max

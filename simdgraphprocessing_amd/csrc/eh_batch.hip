@@ -87,7 +87,7 @@ static bool same_tp(const TuneParams &x, const TuneParams &y) {
   return x.lanes == y.lanes && x.task_work == y.task_work && x.lds_small == y.lds_small && x.lds_big == y.lds_big &&
          x.waves_per_block == y.waves_per_block && x.big_waves == y.big_waves && x.small_deg == y.small_deg &&
          x.stream_ratio == y.stream_ratio && x.bstat_j == y.bstat_j && x.grab == y.grab &&
-         x.local_max == y.local_max && x.bsh_max == y.bsh_max;
+         x.local_max == y.local_max && x.bsh_max == y.bsh_max && x.short_queues == y.short_queues;
 }
 
 static bool same_key(const PassKey &x, const PassKey &y) {

@@ -73,6 +73,9 @@ struct CountArgs {
   uint32_t *err;                     // device error word: a lane whose hash probe or insert
                                      // walked a whole table ORs ERR_HASH_FULL into it once, when
                                      // its task or kernel ends (the host reports it)
+  uint32_t *xq;                      // short rows of a batch arena: bounds of 8 ranges of the
+                                     // row list (xq[0..8]) and their queue heads (XQ_HEADS on);
+                                     // null: one queue at next
 };
 
 // bits of the device error word (CountArgs::err)
@@ -1168,6 +1171,58 @@ __device__ __forceinline__ void mq_finish(MemberQuery (&q)[NQ]) {
 }
 
 constexpr int SMALL_NQ = 4;  // pairs per lane whose searches run interleaved
+// the XCD queues of a short-row list: 9 bounds, then 8 heads on lines of their own
+constexpr uint32_t XQ_HEADS = 64, XQ_STRIDE = 64, XQ_WORDS = XQ_HEADS + 8 * XQ_STRIDE;
+
+// Add a wave's per-row counts c (rows of one chunk, valid lanes only) into the per-graph
+// counters.  The chunk's rows run in row order, so their graph ids form a few runs: one
+// atomic per distinct graph id instead of one per row (these atomics are performed at the
+// memory side and serialise per line).
+__device__ __forceinline__ void add_graph_counts(const CountArgs &a, uint32_t c, uint32_t gid, uint32_t lane) {
+  uint64_t pend = __ballot(c != 0u);
+  while (pend) {
+    const uint32_t g = rdl(gid, (uint32_t)__builtin_ctzll(pend));
+    const bool mine = c != 0u && gid == g;
+    const uint64_t sum = wave_sum_u64(mine ? (uint64_t)c : 0ull);
+    if (lane == 0) atomicAdd(&a.counts[g], (unsigned long long)sum);
+    pend &= ~__ballot(mine);
+  }
+}
+
+// The next (at most) 64 rows of a short-row list for a wave: [c0, c0 + nr); false once
+// every queue is drained.  Without XCD queues the list is one queue whose head is a.next.
+// With them it is 8 ranges of equal work, a.xq[q] .. a.xq[q + 1], with heads
+// a.xq[XQ_HEADS + XQ_STRIDE * q]:
+// workgroups are dispatched to the 8 XCDs in turn, so blockIdx.x % 8 names the block's
+// XCD, and a wave takes from that XCD's range first, then from the next ranges in turn
+// (qs, wave-uniform, counts the ranges it has found drained).  Which range a block takes
+// changes only what its L2 holds, never which rows are counted.
+__device__ __forceinline__ bool take_rows(const CountArgs &a, uint32_t nrows, uint32_t lane, uint32_t &qs,
+                                          uint32_t &c0, uint32_t &nr) {
+  if (!a.xq) {
+    uint32_t c = 0;
+    if (lane == 0) c = atomicAdd(a.next, 64u);
+    c = rfl(c);
+    if (c >= nrows) return false;
+    c0 = c;
+    nr = min(64u, nrows - c);
+    return true;
+  }
+  for (; qs < 8u; qs++) {
+    const uint32_t q = (blockIdx.x + qs) & 7u;
+    const uint32_t b = a.xq[q], e = a.xq[q + 1u];
+    if (b >= e) continue;
+    uint32_t c = 0;
+    if (lane == 0) c = atomicAdd(a.xq + XQ_HEADS + XQ_STRIDE * q, 64u);
+    c = rfl(c);
+    if (c < e - b) {
+      c0 = b + c;
+      nr = min(64u, e - b - c);
+      return true;
+    }
+  }
+  return false;
+}
 
 template <int K, bool W>
 __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t *__restrict__ rows, uint32_t nrows) {
@@ -1178,12 +1233,8 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
   uint32_t *mbase = meta, *moff = meta + 65, *rcnt = meta + 130, *mdeg = meta + 194;
   const bool per_graph = a.graph_of != nullptr;
   uint64_t acc = 0;
-  for (;;) {
-    uint32_t c0 = 0;
-    if (lane == 0) c0 = atomicAdd(a.next, 64u);
-    c0 = rfl(c0);
-    if (c0 >= nrows) break;
-    const uint32_t nr = min(64u, nrows - c0);
+  uint32_t qs = 0, c0 = 0, nr = 0;
+  while (take_rows(a, nrows, lane, qs, c0, nr)) {
     uint32_t items = 0, off = 0, gid = 0, d = 0, skip = 0;
     if (lane < nr) {
       const uint32_t i = rows[c0 + lane];
@@ -1248,7 +1299,7 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
       }
     }
     wave_sync();
-    if (per_graph && lane < nr && rcnt[lane]) atomicAdd(&a.counts[gid], (unsigned long long)rcnt[lane]);
+    if (per_graph) add_graph_counts(a, lane < nr ? rcnt[lane] : 0u, gid, lane);
     wave_sync();
   }
   const uint64_t s = wave_sum_u64(acc);
@@ -1279,12 +1330,8 @@ __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint3
   uint32_t *adj = meta + SMALL4_META_WORDS;
   const bool per_graph = a.graph_of != nullptr;
   uint64_t acc = 0;
-  for (;;) {
-    uint32_t c0 = 0;
-    if (lane == 0) c0 = atomicAdd(a.next, 64u);
-    c0 = rfl(c0);
-    if (c0 >= nrows) break;
-    const uint32_t nr = min(64u, nrows - c0);
+  uint32_t qs = 0, c0 = 0, nr = 0;
+  while (take_rows(a, nrows, lane, qs, c0, nr)) {
     uint32_t off = 0, gid = 0, d = 0;
     if (lane < nr) {
       const uint32_t i = rows[c0 + lane];
@@ -1374,7 +1421,7 @@ __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint3
       pend = pend && !sel;
     }
     wave_sync();
-    if (per_graph && lane < nr && rcnt[lane]) atomicAdd(&a.counts[gid], (unsigned long long)rcnt[lane]);
+    if (per_graph) add_graph_counts(a, lane < nr ? rcnt[lane] : 0u, gid, lane);
     wave_sync();
   }
   const uint64_t s = wave_sum_u64(acc);
@@ -2188,6 +2235,29 @@ __global__ void k_class_flag(const uint32_t *cls, uint32_t n, uint32_t c, uint32
     flag[i] = cls[i] == c ? 1u : 0u;
 }
 
+// per short row of a batch arena's list, its edge-centric items C(d, 2) (the wedges
+// k_small_rows tests, the pairs of k_small_cliques4's local graph): the XCD queues' weights
+__global__ void k_short_items(const uint32_t *rows, uint32_t nr, const uint32_t *card, uint32_t *items) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < nr; p += gridDim.x * blockDim.x) {
+    const uint32_t d = card[rows[p]];
+    items[p] = (d * (d - 1u)) / 2u;
+  }
+}
+
+// the bounds of the 8 XCD queues over a short-row list of nr rows: queue q starts at the
+// first row whose exclusive item prefix reaches q/8 of the total (pre holds nr + 1 prefixes)
+__global__ void k_short_cuts(const uint64_t *pre, uint32_t nr, uint32_t *xq) {
+  const uint32_t q = threadIdx.x;
+  if (q > 8u) return;
+  const uint64_t target = (pre[nr] / 8u) * q + ((pre[nr] % 8u) * q) / 8u;
+  uint32_t lo = 0, hi = nr;
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (pre[mid] < target) lo = mid + 1u; else hi = mid;
+  }
+  xq[q] = (q == 8u) ? nr : lo;
+}
+
 __global__ void k_scatter_small(const uint32_t *is_small, const uint64_t *pos, uint32_t n, uint32_t *rows) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     if (is_small[i]) rows[pos[i]] = i;
@@ -2245,7 +2315,8 @@ TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts
     if (opts->local_max < 0) tp.local_max = 0;
     else if (opts->local_max > 0 && tp.local_max)
       tp.local_max = (uint32_t)std::min<int32_t>(opts->local_max, (int32_t)LOCAL4_MAX);
-    else if (opts->bstat_split > 0) tp.bstat_j = (uint32_t)std::min(std::max(opts->bstat_split, 64), 65536);
+    if (opts->bstat_split > 0) tp.bstat_j = (uint32_t)std::min(std::max(opts->bstat_split, 64), 65536);
+    if (opts->short_queues < 0) tp.short_queues = false;
   }
   // the task kernels are built for at most 256 threads (4 waves) per block, and a
   // wave's slice holds at most 16384 words (64 KB)
@@ -2350,6 +2421,21 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
                                                 s.small_rows.as<uint32_t>());
   s.n_small_rows = nsmall;
   s.small_deg = tp.small_deg;
+  // batch arenas: the list (in row order, so graph by graph) cut into 8 ranges of equal
+  // items, one queue per XCD; a range then keeps its L2 on the few graphs it is working on
+  s.xq.release();
+  if (nsmall && m->dgraph_of.p) {
+    DevBuf items;
+    EH_CHECK(items.alloc(4ull * nsmall + 8));
+    k_short_items<<<grid_for(nsmall), 256, 0, st>>>(s.small_rows.as<uint32_t>(), (uint32_t)nsmall,
+                                                    m->dcard.as<uint32_t>(), items.as<uint32_t>());
+    k_zero_u32<<<1, 64, 0, st>>>(items.as<uint32_t>() + nsmall, 1);
+    EH_CHECK(prim_exscan_u32_to_u64(ctx, items.as<uint32_t>(), spos.as<uint64_t>(), nsmall + 1));
+    EH_CHECK(s.xq.alloc(4 * XQ_WORDS));
+    k_zero_u32<<<1, 256, 0, st>>>(s.xq.as<uint32_t>(), XQ_WORDS);
+    k_short_cuts<<<1, 64, 0, st>>>(spos.as<uint64_t>(), (uint32_t)nsmall, s.xq.as<uint32_t>());
+    EH_HIP(hipGetLastError());
+  }
   EH_CHECK(prim_sort_u64(ctx, key.as<uint64_t>(), key2.as<uint64_t>(), n, 64));
   EH_CHECK(ntsorted.alloc(4ull * n + 8));
   EH_CHECK(toff.alloc(8ull * n + 16));
@@ -2644,10 +2730,11 @@ eh_status launch_count(eh_ctx *ctx, const CountArgs &base, const uint4 *tasks, u
 
 template <int K, bool W>
 eh_status launch_small(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows, uint64_t nrows, uint32_t *d_next,
-                       PartEv *pe) {
+                       uint32_t *xq, PartEv *pe) {
   if (nrows == 0) return EH_OK;
   CountArgs a = base;
   a.next = d_next;
+  a.xq = xq;
   const int waves = 4, threads = 256;
   const size_t lds_bytes =
       (size_t)(K == 4 ? SMALL4_META_WORDS + SMALL_ADJ_WORDS : SMALL_META_WORDS) * 4 * waves;
@@ -2656,7 +2743,8 @@ eh_status launch_small(eh_ctx *ctx, const CountArgs &base, const uint32_t *rows,
   uint64_t blocks = (uint64_t)per_cu * (ctx->num_cus ? ctx->num_cus : 256);
   const uint64_t need = ceil_div(ceil_div(nrows, 64), (uint64_t)waves);
   if (blocks > need) blocks = need;
-  k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
+  if (xq) k_zero_u32<<<1, 256, 0, ctx->stream>>>(xq + XQ_HEADS, 8 * XQ_STRIDE);
+  else k_zero_u32<<<1, 64, 0, ctx->stream>>>(d_next, 1);
   part_begin(ctx, pe);
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(threads), lds_bytes, ctx->stream, a, rows, (uint32_t)nrows);
   EH_HIP(hipGetLastError());
@@ -2758,6 +2846,10 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
   // d_next[7] is the pass's error word (the queue heads use d_next[0..4]); the caller
   // clears it before the pass and reads it after
   a.err = d_next + 7;
+  a.xq = nullptr;
+  // a batch arena's short rows from its per-XCD queues (the schedule's own list only)
+  uint32_t *xq = (hs && tp.short_queues && hs->xq.p && small_rows == hs->small_rows.as<uint32_t>())
+                     ? hs->xq.as<uint32_t>() : nullptr;
   a.jstart = jstart;
   a.lo16 = lo16;
   a.task_cycles = reinterpret_cast<unsigned long long *>(d_task_cycles);
@@ -2782,7 +2874,7 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
     return EH_ERR_INVALID;
   }
   if (k == 4) {
-    EH_CHECK((launch_small<4, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2, pe(EH_PART_SHORT_ROWS))));
+    EH_CHECK((launch_small<4, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2, xq, pe(EH_PART_SHORT_ROWS))));
     EH_CHECK((launch_local<LOCAL4_MID, false>(m->ctx, a, local_rows0, n_local0, d_next + 3, pe(EH_PART_LOCAL_MID))));
     EH_CHECK((launch_local<LOCAL4_MAX, true>(m->ctx, a, local_rows1, n_local1, d_next + 4, pe(EH_PART_LOCAL_BIG))));
     EH_CHECK((launch_count<4, false>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0,
@@ -2790,13 +2882,13 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
     EH_CHECK((launch_count<4, false>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,
                                      n_big, pe(EH_PART_TASKS))));
   } else if (weighted) {
-    EH_CHECK((launch_small<3, true>(m->ctx, a, small_rows, n_small_rows, d_next + 2, pe(EH_PART_SHORT_ROWS))));
+    EH_CHECK((launch_small<3, true>(m->ctx, a, small_rows, n_small_rows, d_next + 2, xq, pe(EH_PART_SHORT_ROWS))));
     EH_CHECK((launch_count<3, true>(m->ctx, a, tasks_big, n_big, lds_big, tp.big_waves, d_next, 0,
                                     pe(EH_PART_TASKS_BIG))));
     EH_CHECK((launch_count<3, true>(m->ctx, a, tasks_small, n_small, lds_small, tp.waves_per_block, d_next + 1,
                                     n_big, pe(EH_PART_TASKS))));
   } else {
-    EH_CHECK((launch_small<3, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2, pe(EH_PART_SHORT_ROWS))));
+    EH_CHECK((launch_small<3, false>(m->ctx, a, small_rows, n_small_rows, d_next + 2, xq, pe(EH_PART_SHORT_ROWS))));
     EH_CHECK(launch_bstat(m->ctx, a, btasks, n_btasks, in_pairs, bstat_j, d_next + 3, pe(EH_PART_BSTAT)));
     if (hs && hs->n_htasks) {
       EH_CHECK(launch_bshl(m->ctx, a, hs->htasks.as<uint4>(), hs->n_htasks, hs->hpairs.as<uint4>(),

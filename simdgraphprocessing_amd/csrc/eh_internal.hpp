@@ -129,6 +129,7 @@ struct TuneParams {
                               // those pairs B-stationary, k_bshl (0: off)
   uint32_t grab = 2;          // tasks a wave takes from the queue per atomic
   uint32_t local_max = 0;     // 4-cliques: rows above small_deg and up to this run k_local4 (0: off)
+  bool short_queues = true;   // batch arenas: short rows from the per-XCD queues (Schedule::xq)
 };
 
 // device-time marks around one kernel class of a counting pass
@@ -195,6 +196,9 @@ struct Schedule {
   uint64_t n_htasks = 0;
   uint32_t bsh_longest = 0;         // the longest of those B rows (ids)
   DevBuf local_rows[2];             // rows of the two k_local4 classes (<= 128, <= 512 entries)
+  // batch arenas: small_rows cut into 8 ranges of equal wedge work, xq[0..8] their bounds and
+  // the queue heads of a pass, each on a line of its own (one queue per XCD; empty for a single graph)
+  DevBuf xq;
   uint64_t n_local[2] = {0, 0};
   uint64_t n_btasks = 0, n_in_pairs = 0;
 };

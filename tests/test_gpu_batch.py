@@ -62,3 +62,22 @@ def test_batch_pass_options_and_recapture(ctx):
     assert t is None and np.array_equal(k, k4)
     t, k, _ = b.count("persistent", "plain")
     assert np.array_equal(t, tri) and np.array_equal(k, k4)
+
+
+@pytest.mark.parametrize("ng", [1, 3, 48])
+def test_batch_short_queues(ctx, ng):
+    """The short rows taken from the 8 per-XCD queues (the default: the row list cut into 8
+    ranges of equal wedge work) and from one queue count the same, and both equal the
+    oracle's; with 1 or 3 graphs several ranges are empty or hold a few rows."""
+    seed = 4
+    b = ctx.generate_batch(seed, ng)
+    t_q, k_q, _ = b.count("persistent", "tuned")
+    t_1, k_1, _ = b.count("persistent", "tuned", short_queues=-1)
+    t_q2, k_q2, _ = b.count("persistent", "tuned")  # the graph recaptured with the queues again
+    assert np.array_equal(t_q, t_1) and np.array_equal(k_q, k_1)
+    assert np.array_equal(t_q, t_q2) and np.array_equal(k_q, k_q2)
+    for g in range(ng):
+        n, off, nbr = oracle_graph(seed, g)
+        om = O.Matrix(n, off, nbr, "hybrid")
+        assert t_q[g] == om.triangles(4)[0], g
+        assert k_q[g] == om.cliques(4, 4)[0], g
