@@ -273,6 +273,12 @@ def config_c4(ctx, with_cpu):
            "triangles_total": int(tri.sum()), "four_cliques_total": int(k4.sum()),
            "alg_bytes_tri": tb, "alg_bytes_k4": kb}
     out.update(byte_figures(tb + kb, ms_pers, traffic_record("c4", "hybrid", params)))
+    # the arena's layouts under the learned selection and every rule (rebuilt in place; the
+    # persistent passes at the same shapes, each with its build time)
+    from simdgraphprocessing_amd import selector
+    if os.path.exists(selector.PARAMS_PATH):
+        ctx.set_selector(selector.load())
+        out["selector"] = selector.compare_batch(ctx, b, steps=5, log=log)
     if with_cpu:
         # all 8192 counts against the oracle in one pass over the union graph
         import oracle as O

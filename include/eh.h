@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define EH_ABI_VERSION 4
+#define EH_ABI_VERSION 5
 
 typedef enum {
   EH_OK = 0,
@@ -379,6 +379,16 @@ eh_status eh_batch_graph_download(eh_batch *b, uint32_t g, uint64_t *row_off, ui
  * the graph is recorded again whenever either pass changes. */
 eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tri_opts,
                          const eh_run_opts *k4_opts, uint64_t *tri, uint64_t *k4, double *ms);
+/* No reference counterpart: build the arena's layouts again from the same graphs, under
+ * `policy`, or with the per-row arms row_types (total_vertices entries, as
+ * eh_matrix_build_typed takes them; policy is then ignored).  The schedules, the
+ * captured graph and the per-graph plans are built again on the next eh_batch_count. */
+eh_status eh_batch_rebuild(eh_batch *b, eh_layout_policy policy, const uint8_t *row_types);
+/* No reference counterpart: the arena's oriented matrix (rows are the union graph's
+ * vertices, every graph's a contiguous range), for eh_matrix_info_get, eh_matrix_row_types,
+ * eh_matrix_features and eh_matrix_download.  Borrowed: valid until eh_batch_rebuild or
+ * eh_batch_free, and never passed to eh_matrix_free. */
+eh_status eh_batch_matrix(eh_batch *b, eh_matrix **out);
 /* No reference counterpart: the algorithmic bytes of the batch's passes, summed over its
  * graphs (B_TC of the triangle pass, B_4C of the 4-clique pass, as in eh_run_stats). */
 eh_status eh_batch_alg_bytes(eh_batch *b, uint64_t *tri_bytes, uint64_t *k4_bytes);
@@ -416,7 +426,8 @@ eh_status eh_selector_set_rule(eh_ctx *ctx, uint32_t guard_words, int32_t fixed_
  * above the split are B-stationary.  Unoriented matrices and batch arenas always count every
  * use.  No counterpart in the reference. */
 eh_status eh_selector_set_schedule(eh_ctx *ctx, int32_t bstat_split, int32_t bsh_max);
-/* per-row features (n x 16 floats) of an oriented matrix's rows */
+/* per-row features (n x 16 floats) of an oriented matrix's rows; in a batch arena each row's
+ * ids, index and n are its own graph's, and feature 15 is 0 (1 for a single graph) */
 eh_status eh_matrix_features(eh_matrix *m, float *features);
 /* A matrix whose rows take the caller's arms (row_types: n values 0..3; a bitset only within
  * the guard of eh_selector_set_rule, rows of fewer than two ids uint): the selector's labels

@@ -96,6 +96,7 @@ EXPORTS = [
     "eh_matrix_download", "eh_matrix_row_types", "eh_matrix_free", "eh_intersect", "eh_intersect_rows",
     "eh_intersect_rows_static", "eh_intersect_row_sets", "eh_union", "eh_difference", "eh_triangle_count", "eh_clique_count", "eh_bfs_path", "eh_count_profile", "eh_batch_generate", "eh_batch_info",
     "eh_batch_graph_params", "eh_batch_graph_download", "eh_batch_count", "eh_batch_alg_bytes", "eh_batch_free",
+    "eh_batch_rebuild", "eh_batch_matrix",
     "eh_selector_set_params", "eh_selector_get_params", "eh_selector_set_rule", "eh_selector_set_schedule",
     "eh_matrix_features",
     "eh_matrix_build_typed", "eh_mlp_forward", "eh_mlp_loss_grad", "eh_selector_train",
@@ -178,6 +179,8 @@ def lib():
     L.eh_batch_alg_bytes.argtypes = [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.eh_batch_free.argtypes = [_vp]
     L.eh_batch_free.restype = None
+    L.eh_batch_rebuild.argtypes = [_vp, C.c_int, _vp]
+    L.eh_batch_matrix.argtypes = [_vp, _pp]
     L.eh_selector_set_params.argtypes = [_vp, _vp]
     L.eh_selector_get_params.argtypes = [_vp, _vp]
     L.eh_selector_set_rule.argtypes = [_vp, C.c_uint32, C.c_int32]
@@ -449,15 +452,16 @@ class Graph:
 
 
 class SparseMatrix:
-    """Built per-row layouts on the device (eh_matrix)."""
+    """Built per-row layouts on the device (eh_matrix).  owner: the batch a borrowed arena
+    matrix belongs to (kept alive here, and the matrix is not freed)."""
 
-    def __init__(self, ctx, h):
-        self.ctx, self.h = ctx, h
+    def __init__(self, ctx, h, owner=None):
+        self.ctx, self.h, self.owner = ctx, h, owner
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self, "owner", None) is None:
             lib().eh_matrix_free(self.h)
-            self.h = None
+        self.h = None
 
     def info(self):
         mi = MatrixInfo()
@@ -583,6 +587,21 @@ class Batch:
         e = C.c_uint64()
         _check(lib().eh_batch_info(self.h, C.byref(g), C.byref(v), C.byref(e)))
         return {"graphs": g.value, "vertices": v.value, "edges": e.value}
+
+    def rebuild(self, policy="hybrid", types=None):
+        """Build the arena's layouts again under `policy`, or with the per-row arms `types`
+        (a uint8 per arena row, as build_matrix_typed takes them).  Matrices borrowed from the
+        batch before (``matrix()``) are invalid afterwards."""
+        t = None if types is None else np.ascontiguousarray(types, np.uint8)
+        if t is not None and len(t) != self.info()["vertices"]:
+            raise ValueError("one type per arena row")
+        _check(lib().eh_batch_rebuild(self.h, LAYOUTS[policy], _ptr(t)))
+
+    def matrix(self):
+        """The arena's oriented matrix (borrowed: valid until rebuild)."""
+        h = C.c_void_p()
+        _check(lib().eh_batch_matrix(self.h, C.byref(h)))
+        return SparseMatrix(self.ctx, h, owner=self)
 
     def alg_bytes(self):
         """Algorithmic bytes of the triangle and 4-clique passes, summed over the graphs."""

@@ -175,6 +175,37 @@ extern "C" eh_status eh_batch_generate(eh_ctx *ctx, uint64_t seed, uint32_t ngra
   return EH_OK;
 }
 
+extern "C" eh_status eh_batch_rebuild(eh_batch *b, eh_layout_policy policy, const uint8_t *row_types) {
+  if (!b || !b->g) { eh_set_error("eh_batch_rebuild: null batch"); return EH_ERR_INVALID; }
+  eh_ctx *ctx = b->ctx;
+  EH_HIP(hipSetDevice(ctx->device));
+  DevBuf dt;
+  if (row_types) {
+    // per-row arms as eh_matrix_build_typed takes them (the LEARNED policy's guard applies)
+    EH_CHECK(dt.alloc((size_t)b->nv_total + 16));
+    EH_HIP(hipMemcpy(dt.p, row_types, b->nv_total, hipMemcpyHostToDevice));
+    policy = EH_LAYOUT_LEARNED;
+  }
+  eh_matrix *M = nullptr;
+  EH_CHECK(matrix_build_dev(ctx, b->g->n, b->g->off.as<uint64_t>(), b->g->nbr.as<uint32_t>(), b->g->entries,
+                            EH_FILTER_LOWER, nullptr, policy, b->vgraph.as<uint32_t>(), b->g->symmetric, &M,
+                            row_types ? dt.as<uint8_t>() : nullptr));
+  // the captured graph and the per-graph plans read the old matrix's buffers: record them again
+  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  b->graph_key[0] = b->graph_key[1] = PassKey();
+  b->plan[0].valid = b->plan[1].valid = false;
+  eh_matrix_free(b->M);
+  b->M = M;
+  return EH_OK;
+}
+
+extern "C" eh_status eh_batch_matrix(eh_batch *b, eh_matrix **out) {
+  if (!b || !out || !b->M) { eh_set_error("eh_batch_matrix: null argument"); return EH_ERR_INVALID; }
+  *out = b->M;
+  return EH_OK;
+}
+
 extern "C" eh_status eh_batch_info(eh_batch *b, uint32_t *ngraphs, uint64_t *total_vertices,
                                    uint64_t *total_edges) {
   if (!b) { eh_set_error("null batch"); return EH_ERR_INVALID; }

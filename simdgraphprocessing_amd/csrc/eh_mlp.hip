@@ -105,15 +105,19 @@ struct RowStats {
 // trained on small graphs reads a large graph's rows on the same scale.  The absolute sizes
 // saturate where the training graphs (RMAT 18-21) end -- a length of FEAT_CARD_CAP, FEAT_USES_CAP
 // uses -- so a larger graph's heaviest rows read as the heaviest rows the network has seen
-// instead of outside its range:
+// instead of outside its range.  In a batch arena, ids, i and n are those of the row's own
+// graph (ids less the graph's first vertex), so an arena row reads as the same row of its graph
+// alone; feature 15 tells the two regimes apart (the arena's rows are read by the batch passes'
+// edge-centric wedge queries, a single graph's by its tuned pass):
 //   0 log length            1 log span / n         2 density (ids per id of span)
 //   3 16-bit partitions per id   4 log uses        5 log (uses x length)
 //   6 log bitset words per id    7 share of ids below n / 64
 //   8 log span per id       9 largest gap / span   10 row id / n
 //   11 first id / n         12 last id / n         13 middle id / n
-//   14 share of ids in the first quarter of the span   15 constant 1
+//   14 share of ids in the first quarter of the span   15 1 (one graph) or 0 (batch arena)
 constexpr float FEAT_CARD_CAP = 1023.0f, FEAT_USES_CAP = 16383.0f;
-__device__ __forceinline__ void write_features(const RowStats &r, uint32_t u_raw, uint32_t i, uint32_t n, float *out) {
+__device__ __forceinline__ void write_features(const RowStats &r, uint32_t u_raw, uint32_t i, uint32_t n, float flag,
+                                               float *out) {
   float f[NI];
   for (int q = 0; q < NI; q++) f[q] = 0.0f;
   const float u = fminf((float)u_raw, FEAT_USES_CAP), fn = (float)n;
@@ -136,16 +140,17 @@ __device__ __forceinline__ void write_features(const RowStats &r, uint32_t u_raw
   f[4] = lg1(u) / 24.0f;
   f[5] = lg1(u * fminf((float)r.c, FEAT_CARD_CAP)) / 40.0f;
   f[10] = (float)i / fn;
-  f[15] = 1.0f;
+  f[15] = flag;
   float4 *o = reinterpret_cast<float4 *>(out);
 #pragma unroll
   for (int q = 0; q < NI / 4; q++) o[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
 }
 
 // the same 16 features into an LDS queue entry (word stores: entries are an odd stride apart)
-__device__ __forceinline__ void write_features_lds(const RowStats &r, uint32_t u, uint32_t i, uint32_t n, float *out) {
+__device__ __forceinline__ void write_features_lds(const RowStats &r, uint32_t u, uint32_t i, uint32_t n, float flag,
+                                                   float *out) {
   float f[NI];
-  write_features(r, u, i, n, f);
+  write_features(r, u, i, n, flag, f);
 #pragma unroll
   for (int q = 0; q < NI; q++) out[q] = f[q];
 }
@@ -162,18 +167,19 @@ __device__ __forceinline__ void wave_sync() {
 
 constexpr uint32_t FEAT_LANE_MAX = 32;  // rows up to this long are read by one lane
 
-// the statistics of a row of at most FEAT_LANE_MAX ids, read by one lane
-__device__ __forceinline__ RowStats lane_stats(const uint32_t *r, uint32_t c, uint32_t lowcut) {
+// the statistics of a row of at most FEAT_LANE_MAX ids, read by one lane; ids are taken less
+// `base` (the first vertex of the row's graph in a batch arena, else 0)
+__device__ __forceinline__ RowStats lane_stats(const uint32_t *r, uint32_t c, uint32_t lowcut, uint32_t base) {
   RowStats st = {c, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
   if (!c) return st;
-  st.lo = r[0];
-  st.hi = r[c - 1];
-  st.mid = r[c / 2];
+  st.lo = r[0] - base;
+  st.hi = r[c - 1] - base;
+  st.mid = r[c / 2] - base;
   st.parts = 1;
   const uint32_t qcut = st.lo + (st.hi - st.lo) / 4u;
   uint32_t prev = st.lo;
   for (uint32_t k = 0; k < c; k++) {
-    const uint32_t x = r[k];
+    const uint32_t x = r[k] - base;
     if (k) {
       st.parts += (x >> 16) != (prev >> 16);
       st.maxgap = max(st.maxgap, x - prev);
@@ -186,14 +192,15 @@ __device__ __forceinline__ RowStats lane_stats(const uint32_t *r, uint32_t c, ui
 }
 
 // the statistics of a longer row, read by the whole wave (every lane returns them)
-__device__ __forceinline__ RowStats wave_stats(const uint32_t *r, uint32_t c, uint32_t lane, uint32_t lowcut) {
-  RowStats st = {c, r[0], r[c - 1], r[c / 2], 0u, 0u, 0u, 0u};
+__device__ __forceinline__ RowStats wave_stats(const uint32_t *r, uint32_t c, uint32_t lane, uint32_t lowcut,
+                                               uint32_t base) {
+  RowStats st = {c, r[0] - base, r[c - 1] - base, r[c / 2] - base, 0u, 0u, 0u, 0u};
   const uint32_t qcut = st.lo + (st.hi - st.lo) / 4u;
   uint32_t parts = 0, low = 0, quart = 0, gap = 0;
   for (uint32_t k = lane; k < c; k += 64) {
-    const uint32_t x = r[k];
+    const uint32_t x = r[k] - base;
     if (k) {
-      const uint32_t pv = r[k - 1];
+      const uint32_t pv = r[k - 1] - base;
       parts += (x >> 16) != (pv >> 16);
       gap = max(gap, x - pv);
     }
@@ -213,28 +220,54 @@ __device__ __forceinline__ RowStats wave_stats(const uint32_t *r, uint32_t c, ui
   return st;
 }
 
+// spans[g] = {first vertex, end} of graph g of a batch arena (graph_of: the graph of every row,
+// ascending, every graph's rows contiguous)
+__global__ void k_graph_spans(const uint32_t *graph_of, uint32_t n, uint2 *spans) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t g = graph_of[i];
+    if (i == 0 || graph_of[i - 1] != g) spans[g].x = (uint32_t)i;
+    if (i + 1 == n || graph_of[i + 1] != g) spans[g].y = (uint32_t)i + 1u;
+  }
+}
+
+// the graph a row's ids and index are taken in: its own graph in a batch arena (graph_of set),
+// else the whole matrix
+struct RowGraph {
+  uint32_t base, n;
+};
+__device__ __forceinline__ RowGraph row_graph(const uint32_t *graph_of, const uint2 *spans, uint64_t i, uint32_t n) {
+  if (!graph_of) return RowGraph{0u, n};
+  const uint2 sp = spans[graph_of[i]];
+  return RowGraph{sp.x, sp.y - sp.x};
+}
+
 // Features of every row, at its own index.  A wave takes 64 consecutive rows: each lane reads
 // a row of at most FEAT_LANE_MAX ids itself, then the wave reads the longer ones together.
 __global__ void __launch_bounds__(256) k_features(const uint32_t *card, const uint32_t *aoff, const uint32_t *unbr,
-                                                  const uint32_t *uses, uint32_t n, float *F) {
+                                                  const uint32_t *uses, uint32_t n, const uint32_t *graph_of,
+                                                  const uint2 *spans, float *F) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwv = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const float flag = graph_of ? 0.0f : 1.0f;
   for (uint64_t base = wid * 64; base < n; base += nwv * 64) {
     const uint64_t i = base + lane;
     const bool valid = i < n;
     const uint32_t c = valid ? card[i] : 0u;
     const uint32_t u = valid ? uses[i] : 0u;
+    const RowGraph rg = valid ? row_graph(graph_of, spans, i, n) : RowGraph{0u, n};
     if (valid && c <= FEAT_LANE_MAX)
-      write_features(lane_stats(unbr + (uint64_t)aoff[i] * 4, c, n >> 6), u, (uint32_t)i, n, F + i * NI);
+      write_features(lane_stats(unbr + (uint64_t)aoff[i] * 4, c, rg.n >> 6, rg.base), u, (uint32_t)i - rg.base, rg.n,
+                     flag, F + i * NI);
     uint64_t lm = __ballot(valid && c > FEAT_LANE_MAX);
     while (lm) {
       const uint32_t q = (uint32_t)__builtin_ctzll(lm);
       lm &= lm - 1;
       const uint32_t ii = (uint32_t)base + q;
       const uint32_t cc = __builtin_amdgcn_readlane(c, q), uu = __builtin_amdgcn_readlane(u, q);
-      const RowStats st = wave_stats(unbr + (uint64_t)aoff[ii] * 4, cc, lane, n >> 6);
-      if (lane == 0) write_features(st, uu, ii, n, F + (uint64_t)ii * NI);
+      const uint32_t gb = __builtin_amdgcn_readlane(rg.base, q), gn = __builtin_amdgcn_readlane(rg.n, q);
+      const RowStats st = wave_stats(unbr + (uint64_t)aoff[ii] * 4, cc, lane, gn >> 6, gb);
+      if (lane == 0) write_features(st, uu, ii - gb, gn, flag, F + (uint64_t)ii * NI);
     }
   }
 }
@@ -332,6 +365,7 @@ __device__ __forceinline__ uint8_t pick_arm(const float *W, const float *x, uint
 
 __global__ void __launch_bounds__(256) k_select_learned(const uint32_t *card, const uint32_t *aoff,
                                                         const uint32_t *unbr, const uint32_t *uses, uint32_t n,
+                                                        const uint32_t *graph_of, const uint2 *spans,
                                                         const float *__restrict__ P, uint32_t guard, int fixed,
                                                         uint8_t *types) {
   __shared__ float Wl[NPARAM];
@@ -345,6 +379,7 @@ __global__ void __launch_bounds__(256) k_select_learned(const uint32_t *card, co
   float *QF = Qf[wv];
   uint32_t *QR = Qr[wv], *QC = Qc[wv], *QW = Qw[wv];
   uint32_t qn = 0;
+  const float flag = graph_of ? 0.0f : 1.0f;
   auto drain = [&](uint32_t cnt) {
     // one queued row per lane
     if (lane < cnt) types[QR[lane]] = pick_arm(Wl, QF + lane * SEL_STRIDE, QC[lane], QW[lane], guard);
@@ -378,11 +413,14 @@ __global__ void __launch_bounds__(256) k_select_learned(const uint32_t *card, co
     }
     const uint64_t m = __ballot(want);
     const uint32_t slot = qn + mbcnt64(m);
+    const RowGraph rg = want ? row_graph(graph_of, spans, i, n) : RowGraph{0u, n};
     if (want) {
       QR[slot] = (uint32_t)i;
       QC[slot] = c;
       QW[slot] = (r[c - 1] >> 6) - (r[0] >> 6) + 1u;
-      if (c <= FEAT_LANE_MAX) write_features_lds(lane_stats(r, c, n >> 6), u, (uint32_t)i, n, QF + slot * SEL_STRIDE);
+      if (c <= FEAT_LANE_MAX)
+        write_features_lds(lane_stats(r, c, rg.n >> 6, rg.base), u, (uint32_t)i - rg.base, rg.n, flag,
+                           QF + slot * SEL_STRIDE);
     }
     uint64_t lm = __ballot(want && c > FEAT_LANE_MAX);
     while (lm) {
@@ -390,9 +428,10 @@ __global__ void __launch_bounds__(256) k_select_learned(const uint32_t *card, co
       lm &= lm - 1;
       const uint32_t cc = __builtin_amdgcn_readlane(c, q), uu = __builtin_amdgcn_readlane(u, q);
       const uint32_t sq = __builtin_amdgcn_readlane(slot, q);
+      const uint32_t gb = __builtin_amdgcn_readlane(rg.base, q), gn = __builtin_amdgcn_readlane(rg.n, q);
       const uint32_t ii = (uint32_t)base + q;
-      const RowStats st = wave_stats(unbr + (uint64_t)aoff[ii] * 4, cc, lane, n >> 6);
-      if (lane == 0) write_features_lds(st, uu, ii, n, QF + sq * SEL_STRIDE);
+      const RowStats st = wave_stats(unbr + (uint64_t)aoff[ii] * 4, cc, lane, gn >> 6, gb);
+      if (lane == 0) write_features_lds(st, uu, ii - gb, gn, flag, QF + sq * SEL_STRIDE);
     }
     qn += (uint32_t)__popcll(m);
     wave_sync();
@@ -536,15 +575,30 @@ static eh_status uses_dev(eh_matrix *m, const uint64_t *sym_off, DevBuf &uses, b
   return EH_OK;
 }
 
+// {first vertex, end} of every graph of a batch arena; spans stays empty for a single graph
+static eh_status graph_spans_dev(eh_matrix *m, DevBuf &spans) {
+  if (!m->dgraph_of.p || m->n == 0) return EH_OK;
+  hipStream_t st = m->ctx->stream;
+  uint32_t last = 0;
+  EH_HIP(hipMemcpyAsync(&last, m->dgraph_of.as<uint32_t>() + (m->n - 1), 4, hipMemcpyDeviceToHost, st));
+  EH_HIP(hipStreamSynchronize(st));
+  EH_CHECK(spans.alloc(8ull * ((uint64_t)last + 1) + 8));
+  k_graph_spans<<<grid_for(m->n), 256, 0, st>>>(m->dgraph_of.as<uint32_t>(), m->n, spans.as<uint2>());
+  EH_HIP(hipGetLastError());
+  return EH_OK;
+}
+
 // features of every row, at its own index
-static eh_status features_dev(eh_matrix *m, DevBuf &F, DevBuf &uses) {
+static eh_status features_dev(eh_matrix *m, DevBuf &F, DevBuf &uses, DevBuf &spans) {
   hipStream_t st = m->ctx->stream;
   const uint32_t n = m->n;
   EH_CHECK(uses_dev(m, nullptr, uses));
+  EH_CHECK(graph_spans_dev(m, spans));
   EH_CHECK(F.alloc(4ull * NI * n + 64));
   k_features<<<grid_for((uint64_t)n, 256), 256, 0, st>>>(m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(),
                                                          m->dunbr.as<uint32_t>(), uses.as<uint32_t>(), n,
-                                                         F.as<float>());
+                                                         spans.p ? m->dgraph_of.as<uint32_t>() : nullptr,
+                                                         spans.as<uint2>(), F.as<float>());
   EH_HIP(hipGetLastError());
   return EH_OK;
 }
@@ -578,8 +632,11 @@ eh_status matrix_select_learned(eh_matrix *m, uint8_t *d_types, const uint64_t *
   }
   // every block stages the parameters (22 KB, from L2) once: a few thousand blocks at most
   const unsigned blocks = std::min(grid_for(n), 8192u);
+  DevBuf spans;
+  EH_CHECK(graph_spans_dev(m, spans));
   k_select_learned<<<blocks, 256, 0, st>>>(m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(), m->dunbr.as<uint32_t>(),
-                                           uses.as<uint32_t>(), n, fixed ? nullptr : ctx->mlp_dev.as<float>(),
+                                           uses.as<uint32_t>(), n, spans.p ? m->dgraph_of.as<uint32_t>() : nullptr,
+                                           spans.as<uint2>(), fixed ? nullptr : ctx->mlp_dev.as<float>(),
                                            ctx->sel_guard, ctx->sel_fixed, d_types);
   EH_HIP(hipGetLastError());
   EH_HIP(hipStreamSynchronize(st));
@@ -625,8 +682,8 @@ extern "C" eh_status eh_selector_set_schedule(eh_ctx *ctx, int32_t bstat_split, 
 extern "C" eh_status eh_matrix_features(eh_matrix *m, float *features) {
   if (!m || !features) { eh_set_error("null argument"); return EH_ERR_INVALID; }
   EH_HIP(hipSetDevice(m->ctx->device));
-  DevBuf F, uses;
-  EH_CHECK(features_dev(m, F, uses));
+  DevBuf F, uses, spans;
+  EH_CHECK(features_dev(m, F, uses, spans));
   EH_HIP(hipStreamSynchronize(m->ctx->stream));
   EH_HIP(hipMemcpy(features, F.p, 4ull * NI * m->n, hipMemcpyDeviceToHost));
   return EH_OK;

@@ -96,13 +96,13 @@ def tuned_uses(off, nbr, split=65536, bsh=512):
     return np.bincount(nbr[sel], minlength=n)
 
 
-def groups_of(card, uses, words, min_rows):
-    """Row groups (lists of row ids) by length, bitset words per id and uses; only rows of at
-    least two ids that some row holds, and groups of at least min_rows."""
+def groups_of(card, uses, words, min_rows, by_uses=True):
+    """Row groups (lists of row ids) by length, bitset words per id and (by_uses) uses; only
+    rows of at least two ids that some row holds, and groups of at least min_rows."""
     cand = (card >= 2) & (uses >= 1)
     cb = np.digitize(card, [4, 8, 16, 32, 64, 256])
     db = np.digitize(words / np.maximum(card, 1), [1.5, 8.5, 64.5])
-    ub = np.digitize(uses, [2, 8, 64, 512])
+    ub = np.digitize(uses, [2, 8, 64, 512]) if by_uses else np.zeros(len(card), np.int64)
     key = (cb * 4 + db) * 5 + ub
     out = []
     for k in np.unique(key[cand]):
@@ -236,6 +236,157 @@ def collect(ctx, scales=(18, 19, 20, 21), seeds=range(10, 20), edge_factor=16, r
                 log(f"  labelled RMAT-{scale} seed {seed}: {len(X)} rows ({time.time() - t0:.1f}s)")
             del g
     return np.concatenate(xs), np.concatenate(ts), np.concatenate(ms), np.concatenate(gs)
+
+
+# Batch arenas (eh_batch): every arena row held by another is read by the batch passes'
+# edge-centric wedge queries (the triangle pass's short rows, k_small_cliques4), where a bitset
+# B row answers a query with one probe and a uint row with a binary search, so the arms cost
+# differently there than in a single graph's A-side pairs.  The arena's rows carry feature 15 =
+# 0 and its graph-relative ids, and are labelled on arenas of their own: ER batches drawn as the
+# 4,096-graph workload is (seed 4), from other seeds.
+BATCH_SEEDS = (10, 11, 12, 13)
+BATCH_GRAPHS = 1024
+# the batch comparison adds every row a bitset (no guard), a rule the arena's small graphs can
+# afford and single large graphs cannot (its payload passes 32-bit offsets at RMAT-22)
+BATCH_RULES = RULES + (("bitset_all", "bs", None, None),)
+
+
+def batch_shapes():
+    """The committed launch shapes of the 4,096-graph workload's two passes."""
+    from .autotune import committed
+    return committed("c4")
+
+
+def _batch_pass(b, shapes):
+    t, k, ms = b.count("persistent", "tuned", tri_opts=shapes.get("tri"), k4_opts=shapes.get("k4"))
+    return np.concatenate([t, k]), ms
+
+
+def label_batch(ctx, seed, ngraphs=BATCH_GRAPHS, reps=6, min_rows=256, per_group=2000, log=None):
+    """Labels of one batch arena (seed, ngraphs ER graphs as eh_batch_generate draws them), timed
+    on the arena's persistent triangle + 4-clique passes at the committed launch shapes.
+
+    The arms interact there: the edge-centric kernels answer a wave's 64 wedge queries together,
+    and a wave whose B rows mix bitsets and uint lists runs both searches, so an arena with one
+    group of rows switched from uint to bitsets runs no faster than all uint (measured), while all
+    bitsets save 14%.  Each arm is therefore timed around an operating point, the density rule's
+    selection (the yardstick): per group (length x bitset words per id; the uses of an ER arena's
+    rows are too alike to split by) and arm, an arena rebuilt with the group in that arm and the
+    rest as the rule has them is timed alternately with the rule's arena, reps times, and a row's
+    label for an arm is its stake's share of the difference between that arm's and uint's times.
+    Build costs are left out: a batch is built once and its captured launch replayed.  Returns
+    (features, targets, weights)."""
+    shapes = batch_shapes()
+    ctx.set_selector_rule(GUARD, -1)
+    b0 = ctx.generate_batch(seed, ngraphs, policy="hybrid")
+    bv = ctx.generate_batch(seed, ngraphs, policy="hybrid")
+    M0 = b0.matrix()
+    F = M0.features()
+    base = M0.row_types()
+    card, uses, words = row_stats(M0)
+    del M0
+    n = len(card)
+    stake = (uses * card).astype(np.float64)
+    groups = groups_of(card, uses, words, min_rows, by_uses=False)
+    ref, _ = _batch_pass(b0, shapes)  # the schedules and the captured graph, then warm passes
+    k0 = float(np.median([_batch_pass(b0, shapes)[1] for _ in range(3)]))
+    bs_ok = words <= GUARD * card
+    T = np.zeros((n, 4), np.float32)
+    Mk = np.zeros((n, 4), np.float32)
+    wt = (1.0 + np.log2(1.0 + stake) / 4.0).astype(np.float32)
+    for rows in groups:
+        Mk[rows, U] = wt[rows]
+        d = {}
+        for arm in (U,) + LEARNED_ARMS:
+            sel = rows[bs_ok[rows]] if arm == 0 else rows
+            if len(sel) == 0:
+                continue
+            t = base.copy()
+            t[sel] = arm
+            bv.rebuild(types=t)
+            c, _ = _batch_pass(bv, shapes)
+            assert np.array_equal(c, ref)
+            dk = []
+            for _ in range(reps):
+                kb = _batch_pass(b0, shapes)[1]
+                kv = _batch_pass(bv, shapes)[1]
+                dk.append(kv - kb)
+            d[arm] = significant(dk, k0)
+        for arm in LEARNED_ARMS:
+            if arm not in d:
+                continue
+            sel = rows[bs_ok[rows]] if arm == 0 else rows
+            w = stake[sel] / max(stake[sel].sum(), 1.0)
+            T[sel, arm] = slog(1e6 * (d[arm] - d[U]) * w)
+            Mk[sel, arm] = wt[sel]
+    rng = np.random.default_rng(seed)
+    keep = np.zeros(n, bool)
+    for rows in groups:
+        keep[rows if len(rows) <= per_group else rng.choice(rows, per_group, replace=False)] = True
+    if log:
+        log(f"    {len(groups)} groups, {int(keep.sum())} rows labelled; density-rule batch passes {k0:.2f} ms")
+    del b0, bv
+    return F[keep], T[keep], Mk[keep]
+
+
+def collect_batch(ctx, seeds=BATCH_SEEDS, ngraphs=BATCH_GRAPHS, log=None):
+    """Engine-timed training rows from batch arenas: (features, targets, mask, seed of each row)."""
+    xs, ts, ms, gs = [], [], [], []
+    for seed in seeds:
+        t0 = time.time()
+        X, T, Mk = label_batch(ctx, seed, ngraphs, log=log)
+        xs.append(X)
+        ts.append(T)
+        ms.append(Mk)
+        gs.append(np.full(len(X), seed, np.int64))
+        if log:
+            log(f"  labelled batch seed {seed} ({ngraphs} graphs): {len(X)} rows ({time.time() - t0:.1f}s)")
+    return np.concatenate(xs), np.concatenate(ts), np.concatenate(ms), np.concatenate(gs)
+
+
+def compare_batch(ctx, b, steps=5, params=None, log=None):
+    """The persistent triangle + 4-clique passes of batch b (committed shapes) under the learned
+    layouts and under every rule of BATCH_RULES, each with its build time (the arena rebuilt in
+    place; median of 3 builds); per-graph counts must agree.  The batch is left under the
+    density rule."""
+    shapes = batch_shapes()
+    if params is not None:
+        ctx.set_selector(params)
+    res = {}
+    ref = None
+    for name, policy, guard, fixed in BATCH_RULES + (("learned", "learned", GUARD, -1),):
+        if policy == "learned":
+            ctx.set_selector_rule(guard, fixed)
+        builds = []
+        for _ in range(3):
+            b.rebuild(policy)
+            builds.append(b.matrix().info()["build_ms"])
+        info = b.matrix().info()
+        c, _ = _batch_pass(b, shapes)
+        ms = []
+        for _ in range(steps):
+            c2, m = _batch_pass(b, shapes)
+            assert np.array_equal(c2, c)
+            ms.append(m)
+        if ref is None:
+            ref = c
+        assert np.array_equal(c, ref), name
+        res[name] = {"kernel_ms": float(np.median(ms)), "build_ms": float(np.median(builds)),
+                     "rows_by_type": info["rows_by_type"]}
+        if log:
+            log(f"  batch {name}: {res[name]['kernel_ms']:.3f} ms, build {res[name]['build_ms']:.2f} ms, "
+                f"{info['rows_by_type']}")
+        ctx.set_selector_rule(GUARD, -1)
+    b.rebuild("hybrid")
+    rules = [r[0] for r in BATCH_RULES]
+    best_pass = min(rules, key=lambda k: res[k]["kernel_ms"])
+    best_total = min(rules, key=lambda k: res[k]["kernel_ms"] + res[k]["build_ms"])
+    L = res["learned"]
+    return {"regime": "batch", "shapes": shapes, "policies": res, "best_rule_pass": best_pass,
+            "learned_vs_best_rule": res[best_pass]["kernel_ms"] / L["kernel_ms"],
+            "best_rule_with_build": best_total,
+            "learned_vs_best_rule_with_build": (res[best_total]["kernel_ms"] + res[best_total]["build_ms"]) /
+                                               (L["kernel_ms"] + L["build_ms"])}
 
 
 def train(ctx, X, T, Mk, epochs=60, batch=512, lr=2e-3, seed=0):

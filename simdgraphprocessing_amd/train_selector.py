@@ -2,12 +2,14 @@
 
     python -m simdgraphprocessing_amd.train_selector [--out PATH] [--report PATH]
 
-Labels RMAT scales 18-21 x seeds 10-19 (edge factor 16) with the tuned pass's own costs,
-trains the FP32 MLP 16 -> 64 -> 64 -> 4 on the device, saves the parameters (default:
+Labels RMAT scales 18-21 x seeds 10-19 (edge factor 16) with the tuned pass's own costs, and
+batch arenas of 1,024 ER graphs (seeds 10-13) with the batch passes' own costs, trains the FP32
+MLP 16 -> 64 -> 64 -> 4 on the device, saves the parameters (default:
 simdgraphprocessing_amd/selector_params.npy), compares its choices with the rules on held-out
-rows, and evaluates triangle counting on RMAT scale 24, edge factor 16, seed 24: learned
-layouts vs every rule on the tuned schedule (pass and build times), and the plain schedule,
-with equal counts asserted.
+rows (the last RMAT seed and the last batch seed), and evaluates triangle counting on RMAT scale
+24, edge factor 16, seed 24 (learned layouts vs every rule on the tuned schedule, pass and build
+times, and the plain schedule) and the 4,096-graph batch of seed 4 (triangles + 4-cliques per
+graph, learned arena layouts vs every rule), with equal counts asserted.
 """
 import argparse
 import json
@@ -21,6 +23,9 @@ import numpy as np  # noqa: E402
 
 import simdgraphprocessing_amd as eh  # noqa: E402
 from simdgraphprocessing_amd import selector  # noqa: E402
+
+
+BATCH_SEED_BASE = 1 << 20
 
 
 def log(*a):
@@ -39,6 +44,9 @@ def main():
     ap.add_argument("--check-scales", default="21,22", help="held-out RMAT scales timed in the A-side regime")
     ap.add_argument("--save-data", default=None, help="write the collected labels (.npz) here")
     ap.add_argument("--data", default=None, help="train on labels written by --save-data instead of collecting")
+    ap.add_argument("--batch-seeds", default="10-13", help="batch arenas labelled (empty: none)")
+    ap.add_argument("--batch-graphs", type=int, default=selector.BATCH_GRAPHS)
+    ap.add_argument("--eval-batch", default="4,4096", help="seed,graphs of the batch evaluated (empty: none)")
     a = ap.parse_args()
     s0, s1 = (int(x) for x in a.seeds.split("-"))
     scales = tuple(int(x) for x in a.scales.split(","))
@@ -50,11 +58,19 @@ def main():
             X, T, Mk, seed_of = z["X"], z["T"], z["Mk"], z["seed_of"]
     else:
         X, T, Mk, seed_of = selector.collect(ctx, scales=scales, seeds=range(s0, s1 + 1), log=log)
+        if a.batch_seeds:
+            b0, b1 = (int(x) for x in a.batch_seeds.split("-"))
+            Xb, Tb, Mb, sb = selector.collect_batch(ctx, seeds=range(b0, b1 + 1), ngraphs=a.batch_graphs, log=log)
+            # batch seeds are told apart from RMAT seeds by an offset
+            X, T, Mk = np.concatenate([X, Xb]), np.concatenate([T, Tb]), np.concatenate([Mk, Mb])
+            seed_of = np.concatenate([seed_of, BATCH_SEED_BASE + sb])
         if a.save_data:
             np.savez_compressed(a.save_data, X=X, T=T, Mk=Mk, seed_of=seed_of)
     t_collect = time.time() - t0
-    # hold out the graphs of the last seed (their rows share no group with the training rows)
-    test = seed_of == s1
+    # hold out the graphs of the last RMAT seed and the last batch seed (their rows share no
+    # group with the training rows)
+    bseeds = seed_of[seed_of >= BATCH_SEED_BASE]
+    test = (seed_of == s1) | ((seed_of >= BATCH_SEED_BASE) & (seed_of == (bseeds.max() if len(bseeds) else -1)))
     log(f"collected {len(X)} rows in {t_collect:.1f}s; training {a.epochs} epochs on {int((~test).sum())} rows")
     t1 = time.time()
     params, loss = selector.train(ctx, X[~test], T[~test], Mk[~test], epochs=a.epochs, batch=512, lr=2e-3, seed=1)
@@ -88,17 +104,39 @@ def main():
             del gc
     ev = selector.evaluate(ctx, scale=a.eval_scale, seed=a.eval_seed, log=log)
     log(json.dumps(ev))
+    ev_batch = None
+    if a.eval_batch:
+        bs_, bg_ = (int(x) for x in a.eval_batch.split(","))
+        b = ctx.generate_batch(bs_, bg_, policy="hybrid")
+        ev_batch = selector.compare_batch(ctx, b, steps=5, log=log)
+        ev_batch["workload"] = f"{bg_} ER graphs, seed {bs_}, triangles + 4-cliques per graph"
+        del b
+        log(json.dumps(ev_batch))
+    # the held-out rows of each regime apart (feature 15: 1 one graph, 0 batch arena)
+    by_regime = {}
+    for name, part in (("graph", X[test][:, 15] > 0.5), ("batch", X[test][:, 15] < 0.5)):
+        if part.any():
+            Tp, Mp, Xp = T[test][part], Mk[test][part], X[test][part]
+            cp = {"learned": selector.choice_cost(Tp, Mp, learned[part])[0],
+                  "density_1_256": selector.choice_cost(Tp, Mp, selector.heuristic_choice(Xp))[0],
+                  "bitset_1_512": selector.choice_cost(Tp, Mp, selector.bitset_choice(Xp, 8))[0],
+                  "uint": 0.0, "best_arm": selector.choice_cost(Tp, Mp, oracle_best[part])[0]}
+            br = min(("density_1_256", "bitset_1_512", "uint"), key=lambda k: cp[k])
+            by_regime[name] = {"rows": int(part.sum()), "ns_vs_uint": cp, "best_rule": br,
+                               "saving_vs_best_rule": -cp["learned"] / max(-cp[br], 1e-9)}
+    log(json.dumps(by_regime))
     report = {"train_rows": int(len(X)), "scales": scales, "seeds": [s0, s1], "epochs": a.epochs,
               "train_loss": loss, "heldout_mse": mse, "collect_s": t_collect, "train_s": t_train,
               "heldout_ns_vs_uint": cost, "heldout_best_rule": best_rule,
               "heldout_saving_vs_best_rule": gain,
               "arm_histogram_learned": np.bincount(learned, minlength=4).tolist(),
               "arm_histogram_best": np.bincount(oracle_best, minlength=4).tolist(),
-              "evaluation": ev}
+              "heldout_by_regime": by_regime,
+              "evaluation": ev, "evaluation_batch": ev_batch}
     if a.report:
         with open(a.report, "w") as f:
             json.dump(report, f, indent=1)
-    log(json.dumps({k: v for k, v in report.items() if k != "evaluation"}))
+    log(json.dumps({k: v for k, v in report.items() if k not in ("evaluation", "evaluation_batch")}))
 
 
 if __name__ == "__main__":

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_status_strings_and_version():
     L = eh.lib()
-    assert L.eh_abi_version() == 4
+    assert L.eh_abi_version() == 5
     for s in range(7):
         assert L.eh_status_str(s)
 

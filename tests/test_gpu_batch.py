@@ -81,3 +81,61 @@ def test_batch_short_queues(ctx, ng):
         om = O.Matrix(n, off, nbr, "hybrid")
         assert t_q[g] == om.triangles(4)[0], g
         assert k_q[g] == om.cliques(4, 4)[0], g
+
+
+def test_arena_features_are_graph_relative(ctx):
+    """An arena row's selector features are those of the same row of its graph alone (ids,
+    index and n taken in the graph), feature 15 apart: 0 in the arena, 1 for a single graph."""
+    b = ctx.generate_batch(5, 40, policy="uint")
+    n_g, _ = b.params()
+    vbase = np.concatenate([[0], np.cumsum(n_g.astype(np.int64))])
+    F = b.matrix().features()
+    assert F.shape == (int(vbase[-1]), 16) and np.all(F[:, 15] == 0)
+    ctx.set_selector_schedule(-1)  # a single graph's every use counts, as the arena's do
+    try:
+        for g in (0, 7, 39):
+            n, off, nbr = b.graph(g)
+            Fs = ctx.matrix_from_csr(n, off, nbr, "lower", "uint").features()
+            Fa = F[vbase[g]:vbase[g + 1]]
+            assert np.all(Fs[:, 15] == 1)
+            assert np.array_equal(Fa[:, :15], Fs[:, :15]), g
+    finally:
+        ctx.set_selector_schedule(0)
+
+
+def test_batch_rebuild(ctx):
+    """The arena's layouts rebuilt in place under other policies and with per-row arms: every
+    graph's counts are unchanged in both modes (the captured graph and the per-graph plans are
+    recorded again over the new matrix), and a typed rebuild gives each row its arm (a bitset
+    only within the guard, rows of fewer than two ids uint)."""
+    from simdgraphprocessing_amd import selector
+    seed, ng = 6, 32
+    ctx.set_selector(selector.load())
+    b = ctx.generate_batch(seed, ng, policy="hybrid")
+    tri, k4, _ = b.count("persistent", "tuned")
+    tg, kg, _ = b.count("per_graph", "tuned")
+    assert np.array_equal(tri, tg) and np.array_equal(k4, kg)
+    for g in range(0, ng, 5):
+        n, off, nbr = oracle_graph(seed, g)
+        om = O.Matrix(n, off, nbr, "hybrid")
+        assert tri[g] == om.triangles(4)[0] and k4[g] == om.cliques(4, 4)[0], g
+    M = b.matrix()
+    card, _, words = selector.row_stats(M)
+    rng = np.random.default_rng(2)
+    t = rng.integers(0, 4, len(card)).astype(np.uint8)
+    for policy, types in (("uint", None), ("bs", None), ("pshort", None), (None, t), ("learned", None),
+                          ("hybrid", None)):
+        if types is not None:
+            b.rebuild(types=types)
+        else:
+            b.rebuild(policy)
+        for mode in ("persistent", "per_graph"):
+            t2, k2, _ = b.count(mode, "tuned")
+            assert np.array_equal(t2, tri) and np.array_equal(k2, k4), (policy, mode)
+        got = b.matrix().row_types()
+        if types is not None:
+            want = np.where((types == 0) & (words > selector.GUARD * card), 2, types)
+            want = np.where(card < 2, 2, want)
+            assert np.array_equal(got, want)
+        elif policy == "uint":
+            assert np.all(got == 2)

@@ -231,3 +231,26 @@ def test_committed_selector_matches_best_rule(ctx):
           ", ".join(f"{k} {v:.4e}" for k, v in cost.items()), flush=True)
     assert cl < 0 and cl <= GAIN_MIN * min(cost[best], 0.0)
     c.close()
+
+
+def test_committed_selector_on_batch_arenas(ctx):
+    """The committed network on the labels of a held-out batch arena (512 ER graphs of seed 40;
+    it trains on seeds 10-13): the arena rows' features are their own graph's and feature 15 is
+    0, and the network's arms save at least GAIN_MIN of what the best rule saves (the arena's
+    labels are the persistent passes' times around the density rule's selection)."""
+    import simdgraphprocessing_amd as eh
+    c = eh.Context(0)
+    params = selector.load()
+    F, Th, Mh = selector.label_batch(c, 40, 512)
+    assert np.all(F[:, 15] == 0)
+    y = c.mlp_forward(params, F)
+    learned = np.argmin(np.where(Mh > 0, y, 1e9), axis=1)
+    cost = {"density_1_256": selector.heuristic_choice(F), "bitset_1_512": selector.bitset_choice(F, 8),
+            "uint": np.full(len(F), selector.U)}
+    cost = {k: selector.choice_cost(Th, Mh, v)[0] for k, v in cost.items()}
+    cl = selector.choice_cost(Th, Mh, learned)[0]
+    best = min(cost, key=cost.get)
+    print(f"committed network, held-out batch: learned {cl:.4e}; " +
+          ", ".join(f"{k} {v:.4e}" for k, v in cost.items()), flush=True)
+    assert cl < 0 and cl <= GAIN_MIN * min(cost[best], 0.0)
+    c.close()
