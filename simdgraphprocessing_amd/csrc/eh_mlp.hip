@@ -326,36 +326,89 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float *__restrict__ P
 // only where its words stay within `guard` per id.
 // Each wave walks 64 consecutive rows at a time and appends its candidates' features to its own
 // LDS queue (short rows read by one lane, long rows by the wave, as in k_features); whenever 64
-// are queued, every lane evaluates the network for one of them with the parameters the block
-// staged in LDS (every lane reads the same word), then the queue shifts down.  No features
-// leave the CU and no candidate list is built in HBM.
+// are queued, the wave evaluates the network for all 64 at once on the matrix cores
+// (mlp_mfma64) with the parameters the block staged in LDS, then the queue shifts down.  No
+// features leave the CU and no candidate list is built in HBM.
 constexpr uint32_t SEL_Q = 128;        // queue entries per wave (below 64 + one step's 64)
 constexpr uint32_t SEL_STRIDE = NI + 1;  // LDS words per entry (odd: lanes read distinct banks)
+static_assert(NI == 16 && NH == 64 && NO == 4, "the matrix-core evaluation's tiling");
 
-__device__ __forceinline__ uint8_t pick_arm(const float *W, const float *x, uint32_t c, uint32_t nw, uint32_t guard) {
-  float xi[NI], h1[NH], y[NO];
+// The network over a wave's 64 queued rows on the matrix cores (v_mfma_f32_16x16x4_f32, FP32
+// in and out), in transposed form so that no activation crosses lanes: H1^T = W1 X^T, H2^T =
+// W2 relu(H1^T), Y^T = W3 relu(H2^T), each a sum of 16x16x4 products.  In a product's result
+// lane l (q = l / 16, c = l % 16) holds rows 4q .. 4q + 3 of column c; the next layer takes
+// those four rows as the k-slot q of four products, so its B operand is the previous result's
+// register as it stands (k-slot q of the product for register r of hidden tile hb is hidden
+// unit 16 hb + 4q + r), and only the weights are read in a matching order.  The block stages the
+// weights in that order (mlp_mfma_order: 96 A operands x 64 lanes, then b1, b2, b3), so every
+// operand is one conflict-free LDS word per lane.  Rows of X^T's columns are the queued rows
+// (x = 16 xb + c), so a queue slot past the rows being drained only affects its own column.
+// y[xb] of lanes 0..15 returns the 4 outputs of row 16 xb + lane.
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr uint32_t MM_A1 = 0, MM_A2 = 16, MM_A3 = 80, MM_OPS = 96, MM_B1 = MM_OPS * 64, MM_B2 = MM_B1 + NH,
+                   MM_B3 = MM_B2 + NH, MM_WORDS = MM_B3 + NO;
+
+__device__ __forceinline__ void mlp_mfma64(const float *Wm, const float *QF, uint32_t lane, v4f (&y)[4]) {
+  const uint32_t q = lane >> 4, c = lane & 15u;
+  v4f h1[4][4];  // [hidden tile][row tile]
 #pragma unroll
-  for (int q = 0; q < NI; q++) xi[q] = x[q];
+  for (int hb = 0; hb < 4; hb++)
 #pragma unroll
-  for (int o = 0; o < NH; o++) {
-    float a = W[OB1 + o];
+    for (int xb = 0; xb < 4; xb++)
 #pragma unroll
-    for (int i = 0; i < NI; i++) a += W[OW1 + o * NI + i] * xi[i];
-    h1[o] = relu(a);
+      for (int r = 0; r < 4; r++) h1[hb][xb][r] = Wm[MM_B1 + hb * 16 + 4 * q + r];
+#pragma unroll
+  for (int kc = 0; kc < NI / 4; kc++) {
+    float b[4];
+#pragma unroll
+    for (int xb = 0; xb < 4; xb++) b[xb] = QF[(xb * 16 + c) * SEL_STRIDE + 4 * kc + q];
+#pragma unroll
+    for (int hb = 0; hb < 4; hb++) {
+      const float a = Wm[(MM_A1 + hb * 4 + kc) * 64 + lane];
+#pragma unroll
+      for (int xb = 0; xb < 4; xb++) h1[hb][xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[xb], h1[hb][xb], 0, 0, 0);
+    }
   }
 #pragma unroll
-  for (int k = 0; k < NO; k++) y[k] = W[OB3 + k];
-  for (int o = 0; o < NH; o++) {
-    float a = W[OB2 + o];
+  for (int hb = 0; hb < 4; hb++)
 #pragma unroll
-    for (int i = 0; i < NH; i++) a += W[OW2 + o * NH + i] * h1[i];
-    const float h2 = relu(a);
+    for (int xb = 0; xb < 4; xb++)
 #pragma unroll
-    for (int k = 0; k < NO; k++) y[k] += W[OW3 + k * NH + o] * h2;
+      for (int r = 0; r < 4; r++) h1[hb][xb][r] = relu(h1[hb][xb][r]);
+#pragma unroll
+  for (int xb = 0; xb < 4; xb++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) y[xb][r] = q == 0 ? Wm[MM_B3 + r] : 0.0f;
+#pragma unroll
+  for (int ob = 0; ob < 4; ob++) {
+    v4f h2[4];
+#pragma unroll
+    for (int xb = 0; xb < 4; xb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) h2[xb][r] = Wm[MM_B2 + ob * 16 + 4 * q + r];
+#pragma unroll
+    for (int hb = 0; hb < 4; hb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const float a = Wm[(MM_A2 + (ob * 4 + hb) * 4 + r) * 64 + lane];
+#pragma unroll
+        for (int xb = 0; xb < 4; xb++) h2[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, h1[hb][xb][r], h2[xb], 0, 0, 0);
+      }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const float a = Wm[(MM_A3 + ob * 4 + r) * 64 + lane];
+#pragma unroll
+      for (int xb = 0; xb < 4; xb++) y[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, relu(h2[xb][r]), y[xb], 0, 0, 0);
+    }
   }
+}
+
+// the arm with the smallest prediction among bitset (within the guard), uint and uint searched
+__device__ __forceinline__ uint8_t arm_of(const v4f &y, uint32_t c, uint32_t nw, uint32_t guard) {
   const bool bs_ok = (uint64_t)nw <= (uint64_t)guard * c;
   uint8_t pick = EH_U;
   float best = 3.0e38f;
+#pragma unroll
   for (uint32_t k = 0; k < NO; k++) {
     if (k == EH_PS || (k == EH_BS && !bs_ok)) continue;
     if (y[k] < best) { best = y[k]; pick = (uint8_t)k; }
@@ -368,12 +421,12 @@ __global__ void __launch_bounds__(256) k_select_learned(const uint32_t *card, co
                                                         const uint32_t *graph_of, const uint2 *spans,
                                                         const float *__restrict__ P, uint32_t guard, int fixed,
                                                         uint8_t *types) {
-  __shared__ float Wl[NPARAM];
+  __shared__ float Wm[MM_WORDS];
   __shared__ float Qf[4][SEL_Q * SEL_STRIDE];
   __shared__ uint32_t Qr[4][SEL_Q], Qc[4][SEL_Q], Qw[4][SEL_Q];
   const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
   if (fixed < 0) {
-    for (uint32_t q = threadIdx.x; q < (uint32_t)NPARAM; q += blockDim.x) Wl[q] = P[q];
+    for (uint32_t q = threadIdx.x; q < MM_WORDS; q += blockDim.x) Wm[q] = P[q];
   }
   __syncthreads();
   float *QF = Qf[wv];
@@ -381,8 +434,14 @@ __global__ void __launch_bounds__(256) k_select_learned(const uint32_t *card, co
   uint32_t qn = 0;
   const float flag = graph_of ? 0.0f : 1.0f;
   auto drain = [&](uint32_t cnt) {
-    // one queued row per lane
-    if (lane < cnt) types[QR[lane]] = pick_arm(Wl, QF + lane * SEL_STRIDE, QC[lane], QW[lane], guard);
+    // the first cnt queued rows (row 16 xb + lane's outputs in lane `lane` < 16)
+    v4f y[4];
+    mlp_mfma64(Wm, QF, lane, y);
+#pragma unroll
+    for (uint32_t xb = 0; xb < 4; xb++) {
+      const uint32_t e = xb * 16u + lane;
+      if (lane < 16u && e < cnt) types[QR[e]] = arm_of(y[xb], QC[e], QW[e], guard);
+    }
     wave_sync();
     const uint32_t rest = qn - cnt;
     for (uint32_t e = lane; e < rest; e += 64) {
@@ -609,6 +668,30 @@ static eh_status mlp_forward_dev(eh_ctx *ctx, const float *d_params, const float
   return EH_OK;
 }
 
+// the parameters in the order mlp_mfma64 reads them: for each of its 96 A operands the 64
+// lanes' values (lane l: q = l / 16, c = l % 16) -- W1 per (hidden tile hb, k chunk kc):
+// W1[16 hb + c][4 kc + q]; W2 per (out tile ob, hidden tile hb, register r): W2[16 ob + c][16 hb +
+// 4 q + r]; W3 per (ob, r): W3[c][16 ob + 4 q + r] for c < 4, else 0 -- then b1, b2, b3
+static void mlp_mfma_order(const float *P, std::vector<float> &wm) {
+  wm.assign(MM_WORDS, 0.0f);
+  for (uint32_t lane = 0; lane < 64; lane++) {
+    const uint32_t q = lane >> 4, c = lane & 15u;
+    for (uint32_t hb = 0; hb < 4; hb++)
+      for (uint32_t kc = 0; kc < NI / 4; kc++)
+        wm[(MM_A1 + hb * 4 + kc) * 64 + lane] = P[OW1 + (hb * 16 + c) * NI + 4 * kc + q];
+    for (uint32_t ob = 0; ob < 4; ob++)
+      for (uint32_t hb = 0; hb < 4; hb++)
+        for (uint32_t r = 0; r < 4; r++)
+          wm[(MM_A2 + (ob * 4 + hb) * 4 + r) * 64 + lane] = P[OW2 + (ob * 16 + c) * NH + hb * 16 + 4 * q + r];
+    for (uint32_t ob = 0; ob < 4; ob++)
+      for (uint32_t r = 0; r < 4; r++)
+        wm[(MM_A3 + ob * 4 + r) * 64 + lane] = c < (uint32_t)NO ? P[OW3 + c * NH + ob * 16 + 4 * q + r] : 0.0f;
+  }
+  for (int o = 0; o < NH; o++) wm[MM_B1 + o] = P[OB1 + o];
+  for (int o = 0; o < NH; o++) wm[MM_B2 + o] = P[OB2 + o];
+  for (int k = 0; k < NO; k++) wm[MM_B3 + k] = P[OB3 + k];
+}
+
 eh_status matrix_select_learned(eh_matrix *m, uint8_t *d_types, const uint64_t *sym_off) {
   eh_ctx *ctx = m->ctx;
   const bool fixed = ctx->sel_fixed >= 0;
@@ -626,9 +709,11 @@ eh_status matrix_select_learned(eh_matrix *m, uint8_t *d_types, const uint64_t *
     EH_HIP(hipStreamSynchronize(st));
     return EH_OK;
   }
+  std::vector<float> wm;
   if (!fixed) {
-    if (ctx->mlp_dev.bytes < 4ull * NPARAM) EH_CHECK(ctx->mlp_dev.alloc(4ull * NPARAM));
-    EH_HIP(hipMemcpyAsync(ctx->mlp_dev.p, ctx->mlp.data(), 4ull * NPARAM, hipMemcpyHostToDevice, st));
+    mlp_mfma_order(ctx->mlp.data(), wm);
+    if (ctx->mlp_dev.bytes < 4ull * MM_WORDS) EH_CHECK(ctx->mlp_dev.alloc(4ull * MM_WORDS));
+    EH_HIP(hipMemcpyAsync(ctx->mlp_dev.p, wm.data(), 4ull * MM_WORDS, hipMemcpyHostToDevice, st));
   }
   // every block stages the parameters (22 KB, from L2) once: a few thousand blocks at most
   const unsigned blocks = std::min(grid_for(n), 8192u);
