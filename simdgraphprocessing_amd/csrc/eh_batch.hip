@@ -321,6 +321,17 @@ extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tr
     key[q].lds_small = S.lds_small;
     key[q].lds_big = S.lds_big;
   }
+  // Both passes in one: when both are tuned and every row of both is a short row (no tasks, no
+  // local-graph rows, no B-stationary part), the two short-row lists hold the same rows (one
+  // rule, one bound), and the 4-clique kernel's local-edge hits are the triangles of those rows:
+  // k_small_cliques4 then counts both and the triangle pass is not launched
+  bool fuse = false;
+  if (want[0] && want[1] && key[0].sched == EH_SCHED_TUNED && key[1].sched == EH_SCHED_TUNED) {
+    const Schedule &s3 = M->sched[0][EH_SCHED_TUNED], &s4 = M->sched[1][EH_SCHED_TUNED];
+    fuse = s3.n_big == 0 && s3.n_small == 0 && s3.n_btasks == 0 && s3.n_htasks == 0 && s3.bstat_j == 0 &&
+           s4.n_big == 0 && s4.n_small == 0 && s4.n_local[0] == 0 && s4.n_local[1] == 0 &&
+           s3.small_deg == s4.small_deg && s3.n_small_rows == s4.n_small_rows && s3.n_small_rows > 0;
+  }
   unsigned long long *cnt = b->counts.as<unsigned long long>();
   uint32_t *nx = b->next.as<uint32_t>();
   float elapsed = 0;
@@ -335,9 +346,10 @@ extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tr
       k_zero_err<<<1, 64, 0, st>>>(nx);
       eh_status s = EH_OK;
       for (int q = 0; q < 2 && s == EH_OK; q++)
-        if (want[q])
+        if (want[q] && !(fuse && q == 0))
           s = schedule_launch(M, kk[q], key[q].sched, key[q].tp, M->sched[q][key[q].sched],
-                              b->counts.as<uint64_t>() + (uint64_t)q * ng, nx + 8 * q, nullptr);
+                              b->counts.as<uint64_t>() + (uint64_t)q * ng, nx + 8 * q, nullptr, nullptr,
+                              fuse ? b->counts.as<uint64_t>() : nullptr);
       hipGraph_t gr = nullptr;
       hipError_t e = hipStreamEndCapture(st, &gr);
       if (s != EH_OK) { if (gr) (void)hipGraphDestroy(gr); return s; }
@@ -360,7 +372,7 @@ extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tr
     k_zero_err<<<1, 64, 0, st>>>(nx);
     for (uint32_t g = 0; g < ng; g++) {
       for (int q = 0; q < 2; q++) {
-        if (!want[q]) continue;
+        if (!want[q] || (fuse && q == 0)) continue;
         const eh_batch::Plan &P = b->plan[q];
         EH_CHECK(launch_parts(M, kk[q], key[q].sched, key[q].tp, P.small_rows.as<uint32_t>() + P.sr_off[g],
                               P.sr_off[g + 1] - P.sr_off[g], P.tasks_big.as<uint4>() + P.tb_off[g],
@@ -369,7 +381,8 @@ extern "C" eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tr
                               b->counts.as<uint64_t>() + (uint64_t)q * ng, nx + 8 * q, nullptr, nullptr, 0, nullptr,
                               0, nullptr, nullptr, P.local_rows[0].as<uint32_t>() + P.lr_off[0][g],
                               P.lr_off[0][g + 1] - P.lr_off[0][g], P.local_rows[1].as<uint32_t>() + P.lr_off[1][g],
-                              P.lr_off[1][g + 1] - P.lr_off[1][g]));
+                              P.lr_off[1][g + 1] - P.lr_off[1][g], nullptr, nullptr,
+                              fuse ? b->counts.as<uint64_t>() : nullptr));
       }
     }
     EH_HIP(hipEventRecord(ctx->ev1, st));

@@ -76,6 +76,9 @@ struct CountArgs {
   uint32_t *xq;                      // short rows of a batch arena: bounds of 8 ranges of the
                                      // row list (xq[0..8]) and their queue heads (XQ_HEADS on);
                                      // null: one queue at next
+  unsigned long long *counts3;       // k_small_cliques4 with the triangle pass fused in: per-graph
+                                     // (or total) triangle counters for its local-edge hits (null:
+                                     // 4-cliques only)
 };
 
 // bits of the device error word (CountArgs::err)
@@ -1178,13 +1181,13 @@ constexpr uint32_t XQ_HEADS = 64, XQ_STRIDE = 64, XQ_WORDS = XQ_HEADS + 8 * XQ_S
 // counters.  The chunk's rows run in row order, so their graph ids form a few runs: one
 // atomic per distinct graph id instead of one per row (these atomics are performed at the
 // memory side and serialise per line).
-__device__ __forceinline__ void add_graph_counts(const CountArgs &a, uint32_t c, uint32_t gid, uint32_t lane) {
+__device__ __forceinline__ void add_graph_counts(unsigned long long *counts, uint32_t c, uint32_t gid, uint32_t lane) {
   uint64_t pend = __ballot(c != 0u);
   while (pend) {
     const uint32_t g = rdl(gid, (uint32_t)__builtin_ctzll(pend));
     const bool mine = c != 0u && gid == g;
     const uint64_t sum = wave_sum_u64(mine ? (uint64_t)c : 0ull);
-    if (lane == 0) atomicAdd(&a.counts[g], (unsigned long long)sum);
+    if (lane == 0) atomicAdd(&counts[g], (unsigned long long)sum);
     pend &= ~__ballot(mine);
   }
 }
@@ -1299,7 +1302,7 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
       }
     }
     wave_sync();
-    if (per_graph) add_graph_counts(a, lane < nr ? rcnt[lane] : 0u, gid, lane);
+    if (per_graph) add_graph_counts(a.counts, lane < nr ? rcnt[lane] : 0u, gid, lane);
     wave_sync();
   }
   const uint64_t s = wave_sum_u64(acc);
@@ -1316,7 +1319,10 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
 // both.  The extension over later entries is then LDS work, not more HBM searches.
 // A wave takes 64 rows at a time and runs them in sub-batches whose bit-rows fit
 // its SMALL_ADJ_WORDS (a row of up to 64 entries needs at most 128 words).
-constexpr uint32_t SMALL4_META_WORDS = 324;
+// With a.counts3 set the kernel is the triangle pass of the same rows too: every hit of
+// pass 1 is a triangle {i, R[p], R[c]}, so each row's hits go to its graph's triangle
+// counter, and rows of two entries (one wedge, no 4-clique) run pass 1 as well.
+constexpr uint32_t SMALL4_META_WORDS = 388;
 constexpr uint32_t LOCAL4_META = 196;  // k_local4: pair scan bases (65), offsets (64), columns (64)
 constexpr uint32_t LOCAL4_MID = 128, LOCAL4_MAX = 512;  // the two k_local4 row classes
 constexpr uint32_t SMALL_ADJ_WORDS = 1024;
@@ -1327,9 +1333,11 @@ __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint3
   const uint32_t lane = lane_id();
   uint32_t *meta = lds_all + (threadIdx.x >> 6) * (SMALL4_META_WORDS + SMALL_ADJ_WORDS);
   uint32_t *mbase = meta, *moff = meta + 65, *rcnt = meta + 130, *mdeg = meta + 194, *mwb = meta + 258;
+  uint32_t *rcnt3 = meta + 322;
   uint32_t *adj = meta + SMALL4_META_WORDS;
   const bool per_graph = a.graph_of != nullptr;
-  uint64_t acc = 0;
+  const bool tri = a.counts3 != nullptr;
+  uint64_t acc = 0, acc3 = 0;
   uint32_t qs = 0, c0 = 0, nr = 0;
   while (take_rows(a, nrows, lane, qs, c0, nr)) {
     uint32_t off = 0, gid = 0, d = 0;
@@ -1340,7 +1348,9 @@ __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint3
       if (per_graph) gid = a.graph_of[i];
     }
     rcnt[lane] = 0;
-    bool pend = lane < nr && d >= 3u;  // fewer than 3 entries close no 4-clique
+    rcnt3[lane] = 0;
+    // fewer than 3 entries close no 4-clique; 2 close a triangle
+    bool pend = lane < nr && d >= (tri ? 2u : 3u);
     while (__ballot(pend)) {
       // the pending rows whose bit-rows fit form a prefix of the lanes (at least one row)
       const uint32_t wpr = (d + 31u) >> 5;
@@ -1392,6 +1402,10 @@ __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint3
           if (m[k].hit) {
             const uint32_t wq = (mdeg[q[k]] + 31u) >> 5;
             atomicOr(&adj[mwb[q[k]] + cc[k] * wq + (pp[k] >> 5)], 1u << (pp[k] & 31u));
+            if (tri) {
+              if (per_graph) atomicAdd(&rcnt3[q[k]], 1u);
+              else acc3++;
+            }
           }
         }
       }
@@ -1421,11 +1435,16 @@ __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint3
       pend = pend && !sel;
     }
     wave_sync();
-    if (per_graph) add_graph_counts(a, lane < nr ? rcnt[lane] : 0u, gid, lane);
+    if (per_graph) add_graph_counts(a.counts, lane < nr ? rcnt[lane] : 0u, gid, lane);
+    if (per_graph && tri) add_graph_counts(a.counts3, lane < nr ? rcnt3[lane] : 0u, gid, lane);
     wave_sync();
   }
   const uint64_t s = wave_sum_u64(acc);
   if (lane == 0 && s) atomicAdd(&a.counts[0], (unsigned long long)s);
+  if (tri) {
+    const uint64_t s3 = wave_sum_u64(acc3);
+    if (lane == 0 && s3) atomicAdd(&a.counts3[0], (unsigned long long)s3);
+  }
 }
 
 // B-stationary triangle pairs.  On an oriented matrix row(j) only holds ids below j,
@@ -2838,10 +2857,15 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
                        uint64_t *d_task_cycles, const uint4 *btasks, uint64_t n_btasks, const uint2 *in_pairs,
                        uint32_t bstat_j, const uint32_t *jstart, const uint4 *lo16, const uint32_t *local_rows0,
                        uint64_t n_local0, const uint32_t *local_rows1, uint64_t n_local1, const Schedule *hs,
-                       PartEv *pev) {
+                       PartEv *pev, uint64_t *d_counts3) {
   // pev (may be null): one pair of events per kernel class, indexed by EH_PART_*
   auto pe = [pev](int q) -> PartEv * { return pev ? pev + q : nullptr; };
+  if (d_counts3 && (k != 4 || n_big || n_small || n_local0 || n_local1)) {
+    eh_set_error("the triangle pass is fused into a 4-clique pass of short rows only");
+    return EH_ERR_INVALID;
+  }
   CountArgs a;
+  a.counts3 = reinterpret_cast<unsigned long long *>(d_counts3);
   a.grab = tp.grab ? tp.grab : 1u;
   // d_next[7] is the pass's error word (the queue heads use d_next[0..4]); the caller
   // clears it before the pass and reads it after
@@ -2905,14 +2929,15 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
 }
 
 eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, Schedule &s,
-                          uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles, PartEv *pev) {
+                          uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles, PartEv *pev,
+                          uint64_t *d_counts3) {
   const uint4 *t = s.tasks.as<uint4>();
   return launch_parts(m, k, sched, tp, s.small_rows.as<uint32_t>(), s.n_small_rows, t, s.n_big, t + s.n_big,
                       s.n_small, s.lds_small, s.lds_big, d_counts, d_next, d_task_cycles, s.btasks.as<uint4>(),
                       s.n_btasks, s.in_pairs.as<uint2>(), s.bstat_j,
                       s.bstat_j ? s.jstart.as<uint32_t>() : nullptr, s.bstat_j ? s.lo16.as<uint4>() : nullptr,
                       s.local_rows[0].as<uint32_t>(), s.n_local[0], s.local_rows[1].as<uint32_t>(), s.n_local[1],
-                      &s, pev);
+                      &s, pev, d_counts3);
 }
 
 // Sum over the kept pairs (i, j) of sum over x in row(i) & row(j) of card(x): over the

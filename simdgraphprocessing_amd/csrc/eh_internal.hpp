@@ -268,8 +268,12 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
 // the schedule of a k-pass under opts: built, or reused while every parameter it was built for
 // holds; *sched_ms = its build time when built here, else 0
 eh_status schedule_ensure(eh_matrix *m, int k, const eh_run_opts *opts, double *sched_ms);
+// d_counts3 (4-cliques only, may be null): the triangle counters of the same rows, counted in
+// the 4-clique pass (every row of the schedule must be a short row: the triangle pass is then
+// not run); see k_small_cliques4
 eh_status schedule_launch(eh_matrix *m, int k, int sched, const TuneParams &tp, Schedule &s,
-                          uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles, PartEv *pev = nullptr);
+                          uint64_t *d_counts, uint32_t *d_next, uint64_t *d_task_cycles, PartEv *pev = nullptr,
+                          uint64_t *d_counts3 = nullptr);
 TuneParams tune_defaults(eh_matrix *m, int sched, int k, const eh_run_opts *opts);
 eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, const uint32_t *small_rows,
                        uint64_t n_small_rows, const uint4 *tasks_big, uint64_t n_big, const uint4 *tasks_small,
@@ -278,7 +282,7 @@ eh_status launch_parts(eh_matrix *m, int k, int sched, const TuneParams &tp, con
                        const uint2 *in_pairs = nullptr, uint32_t bstat_j = 0, const uint32_t *jstart = nullptr,
                        const uint4 *lo16 = nullptr, const uint32_t *local_rows0 = nullptr, uint64_t n_local0 = 0,
                        const uint32_t *local_rows1 = nullptr, uint64_t n_local1 = 0,
-                       const Schedule *hs = nullptr, PartEv *pev = nullptr);
+                       const Schedule *hs = nullptr, PartEv *pev = nullptr, uint64_t *d_counts3 = nullptr);
 eh_status matrix_triangle_weight(eh_matrix *m, uint32_t ncounts, uint64_t *out);
 eh_status ctx_part_events(eh_ctx *ctx);
 

@@ -51,7 +51,8 @@ def test_batch_pass_options_and_recapture(ctx):
     shapes = (({"small_degree": 64}, {"small_degree": 8}),
               ({"small_degree": 16}, {"small_degree": 64}),
               ({"small_degree": -1}, {"small_degree": -1, "lds_words": 3072, "waves": 2}),
-              ({"small_degree": 64}, {"small_degree": 48}))
+              ({"small_degree": 64}, {"small_degree": 48}),
+              ({"small_degree": 64, "lds_words": 3072, "waves": 2}, {"small_degree": 64}))
     for t_o, k_o in shapes:
         for mode in ("persistent", "per_graph", "persistent"):
             t, k, _ = b.count(mode, "tuned", tri_opts=t_o, k4_opts=k_o)
@@ -139,3 +140,23 @@ def test_batch_rebuild(ctx):
             assert np.array_equal(got, want)
         elif policy == "uint":
             assert np.all(got == 2)
+
+
+@pytest.mark.parametrize("seed,ng", [(4, 48), (7, 5)])
+def test_batch_fused_pass(ctx, seed, ng):
+    """At the batch workload's shapes every row of both passes is a short row, and the triangle
+    pass runs inside the 4-clique kernel (its local-edge hits are the triangles): the counts of
+    the fused launch equal those of each pass run alone and the oracle's, in both modes."""
+    from simdgraphprocessing_amd import selector
+    sh = selector.batch_shapes()
+    b = ctx.generate_batch(seed, ng)
+    kw = {"tri_opts": sh["tri"], "k4_opts": sh["k4"]}
+    for mode in ("persistent", "per_graph"):
+        t, k, _ = b.count(mode, "tuned", **kw)
+        t1, _, _ = b.count(mode, "tuned", cliques=False, **kw)
+        _, k1, _ = b.count(mode, "tuned", triangles=False, **kw)
+        assert np.array_equal(t, t1) and np.array_equal(k, k1), mode
+    for g in range(ng):
+        n, off, nbr = oracle_graph(seed, g)
+        om = O.Matrix(n, off, nbr, "hybrid")
+        assert t[g] == om.triangles(4)[0] and k[g] == om.cliques(4, 4)[0], g
