@@ -92,6 +92,7 @@ extern "C" void eh_ctx_destroy(eh_ctx *ctx) {
   }
   ctx->scratch.release();
   ctx->counter.release();
+  ctx->cache.clear();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

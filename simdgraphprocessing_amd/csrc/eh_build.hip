@@ -502,8 +502,8 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   DevBuf pc, poff, type, parts, bs_n, ps_n, blk_n, ntu_n, bsoff, psoff, blkoff, ntuoff, refb, stats;
   if ((s = m->dcard.alloc(4ull * n + 4)) != EH_OK) return fail(s);
   if ((s = m->daoff.alloc(4ull * n + 4)) != EH_OK) return fail(s);
-  if ((s = pc.alloc(4ull * n + 8)) != EH_OK) return fail(s);
-  if ((s = poff.alloc(8ull * n + 16)) != EH_OK) return fail(s);
+  if ((s = pc.alloc(4ull * n + 8, &ctx->cache)) != EH_OK) return fail(s);
+  if ((s = poff.alloc(8ull * n + 16, &ctx->cache)) != EH_OK) return fail(s);
   if (hipEventRecord(ctx->ev0, st) != hipSuccess) return fail(EH_ERR_HIP);
   k_filter_count<<<grid_for(n), 256, 0, st>>>(d_off, d_nbr, n, filter, d_mask, m->dcard.as<uint32_t>());
   k_pad4<<<grid_for((uint64_t)n + 1), 256, 0, st>>>(m->dcard.as<uint32_t>(), n, pc.as<uint32_t>());
@@ -526,7 +526,7 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   // learned types are computed from the packed rows before selection
   DevBuf forced;
   if (policy == EH_LAYOUT_LEARNED) {
-    if ((s = forced.alloc((size_t)n + 16)) != EH_OK) return fail(s);
+    if ((s = forced.alloc((size_t)n + 16, &ctx->cache)) != EH_OK) return fail(s);
     m->card = 0;
     if (d_types_in) {
       k_guard_types<<<grid_for(n), 256, 0, st>>>(m->dcard.as<uint32_t>(), poff.as<uint64_t>(), m->dunbr.as<uint32_t>(),
@@ -535,14 +535,14 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
       return fail(s);
     }
   }
-  if ((s = type.alloc((size_t)n + 16)) != EH_OK) return fail(s);
-  if ((s = parts.alloc(4ull * n + 4)) != EH_OK) return fail(s);
-  if ((s = bs_n.alloc(4ull * n + 8)) != EH_OK) return fail(s);
-  if ((s = ps_n.alloc(4ull * n + 8)) != EH_OK) return fail(s);
-  if ((s = blk_n.alloc(4ull * n + 8)) != EH_OK) return fail(s);
-  if ((s = ntu_n.alloc(4ull * n + 8)) != EH_OK) return fail(s);
-  if ((s = refb.alloc(8ull * n + 8)) != EH_OK) return fail(s);
-  if ((s = stats.alloc(64)) != EH_OK) return fail(s);
+  if ((s = type.alloc((size_t)n + 16, &ctx->cache)) != EH_OK) return fail(s);
+  if ((s = parts.alloc(4ull * n + 4, &ctx->cache)) != EH_OK) return fail(s);
+  if ((s = bs_n.alloc(4ull * n + 8, &ctx->cache)) != EH_OK) return fail(s);
+  if ((s = ps_n.alloc(4ull * n + 8, &ctx->cache)) != EH_OK) return fail(s);
+  if ((s = blk_n.alloc(4ull * n + 8, &ctx->cache)) != EH_OK) return fail(s);
+  if ((s = ntu_n.alloc(4ull * n + 8, &ctx->cache)) != EH_OK) return fail(s);
+  if ((s = refb.alloc(8ull * n + 8, &ctx->cache)) != EH_OK) return fail(s);
+  if ((s = stats.alloc(64, &ctx->cache)) != EH_OK) return fail(s);
   if (hipMemsetAsync(stats.p, 0, 64, st) != hipSuccess) return fail(EH_ERR_HIP);
   if (hipMemsetAsync(bs_n.p, 0, 4ull * n + 8, st) != hipSuccess) return fail(EH_ERR_HIP);
   if (hipMemsetAsync(ps_n.p, 0, 4ull * n + 8, st) != hipSuccess) return fail(EH_ERR_HIP);
@@ -558,7 +558,7 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
                       policy == EH_LAYOUT_LEARNED;
   DevBuf parts_in;
   if (any_ps) {
-    if ((s = parts_in.alloc(4ull * n + 4)) != EH_OK) return fail(s);
+    if ((s = parts_in.alloc(4ull * n + 4, &ctx->cache)) != EH_OK) return fail(s);
     k_parts<<<grid_for(n), 256, 0, st>>>(m->dcard.as<uint32_t>(), poff.as<uint64_t>(), m->dunbr.as<uint32_t>(), n,
                                          policy == EH_LAYOUT_LEARNED ? forced.as<uint8_t>() : nullptr,
                                          parts_in.as<uint32_t>());
@@ -576,12 +576,12 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   ta.type_rows = stats.as<unsigned long long>();
   const unsigned bounded = std::min(grid_for(n), (unsigned)(ctx->num_cus ? ctx->num_cus : 256) * 8u);
   k_select<<<bounded, 256, 0, st>>>(ta);
-  if ((s = bsoff.alloc(8ull * n + 16)) != EH_OK) return fail(s);
-  if ((s = psoff.alloc(8ull * n + 16)) != EH_OK) return fail(s);
+  if ((s = bsoff.alloc(8ull * n + 16, &ctx->cache)) != EH_OK) return fail(s);
+  if ((s = psoff.alloc(8ull * n + 16, &ctx->cache)) != EH_OK) return fail(s);
   if ((s = prim_exscan_u32_to_u64(ctx, bs_n.as<uint32_t>(), bsoff.as<uint64_t>(), (uint64_t)n + 1)) != EH_OK) return fail(s);
   if ((s = prim_exscan_u32_to_u64(ctx, ps_n.as<uint32_t>(), psoff.as<uint64_t>(), (uint64_t)n + 1)) != EH_OK) return fail(s);
-  if ((s = blkoff.alloc(8ull * n + 16)) != EH_OK) return fail(s);
-  if ((s = ntuoff.alloc(8ull * n + 16)) != EH_OK) return fail(s);
+  if ((s = blkoff.alloc(8ull * n + 16, &ctx->cache)) != EH_OK) return fail(s);
+  if ((s = ntuoff.alloc(8ull * n + 16, &ctx->cache)) != EH_OK) return fail(s);
   if ((s = prim_exscan_u32_to_u64(ctx, blk_n.as<uint32_t>(), blkoff.as<uint64_t>(), (uint64_t)n + 1)) != EH_OK) return fail(s);
   if ((s = prim_exscan_u32_to_u64(ctx, ntu_n.as<uint32_t>(), ntuoff.as<uint64_t>(), (uint64_t)n + 1)) != EH_OK) return fail(s);
   uint64_t hb[4] = {0, 0, 0, 0};
@@ -636,7 +636,7 @@ eh_status matrix_build_dev(eh_ctx *ctx, uint32_t n, const uint64_t *d_off, const
   // kept entries = sum of card
   {
     DevBuf c64;
-    if ((s = c64.alloc(8ull * n + 8)) != EH_OK) return fail(s);
+    if ((s = c64.alloc(8ull * n + 8, &ctx->cache)) != EH_OK) return fail(s);
     k_u32_to_u64_plus<<<grid_for(n), 256, 0, st>>>(m->dcard.as<uint32_t>(), n, c64.as<uint64_t>());
     if ((s = prim_reduce_sum_u64(ctx, c64.as<uint64_t>(), n, &m->card)) != EH_OK) return fail(s);
   }

@@ -2359,26 +2359,26 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   // prefix identity needs every row(j) below j)
   const uint32_t bstat_j = (sched == EH_SCHED_TUNED && k == 3 && m->filter == EH_FILTER_LOWER &&
                             !m->dgraph_of.p) ? std::min(tp.bstat_j, n) : 0u;
-  EH_CHECK(jstart.alloc(4ull * n + 8));
+  EH_CHECK(jstart.alloc(4ull * n + 8));  // kept with the schedule
   DevBuf jbelow, hcount;
-  EH_CHECK(jbelow.alloc(4ull * n + 8));
+  EH_CHECK(jbelow.alloc(4ull * n + 8, &ctx->cache));
   // rows whose B rows at or above the split are all short go wholly B-stationary (hashed)
   const uint32_t bsh_max = (bstat_j && bstat_j < n) ? tp.bsh_max : 0u;
   const uint32_t nh = bsh_max ? n - bstat_j : 0u;
-  if (bstat_j) EH_CHECK(bpairs.alloc(4ull * n + 8));
+  if (bstat_j) EH_CHECK(bpairs.alloc(4ull * n + 8, &ctx->cache));
   if (nh) {
-    EH_CHECK(hcount.alloc(4ull * nh + 8));
+    EH_CHECK(hcount.alloc(4ull * nh + 8, &ctx->cache));
     k_zero_u32<<<grid_for((uint64_t)nh + 1), 256, 0, st>>>(hcount.as<uint32_t>(), nh + 1);
   }
-  EH_CHECK(is_small.alloc(4ull * n + 8));
-  EH_CHECK(spos.alloc(8ull * n + 16));
+  EH_CHECK(is_small.alloc(4ull * n + 8, &ctx->cache));
+  EH_CHECK(spos.alloc(8ull * n + 16, &ctx->cache));
   k_zero_u32<<<grid_for((uint64_t)n + 1), 256, 0, st>>>(is_small.as<uint32_t>(), n + 1);
-  EH_CHECK(key.alloc(8ull * n + 8));
-  EH_CHECK(key2.alloc(8ull * n + 8));
-  EH_CHECK(ntask.alloc(4ull * n + 4));
-  EH_CHECK(rowinfo.alloc(4ull * n + 4));
-  EH_CHECK(stats.alloc(64));
-  EH_CHECK(cnt2.alloc(8));
+  EH_CHECK(key.alloc(8ull * n + 8, &ctx->cache));
+  EH_CHECK(key2.alloc(8ull * n + 8, &ctx->cache));
+  EH_CHECK(ntask.alloc(4ull * n + 4, &ctx->cache));
+  EH_CHECK(rowinfo.alloc(4ull * n + 4, &ctx->cache));
+  EH_CHECK(stats.alloc(64, &ctx->cache));
+  EH_CHECK(cnt2.alloc(8, &ctx->cache));
   EH_HIP(hipEventRecord(ctx->ev0, st));
   k_zero_u64<<<1, 64, 0, st>>>(stats.as<unsigned long long>(), 8);
   k_zero_u32<<<1, 64, 0, st>>>(cnt2.as<uint32_t>(), 2);
@@ -2416,7 +2416,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     s.n_local[c] = 0;
     if (!local_max) continue;
     DevBuf flag;
-    EH_CHECK(flag.alloc(4ull * n + 8));
+    EH_CHECK(flag.alloc(4ull * n + 8, &ctx->cache));
     k_class_flag<<<grid_for(n), 256, 0, st>>>(is_small.as<uint32_t>(), n, 2u + c, flag.as<uint32_t>());
     k_zero_u32<<<1, 64, 0, st>>>(flag.as<uint32_t>() + n, 1);
     EH_CHECK(prim_exscan_u32_to_u64(ctx, flag.as<uint32_t>(), spos.as<uint64_t>(), (uint64_t)n + 1));
@@ -2445,7 +2445,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
   s.xq.release();
   if (nsmall && m->dgraph_of.p) {
     DevBuf items;
-    EH_CHECK(items.alloc(4ull * nsmall + 8));
+    EH_CHECK(items.alloc(4ull * nsmall + 8, &ctx->cache));
     k_short_items<<<grid_for(nsmall), 256, 0, st>>>(s.small_rows.as<uint32_t>(), (uint32_t)nsmall,
                                                     m->dcard.as<uint32_t>(), items.as<uint32_t>());
     k_zero_u32<<<1, 64, 0, st>>>(items.as<uint32_t>() + nsmall, 1);
@@ -2456,8 +2456,8 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     EH_HIP(hipGetLastError());
   }
   EH_CHECK(prim_sort_u64(ctx, key.as<uint64_t>(), key2.as<uint64_t>(), n, 64));
-  EH_CHECK(ntsorted.alloc(4ull * n + 8));
-  EH_CHECK(toff.alloc(8ull * n + 16));
+  EH_CHECK(ntsorted.alloc(4ull * n + 8, &ctx->cache));
+  EH_CHECK(toff.alloc(8ull * n + 16, &ctx->cache));
   k_gather_ntask<<<grid_for(n), 256, 0, st>>>(key.as<uint64_t>(), ntask.as<uint32_t>(), n,
                                              ntsorted.as<uint32_t>(), cnt2.as<uint32_t>());
   EH_CHECK(prim_exscan_u32_to_u64(ctx, ntsorted.as<uint32_t>(), toff.as<uint64_t>(), (uint64_t)n + 1));
@@ -2488,7 +2488,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     // 512 3.54, 1024 3.47, 2048 3.55)
     constexpr uint32_t BSTAT_CHUNK = 1024;
     DevBuf inoff, inend, incount, rowpos, ntb, boff;
-    EH_CHECK(rowpos.alloc(8ull * n + 16));
+    EH_CHECK(rowpos.alloc(8ull * n + 16, &ctx->cache));
     k_zero_u32<<<1, 64, 0, st>>>(bpairs.as<uint32_t>() + n, 1);
     EH_CHECK(prim_exscan_u32_to_u64(ctx, bpairs.as<uint32_t>(), rowpos.as<uint64_t>(), (uint64_t)n + 1));
     uint64_t npairs = 0;
@@ -2500,23 +2500,23 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     }
     EH_CHECK(s.in_pairs.alloc(8 * npairs + 16));
     DevBuf pkeys, pkeys2;
-    EH_CHECK(pkeys.alloc(8 * npairs + 16));
-    EH_CHECK(pkeys2.alloc(8 * npairs + 16));
-    EH_CHECK(inoff.alloc(8ull * bstat_j + 16));
-    EH_CHECK(inend.alloc(4ull * bstat_j + 8));
-    EH_CHECK(incount.alloc(4ull * bstat_j + 8));
-    EH_CHECK(ntb.alloc(4ull * bstat_j + 8));
-    EH_CHECK(boff.alloc(8ull * bstat_j + 16));
+    EH_CHECK(pkeys.alloc(8 * npairs + 16, &ctx->cache));
+    EH_CHECK(pkeys2.alloc(8 * npairs + 16, &ctx->cache));
+    EH_CHECK(inoff.alloc(8ull * bstat_j + 16, &ctx->cache));
+    EH_CHECK(inend.alloc(4ull * bstat_j + 8, &ctx->cache));
+    EH_CHECK(incount.alloc(4ull * bstat_j + 8, &ctx->cache));
+    EH_CHECK(ntb.alloc(4ull * bstat_j + 8, &ctx->cache));
+    EH_CHECK(boff.alloc(8ull * bstat_j + 16, &ctx->cache));
     k_zero_u32<<<grid_for(bstat_j + 1), 256, 0, st>>>(inend.as<uint32_t>(), bstat_j + 1);
     // 16-bit prefixes
     DevBuf nvec;
-    EH_CHECK(nvec.alloc(4ull * n + 8));
-    EH_CHECK(lo16off.alloc(8ull * n + 16));
+    EH_CHECK(nvec.alloc(4ull * n + 8, &ctx->cache));
+    EH_CHECK(lo16off.alloc(8ull * n + 16, &ctx->cache));
     {
       DevBuf nvec2, obig, osmall;
-      EH_CHECK(nvec2.alloc(4ull * n + 8));
-      EH_CHECK(obig.alloc(8ull * n + 16));
-      EH_CHECK(osmall.alloc(8ull * n + 16));
+      EH_CHECK(nvec2.alloc(4ull * n + 8, &ctx->cache));
+      EH_CHECK(obig.alloc(8ull * n + 16, &ctx->cache));
+      EH_CHECK(osmall.alloc(8ull * n + 16, &ctx->cache));
       k_lo16_count<<<grid_for(n), 256, 0, st>>>(jbelow.as<uint32_t>(), n, nvec.as<uint32_t>(), nvec2.as<uint32_t>());
       k_zero_u32<<<1, 64, 0, st>>>(nvec.as<uint32_t>() + n, 1);
       k_zero_u32<<<1, 64, 0, st>>>(nvec2.as<uint32_t>() + n, 1);
@@ -2574,9 +2574,9 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
       // read the prefixes of nearby rows (with 256-pair tasks the L2 hit rate on RMAT-22 rose
       // from 16% to 38%; with 1024-pair tasks the order is worth 0.5% on RMAT-24 only)
       DevBuf tk, tk2, bt2;
-      EH_CHECK(tk.alloc(8 * nbt + 8));
-      EH_CHECK(tk2.alloc(8 * nbt + 8));
-      EH_CHECK(bt2.alloc(16 * nbt + 16));
+      EH_CHECK(tk.alloc(8 * nbt + 8, &ctx->cache));
+      EH_CHECK(tk2.alloc(8 * nbt + 8, &ctx->cache));
+      EH_CHECK(bt2.alloc(16 * nbt + 16, &ctx->cache));
       k_bstat_task_keys<<<grid_for(nbt), 256, 0, st>>>(s.btasks.as<uint4>(), nbt, s.in_pairs.as<uint2>(),
                                                        tk.as<uint64_t>());
       EH_CHECK(prim_sort_u64(ctx, tk.as<uint64_t>(), tk2.as<uint64_t>(), nbt, 64));
@@ -2597,8 +2597,8 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     // bins of them by table words, the in-pairs of each, tasks of at most BSH_CHUNK pairs
     constexpr uint32_t BSH_CHUNK = 256;
     DevBuf hflag, hpos, hc, hw, pw, bflag, bpos, hfill, ntb, boff;
-    EH_CHECK(hflag.alloc(4ull * nh + 8));
-    EH_CHECK(hpos.alloc(8ull * nh + 16));
+    EH_CHECK(hflag.alloc(4ull * nh + 8, &ctx->cache));
+    EH_CHECK(hpos.alloc(8ull * nh + 16, &ctx->cache));
     k_nonzero_flag<<<grid_for(nh), 256, 0, st>>>(hcount.as<uint32_t>(), nh, hflag.as<uint32_t>());
     k_zero_u32<<<1, 64, 0, st>>>(hflag.as<uint32_t>() + nh, 1);
     EH_CHECK(prim_exscan_u32_to_u64(ctx, hflag.as<uint32_t>(), hpos.as<uint64_t>(), (uint64_t)nh + 1));
@@ -2608,13 +2608,13 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     if (nhr) {
       EH_CHECK(s.hj.alloc(4 * nhr + 16));
       EH_CHECK(s.hjlo.alloc(4 * nhr + 16));
-      EH_CHECK(hc.alloc(4 * nhr + 16));
-      EH_CHECK(hw.alloc(4 * nhr + 16));
+      EH_CHECK(hc.alloc(4 * nhr + 16, &ctx->cache));
+      EH_CHECK(hw.alloc(4 * nhr + 16, &ctx->cache));
       EH_CHECK(s.hoff.alloc(8 * nhr + 16));
-      EH_CHECK(pw.alloc(8 * nhr + 16));
-      EH_CHECK(bflag.alloc(4 * nhr + 16));
-      EH_CHECK(bpos.alloc(8 * nhr + 16));
-      EH_CHECK(hfill.alloc(4 * nhr + 16));
+      EH_CHECK(pw.alloc(8 * nhr + 16, &ctx->cache));
+      EH_CHECK(bflag.alloc(4 * nhr + 16, &ctx->cache));
+      EH_CHECK(bpos.alloc(8 * nhr + 16, &ctx->cache));
+      EH_CHECK(hfill.alloc(4 * nhr + 16, &ctx->cache));
       k_zero_u32<<<1, 64, 0, st>>>(cnt2.as<uint32_t>(), 1);
       k_bsh_rows<<<grid_for(nh), 256, 0, st>>>(hcount.as<uint32_t>(), hpos.as<uint64_t>(), nh, bstat_j,
                                                m->dcard.as<uint32_t>(), jbelow.as<uint32_t>(), s.hj.as<uint32_t>(),
@@ -2654,8 +2654,8 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
           m->dcard.as<uint32_t>(), m->daoff.as<uint32_t>(), m->dunbr.as<uint32_t>(), m->ddesc.as<RowDesc>(),
           jstart.as<uint32_t>(), jbelow.as<uint32_t>(), n, bstat_j, hpos.as<uint64_t>(), s.hoff.as<uint64_t>(),
           hfill.as<uint32_t>(), lo16off.as<uint64_t>(), s.hpairs.as<uint4>());
-      EH_CHECK(ntb.alloc(4 * nb + 16));
-      EH_CHECK(boff.alloc(8 * nb + 16));
+      EH_CHECK(ntb.alloc(4 * nb + 16, &ctx->cache));
+      EH_CHECK(boff.alloc(8 * nb + 16, &ctx->cache));
       k_bsh_bin_ntask<<<grid_for(nb), 256, 0, st>>>(s.bstart.as<uint32_t>(), s.hoff.as<uint64_t>(), (uint32_t)nb,
                                                     BSH_CHUNK, ntb.as<uint32_t>());
       k_zero_u32<<<1, 64, 0, st>>>(ntb.as<uint32_t>() + nb, 1);

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -81,26 +83,114 @@ void eh_set_error(const char *fmt, ...);
 
 // ---------------------------------------------------------------- device buffers
 
+// A context's cache of device blocks for the temporaries of its builds and schedules: a
+// released block goes back here instead of to hipFree (which waits for the whole device), and
+// a later request it fits takes it again.  Every use of a context's memory is ordered on the
+// context's stream (a blocking stream: copies on the null stream wait for it), so a block
+// handed out again is only touched by work queued after the work that used it before.  Blocks
+// are sized in 256-byte steps below 1 MiB and 1 MiB steps above; a request takes the smallest
+// cached block of at least its size if that is at most twice it.  Beyond CAP bytes the largest
+// cached blocks are freed.
+struct BlockCache {
+  static constexpr size_t CAP = 8ull << 30;
+  std::mutex mu;
+  std::multimap<size_t, void *> blocks;  // by size
+  size_t cached = 0;
+  static size_t round(size_t b) {
+    const size_t q = b < (1u << 20) ? 256u : (1u << 20);
+    return (b + q - 1) / q * q;
+  }
+  void *get(size_t b, size_t *got) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = blocks.lower_bound(b);
+    if (it == blocks.end() || it->first > 2 * b) return nullptr;
+    void *p = it->second;
+    *got = it->first;
+    cached -= it->first;
+    blocks.erase(it);
+    return p;
+  }
+  void put(void *p, size_t b) {
+    std::lock_guard<std::mutex> g(mu);
+    blocks.emplace(b, p);
+    cached += b;
+    while (cached > CAP && !blocks.empty()) {
+      auto last = std::prev(blocks.end());
+      (void)hipFree(last->second);
+      cached -= last->first;
+      blocks.erase(last);
+    }
+  }
+  void clear() {
+    std::lock_guard<std::mutex> g(mu);
+    for (auto &kv : blocks) (void)hipFree(kv.second);
+    blocks.clear();
+    cached = 0;
+  }
+};
+
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  BlockCache *cache = nullptr;  // the block returns here on release (a context's temporaries)
+  size_t block = 0;             // its cached size
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
-  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), cache(o.cache), block(o.block) {
+    o.p = nullptr;
+    o.bytes = 0;
+    o.cache = nullptr;
+    o.block = 0;
+  }
   DevBuf &operator=(DevBuf &&o) noexcept {
-    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      cache = o.cache;
+      block = o.block;
+      o.p = nullptr;
+      o.bytes = 0;
+      o.cache = nullptr;
+      o.block = 0;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (cache) cache->put(p, block);
+      else (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
+    cache = nullptr;
+    block = 0;
   }
-  eh_status alloc(size_t b) {
+  // c: take the block from (and later return it to) a context's cache
+  eh_status alloc(size_t b, BlockCache *c = nullptr) {
     release();
     if (b == 0) b = 16;
+    if (c) {
+      size_t got = 0;
+      if ((p = c->get(b, &got)) != nullptr) {
+        bytes = b;
+        cache = c;
+        block = got;
+        return EH_OK;
+      }
+      const size_t r = BlockCache::round(b);
+      if (hipMalloc(&p, r) == hipSuccess) {
+        bytes = b;
+        cache = c;
+        block = r;
+        return EH_OK;
+      }
+      (void)hipGetLastError();
+      // out of memory with blocks cached: give them back and try once more
+      c->clear();
+    }
     hipError_t e = hipMalloc(&p, b);
     if (e != hipSuccess) {
       p = nullptr;
@@ -153,6 +243,7 @@ struct eh_ctx {
   // k_bshl row bound; split 0: every pair A-side): only rows some A-side or edge-centric pair of
   // that schedule reads are candidates
   uint32_t sel_split = 65536, sel_bsh = 512;
+  BlockCache cache;        // device blocks of the builds' and schedules' temporaries
   DevBuf scratch;          // reusable temp storage for primitives
   DevBuf counter;          // small device counters
   DevBuf pass_words;       // a counting pass's queue heads, error word and count (kept between
