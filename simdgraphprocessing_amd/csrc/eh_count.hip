@@ -2057,7 +2057,7 @@ __global__ void __launch_bounds__(256) k_bstat_fill(const uint32_t *aoff, const 
     const uint32_t jb = jbelow[i];
     if (jb <= 1) continue;
     const uint32_t *G = unbr + (uint64_t)aoff[i] * 4;
-    const uint64_t pre = (uint64_t)(uint32_t)lo16off[i] << 16;
+    const uint64_t pre = (uint64_t)(uint32_t)lo16off[i];
     uint64_t pos = rowpos[i];
     for (uint32_t k0 = 0; k0 < jb; k0 += 64) {
       const uint32_t k = k0 + lane;
@@ -2068,10 +2068,10 @@ __global__ void __launch_bounds__(256) k_bstat_fill(const uint32_t *aoff, const 
         q = desc[j].card != 0;
       }
       const uint64_t m = __ballot(q);
-      // j (16 bits) | row i's prefix vector offset (32 bits) | p (16 bits, p < j < 2^16):
-      // sorting the keys orders every in-list by prefix offset (row order within each region
-      // of the prefix copy)
-      if (q) keys[pos + mbcnt(m)] = ((uint64_t)j << 48) | pre | k;
+      // p (16 bits, p < j < 2^16) | j (16 bits) | row i's prefix vector offset (32 bits):
+      // sorting the keys on their low 48 bits orders every in-list by prefix offset (row order
+      // within each region of the prefix copy); p rides along in the bits the sort skips
+      if (q) keys[pos + mbcnt(m)] = ((uint64_t)k << 48) | ((uint64_t)j << 32) | pre;
       pos += (uint64_t)__popcll(m);
     }
   }
@@ -2081,9 +2081,9 @@ __global__ void __launch_bounds__(256) k_bstat_fill(const uint32_t *aoff, const 
 // (incount starts at zero; j without keys keep count 0)
 __global__ void k_bstat_bounds(const uint64_t *keys, uint64_t np, uint64_t *inoff, uint32_t *inend) {
   for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < np; q += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t j = (uint32_t)(keys[q] >> 48);
-    if (q == 0 || (uint32_t)(keys[q - 1] >> 48) != j) inoff[j] = q;
-    if (q + 1 == np || (uint32_t)(keys[q + 1] >> 48) != j) inend[j] = (uint32_t)(q + 1);
+    const uint32_t j = (uint32_t)(keys[q] >> 32) & 0xFFFFu;
+    if (q == 0 || ((uint32_t)(keys[q - 1] >> 32) & 0xFFFFu) != j) inoff[j] = q;
+    if (q + 1 == np || ((uint32_t)(keys[q + 1] >> 32) & 0xFFFFu) != j) inend[j] = (uint32_t)(q + 1);
   }
 }
 
@@ -2095,7 +2095,7 @@ __global__ void k_bstat_counts(const uint64_t *inoff, const uint32_t *inend, uin
 __global__ void k_bstat_unpack(const uint64_t *keys, uint64_t np, uint2 *in_pairs) {
   for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < np; q += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t x = keys[q];
-    in_pairs[q] = make_uint2((uint32_t)(x >> 16), (uint32_t)(x & 0xFFFFu));
+    in_pairs[q] = make_uint2((uint32_t)x, (uint32_t)(x >> 48));
   }
 }
 
@@ -2541,7 +2541,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
                                                            m->ddesc.as<RowDesc>(), jbelow.as<uint32_t>(), n,
                                                            rowpos.as<uint64_t>(), lo16off.as<uint64_t>(),
                                                            pkeys.as<uint64_t>());
-    EH_CHECK(prim_sort_u64(ctx, pkeys.as<uint64_t>(), pkeys2.as<uint64_t>(), npairs, 64));
+    EH_CHECK(prim_sort_u64(ctx, pkeys.as<uint64_t>(), pkeys2.as<uint64_t>(), npairs, 48));
     k_bstat_unpack<<<grid_for(npairs), 256, 0, st>>>(pkeys.as<uint64_t>(), npairs, s.in_pairs.as<uint2>());
     if (npairs)
       k_bstat_bounds<<<grid_for(npairs), 256, 0, st>>>(pkeys.as<uint64_t>(), npairs, inoff.as<uint64_t>(),
