@@ -2068,9 +2068,8 @@ __global__ void __launch_bounds__(256) k_bstat_fill(const uint32_t *aoff, const 
         q = desc[j].card != 0;
       }
       const uint64_t m = __ballot(q);
-      // p (16 bits, p < j < 2^16) | j (16 bits) | row i's prefix vector offset (32 bits):
-      // sorting the keys on their low 48 bits orders every in-list by prefix offset (row order
-      // within each region of the prefix copy); p rides along in the bits the sort skips
+      // p (16 bits, p < j < 2^16) | j (16 bits) | row i's prefix vector offset (32 bits),
+      // written in row order: a stable sort on j's bits alone leaves every in-list in row order
       if (q) keys[pos + mbcnt(m)] = ((uint64_t)k << 48) | ((uint64_t)j << 32) | pre;
       pos += (uint64_t)__popcll(m);
     }
@@ -2541,7 +2540,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
                                                            m->ddesc.as<RowDesc>(), jbelow.as<uint32_t>(), n,
                                                            rowpos.as<uint64_t>(), lo16off.as<uint64_t>(),
                                                            pkeys.as<uint64_t>());
-    EH_CHECK(prim_sort_u64(ctx, pkeys.as<uint64_t>(), pkeys2.as<uint64_t>(), npairs, 48));
+    EH_CHECK(prim_sort_u64_range(ctx, pkeys.as<uint64_t>(), pkeys2.as<uint64_t>(), npairs, 32, 48));
     k_bstat_unpack<<<grid_for(npairs), 256, 0, st>>>(pkeys.as<uint64_t>(), npairs, s.in_pairs.as<uint2>());
     if (npairs)
       k_bstat_bounds<<<grid_for(npairs), 256, 0, st>>>(pkeys.as<uint64_t>(), npairs, inoff.as<uint64_t>(),

@@ -329,6 +329,7 @@ eh_status clique_count_deep(eh_matrix *m, uint32_t k, uint64_t *count, double *m
 // inputs up to its merge-sort limit, builds its digit mask as (1 << (begin_bit + bits)) - 1, a
 // shift by 64 for a range ending at bit 64 with begin_bit > 0, and then orders by the wrong bits)
 eh_status prim_sort_u64(eh_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int end_bit);
+eh_status prim_sort_u64_range(eh_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int begin_bit, int end_bit);
 eh_status prim_sort_pairs_u32(eh_ctx *ctx, uint32_t *keys, uint32_t *keys_tmp, uint32_t *vals,
                               uint32_t *vals_tmp, uint64_t n);
 eh_status prim_sort_pairs_u64_u32(eh_ctx *ctx, uint64_t *keys, uint64_t *keys_tmp, uint32_t *vals,

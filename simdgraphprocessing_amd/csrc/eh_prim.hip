@@ -28,6 +28,24 @@ eh_status prim_sort_u64(eh_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, 
   return EH_OK;
 }
 
+// keys ordered by bits [begin_bit, end_bit) only, stably (an LSD radix sort keeps the input
+// order among equal digits); end_bit must stay below 64 when begin_bit > 0 (see prim_sort_u64)
+eh_status prim_sort_u64_range(eh_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int begin_bit, int end_bit) {
+  if (n < 2) return EH_OK;
+  if (begin_bit < 0 || end_bit > 64 || begin_bit >= end_bit || (begin_bit > 0 && end_bit == 64)) {
+    eh_set_error("prim_sort_u64_range: bit range [%d, %d) not supported", begin_bit, end_bit);
+    return EH_ERR_INVALID;
+  }
+  size_t bytes = 0;
+  EH_HIP(rocprim::radix_sort_keys(nullptr, bytes, keys, tmp, (size_t)n, (unsigned)begin_bit, (unsigned)end_bit,
+                                  ctx->stream));
+  EH_CHECK(ensure_scratch(ctx, bytes));
+  EH_HIP(rocprim::radix_sort_keys(ctx->scratch.p, bytes, keys, tmp, (size_t)n, (unsigned)begin_bit,
+                                  (unsigned)end_bit, ctx->stream));
+  EH_HIP(hipMemcpyAsync(keys, tmp, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  return EH_OK;
+}
+
 eh_status prim_sort_pairs_u32(eh_ctx *ctx, uint32_t *keys, uint32_t *keys_tmp, uint32_t *vals,
                               uint32_t *vals_tmp, uint64_t n) {
   if (n < 2) return EH_OK;
