@@ -67,6 +67,9 @@ def lib():
     L.orc_bfs_path.argtypes = [C.c_uint32, _u64p, _u32p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                C.c_uint32, C.c_void_p]
     L.orc_bfs_path.restype = C.c_uint32
+    L.orc_bfs_path_mt.argtypes = [C.c_uint32, _u64p, _u32p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                  C.c_uint32, C.c_void_p, C.c_int]
+    L.orc_bfs_path_mt.restype = C.c_uint32
     L.orc_bitset_difference.argtypes = [_u8p, C.c_size_t, _u8p, C.c_size_t, _u32p]
     L.orc_bitset_difference.restype = C.c_size_t
     L.orc_row_image_bound.restype = C.c_size_t
@@ -600,13 +603,18 @@ def load_directed_binary(path_or_bytes):
     return n, outs, ins, ids
 
 
-def bfs_path(n, off, nbr, start, max_depth=0xEFFFFFF, cap=64):
+def bfs_path(n, off, nbr, start, max_depth=0xEFFFFFF, cap=64, threads=1):
     """n_path's BFS (apps/n_path.cpp:41-85) over rows [0, n): returns
-    {path_length, rounds, level_sizes, reached}."""
+    {path_length, rounds, level_sizes, reached}.  threads > 1: each round's union over that many
+    threads (apps/n_path.cpp:48-77), same results."""
     rounds, reached = C.c_uint32(), C.c_uint64()
     sizes = np.zeros(cap, np.uint64)
-    p = lib().orc_bfs_path(n, _u64(off), _u32(nbr), start, max_depth, C.addressof(rounds), sizes.ctypes.data,
-                           cap, C.addressof(reached))
+    if threads > 1:
+        p = lib().orc_bfs_path_mt(n, _u64(off), _u32(nbr), start, max_depth, C.addressof(rounds),
+                                  sizes.ctypes.data, cap, C.addressof(reached), threads)
+    else:
+        p = lib().orc_bfs_path(n, _u64(off), _u32(nbr), start, max_depth, C.addressof(rounds), sizes.ctypes.data,
+                               cap, C.addressof(reached))
     return {"path_length": int(p), "rounds": rounds.value, "level_sizes": sizes[:min(rounds.value, cap)].tolist(),
             "reached": reached.value}
 

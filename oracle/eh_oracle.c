@@ -1577,6 +1577,72 @@ uint32_t orc_bfs_path(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint
   return path;
 }
 
+/* The same search with the frontier's set_union spread over threads as n_path does it
+ * (apps/n_path.cpp:48-77: frontier.par_foreach into per-thread visited bitsets, then their
+ * union): chunks of 100 frontier rows (par_for_range), each thread ORs its rows into its own
+ * bitset, the bitsets are merged into the visited set, and the new frontier is the
+ * difference with the old visited set, in id order.  Same results as orc_bfs_path. */
+typedef struct {
+  const uint64_t *off;
+  const uint32_t *nbr;
+  const uint32_t *frontier;
+  uint64_t **bufs;
+} bfs_par_ctx;
+
+static void bfs_par_body(void *c, int tid, uint64_t f) {
+  const bfs_par_ctx *b = (const bfs_par_ctx *)c;
+  uint64_t *vb = b->bufs[tid];
+  const uint32_t v = b->frontier[f];
+  for (uint64_t k = b->off[v]; k < b->off[v + 1]; k++) vb[b->nbr[k] >> 6] |= 1ull << (b->nbr[k] & 63);
+}
+
+uint32_t orc_bfs_path_mt(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint32_t start, uint32_t max_depth,
+                         uint32_t *rounds, uint64_t *level_sizes, uint32_t cap, uint64_t *reached, int threads) {
+  if (threads < 1) threads = 1;
+  const size_t words = (size_t)n / 64 + 2;
+  uint64_t *visited = (uint64_t *)calloc(words, 8), *old = (uint64_t *)calloc(words, 8);
+  uint64_t **bufs = (uint64_t **)malloc(sizeof(uint64_t *) * (size_t)threads);
+  for (int t = 0; t < threads; t++) bufs[t] = (uint64_t *)calloc(words, 8);
+  uint32_t *frontier = (uint32_t *)malloc(4 * ((size_t)n + 1));
+  size_t nf = 1;
+  uint32_t path = 0, r = 0;
+  frontier[0] = start;
+  visited[start >> 6] |= 1ull << (start & 63);
+  bfs_par_ctx c = {off, nbr, frontier, bufs};
+  for (;;) {
+    memcpy(old, visited, 8 * words);
+    const int used = par_for_range(threads, 0, nf, 100, bfs_par_body, &c);
+    for (int t = 0; t < used; t++) {
+      for (size_t w = 0; w < words; w++) visited[w] |= bufs[t][w];
+      memset(bufs[t], 0, 8 * words);
+    }
+    nf = 0;
+    for (size_t w = 0; w < words; w++) {
+      uint64_t d = visited[w] & ~old[w];
+      while (d) {
+        frontier[nf++] = (uint32_t)(64 * w + (size_t)__builtin_ctzll(d));
+        d &= d - 1;
+      }
+    }
+    if (r < cap && level_sizes) level_sizes[r] = nf;
+    r++;
+    if (nf == 0 || path >= max_depth) break;
+    path++;
+  }
+  if (rounds) *rounds = r;
+  if (reached) {
+    uint64_t cnt = 0;
+    for (size_t w = 0; w < words; w++) cnt += (uint64_t)__builtin_popcountll(visited[w]);
+    *reached = cnt;
+  }
+  for (int t = 0; t < threads; t++) free(bufs[t]);
+  free(bufs);
+  free(visited);
+  free(old);
+  free(frontier);
+  return path;
+}
+
 /* set_difference (src/set/ops/difference.hpp:5-84) on bitset images: A's words AND NOT
  * B's over their common span, A's words outside it as they are */
 size_t orc_bitset_difference(const uint8_t *a, size_t a_bytes, const uint8_t *b, size_t b_bytes, uint32_t *out) {

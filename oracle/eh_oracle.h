@@ -120,6 +120,10 @@ uint64_t orc_count_rows(const orc_matrix *M, int k, int threads, uint64_t *per_r
  * visited vertices. */
 uint32_t orc_bfs_path(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint32_t start, uint32_t max_depth,
                       uint32_t *rounds, uint64_t *level_sizes, uint32_t cap, uint64_t *reached);
+/* the same search with each round's set_union over `threads` threads (per-thread visited
+ * bitsets merged after the round, as apps/n_path.cpp:48-77 does); same results */
+uint32_t orc_bfs_path_mt(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint32_t start, uint32_t max_depth,
+                         uint32_t *rounds, uint64_t *level_sizes, uint32_t cap, uint64_t *reached, int threads);
 /* set_difference(C, A, B) of two bitset images [u64 first word][words] into a sorted
  * uint list: the elements of A not in B.  Returns the count. */
 size_t orc_bitset_difference(const uint8_t *a, size_t a_bytes, const uint8_t *b, size_t b_bytes, uint32_t *out);

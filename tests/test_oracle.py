@@ -773,6 +773,18 @@ def test_bfs_levels_match_independent_bfs():
                     assert O.bfs_path(n, off, nbr, start, depth)["path_length"] == min(ecc, depth)
 
 
+def test_bfs_threaded_equals_single():
+    """The threaded search (per-thread union bitsets merged each round, as n_path does) gives the
+    single-threaded one's results, including frontiers of more than one 100-row chunk."""
+    n = 1 << 14
+    off, nbr = O.canonical_oriented(n, *O.gen_rmat(14, 8, 5))["sym"]
+    for start in (0, 17, 999, n - 1):
+        for depth in (1, 2, 0xEFFFFFF):
+            want = O.bfs_path(n, off, nbr, start, depth)
+            for threads in (2, 7):
+                assert O.bfs_path(n, off, nbr, start, depth, threads=threads) == want, (start, depth, threads)
+
+
 def test_bfs_closed_forms():
     """A path of k vertices from one end has eccentricity k-1; a star's centre 1, a leaf 2;
     an isolated start 0."""
