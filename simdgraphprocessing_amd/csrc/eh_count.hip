@@ -1322,7 +1322,71 @@ __global__ void __launch_bounds__(256) k_small_rows(CountArgs a, const uint32_t 
 // With a.counts3 set the kernel is the triangle pass of the same rows too: every hit of
 // pass 1 is a triangle {i, R[p], R[c]}, so each row's hits go to its graph's triangle
 // counter, and rows of two entries (one wedge, no 4-clique) run pass 1 as well.
-constexpr uint32_t SMALL4_META_WORDS = 388;
+constexpr uint32_t SMALL4_META_WORDS = 456;
+
+// Pass 1 of k_small_cliques4 in chunks: the pairs (p, c), p < c, of a row of d entries as
+// chunks of up to eight p's of one column c (p0 = 0, 8, ...): column c has ceil(c / 8) of
+// them, the columns 8(b-1)+1 .. 8b have b each, so with D = d - 1 = 8F + R the row holds
+// 4F(F+1) + R(F+1).  A chunk reads its B row's descriptor once and tests its eight ids
+// together (one aligned 32-byte read of the row), where one query per pair re-read both.
+__device__ __forceinline__ uint32_t chunk_count(uint32_t d) {
+  if (d < 2u) return 0u;
+  const uint32_t D = d - 1u, F = D >> 3, R = D & 7u;
+  return 4u * F * (F + 1u) + R * (F + 1u);
+}
+
+// chunk r of a row -> (column c, first p): block b holds the chunks 4b(b-1) .. 4b(b+1) - 1
+__device__ __forceinline__ void chunk_decode(uint32_t r, uint32_t *c, uint32_t *p0) {
+  uint32_t b = (uint32_t)((1.0f + sqrtf(1.0f + (float)r)) * 0.5f);
+  if (b < 1u) b = 1u;
+  while (4u * b * (b + 1u) <= r) b++;
+  while (b > 1u && 4u * b * (b - 1u) > r) b--;
+  const uint32_t o = r - 4u * b * (b - 1u);
+  *c = 8u * (b - 1u) + 1u + o / b;
+  *p0 = (o % b) * 8u;
+}
+
+// bit u set when x[u] (u < np) is in row(j) (descriptor d); the ids are tested together
+__device__ __forceinline__ uint32_t chunk_hits(const CountArgs &a, const RowDesc &d, const uint32_t (&x)[8],
+                                               uint32_t np) {
+  if (d.card == 0) return 0u;
+  const uint32_t t = desc_type(d);
+  uint32_t h = 0;
+  if (t == EH_BS) {
+    uint32_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const uint32_t w = (x[u] >> 6) - d.w0;
+      v[u] = ((uint32_t)u < np && w < desc_aux(d))
+                 ? reinterpret_cast<const uint32_t *>(a.bsw)[2ull * (d.off + w) + ((x[u] >> 5) & 1u)]
+                 : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) h |= ((v[u] >> (x[u] & 31u)) & 1u) << u;
+    return h;
+  }
+  if (t == EH_PS) {
+    for (uint32_t u = 0; u < np; u++) h |= row_contains(a, d, x[u]) << u;
+    return h;
+  }
+  // uint, uint searched and the block rows' canonical lists: up to eight searches of one sorted
+  // list in lockstep (one length for all); the slots past np issue no loads
+  const uint32_t *L = a.unbr + (uint64_t)d.off * 4;
+  uint32_t b[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  uint32_t len = d.card;
+  while (len > 1u) {
+    const uint32_t half = len >> 1;
+    uint32_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = ((uint32_t)u < np) ? L[b[u] + half] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int u = 0; u < 8; u++) b[u] = (v[u] <= x[u]) ? b[u] + half : b[u];
+    len -= half;
+  }
+#pragma unroll
+  for (int u = 0; u < 8; u++) h |= ((uint32_t)u < np && L[b[u]] == x[u]) ? (1u << u) : 0u;
+  return h;
+}
 constexpr uint32_t LOCAL4_META = 196;  // k_local4: pair scan bases (65), offsets (64), columns (64)
 constexpr uint32_t LOCAL4_MID = 128, LOCAL4_MAX = 512;  // the two k_local4 row classes
 constexpr uint32_t SMALL_ADJ_WORDS = 1024;
@@ -1333,7 +1397,7 @@ __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint3
   const uint32_t lane = lane_id();
   uint32_t *meta = lds_all + (threadIdx.x >> 6) * (SMALL4_META_WORDS + SMALL_ADJ_WORDS);
   uint32_t *mbase = meta, *moff = meta + 65, *rcnt = meta + 130, *mdeg = meta + 194, *mwb = meta + 258;
-  uint32_t *rcnt3 = meta + 322;
+  uint32_t *rcnt3 = meta + 322, *cbase = meta + 386;
   uint32_t *adj = meta + SMALL4_META_WORDS;
   const bool per_graph = a.graph_of != nullptr;
   const bool tri = a.counts3 != nullptr;
@@ -1359,53 +1423,44 @@ __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint3
       const uint32_t wb = wave_excl_scan(words, lane, &wtot);
       const bool sel = pend && wb + words <= SMALL_ADJ_WORDS;
       const uint32_t items = sel ? (d * (d - 1u)) / 2u : 0u;
-      uint32_t tot;
+      uint32_t tot, ctot;
       const uint32_t base = wave_excl_scan(items, lane, &tot);
       tot = rfl(tot);
+      const uint32_t cb = wave_excl_scan(sel ? chunk_count(d) : 0u, lane, &ctot);
+      ctot = rfl(ctot);
       mbase[lane] = base;
+      cbase[lane] = cb;
       moff[lane] = off;
       mdeg[lane] = sel ? d : 0u;
       mwb[lane] = wb;
-      if (lane == 0) mbase[64] = tot;
+      if (lane == 0) {
+        mbase[64] = tot;
+        cbase[64] = ctot;
+      }
       for (uint32_t w = lane; w < SMALL_ADJ_WORDS; w += 64) adj[w] = 0u;
       wave_sync();
-      // pass 1: local edges, SMALL_NQ membership tests per lane in flight (a lane's row
-      // index moves forward with its slots, as in k_small_rows)
+      // pass 1: local edges, a chunk of up to eight pairs of one column per lane (a lane's
+      // row index moves forward with its chunks, as in k_small_rows); a chunk's hits are one
+      // byte of adj(c): p0 is a multiple of 8, so its bits never cross a word
       uint32_t qc = 0;
-      for (uint32_t t = lane; t < tot; t += 64 * SMALL_NQ) {
-        uint32_t q[SMALL_NQ], x[SMALL_NQ], j[SMALL_NQ], pp[SMALL_NQ], cc[SMALL_NQ];
-#pragma unroll
-        for (int k = 0; k < SMALL_NQ; k++) {
-          const uint32_t tk = t + 64u * k;
-          while (qc < 63u && mbase[qc + 1u] <= tk) qc++;
-          q[k] = qc;
-          x[k] = 0;
-          j[k] = 0;
-          pp[k] = 0;
-          cc[k] = 0;
-          if (tk < tot) {
-            tri_decode(tk - mbase[q[k]], &pp[k], &cc[k]);
-            const uint32_t *R = a.unbr + (uint64_t)moff[q[k]] * 4;
-            x[k] = R[pp[k]];
-            j[k] = R[cc[k]];
-          }
-        }
-        MemberQuery m[SMALL_NQ];
-#pragma unroll
-        for (int k = 0; k < SMALL_NQ; k++) {
-          const RowDesc dd = (t + 64u * k < tot) ? a.desc[j[k]] : RowDesc{0u, 0u, 0u, 0u};
-          mq_start(a, dd, x[k], m[k]);
-        }
-        mq_finish<SMALL_NQ>(m);
-#pragma unroll
-        for (int k = 0; k < SMALL_NQ; k++) {
-          if (m[k].hit) {
-            const uint32_t wq = (mdeg[q[k]] + 31u) >> 5;
-            atomicOr(&adj[mwb[q[k]] + cc[k] * wq + (pp[k] >> 5)], 1u << (pp[k] & 31u));
-            if (tri) {
-              if (per_graph) atomicAdd(&rcnt3[q[k]], 1u);
-              else acc3++;
-            }
+      for (uint32_t t = lane; t < ctot; t += 64) {
+        while (qc < 63u && cbase[qc + 1u] <= t) qc++;
+        const uint32_t q = qc;
+        uint32_t c, p0;
+        chunk_decode(t - cbase[q], &c, &p0);
+        // the row's entries p0 .. p0 + 7 (aligned: rows start on 16 bytes, p0 is a multiple of
+        // 8); those past c are not tested, and a read past the row's end stays in the arena
+        const uint32_t *R = a.unbr + (uint64_t)moff[q] * 4;
+        const uint4 xa = *reinterpret_cast<const uint4 *>(R + p0), xb = *reinterpret_cast<const uint4 *>(R + p0 + 4);
+        const uint32_t x[8] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
+        const RowDesc dd = a.desc[R[c]];
+        const uint32_t h = chunk_hits(a, dd, x, min(8u, c - p0));
+        if (h) {
+          const uint32_t wq = (mdeg[q] + 31u) >> 5;
+          atomicOr(&adj[mwb[q] + c * wq + (p0 >> 5)], h << (p0 & 31u));
+          if (tri) {
+            if (per_graph) atomicAdd(&rcnt3[q], (uint32_t)__popc(h));
+            else acc3 += (uint32_t)__popc(h);
           }
         }
       }
