@@ -1422,7 +1422,8 @@ __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint3
       uint32_t wtot;
       const uint32_t wb = wave_excl_scan(words, lane, &wtot);
       const bool sel = pend && wb + words <= SMALL_ADJ_WORDS;
-      const uint32_t items = sel ? (d * (d - 1u)) / 2u : 0u;
+      // pass 2 visits every column c >= 1 of a row: d - 1 of them
+      const uint32_t items = sel ? d - 1u : 0u;
       uint32_t tot, ctot;
       const uint32_t base = wave_excl_scan(items, lane, &tot);
       tot = rfl(tot);
@@ -1465,20 +1466,26 @@ __global__ void __launch_bounds__(256) k_small_cliques4(CountArgs a, const uint3
         }
       }
       wave_sync();
-      // pass 2: every local edge (b, c) adds |adj(b) & adj(c)|
+      // pass 2: every local edge (b, c) adds |adj(b) & adj(c)|; a lane takes a column c and
+      // walks the set bits b of adj(c), its local edges, instead of every pair (b, c)
       qc = 0;
       for (uint32_t t = lane; t < tot; t += 64) {
         while (qc < 63u && mbase[qc + 1u] <= t) qc++;
         const uint32_t q = qc;
-        uint32_t b, c;
-        tri_decode(t - mbase[q], &b, &c);
+        const uint32_t c = t - mbase[q] + 1u;
         const uint32_t wq = (mdeg[q] + 31u) >> 5;
         const uint32_t *ac = adj + mwb[q] + c * wq;
         uint32_t h = 0;
-        if ((ac[b >> 5] >> (b & 31u)) & 1u) {
-          const uint32_t *ab = adj + mwb[q] + b * wq;
-          // adj(b) only holds bits below b: words past b's word are zero
-          for (uint32_t w = 0; w <= (b >> 5); w++) h += __popc(ab[w] & ac[w]);
+        // adj(c) only holds bits below c (c < 64: at most two words)
+        for (uint32_t wc = 0; wc <= (c >> 5); wc++) {
+          uint32_t bits = ac[wc];
+          while (bits) {
+            const uint32_t b = 32u * wc + (uint32_t)__builtin_ctz(bits);
+            bits &= bits - 1u;
+            const uint32_t *ab = adj + mwb[q] + b * wq;
+            // adj(b) only holds bits below b: words past b's word are zero
+            for (uint32_t w = 0; w <= (b >> 5); w++) h += __popc(ab[w] & ac[w]);
+          }
         }
         if (per_graph) {
           if (h) atomicAdd(&rcnt[q], h);
