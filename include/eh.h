@@ -376,7 +376,10 @@ eh_status eh_batch_graph_download(eh_batch *b, uint32_t g, uint64_t *row_off, ui
  * hipGraph; mode 1: one launch sequence per graph.  tri/k4 get ngraphs counts
  * each (either may be NULL: that pass is not run).  tri_opts / k4_opts set the
  * schedule and launch shape of the triangle and 4-clique passes (NULL: defaults);
- * the graph is recorded again whenever either pass changes. */
+ * the graph is recorded again whenever either pass changes.  When both passes are tuned
+ * and every row of both is a short row (one small_degree for both), the 4-clique kernel
+ * counts the triangles too and the triangle pass is not launched; the counts are those of
+ * the two passes run apart. */
 eh_status eh_batch_count(eh_batch *b, int mode, const eh_run_opts *tri_opts,
                          const eh_run_opts *k4_opts, uint64_t *tri, uint64_t *k4, double *ms);
 /* No reference counterpart: build the arena's layouts again from the same graphs, under
