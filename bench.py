@@ -245,9 +245,12 @@ def config_c3(ctx, with_cpu):
 
 
 def config_c4(ctx, with_cpu):
-    """4096 ER graphs (n in [64,8192], mean degree in [4,64)), seed 4: per-graph triangles + 4-cliques."""
+    """4096 ER graphs (n in [64,8192], mean degree in [4,64)), seed 4: per-graph triangles + 4-cliques,
+    the arena under the learned layouts when the trained selector is there (the density rule
+    otherwise); every rule's layouts are timed beside them (``selector``)."""
     import numpy as np
-    _, b = build(ctx, "c4", "hybrid")
+    policy = "learned" if load_selector(ctx) else "hybrid"
+    _, b = build(ctx, "c4", policy)
     info = b.info()
     params = committed_shape("c4")
     kw = {"tri_opts": params["tri"], "k4_opts": params["k4"]}
@@ -269,17 +272,16 @@ def config_c4(ctx, with_cpu):
     ms_g = min(per_graph.values())
     tb, kb = b.alg_bytes()
     out = {"workload": "4096 ER graphs n~U[64,8192] mean degree~U[4,64) seed 4, triangles + 4-cliques per graph",
-           "graphs": info["graphs"], "vertices": info["vertices"], "edges": info["edges"],
+           "layout_policy": policy, "graphs": info["graphs"], "vertices": info["vertices"], "edges": info["edges"],
            "persistent_ms": ms_pers, "persistent_runs_ms": ms_p, "per_graph_ms": ms_g,
            "per_graph_by_shape_ms": per_graph, "speedup_vs_per_graph": ms_g / ms_pers, "params": params,
            "triangles_total": int(tri.sum()), "four_cliques_total": int(k4.sum()),
            "alg_bytes_tri": tb, "alg_bytes_k4": kb}
-    out.update(byte_figures(tb + kb, ms_pers, traffic_record("c4", "hybrid", params)))
+    out.update(byte_figures(tb + kb, ms_pers, traffic_record("c4", policy, params)))
     # the arena's layouts under the learned selection and every rule (rebuilt in place; the
     # persistent passes at the same shapes, each with its build time)
     from simdgraphprocessing_amd import selector
-    if os.path.exists(selector.PARAMS_PATH):
-        ctx.set_selector(selector.load())
+    if policy == "learned":
         out["selector"] = selector.compare_batch(ctx, b, steps=5, log=log)
     if with_cpu:
         # all 8192 counts against the oracle in one pass over the union graph
