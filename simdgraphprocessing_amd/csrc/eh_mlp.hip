@@ -165,28 +165,40 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr uint32_t FEAT_LANE_MAX = 32;  // rows up to this long are read by one lane
+constexpr uint32_t FEAT_LANE_MAX = 64;  // rows up to this long are read by one lane
 
 // the statistics of a row of at most FEAT_LANE_MAX ids, read by one lane; ids are taken less
-// `base` (the first vertex of the row's graph in a batch arena, else 0)
+// `base` (the first vertex of the row's graph in a batch arena, else 0).  The row is read as
+// 16-byte vectors (rows start on 16 bytes and are padded to four ids), 32 ids at a time with
+// every load of a round issued before any id is used
 __device__ __forceinline__ RowStats lane_stats(const uint32_t *r, uint32_t c, uint32_t lowcut, uint32_t base) {
   RowStats st = {c, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
   if (!c) return st;
-  st.lo = r[0] - base;
   st.hi = r[c - 1] - base;
   st.mid = r[c / 2] - base;
+  st.lo = r[0] - base;
   st.parts = 1;
   const uint32_t qcut = st.lo + (st.hi - st.lo) / 4u;
   uint32_t prev = st.lo;
-  for (uint32_t k = 0; k < c; k++) {
-    const uint32_t x = r[k] - base;
-    if (k) {
-      st.parts += (x >> 16) != (prev >> 16);
-      st.maxgap = max(st.maxgap, x - prev);
+  for (uint32_t k0 = 0; k0 < c; k0 += 32u) {
+    uint4 v[8];
+#pragma unroll
+    for (uint32_t q = 0; q < 8u; q++)
+      v[q] = (k0 + 4u * q < c) ? reinterpret_cast<const uint4 *>(r + k0)[q] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (uint32_t k = 0; k < 32u; k++) {
+      if (k0 + k < c) {
+        const uint4 w = v[k >> 2];
+        const uint32_t x = ((k & 3u) == 0u ? w.x : (k & 3u) == 1u ? w.y : (k & 3u) == 2u ? w.z : w.w) - base;
+        if (k0 + k) {
+          st.parts += (x >> 16) != (prev >> 16);
+          st.maxgap = max(st.maxgap, x - prev);
+        }
+        st.low += x < lowcut;
+        st.quarter += x <= qcut;
+        prev = x;
+      }
     }
-    st.low += x < lowcut;
-    st.quarter += x <= qcut;
-    prev = x;
   }
   return st;
 }
