@@ -160,6 +160,27 @@ def cpu_triangles(n, off, nbr, gpu_count, runs=5, budget_s=30.0):
             "oracle": "oracle/eh_oracle.c hybrid policy, dynamic chunks of 100 (common.hpp:213-276)"}
 
 
+def cpu_reference_triangles(n, off, nbr, gpu_count, runs=3):
+    """The reference's own triangle program (apps/undirected_triangle_counting.cpp, built from its
+    sources into oracle/_ref by the oracle's Makefile) on the same graph and every host CPU: the
+    median over `runs` runs of the counting phase it times itself (common::stopClock, its "Time[...]"
+    line), the count checked; None when the program was not built."""
+    import oracle as O
+    if not O.ref_available():
+        return None
+    threads = cpu_threads()
+    stats = [O.run_ref_stats("undirected_triangle_counting", n, off, nbr, threads=threads, layout="hybrid")
+             for _ in range(runs)]
+    for st in stats:
+        assert st["count"] == gpu_count, f"reference count {st['count']} != GPU count {gpu_count}"
+    secs = sorted(st["seconds"] for st in stats)[len(stats) // 2]
+    edges = int(off[-1]) // 2
+    return {"value": edges / secs, "unit": "edges/s", "cores": threads, "kind": "reference",
+            "sample": "full graph", "seconds": secs, "runs_s": [st["seconds"] for st in stats],
+            "timing": f"median of {runs}, the program's own counting-phase timer", "count_checked": True,
+            "program": "oracle/_ref/undirected_triangle_counting --layout=hybrid"}
+
+
 # ------------------------------------------------------------------ configs
 
 def config_c1(ctx, with_cpu):
@@ -549,6 +570,17 @@ def main():
         cpu = cpu_triangles(gi["n"], off, nbr, cp)
         log(f"cpu baseline: {cpu['value']:.3e} edges/s on {cpu['cores']} threads ({cpu['sample']}, "
             f"{cpu['timing']})")
+        # the reference's own program on the same graph and threads; the faster of the two is the
+        # reported baseline, the other rides beside it
+        ref = cpu_reference_triangles(gi["n"], off, nbr, cp)
+        if ref is not None:
+            log(f"reference program: {ref['value']:.3e} edges/s on {ref['cores']} threads ({ref['seconds']:.3f} s)")
+            if ref["value"] > cpu["value"]:
+                ref["cpu"] = cpu.pop("cpu")
+                ref["port"] = cpu
+                cpu = ref
+            else:
+                cpu["reference_program"] = ref
         del off, nbr
     policies = {k: {"params": v["params"], "build_ms": v["build_ms"], "sched_ms": v["sched_ms"],
                     "one_shot_ms": v["one_shot_ms"], "first_pass_ms": v["first_pass_ms"]} for k, v in legs.items()}
