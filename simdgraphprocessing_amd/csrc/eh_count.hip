@@ -3107,17 +3107,24 @@ eh_status count_run(eh_matrix *m, int k, const eh_run_opts *opts, uint64_t *d_co
         }
   }
   uint32_t dev_err = 0;
-  EH_HIP(hipMemcpy(&dev_err, nx + 7, 4, hipMemcpyDeviceToHost));
+  std::vector<uint64_t> h(ncounts ? ncounts : 1);
+  if (!d_counts && count) {
+    // the error word and the counts sit in one buffer: one copy brings both
+    std::vector<uint64_t> words(16 + h.size());
+    EH_HIP(hipMemcpy(words.data(), pw.p, 8 * words.size(), hipMemcpyDeviceToHost));
+    dev_err = reinterpret_cast<const uint32_t *>(words.data())[7];
+    std::copy(words.begin() + 16, words.end(), h.begin());
+  } else {
+    EH_HIP(hipMemcpy(&dev_err, nx + 7, 4, hipMemcpyDeviceToHost));
+    if (count) EH_HIP(hipMemcpy(h.data(), dc, 8 * h.size(), hipMemcpyDeviceToHost));
+  }
   if (dev_err) {
     eh_set_error("counting pass: an LDS hash table had no free slot (device error word %#x); the count is "
                  "not valid", dev_err);
     return EH_ERR_INVALID;
   }
-  if (count) {
-    std::vector<uint64_t> h(ncounts ? ncounts : 1);
-    EH_HIP(hipMemcpy(h.data(), dc, 8 * h.size(), hipMemcpyDeviceToHost));
+  if (count)
     for (size_t q = 0; q < h.size(); q++) count[q] = h[q];
-  }
   if (stats) {
     stats->kernel_ms = total_ms / repeat;
     stats->sched_ms = sched_ms;
