@@ -1939,7 +1939,8 @@ struct SchedArgs {
   uint32_t *jstart;       // out: first entry of row i its A-side tasks cover
   uint32_t *jbelow;       // out: entries of row i below bstat_j
   uint32_t *is_small;     // out: 1 for those rows, 2 / 3 for the k_local4 classes
-  uint64_t *sortkey;      // out: (class << 62) | (order << 32) | i
+  uint64_t *sortkey;      // out: (i << 32) | (class << 30) | order, in row order (a stable sort on
+                          // the low 32 bits orders the rows by class, then order, then i)
   uint32_t *ntask;        // out: tasks of row i
   uint32_t *rowinfo;      // out: kind << 28 | graph id
   unsigned long long *stats;  // [0..3] pair types, [4] work units, [5] wedges
@@ -2006,7 +2007,7 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
     const bool local = !small && dA > 2 && dA <= s.local_max;
     s.is_small[i] = small ? 1u : local ? (dA <= LOCAL4_MID ? 2u : 3u) : 0u;
     if (dA == 0 || small || local || below == dA) {
-      s.sortkey[i] = (3ull << 62) | i;
+      s.sortkey[i] = ((uint64_t)i << 32) | (3u << 30);
       s.ntask[i] = 0;
       s.rowinfo[i] = 0;
       continue;
@@ -2057,7 +2058,7 @@ __global__ void __launch_bounds__(256) k_row_work(SchedArgs s) {
     }
     s.ntask[i] = nt;
     const uint64_t order = s.tuned ? (0x3FFFFFFFull - min<uint64_t>(work, 0x3FFFFFFFull)) : 0ull;
-    s.sortkey[i] = ((uint64_t)cls << 62) | (order << 32) | i;
+    s.sortkey[i] = ((uint64_t)i << 32) | ((uint64_t)cls << 30) | order;
   }
   // the statistics: summed over the block's waves in LDS, then one atomic per block and value
   // (one per wave had every wave of the grid queue on the same six words)
@@ -2079,9 +2080,10 @@ __global__ void k_gather_ntask(const uint64_t *sorted, const uint32_t *ntask, ui
                                uint32_t *nbig_rows) {
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
     const uint64_t key = sorted[q];
-    nt_sorted[q] = ntask[(uint32_t)key];
-    if ((key >> 62) == 0) atomicAdd(nbig_rows, 1u);
-    if ((key >> 62) == 2) atomicAdd(nbig_rows + 1, 1u);
+    nt_sorted[q] = ntask[(uint32_t)(key >> 32)];
+    const uint32_t cls = ((uint32_t)key >> 30) & 3u;
+    if (cls == 0) atomicAdd(nbig_rows, 1u);
+    if (cls == 2) atomicAdd(nbig_rows + 1, 1u);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) nt_sorted[n] = 0;
 }
@@ -2092,7 +2094,7 @@ __global__ void k_expand_tasks(const uint64_t *sorted, const uint32_t *nt_sorted
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
     const uint32_t nt = nt_sorted[q];
     if (!nt) continue;
-    const uint32_t i = (uint32_t)sorted[q];
+    const uint32_t i = (uint32_t)(sorted[q] >> 32);
     const uint32_t js = jstart[i];
     const uint32_t d = card[i] - js;
     const uint64_t base = toff[q];
@@ -2516,7 +2518,7 @@ eh_status schedule_build(eh_matrix *m, int sched, int k, const TuneParams &tp, S
     k_short_cuts<<<1, 64, 0, st>>>(spos.as<uint64_t>(), (uint32_t)nsmall, s.xq.as<uint32_t>());
     EH_HIP(hipGetLastError());
   }
-  EH_CHECK(prim_sort_u64(ctx, key.as<uint64_t>(), key2.as<uint64_t>(), n, 64));
+  EH_CHECK(prim_sort_u64(ctx, key.as<uint64_t>(), key2.as<uint64_t>(), n, 32));
   EH_CHECK(ntsorted.alloc(4ull * n + 8, &ctx->cache));
   EH_CHECK(toff.alloc(8ull * n + 16, &ctx->cache));
   k_gather_ntask<<<grid_for(n), 256, 0, st>>>(key.as<uint64_t>(), ntask.as<uint32_t>(), n,
